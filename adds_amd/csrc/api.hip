@@ -1,0 +1,803 @@
+// C ABI of the ADDS solver (include/adds.h): device-resident graphs, solver workspace, solve
+// orchestration (dist fill + persistent kernel + cleanup on one stream), status and watchdog,
+// batched/union solves and HIP-graph capture.
+//
+// Reference seams replaced here: CSRGraph::copy_to_gpu / allocOnDevice (csr_graph.cu:86-105, 309-328),
+// gg_main's grid sizing, worklist sizing and timed solve (ads_int/kernel.cu:185-288), worklist::alloc /
+// reinit / set_param (wl.h:203-449) and the weight profiler (kernel.cu:151-183, 238-257).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/adds.h"
+#include "device.h"
+
+using namespace adds;
+
+static thread_local std::string g_detail;
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) {                                                                 \
+      g_detail = std::string(#expr) + ": " + hipGetErrorString(_e);                         \
+      return (_e == hipErrorOutOfMemory || _e == hipErrorMemoryAllocation) ? ADDS_ENOMEM    \
+                                                                           : ADDS_EHIP;     \
+    }                                                                                       \
+  } while (0)
+
+struct Workspace {
+  uint32_t* st = nullptr;
+  uint64_t* pool = nullptr;
+  uint32_t* tab = nullptr;
+  uint4* tasks = nullptr;
+  uint32_t* srcs = nullptr;
+  uint32_t nq = 0, maxv = 0, pool_chunks = 0, task_cap = 0, srcs_cap = 0;
+  uint32_t* host_state = nullptr;       // pinned copy of the first 8 state lines
+  uint32_t* host_abort = nullptr;       // pinned, device-visible abort word
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // solve start, solve end, state copied
+  hipStream_t last_stream = nullptr;
+  bool pending = false;                 // a NO_SYNC solve has not been collected
+  double timeout_ms = 0;
+};
+
+struct adds_graph {
+  int device = 0;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  adds_wtype wt = ADDS_W_U32;
+  uint32_t* row = nullptr;
+  uint32_t* col = nullptr;
+  uint32_t* w = nullptr;
+  float ave_wt = 1.f;  // mean sampled weight (the reference's profiler statistic)
+  Workspace ws;
+  uint32_t* outbuf = nullptr;  // device staging for host outputs
+  uint64_t outbuf_cap = 0;
+  // transpose (backward)
+  uint32_t* rrow = nullptr;
+  uint32_t* reid = nullptr;
+  uint32_t* rsrc = nullptr;
+  // batched union cache
+  std::vector<const adds_graph*> union_key;
+  adds_graph* union_g = nullptr;
+  std::vector<uint32_t> union_off;
+};
+
+struct adds_solve_graph {
+  adds_graph* g = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" const char* adds_strerror(adds_status s) {
+  switch (s) {
+    case ADDS_OK: return "ok";
+    case ADDS_EINVAL: return "invalid argument";
+    case ADDS_ENOMEM: return "out of memory";
+    case ADDS_EOVERFLOW: return "worklist capacity exceeded";
+    case ADDS_ETIMEOUT: return "solve timed out or was aborted";
+    case ADDS_EHIP: return "HIP runtime error";
+    case ADDS_EIO: return "I/O error";
+    case ADDS_ENODEV: return "no GPU device";
+  }
+  return "unknown status";
+}
+
+extern "C" const char* adds_last_error_detail(void) { return g_detail.c_str(); }
+
+extern "C" adds_status adds_set_device(int device) {
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return ADDS_ENODEV;
+  if (device < 0 || device >= cnt) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(device));
+  return ADDS_OK;
+}
+
+extern "C" uint32_t adds_device_cu_count(void) {
+  int dev = 0, cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return (uint32_t)cu;
+}
+
+static adds_status free_workspace(Workspace& ws) {
+  if (ws.st) (void)hipFree(ws.st);
+  if (ws.pool) (void)hipFree(ws.pool);
+  if (ws.tab) (void)hipFree(ws.tab);
+  if (ws.tasks) (void)hipFree(ws.tasks);
+  if (ws.srcs) (void)hipFree(ws.srcs);
+  if (ws.host_state) (void)hipHostFree(ws.host_state);
+  if (ws.host_abort) (void)hipHostFree(ws.host_abort);
+  if (ws.ev0) (void)hipEventDestroy(ws.ev0);
+  if (ws.ev1) (void)hipEventDestroy(ws.ev1);
+  if (ws.ev2) (void)hipEventDestroy(ws.ev2);
+  ws = Workspace();
+  return ADDS_OK;
+}
+
+static void release_graph_memory(adds_graph* g) {
+  free_workspace(g->ws);
+  if (g->row) (void)hipFree(g->row);
+  if (g->col) (void)hipFree(g->col);
+  if (g->w) (void)hipFree(g->w);
+  if (g->outbuf) (void)hipFree(g->outbuf);
+  if (g->rrow) (void)hipFree(g->rrow);
+  if (g->reid) (void)hipFree(g->reid);
+  if (g->rsrc) (void)hipFree(g->rsrc);
+  if (g->union_g) {
+    release_graph_memory(g->union_g);
+    delete g->union_g;
+  }
+  g->row = g->col = g->w = g->outbuf = g->rrow = g->reid = g->rsrc = nullptr;
+  g->union_g = nullptr;
+}
+
+// Mean of up to 2^20 evenly spaced weights: the statistic the reference's profiler_kernel samples
+// (ads_int/kernel.cu:151-183), computed once per graph on the host copy.
+static float sample_mean_weight(const uint32_t* w, uint64_t m, adds_wtype wt) {
+  if (m == 0) return 1.f;
+  const uint64_t samples = std::min<uint64_t>(m, 1u << 20);
+  double s = 0;
+  for (uint64_t i = 0; i < samples; i++) {
+    uint32_t x = w[(i * m) / samples];
+    if (wt == ADDS_W_F32) {
+      float f;
+      memcpy(&f, &x, 4);
+      s += f;
+    } else {
+      s += x;
+    }
+  }
+  double mean = s / (double)samples;
+  return mean > 0 ? (float)mean : 1.f;
+}
+
+static adds_status alloc_graph(uint32_t n, uint64_t m, adds_wtype wt, adds_graph** out) {
+  if (!out || n == 0 || n >= 0xFFFFFFFFu || m >= 0xFFFFFFFFull) return ADDS_EINVAL;
+  int dev = 0;
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return ADDS_ENODEV;
+  HIPCHK(hipGetDevice(&dev));
+  adds_graph* g = new adds_graph();
+  g->device = dev;
+  g->n = n;
+  g->m = m;
+  g->wt = wt;
+  hipError_t e = hipMalloc(&g->row, ((size_t)n + 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->col, std::max<uint64_t>(m, 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->w, std::max<uint64_t>(m, 1) * 4);
+  if (e != hipSuccess) {
+    g_detail = std::string("graph allocation: ") + hipGetErrorString(e);
+    release_graph_memory(g);
+    delete g;
+    return ADDS_ENOMEM;
+  }
+  *out = g;
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_graph_create(const uint32_t* row_ptr, const uint32_t* col, const void* w, uint32_t n,
+                                         uint64_t m, adds_wtype wt, adds_graph** out) {
+  if (!row_ptr || (m && (!col || !w)) || !out) return ADDS_EINVAL;
+  if (row_ptr[0] != 0 || row_ptr[n] != m) {
+    g_detail = "row_ptr[0] must be 0 and row_ptr[n] must equal m";
+    return ADDS_EINVAL;
+  }
+  for (uint32_t v = 0; v < n; v++)
+    if (row_ptr[v + 1] < row_ptr[v]) {
+      g_detail = "row_ptr is not monotone";
+      return ADDS_EINVAL;
+    }
+  const uint32_t* wb = (const uint32_t*)w;
+  for (uint64_t i = 0; i < m; i++) {
+    if (col[i] >= n) {
+      g_detail = "edge destination out of range";
+      return ADDS_EINVAL;
+    }
+    if (wt == ADDS_W_F32) {
+      uint32_t x = wb[i] & 0x7FFFFFFFu;  // -0.0 is accepted (it adds like +0.0)
+      if (x >= 0x7F800000u || (wb[i] >> 31 && x != 0)) {
+        g_detail = "float weights must be finite and not negative";
+        return ADDS_EINVAL;
+      }
+    }
+  }
+  adds_status rc = alloc_graph(n, m, wt, out);
+  if (rc != ADDS_OK) return rc;
+  adds_graph* g = *out;
+  HIPCHK(hipMemcpy(g->row, row_ptr, ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
+  if (m) {
+    HIPCHK(hipMemcpy(g->col, col, m * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g->w, w, m * 4, hipMemcpyHostToDevice));
+  }
+  g->ave_wt = sample_mean_weight(wb, m, wt);
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_graph_create_device(const uint32_t* row_ptr_dev, const uint32_t* col_dev,
+                                                const void* w_dev, uint32_t n, uint64_t m, adds_wtype wt,
+                                                adds_graph** out) {
+  if (!row_ptr_dev || (m && (!col_dev || !w_dev)) || !out) return ADDS_EINVAL;
+  adds_status rc = alloc_graph(n, m, wt, out);
+  if (rc != ADDS_OK) return rc;
+  adds_graph* g = *out;
+  HIPCHK(hipMemcpy(g->row, row_ptr_dev, ((size_t)n + 1) * 4, hipMemcpyDeviceToDevice));
+  if (m) {
+    HIPCHK(hipMemcpy(g->col, col_dev, m * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(g->w, w_dev, m * 4, hipMemcpyDeviceToDevice));
+    const uint64_t samples = std::min<uint64_t>(m, 1u << 16);
+    std::vector<uint32_t> hw(samples);
+    for (uint64_t i = 0; i < samples; i++)
+      HIPCHK(hipMemcpy(&hw[i], (const uint32_t*)w_dev + (i * m) / samples, 4, hipMemcpyDeviceToHost));
+    g->ave_wt = sample_mean_weight(hw.data(), samples, wt);
+  }
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* w_dev, void* stream) {
+  if (!g || (!w_dev && g->m)) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  if (g->m) HIPCHK(hipMemcpyAsync(g->w, w_dev, g->m * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  if (g->union_g) {  // weights changed: the union copy is stale
+    release_graph_memory(g->union_g);
+    delete g->union_g;
+    g->union_g = nullptr;
+    g->union_key.clear();
+  }
+  return ADDS_OK;
+}
+
+extern "C" void adds_graph_destroy(adds_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  (void)hipDeviceSynchronize();
+  release_graph_memory(g);
+  delete g;
+}
+
+extern "C" adds_status adds_graph_info(const adds_graph* g, uint32_t* n, uint64_t* m, adds_wtype* wt) {
+  if (!g) return ADDS_EINVAL;
+  if (n) *n = g->n;
+  if (m) *m = g->m;
+  if (wt) *wt = g->wt;
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_graph_device_arrays(const adds_graph* g, const uint32_t** row_ptr,
+                                                const uint32_t** col, const void** w) {
+  if (!g) return ADDS_EINVAL;
+  if (row_ptr) *row_ptr = g->row;
+  if (col) *col = g->col;
+  if (w) *w = g->w;
+  return ADDS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// parameters and workspace
+// ------------------------------------------------------------------------------------------------
+struct Resolved {
+  float delta;
+  uint32_t wg_per_cu, chunk, nb, parts, hub_degree, task_edges, concur, flags;
+  uint64_t pool_entries;
+  bool pool_fixed;
+  double timeout_ms;
+};
+
+// Default bucket width.  The reference derives it from the mean weight and degree
+// (wl.h:427-449: nearest power of two to ave_wt/ave_degree/32, at least 4, then adapted online); this
+// solver has no online adaptation and uses a wider default, ave_wt * 8 / ave_degree, chosen by
+// measurement.  For int graphs it is rounded to a power of two like the reference's shift.
+static float default_delta(const adds_graph* g) {
+  double ave_deg = g->n ? (double)g->m / (double)g->n : 1.0;
+  if (ave_deg < 1) ave_deg = 1;
+  double d = (double)g->ave_wt * 8.0 / ave_deg;
+  if (g->wt == ADDS_W_U32) {
+    double p = std::exp2(std::round(std::log2(std::max(d, 1.0))));
+    return (float)p;
+  }
+  return (float)std::max(d, 1e-30);
+}
+
+static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t total_n, Resolved& r) {
+  adds_params z;
+  memset(&z, 0, sizeof(z));
+  if (!p) p = &z;
+  r.delta = p->delta > 0 ? p->delta : default_delta(g);
+  r.wg_per_cu = p->wg_per_cu ? p->wg_per_cu : 4;
+  r.chunk = p->chunk ? p->chunk : 64;
+  r.nb = p->nbuckets ? p->nbuckets : 64;
+  r.parts = p->parts ? p->parts : 8;
+  r.hub_degree = p->hub_degree ? p->hub_degree : 512;
+  r.task_edges = p->task_edges ? p->task_edges : 1024;
+  r.concur = p->concurrency ? p->concurrency : 1;
+  r.flags = p->flags;
+  r.timeout_ms = p->timeout_ms ? (double)p->timeout_ms : 120000.0;
+  r.pool_fixed = p->pool_entries != 0;
+  r.pool_entries = p->pool_entries ? p->pool_entries : 3 * total_n + (4u << 20);
+  if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 64 ||
+      (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
+      r.parts != 8) || r.concur < 1 || r.concur > 4 || r.task_edges == 0 || r.hub_degree < 64) {
+    g_detail = "parameter out of range";
+    return ADDS_EINVAL;
+  }
+  if (r.pool_entries < CH) r.pool_entries = CH;
+  return ADDS_OK;
+}
+
+static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t nsrc) {
+  Workspace& ws = g->ws;
+  const uint32_t nq = r.nb * r.parts;
+  const uint64_t chunks64 = (r.pool_entries + CH - 1) / CH;
+  if (chunks64 >= TAB_BAD) return ADDS_EINVAL;
+  const uint32_t chunks = (uint32_t)chunks64;
+  const uint64_t task_cap64 = std::max<uint64_t>(1u << 16, 4 * (g->m / r.task_edges) + 4096);
+  const uint32_t task_cap = (uint32_t)std::min<uint64_t>(task_cap64, 0x7FFFFFFFull);
+  if (!ws.host_state) {
+    HIPCHK(hipHostMalloc((void**)&ws.host_state, 8 * LINE * 4, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&ws.host_abort, 256, hipHostMallocCoherent | hipHostMallocMapped));
+    *ws.host_abort = 0;
+    HIPCHK(hipEventCreate(&ws.ev0));
+    HIPCHK(hipEventCreate(&ws.ev1));
+    HIPCHK(hipEventCreate(&ws.ev2));
+  }
+  if (ws.nq != nq || ws.pool_chunks < chunks || ws.task_cap < task_cap || !ws.st) {
+    // (re)allocate state, pool, chunk maps and task ring, then put them in their empty state
+    HIPCHK(hipDeviceSynchronize());
+    if (ws.st) (void)hipFree(ws.st);
+    if (ws.pool) (void)hipFree(ws.pool);
+    if (ws.tab) (void)hipFree(ws.tab);
+    if (ws.tasks) (void)hipFree(ws.tasks);
+    ws.st = nullptr;
+    ws.pool = nullptr;
+    ws.tab = nullptr;
+    ws.tasks = nullptr;
+    const uint32_t pc = std::max(chunks, ws.pool_chunks);
+    const uint32_t tc = std::max(task_cap, ws.task_cap);
+    HIPCHK(hipMalloc(&ws.st, (size_t)state_words(nq) * 4));
+    HIPCHK(hipMalloc(&ws.pool, (size_t)pc * CH * 8));
+    HIPCHK(hipMalloc(&ws.tab, (size_t)nq * pc * 4));
+    HIPCHK(hipMalloc(&ws.tasks, (size_t)tc * 16));
+    HIPCHK(launch_fill_u32(ws.st, state_words(nq), 0, nullptr));
+    HIPCHK(launch_fill_u64(ws.pool, (uint64_t)pc * CH, EMPTY64, nullptr));
+    HIPCHK(launch_fill_u32(ws.tab, (uint64_t)nq * pc, TAB_UNSET, nullptr));
+    HIPCHK(launch_fill_u32((uint32_t*)ws.tasks, (uint64_t)tc * 4, EMPTY32, nullptr));
+    HIPCHK(hipDeviceSynchronize());
+    ws.nq = nq;
+    ws.pool_chunks = pc;
+    ws.maxv = pc;
+    ws.task_cap = tc;
+  }
+  if (ws.srcs_cap < nsrc) {
+    if (ws.srcs) (void)hipFree(ws.srcs);
+    ws.srcs = nullptr;
+    HIPCHK(hipMalloc(&ws.srcs, (size_t)nsrc * 4));
+    ws.srcs_cap = nsrc;
+  }
+  return ADDS_OK;
+}
+
+static uint32_t inf_bits(adds_wtype wt) { return wt == ADDS_W_F32 ? ADDS_INF_F32_BITS : ADDS_INF_U32; }
+
+static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_t total_n, uint32_t grid) {
+  Workspace& ws = g->ws;
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.row_ptr = g->row;
+  a.col = g->col;
+  a.w = g->w;
+  a.dist = dist;
+  a.st = ws.st;
+  a.pool = ws.pool;
+  a.tab = ws.tab;
+  a.tasks = ws.tasks;
+  a.host_abort = ws.host_abort;
+  a.n = g->n;
+  a.total_n = (uint32_t)total_n;
+  a.pool_chunks = (uint32_t)std::min<uint64_t>(ws.pool_chunks, (r.pool_entries + CH - 1) / CH);
+  a.maxv = ws.maxv;
+  a.task_cap = ws.task_cap;
+  a.nb = r.nb;
+  a.parts = r.parts;
+  a.inv_delta = 1.0f / r.delta;
+  a.chunk = r.chunk;
+  a.hub_degree = r.hub_degree;
+  a.task_edges = r.task_edges;
+  a.concur = std::min(r.concur, r.nb - 2);
+  a.total_waves = grid * WAVES_PER_WG;
+  int rate_khz = 100000;
+  (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->device);
+  if (rate_khz <= 0) rate_khz = 100000;
+  a.timeout_ticks = (uint64_t)(r.timeout_ms * (double)rate_khz);
+  return a;
+}
+
+static uint32_t grid_for(const Resolved& r) {
+  uint32_t cu = adds_device_cu_count();
+  if (cu == 0) cu = 1;
+  return cu * r.wg_per_cu;
+}
+
+// Enqueues dist fill + sources, the persistent solve and the cleanup on `s`.
+static adds_status enqueue_solve(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_t total_n,
+                                 const uint32_t* srcs_dev, uint32_t k, bool replicated, hipStream_t s) {
+  const uint32_t grid = grid_for(r);
+  KArgs a = make_args(g, r, dist, total_n, grid);
+  HIPCHK(launch_init(a, srcs_dev, k, replicated ? 1 : 0, inf_bits(g->wt), s));
+  HIPCHK(launch_solve(a, g->wt == ADDS_W_F32, replicated, grid, s));
+  HIPCHK(launch_cleanup(a, s));
+  return ADDS_OK;
+}
+
+// Waits for ev1 with a host-side watchdog: past the device timeout (+5 s) the abort word is raised,
+// which the persistent kernel polls.
+static adds_status wait_solve(Workspace& ws) {
+  auto t0 = std::chrono::steady_clock::now();
+  bool aborted = false;
+  for (;;) {
+    hipError_t e = hipEventQuery(ws.ev2);
+    if (e == hipSuccess) break;
+    if (e != hipErrorNotReady) {
+      g_detail = std::string("solve: ") + hipGetErrorString(e);
+      return ADDS_EHIP;
+    }
+    double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (!aborted && el > ws.timeout_ms + 5000.0) {
+      __atomic_store_n(ws.host_abort, 1u, __ATOMIC_SEQ_CST);
+      aborted = true;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(el < 10 ? 5 : 200));
+  }
+  if (aborted) __atomic_store_n(ws.host_abort, 0u, __ATOMIC_SEQ_CST);
+  return ADDS_OK;
+}
+
+static adds_status collect(adds_graph* g, adds_stats* stats) {
+  Workspace& ws = g->ws;
+  adds_status rc = wait_solve(ws);
+  ws.pending = false;
+  if (rc != ADDS_OK) return rc;
+  const uint32_t* hs = ws.host_state;
+  const uint32_t err = hs[S_DONE + 1];
+  if (stats) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ws.ev0, ws.ev1);
+    const uint64_t* sv = (const uint64_t*)(hs + S_STATS);
+    stats->ms = ms;
+    stats->vertices_expanded = sv[ST_EXPANDED];
+    stats->edges_scanned = sv[ST_EDGES];
+    stats->relaxations = sv[ST_RELAX];
+    stats->pushes = sv[ST_PUSH];
+    stats->stale_skips = sv[ST_STALE];
+    stats->hub_tasks = sv[ST_TASKS];
+    stats->rebuckets = sv[ST_REBUCKET];
+    stats->pool_chunks_used = hs[S_POOL];
+  }
+  if (err == DERR_OVERFLOW) return ADDS_EOVERFLOW;
+  if (err == DERR_TIMEOUT || err == DERR_ABORT) return ADDS_ETIMEOUT;
+  if (err != 0) return ADDS_EHIP;
+  if (hs[S_DONE] != 1) {
+    g_detail = "solver exited without completing";
+    return ADDS_EHIP;
+  }
+  return ADDS_OK;
+}
+
+// One synchronous-or-not solve of k sources into dist (device, [k][n] when replicated).
+static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t k, uint32_t* dist_dev,
+                              bool replicated, const adds_params* p, adds_stats* stats, hipStream_t s,
+                              bool sync) {
+  const uint64_t total_n = replicated ? (uint64_t)k * g->n : g->n;
+  if (total_n >= 0xFFFFFFFFull) return ADDS_EINVAL;
+  Resolved r;
+  adds_status rc = resolve(g, p, total_n, r);
+  if (rc != ADDS_OK) return rc;
+  for (uint32_t i = 0; i < k; i++)
+    if (srcs_host[i] >= g->n) {
+      g_detail = "source out of range";
+      return ADDS_EINVAL;
+    }
+  if ((uint64_t)k > r.pool_entries) {
+    g_detail = "more sources than worklist pool entries";
+    return ADDS_EINVAL;
+  }
+  for (int attempt = 0; attempt < 3; attempt++) {
+    rc = ensure_workspace(g, r, k);
+    if (rc != ADDS_OK) return rc;
+    Workspace& ws = g->ws;
+    ws.timeout_ms = r.timeout_ms;
+    HIPCHK(hipMemcpyAsync(ws.srcs, srcs_host, (size_t)k * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(ws.ev0, s));
+    rc = enqueue_solve(g, r, dist_dev, total_n, ws.srcs, k, replicated, s);
+    if (rc != ADDS_OK) return rc;
+    HIPCHK(hipEventRecord(ws.ev1, s));
+    HIPCHK(hipMemcpyAsync(ws.host_state, ws.st, 8 * LINE * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(ws.ev2, s));
+    ws.last_stream = s;
+    ws.pending = true;
+    if (!sync) return ADDS_OK;
+    rc = collect(g, stats);
+    if (rc != ADDS_EOVERFLOW || r.pool_fixed) return rc;
+    r.pool_entries *= 2;  // grow the pool and solve again
+  }
+  return rc;
+}
+
+static adds_status ensure_outbuf(adds_graph* g, uint64_t words) {
+  if (g->outbuf_cap >= words) return ADDS_OK;
+  if (g->outbuf) (void)hipFree(g->outbuf);
+  g->outbuf = nullptr;
+  g->outbuf_cap = 0;
+  HIPCHK(hipMalloc(&g->outbuf, words * 4));
+  g->outbuf_cap = words;
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_sssp_multi(adds_graph* g, const uint32_t* srcs, uint32_t k, void* dist,
+                                       const adds_params* p, adds_stats* stats, void* stream) {
+  if (!g || !srcs || k == 0 || !dist) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t flags = p ? p->flags : 0;
+  const bool host_out = flags & ADDS_F_HOST_OUT;
+  const bool sync = host_out || !(flags & ADDS_F_NO_SYNC);
+  uint32_t* d = (uint32_t*)dist;
+  if (host_out) {
+    adds_status rc = ensure_outbuf(g, (uint64_t)k * g->n);
+    if (rc != ADDS_OK) return rc;
+    d = g->outbuf;
+  }
+  adds_status rc = solve_core(g, srcs, k, d, k > 1, p, stats, s, sync);
+  if (rc != ADDS_OK) return rc;
+  if (host_out) {
+    HIPCHK(hipMemcpyAsync(dist, d, (size_t)k * g->n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_sssp(adds_graph* g, uint32_t src, void* dist, const adds_params* p, adds_stats* stats,
+                                 void* stream) {
+  return adds_sssp_multi(g, &src, 1, dist, p, stats, stream);
+}
+
+extern "C" adds_status adds_last_status(adds_graph* g, adds_stats* stats) {
+  if (!g) return ADDS_EINVAL;
+  if (!g->ws.pending) return ADDS_OK;
+  HIPCHK(hipSetDevice(g->device));
+  return collect(g, stats);
+}
+
+// ---- batched over different graphs: block-diagonal union CSR ----
+__global__ static void k_offset_copy(uint32_t* dst, const uint32_t* src, uint64_t count, uint32_t add) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i] + add;
+}
+
+static adds_status build_union(adds_graph* const* graphs, uint32_t k, adds_graph** out, std::vector<uint32_t>& off) {
+  uint64_t N = 0, M = 0;
+  off.assign(k + 1, 0);
+  std::vector<uint64_t> eoff(k + 1, 0);
+  for (uint32_t i = 0; i < k; i++) {
+    if (graphs[i]->wt != graphs[0]->wt) return ADDS_EINVAL;
+    // a graph repeated in the list gets its own copy so that every instance has its own vertices
+    N += graphs[i]->n;
+    M += graphs[i]->m;
+    if (N >= 0xFFFFFFFFull || M >= 0xFFFFFFFFull) return ADDS_EINVAL;
+    off[i + 1] = (uint32_t)N;
+    eoff[i + 1] = M;
+  }
+  adds_graph* u = nullptr;
+  adds_status rc = alloc_graph((uint32_t)N, M, graphs[0]->wt, &u);
+  if (rc != ADDS_OK) return rc;
+  double wsum = 0;
+  for (uint32_t i = 0; i < k; i++) {
+    const adds_graph* g = graphs[i];
+    hipLaunchKernelGGL(k_offset_copy, dim3(256), dim3(256), 0, nullptr, u->row + off[i], g->row, (uint64_t)g->n,
+                       (uint32_t)eoff[i]);
+    if (g->m) {
+      hipLaunchKernelGGL(k_offset_copy, dim3(1024), dim3(256), 0, nullptr, u->col + eoff[i], g->col, g->m, off[i]);
+      HIPCHK(hipMemcpy(u->w + eoff[i], g->w, g->m * 4, hipMemcpyDeviceToDevice));
+    }
+    wsum += (double)g->ave_wt * (double)g->m;
+  }
+  const uint32_t last = (uint32_t)M;
+  HIPCHK(hipMemcpy(u->row + N, &last, 4, hipMemcpyHostToDevice));
+  HIPCHK(hipDeviceSynchronize());
+  u->ave_wt = M ? (float)(wsum / (double)M) : 1.f;
+  *out = u;
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_sssp_batched(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k,
+                                         void* const* dists, const adds_params* p, adds_stats* stats, void* stream) {
+  if (!graphs || !srcs || !dists || k == 0) return ADDS_EINVAL;
+  bool same = true;
+  for (uint32_t i = 0; i < k; i++) {
+    if (!graphs[i] || !dists[i]) return ADDS_EINVAL;
+    same &= graphs[i] == graphs[0];
+  }
+  adds_graph* g0 = graphs[0];
+  HIPCHK(hipSetDevice(g0->device));
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t flags = p ? p->flags : 0;
+  const bool host_out = flags & ADDS_F_HOST_OUT;
+  const hipMemcpyKind kind = host_out ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (same) {
+    // contiguous [k][n] solve, then one copy per instance
+    adds_status rc = ensure_outbuf(g0, (uint64_t)k * g0->n);
+    if (rc != ADDS_OK) return rc;
+    adds_params q = p ? *p : adds_params{};
+    q.flags &= ~(ADDS_F_HOST_OUT | ADDS_F_NO_SYNC);
+    rc = solve_core(g0, srcs, k, g0->outbuf, k > 1, &q, stats, s, true);
+    if (rc != ADDS_OK) return rc;
+    for (uint32_t i = 0; i < k; i++)
+      HIPCHK(hipMemcpyAsync(dists[i], g0->outbuf + (size_t)i * g0->n, (size_t)g0->n * 4, kind, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ADDS_OK;
+  }
+  std::vector<const adds_graph*> key(graphs, graphs + k);
+  if (!g0->union_g || g0->union_key != key) {
+    if (g0->union_g) {
+      release_graph_memory(g0->union_g);
+      delete g0->union_g;
+      g0->union_g = nullptr;
+    }
+    adds_status rc = build_union(graphs, k, &g0->union_g, g0->union_off);
+    if (rc != ADDS_OK) return rc;
+    g0->union_key = key;
+  }
+  adds_graph* u = g0->union_g;
+  std::vector<uint32_t> usrc(k);
+  for (uint32_t i = 0; i < k; i++) {
+    if (srcs[i] >= graphs[i]->n) return ADDS_EINVAL;
+    usrc[i] = g0->union_off[i] + srcs[i];
+  }
+  adds_status rc = ensure_outbuf(u, u->n);
+  if (rc != ADDS_OK) return rc;
+  adds_params q = p ? *p : adds_params{};
+  q.flags &= ~(ADDS_F_HOST_OUT | ADDS_F_NO_SYNC);
+  rc = solve_core(u, usrc.data(), k, u->outbuf, false, &q, stats, s, true);
+  if (rc != ADDS_OK) return rc;
+  for (uint32_t i = 0; i < k; i++)
+    HIPCHK(hipMemcpyAsync(dists[i], u->outbuf + g0->union_off[i], (size_t)graphs[i]->n * 4, kind, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return ADDS_OK;
+}
+
+// ---- HIP graph capture of a multi-source solve ----
+extern "C" adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* srcs_dev, uint32_t k, void* dist_dev,
+                                               const adds_params* p, adds_solve_graph** out) {
+  if (!g || !srcs_dev || !k || !dist_dev || !out) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  const bool repl = k > 1;
+  const uint64_t total_n = repl ? (uint64_t)k * g->n : g->n;
+  if (total_n >= 0xFFFFFFFFull) return ADDS_EINVAL;
+  Resolved r;
+  adds_status rc = resolve(g, p, total_n, r);
+  if (rc != ADDS_OK) return rc;
+  rc = ensure_workspace(g, r, k);
+  if (rc != ADDS_OK) return rc;
+  g->ws.timeout_ms = r.timeout_ms;
+  hipStream_t s;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  adds_solve_graph* sg = new adds_solve_graph();
+  sg->g = g;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = enqueue_solve(g, r, (uint32_t*)dist_dev, total_n, srcs_dev, k, repl, s);
+    hipError_t e2 = hipStreamEndCapture(s, &sg->graph);
+    if (rc == ADDS_OK && e2 != hipSuccess) e = e2;
+  }
+  if (e == hipSuccess && rc == ADDS_OK) e = hipGraphInstantiate(&sg->exec, sg->graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(s);
+  if (e != hipSuccess || rc != ADDS_OK) {
+    if (e != hipSuccess) g_detail = std::string("graph capture: ") + hipGetErrorString(e);
+    adds_solve_graph_destroy(sg);
+    return rc != ADDS_OK ? rc : ADDS_EHIP;
+  }
+  *out = sg;
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* stream) {
+  if (!sg || !sg->exec) return ADDS_EINVAL;
+  adds_graph* g = sg->g;
+  HIPCHK(hipSetDevice(g->device));
+  hipStream_t s = (hipStream_t)stream;
+  Workspace& ws = g->ws;
+  HIPCHK(hipEventRecord(ws.ev0, s));
+  HIPCHK(hipGraphLaunch(sg->exec, s));
+  HIPCHK(hipEventRecord(ws.ev1, s));
+  HIPCHK(hipMemcpyAsync(ws.host_state, ws.st, 8 * LINE * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(ws.ev2, s));
+  ws.pending = true;
+  return ADDS_OK;
+}
+
+extern "C" void adds_solve_graph_destroy(adds_solve_graph* sg) {
+  if (!sg) return;
+  if (sg->exec) (void)hipGraphExecDestroy(sg->exec);
+  if (sg->graph) (void)hipGraphDestroy(sg->graph);
+  delete sg;
+}
+
+// ---- backward ----
+static adds_status ensure_transpose(adds_graph* g) {
+  if (g->rrow) return ADDS_OK;
+  const uint32_t n = g->n;
+  const uint64_t m = g->m;
+  std::vector<uint32_t> row(n + 1), col(m);
+  HIPCHK(hipMemcpy(row.data(), g->row, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+  if (m) HIPCHK(hipMemcpy(col.data(), g->col, m * 4, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> rrow(n + 1, 0), reid(m), rsrc(m);
+  for (uint64_t e = 0; e < m; e++) rrow[col[e] + 1]++;
+  for (uint32_t v = 0; v < n; v++) rrow[v + 1] += rrow[v];
+  std::vector<uint32_t> cur(rrow.begin(), rrow.end() - 1);
+  for (uint32_t u = 0; u < n; u++)
+    for (uint32_t e = row[u]; e < row[u + 1]; e++) {  // ascending e per destination row
+      uint32_t p = cur[col[e]]++;
+      reid[p] = e;
+      rsrc[p] = u;
+    }
+  HIPCHK(hipMalloc(&g->rrow, ((size_t)n + 1) * 4));
+  HIPCHK(hipMalloc(&g->reid, std::max<uint64_t>(m, 1) * 4));
+  HIPCHK(hipMalloc(&g->rsrc, std::max<uint64_t>(m, 1) * 4));
+  HIPCHK(hipMemcpy(g->rrow, rrow.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
+  if (m) {
+    HIPCHK(hipMemcpy(g->reid, reid.data(), m * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g->rsrc, rsrc.data(), m * 4, hipMemcpyHostToDevice));
+  }
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_sssp_backward_f32(adds_graph* g, uint32_t k, const float* dist, const float* grad_dist,
+                                              float* grad_w, double* ms, void* stream) {
+  if (!g || !k || !dist || !grad_dist || !grad_w) return ADDS_EINVAL;
+  if (g->wt != ADDS_W_F32) {
+    g_detail = "backward needs a float graph";
+    return ADDS_EINVAL;
+  }
+  if ((uint64_t)k * g->n >= 0xFFFFFFFFull) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  adds_status rc = ensure_transpose(g);
+  if (rc != ADDS_OK) return rc;
+  const uint64_t kn = (uint64_t)k * g->n;
+  rc = ensure_outbuf(g, 4 * kn);
+  if (rc != ADDS_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  BwdArgs b;
+  b.row_ptr = g->row;
+  b.col = g->col;
+  b.w = (const float*)g->w;
+  b.rrow = g->rrow;
+  b.reid = g->reid;
+  b.rsrc = g->rsrc;
+  b.dist = dist;
+  b.gdist = grad_dist;
+  b.pe = g->outbuf;
+  b.cnt = g->outbuf + kn;
+  b.adj = (float*)(g->outbuf + 2 * kn);
+  b.gw = grad_w;
+  b.n = g->n;
+  b.k = k;
+  b.m = g->m;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  HIPCHK(launch_backward(b, s));
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ADDS_OK;
+}

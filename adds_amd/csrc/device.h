@@ -1,0 +1,88 @@
+// Shared declarations between the solver kernels (sssp_kernels.hip) and the C ABI (api.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace adds {
+
+// ---- worklist geometry ----
+constexpr uint32_t CH_SHIFT = 11;  // pool chunk = 2048 entries (16 KiB)
+constexpr uint32_t CH = 1u << CH_SHIFT;
+constexpr uint64_t EMPTY64 = ~0ull;      // unwritten pool entry
+constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;  // unwritten task word
+constexpr uint32_t TAB_UNSET = 0xFFFFFFFFu;
+constexpr uint32_t TAB_BAD = 0xFFFFFFFEu;
+constexpr uint32_t WG_THREADS = 256;  // 4 wavefronts of 64
+constexpr uint32_t WAVES_PER_WG = WG_THREADS / 64;
+
+// ---- state words (uint32 indices); every hot counter owns a 128-byte line (32 words) ----
+constexpr uint32_t LINE = 32;
+constexpr uint32_t S_CUR = 0 * LINE;        // absolute index of the head bucket
+constexpr uint32_t S_VER = 1 * LINE;        // u64: (busy transitions << 32) | idle wavefronts
+constexpr uint32_t S_DONE = 2 * LINE;       // [0] done flag, [1] error code
+constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
+constexpr uint32_t S_TASK_H = 4 * LINE;     // hub task ring claim index
+constexpr uint32_t S_TASK_T = 5 * LINE;     // hub task ring reserve index
+constexpr uint32_t S_STATS = 6 * LINE;      // u64 stats[8]
+constexpr uint32_t S_QBASE = 8 * LINE;      // queue q: head at S_QBASE + 2q*LINE, tail at +LINE
+constexpr uint32_t state_words(uint32_t nq) { return S_QBASE + 2 * nq * LINE; }
+
+enum StatIdx { ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET, ST_SPARE };
+
+enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3 };
+
+struct KArgs {
+  const uint32_t* row_ptr;
+  const uint32_t* col;
+  const uint32_t* w;  // 4-byte weights (uint32 or float bits)
+  uint32_t* dist;     // distance words, total_n of them
+  uint32_t* st;       // state words
+  uint64_t* pool;     // pool_chunks * CH entries: (dist << 32) | vertex-instance id
+  uint32_t* tab;      // per queue: maxv virtual-chunk -> physical-chunk map
+  uint4* tasks;       // hub edge tasks {gid, dist, edge_begin, edge_end}
+  const uint32_t* host_abort;  // host-visible abort word (may be null)
+  uint32_t n;         // vertices of the graph (per replica)
+  uint32_t total_n;   // replicas * n
+  uint32_t pool_chunks;
+  uint32_t maxv;      // virtual chunks per queue
+  uint32_t task_cap;
+  uint32_t nb;        // bucket ring length (power of two)
+  uint32_t parts;     // sub-queues per bucket
+  float inv_delta;    // 1 / bucket width
+  uint32_t chunk;     // items claimed per wavefront
+  uint32_t hub_degree;
+  uint32_t task_edges;
+  uint32_t concur;
+  uint32_t total_waves;
+  uint64_t timeout_ticks;  // wall_clock64 ticks
+};
+
+// Host launchers (sssp_kernels.hip)
+hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t v, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t count, uint64_t v, hipStream_t s);
+hipError_t launch_init(const KArgs& a, const uint32_t* srcs, uint32_t k, int replicated, uint32_t inf_bits,
+                       hipStream_t s);
+hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s);
+hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
+
+// Backward (backward_kernels.hip)
+struct BwdArgs {
+  const uint32_t* row_ptr;
+  const uint32_t* col;
+  const float* w;
+  const uint32_t* rrow;  // transpose: in-edges of v at rrow[v]..rrow[v+1]
+  const uint32_t* reid;  // transpose: forward edge id of each in-edge (sorted ascending per row)
+  const uint32_t* rsrc;  // transpose: tail vertex of each in-edge
+  const float* dist;     // [k][n]
+  const float* gdist;    // [k][n]
+  uint32_t* pe;          // [k][n] parent edge (or ~0)
+  uint32_t* cnt;         // [k][n] pending children
+  float* adj;            // [k][n] adjoint
+  float* gw;             // [m]
+  uint32_t n;
+  uint32_t k;
+  uint64_t m;
+};
+hipError_t launch_backward(const BwdArgs& b, hipStream_t s);
+
+}  // namespace adds

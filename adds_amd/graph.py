@@ -1,0 +1,259 @@
+"""Host graphs (generators, .gr files, distance text) and device-resident graph handles.
+
+This is the host-side mirror of the reference's CSRGraph (ads_int/csr_graph.h:20-97): ``HostGraph``
+is the host CSR that CSRGraph::read produces (csr_graph.cu:203-298) and ``Graph`` is the
+device-resident copy that copy_to_gpu makes (csr_graph.cu:309-328), solved through the C ABI.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+WKINDS = {"u32": L.WK_U32_UNIFORM, "f32": L.WK_F32_UNIFORM, "lognormal": L.WK_F32_LOGNORMAL}
+
+
+def _wtype(w):
+    if w in (L.W_U32, "int", "u32", np.uint32):
+        return L.W_U32
+    if w in (L.W_F32, "float", "f32", np.float32):
+        return L.W_F32
+    raise ValueError(f"unknown weight type {w!r}")
+
+
+class HostGraph:
+    """A CSR graph in host memory (row_ptr[n+1], col[m], 4-byte weights w[m])."""
+
+    def __init__(self, csr: L.HostCSR):
+        self._csr = csr
+        n, m = csr.n, csr.m
+        self.n, self.m = int(n), int(m)
+        self.wtype = int(csr.wtype)
+        self.row_ptr = np.ctypeslib.as_array(csr.row_ptr, shape=(n + 1,))
+        self.col = np.ctypeslib.as_array(csr.col, shape=(max(m, 1),))[:m]
+        wbits = np.ctypeslib.as_array(csr.w, shape=(max(m, 1),))[:m]
+        self.w_bits = wbits
+        self.w = wbits.view(np.float32) if self.wtype == L.W_F32 else wbits
+
+    def __del__(self):
+        csr = getattr(self, "_csr", None)
+        if csr is not None and csr.row_ptr:
+            L._lib().adds_host_csr_free(ctypes.byref(csr))
+            self._csr = None
+
+    # ---- construction ----
+    @classmethod
+    def _from_call(cls, fn, *args, what="generator"):
+        csr = L.HostCSR()
+        L.check(fn(*args, ctypes.byref(csr)), what)
+        return cls(csr)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, seed, wkind="u32", lo=1, hi=100, a=0.57, b=0.19, c=0.19):
+        return cls._from_call(L._lib().adds_gen_rmat, scale, edge_factor, a, b, c, seed, WKINDS[wkind],
+                              float(lo), float(hi), what="rmat")
+
+    @classmethod
+    def grid(cls, rows, cols, shortcuts, seed, wkind="f32", lo=1.0, hi=2.0):
+        return cls._from_call(L._lib().adds_gen_grid, rows, cols, shortcuts, seed, WKINDS[wkind], float(lo),
+                              float(hi), what="grid")
+
+    @classmethod
+    def er(cls, n, avg_degree, seed, wkind="u32", lo=1, hi=1000):
+        return cls._from_call(L._lib().adds_gen_er, n, float(avg_degree), seed, WKINDS[wkind], float(lo),
+                              float(hi), what="er")
+
+    @classmethod
+    def read_gr(cls, path, wtype="int"):
+        return cls._from_call(L._lib().adds_read_gr, str(path).encode(), _wtype(wtype), what=str(path))
+
+    @classmethod
+    def from_arrays(cls, row_ptr, col, w, wtype):
+        """Copies numpy arrays into a library-owned host CSR (through a .gr-free path)."""
+        wt = _wtype(wtype)
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        wb = np.ascontiguousarray(w, dtype=np.float32 if wt == L.W_F32 else np.uint32).view(np.uint32)
+        obj = cls.__new__(cls)
+        obj._csr = None
+        obj.n, obj.m, obj.wtype = len(row_ptr) - 1, len(col), wt
+        obj.row_ptr, obj.col, obj.w_bits = row_ptr, col, wb
+        obj.w = wb.view(np.float32) if wt == L.W_F32 else wb
+        return obj
+
+    def _csr_view(self):
+        c = L.HostCSR()
+        c.n, c.m = self.n, self.m
+        c.row_ptr = self.row_ptr.ctypes.data_as(L.c_u32p)
+        c.col = self.col.ctypes.data_as(L.c_u32p)
+        c.w = self.w_bits.ctypes.data_as(L.c_u32p)
+        c.wtype = self.wtype
+        return c
+
+    # ---- utilities ----
+    @property
+    def is_float(self):
+        return self.wtype == L.W_F32
+
+    def degrees(self):
+        return np.diff(self.row_ptr.astype(np.int64))
+
+    def write_gr(self, path):
+        c = self._csr_view()
+        L.check(L._lib().adds_write_gr(str(path).encode(), ctypes.byref(c)), str(path))
+
+    def max_degree_vertex(self):
+        c = self._csr_view()
+        return int(L._lib().adds_max_degree_vertex(ctypes.byref(c)))
+
+    def sample_component(self, root, k, seed):
+        out = np.zeros(k, dtype=np.uint32)
+        c = self._csr_view()
+        got = L._lib().adds_sample_component(ctypes.byref(c), root, k, seed, out.ctypes.data_as(L.c_u32p))
+        return out[:got]
+
+    def reached_counts(self, dist_bits):
+        d = np.ascontiguousarray(dist_bits).view(np.uint32)
+        c = self._csr_view()
+        nr, mr = ctypes.c_uint64(), ctypes.c_uint64()
+        L._lib().adds_reached_counts(ctypes.byref(c), d.ctypes.data_as(L.c_u32p), ctypes.byref(nr),
+                                     ctypes.byref(mr))
+        return int(nr.value), int(mr.value)
+
+
+def write_dist_text(path, dist, wtype):
+    d = np.ascontiguousarray(dist).view(np.uint32)
+    L.check(L._lib().adds_write_dist_text(str(path).encode(), d.ctypes.data_as(L.c_u32p), len(d),
+                                          _wtype(wtype)), str(path))
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+class Graph:
+    """A device-resident graph handle (adds_graph) with the solver entry points."""
+
+    def __init__(self, host: HostGraph, device=0):
+        L.check(L._lib().adds_set_device(device), "device")
+        h = ctypes.c_void_p()
+        L.check(L._lib().adds_graph_create(host.row_ptr.ctypes.data_as(L.c_u32p),
+                                           host.col.ctypes.data_as(L.c_u32p),
+                                           host.w_bits.ctypes.data_as(ctypes.c_void_p),
+                                           host.n, host.m, host.wtype, ctypes.byref(h)), "graph")
+        self._h = h
+        self.n, self.m, self.wtype = host.n, host.m, host.wtype
+        self.host = host
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            L._lib().adds_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def dist_dtype(self):
+        return np.float32 if self.wtype == L.W_F32 else np.uint32
+
+    # ---- solves with host output ----
+    def sssp(self, src, **params):
+        """Single-source solve; returns (dist as numpy, stats dict)."""
+        out = np.empty(self.n, dtype=np.uint32)
+        p = L.Params.make(**params)
+        p.flags |= L.F_HOST_OUT
+        st = L.Stats()
+        L.check(L._lib().adds_sssp(self._h, src, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(p),
+                                   ctypes.byref(st), None), "sssp")
+        return out.view(self.dist_dtype), st.as_dict()
+
+    def sssp_multi(self, srcs, **params):
+        srcs = np.ascontiguousarray(srcs, dtype=np.uint32)
+        out = np.empty((len(srcs), self.n), dtype=np.uint32)
+        p = L.Params.make(**params)
+        p.flags |= L.F_HOST_OUT
+        st = L.Stats()
+        L.check(L._lib().adds_sssp_multi(self._h, srcs.ctypes.data_as(L.c_u32p), len(srcs),
+                                         out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(p),
+                                         ctypes.byref(st), None), "sssp_multi")
+        return out.view(self.dist_dtype), st.as_dict()
+
+    # ---- solves into device buffers (torch tensors as plumbing) ----
+    def sssp_device(self, src, dist_ptr, stream=None, **params):
+        p = L.Params.make(**params)
+        st = L.Stats()
+        L.check(L._lib().adds_sssp(self._h, src, ctypes.c_void_p(dist_ptr), ctypes.byref(p), ctypes.byref(st),
+                                   _stream_ptr(stream)), "sssp")
+        return st.as_dict()
+
+    def sssp_multi_device(self, srcs, dist_ptr, stream=None, **params):
+        srcs = np.ascontiguousarray(srcs, dtype=np.uint32)
+        p = L.Params.make(**params)
+        st = L.Stats()
+        L.check(L._lib().adds_sssp_multi(self._h, srcs.ctypes.data_as(L.c_u32p), len(srcs),
+                                         ctypes.c_void_p(dist_ptr), ctypes.byref(p), ctypes.byref(st),
+                                         _stream_ptr(stream)), "sssp_multi")
+        return st.as_dict()
+
+    def set_weights_device(self, w_ptr, stream=None):
+        L.check(L._lib().adds_graph_set_weights_device(self._h, ctypes.c_void_p(w_ptr), _stream_ptr(stream)),
+                "set_weights")
+
+    def backward_device(self, k, dist_ptr, gdist_ptr, gw_ptr, stream=None):
+        ms = ctypes.c_double()
+        L.check(L._lib().adds_sssp_backward_f32(self._h, k, ctypes.c_void_p(dist_ptr), ctypes.c_void_p(gdist_ptr),
+                                                ctypes.c_void_p(gw_ptr), ctypes.byref(ms), _stream_ptr(stream)),
+                "backward")
+        return ms.value
+
+
+def sssp_batched(graphs, srcs, **params):
+    """One launch over k (graph, source) instances; returns a list of numpy dist arrays."""
+    k = len(graphs)
+    srcs = np.ascontiguousarray(srcs, dtype=np.uint32)
+    outs = [np.empty(g.n, dtype=np.uint32) for g in graphs]
+    hs = (ctypes.c_void_p * k)(*[g.handle.value for g in graphs])
+    ds = (ctypes.c_void_p * k)(*[o.ctypes.data for o in outs])
+    p = L.Params.make(**params)
+    p.flags |= L.F_HOST_OUT
+    st = L.Stats()
+    L.check(L._lib().adds_sssp_batched(hs, srcs.ctypes.data_as(L.c_u32p), k, ds, ctypes.byref(p),
+                                       ctypes.byref(st), None), "sssp_batched")
+    return [o.view(g.dist_dtype) for o, g in zip(outs, graphs)], st.as_dict()
+
+
+class SolveGraph:
+    """A multi-source solve captured in a HIP graph over fixed device buffers."""
+
+    def __init__(self, graph: Graph, srcs_ptr, k, dist_ptr, **params):
+        p = L.Params.make(**params)
+        h = ctypes.c_void_p()
+        L.check(L._lib().adds_solve_graph_create(graph.handle, ctypes.c_void_p(srcs_ptr), k, ctypes.c_void_p(dist_ptr),
+                                                 ctypes.byref(p), ctypes.byref(h)), "solve_graph_create")
+        self._h = h
+        self.graph = graph
+
+    def launch(self, stream=None):
+        L.check(L._lib().adds_solve_graph_launch(self._h, _stream_ptr(stream)), "solve_graph_launch")
+
+    def status(self):
+        st = L.Stats()
+        L.check(L._lib().adds_last_status(self.graph.handle, ctypes.byref(st)), "solve_graph")
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            L._lib().adds_solve_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

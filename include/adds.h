@@ -1,0 +1,178 @@
+/* adds.h -- C ABI of the MI355X-native ADDS single-source shortest-path solver.
+ *
+ * The reference exposes this path as a process (`sssp [-g gpu] [-o out] [-s src] graph.gr`) around one
+ * host function, gg_main(CSRGraph& hg, CSRGraph& gg) (ads_int/kernel.cu:185, ads_float/kernel.cu:196),
+ * whose CSRGraph is a struct of raw pointers (ads_int/csr_graph.h:90-97).  This header is that seam as a
+ * C ABI: plain pointers and sizes, status codes instead of exit()/assert, stream-ordered calls.
+ *
+ * Threading: a graph handle is not internally synchronised; use one handle per host thread.  Every call
+ * is ordered on the HIP stream it is given (NULL = the default stream).  The caller owns output buffers;
+ * the handle owns the device-resident CSR and the solver workspace.
+ *
+ * Distances are 32-bit words: uint32 for ADDS_W_U32 graphs (unreached = ADDS_INF_U32) and IEEE binary32
+ * for ADDS_W_F32 graphs (unreached = +infinity, bits ADDS_INF_F32_BITS).  Int distances follow the
+ * reference's int32 arithmetic for every finite distance below 2^31.
+ */
+#ifndef ADDS_H
+#define ADDS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADDS_INF_U32 0xFFFFFFFFu
+#define ADDS_INF_F32_BITS 0x7F800000u
+
+typedef enum {
+  ADDS_OK = 0,
+  ADDS_EINVAL = 1,    /* bad argument, malformed graph, source out of range */
+  ADDS_ENOMEM = 2,    /* host or device allocation failed */
+  ADDS_EOVERFLOW = 3, /* worklist pool or hub-task ring exhausted during a solve */
+  ADDS_ETIMEOUT = 4,  /* device watchdog or host abort ended a solve */
+  ADDS_EHIP = 5,      /* a HIP runtime call failed */
+  ADDS_EIO = 6,       /* file could not be opened, read or written */
+  ADDS_ENODEV = 7     /* no usable GPU */
+} adds_status;
+
+typedef enum { ADDS_W_U32 = 0, ADDS_W_F32 = 1 } adds_wtype;
+
+typedef struct adds_graph adds_graph; /* opaque; owns the device-resident CSR */
+
+/* Solver tuning.  Zero in any field selects the default (the reference's weight-profiler heuristic for
+ * delta, ads_int/kernel.cu:238-257 + wl.h:395-449).  Results never depend on these values. */
+typedef struct {
+  float delta;           /* bucket width in distance units */
+  uint32_t wg_per_cu;    /* persistent workgroups per compute unit */
+  uint32_t chunk;        /* worklist items claimed per wavefront (1..64) */
+  uint32_t flags;        /* ADDS_F_* */
+  uint32_t nbuckets;     /* bucket ring length (power of two, 4..256) */
+  uint32_t parts;        /* sub-queues per bucket (1 or 8: one per XCD) */
+  uint32_t hub_degree;   /* out-degree above which a vertex is split into edge tasks */
+  uint32_t task_edges;   /* edges per hub edge task */
+  uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
+  uint32_t timeout_ms;   /* device watchdog (default 120000) */
+  uint64_t pool_entries; /* worklist pool capacity in entries (default 3 per vertex-instance + 4M) */
+} adds_params;
+
+/* flags */
+#define ADDS_F_HOST_OUT 0x1u  /* dist output buffers are host memory (copied back, then synchronised) */
+#define ADDS_F_NO_SYNC 0x2u   /* do not synchronise; status of the solve is read by adds_last_status */
+
+typedef struct {
+  double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */
+  uint64_t vertices_expanded; /* worklist items whose edges were scanned (reference "work", int) */
+  uint64_t edges_scanned;     /* edges relaxed (reference "work", float) */
+  uint64_t relaxations;       /* atomic min operations issued on dist */
+  uint64_t pushes;            /* worklist entries written (including re-bucketed ones) */
+  uint64_t stale_skips;       /* popped entries superseded by a smaller distance */
+  uint64_t hub_tasks;         /* edge tasks created for high-degree vertices */
+  uint64_t rebuckets;         /* popped entries moved to a later bucket */
+  uint64_t pool_chunks_used;  /* worklist pool chunks consumed */
+} adds_stats;
+
+/* ---- errors / device ---- */
+const char* adds_strerror(adds_status s);
+/* Selects the HIP device used by later calls made from this thread (reference -g, support.cu:48-56). */
+adds_status adds_set_device(int device);
+/* Number of compute units of the current device (0 if none). */
+uint32_t adds_device_cu_count(void);
+
+/* ---- graphs ----
+ * Replaces CSRGraph::copy_to_gpu (csr_graph.cu:309-328): copies a host CSR (row_ptr[n+1], col[m],
+ * w[m] 4-byte weights) into device memory owned by the handle.  Validates monotone row_ptr, row_ptr[n]
+ * == m, col < n, and (float) weights that are finite and not negative. */
+adds_status adds_graph_create(const uint32_t* row_ptr, const uint32_t* col, const void* w, uint32_t n,
+                              uint64_t m, adds_wtype wt, adds_graph** out);
+/* Same from device pointers (copied device-to-device; no validation of contents). */
+adds_status adds_graph_create_device(const uint32_t* row_ptr_dev, const uint32_t* col_dev,
+                                     const void* w_dev, uint32_t n, uint64_t m, adds_wtype wt,
+                                     adds_graph** out);
+/* Overwrites the edge weights from a device buffer of m 4-byte words (used by the autograd binding). */
+adds_status adds_graph_set_weights_device(adds_graph* g, const void* w_dev, void* stream);
+void adds_graph_destroy(adds_graph* g);
+adds_status adds_graph_info(const adds_graph* g, uint32_t* n, uint64_t* m, adds_wtype* wt);
+/* Device pointers of the resident CSR (row_ptr, col, w); valid until the handle is destroyed. */
+adds_status adds_graph_device_arrays(const adds_graph* g, const uint32_t** row_ptr, const uint32_t** col,
+                                     const void** w);
+
+/* ---- solves ----
+ * Replaces gg_main's timed solve (ads_int/kernel.cu:223-262): dist[v] = shortest distance from src.
+ * dist is a device buffer of n words, or host memory with ADDS_F_HOST_OUT. */
+adds_status adds_sssp(adds_graph* g, uint32_t src, void* dist, const adds_params* p, adds_stats* stats,
+                      void* stream);
+/* k sources on one graph in one persistent launch with a shared worklist; dist is [k][n] (device, or
+ * host with ADDS_F_HOST_OUT).  No reference counterpart (batched form). */
+adds_status adds_sssp_multi(adds_graph* g, const uint32_t* srcs, uint32_t k, void* dist,
+                            const adds_params* p, adds_stats* stats, void* stream);
+/* k instances on possibly different graphs in one launch: when every handle is the same graph this is
+ * adds_sssp_multi; otherwise the graphs are solved as one block-diagonal union CSR (built on first use and
+ * cached in graphs[0]) from k sources.  dists[i] receives n_i words.  No reference counterpart. */
+adds_status adds_sssp_batched(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k,
+                              void* const* dists, const adds_params* p, adds_stats* stats, void* stream);
+/* Status of the last solve issued with ADDS_F_NO_SYNC on this handle (synchronises its stream). */
+adds_status adds_last_status(adds_graph* g, adds_stats* stats);
+
+/* Captures one multi-source solve (dist fill, persistent kernel, worklist cleanup) into a HIP graph
+ * bound to the given device buffers; adds_solve_graph_launch replays it.  No reference counterpart. */
+typedef struct adds_solve_graph adds_solve_graph;
+adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* srcs_dev, uint32_t k, void* dist_dev,
+                                    const adds_params* p, adds_solve_graph** out);
+adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* stream);
+void adds_solve_graph_destroy(adds_solve_graph* sg);
+
+/* ---- backward (float graphs) ----
+ * Given forward distances dist [k][n] from srcs and dL/d(dist) grad_dist [k][n] (device), writes
+ * dL/dw into grad_w [m] (device).  The tree is the tight-predecessor forest of each source: parent edge
+ * of v = the smallest edge id e=(u->v) with fl(d[u]+w_e) == d[v] and d[u] < d[v].  No reference
+ * counterpart. */
+adds_status adds_sssp_backward_f32(adds_graph* g, uint32_t k, const float* dist, const float* grad_dist,
+                                   float* grad_w, double* ms, void* stream);
+
+/* ---- host CSR, .gr files and distance text (reference csr_graph.cu / support.cu) ---- */
+typedef struct {
+  uint32_t n;
+  uint64_t m;
+  uint32_t* row_ptr; /* n+1 */
+  uint32_t* col;     /* m */
+  uint32_t* w;       /* m 4-byte weights (uint32 or float bits) */
+  adds_wtype wtype;
+} adds_host_csr;
+
+void adds_host_csr_free(adds_host_csr* g);
+/* Galois .gr v1 reader (csr_graph.cu:203-296).  sizeEdgeTy == 0 gives all-zero weights. */
+adds_status adds_read_gr(const char* path, adds_wtype wt, adds_host_csr* out);
+adds_status adds_write_gr(const char* path, const adds_host_csr* g);
+/* Per-vertex text output (support.cu:78-96): "%d %u" (int), "%d %f" (float), "%d INF".  path "-" =
+ * stdout. */
+adds_status adds_write_dist_text(const char* path, const uint32_t* dist, uint32_t n, adds_wtype wt);
+
+/* ---- synthetic graphs (seeded, deterministic; symmetric, no self loops, no duplicate edges) ---- */
+typedef enum {
+  ADDS_WK_U32_UNIFORM = 0, /* uint32 uniform in [lo, hi] */
+  ADDS_WK_F32_UNIFORM = 1, /* float uniform in [lo, hi) */
+  ADDS_WK_F32_LOGNORMAL = 2 /* float exp(N(0, lo^2)) */
+} adds_wkind;
+
+adds_status adds_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                          uint64_t seed, adds_wkind wk, double lo, double hi, adds_host_csr* out);
+/* rows x cols 4-neighbour grid plus `shortcuts` random long-range undirected pairs. */
+adds_status adds_gen_grid(uint32_t rows, uint32_t cols, uint64_t shortcuts, uint64_t seed, adds_wkind wk,
+                          double lo, double hi, adds_host_csr* out);
+/* Erdos-Renyi-style graph: n*avg_degree/2 uniform random undirected pairs. */
+adds_status adds_gen_er(uint32_t n, double avg_degree, uint64_t seed, adds_wkind wk, double lo, double hi,
+                        adds_host_csr* out);
+/* Highest out-degree vertex (lowest id on ties). */
+uint32_t adds_max_degree_vertex(const adds_host_csr* g);
+/* Up to k distinct vertices sampled uniformly (seeded) from the component reached from `root`. */
+uint32_t adds_sample_component(const adds_host_csr* g, uint32_t root, uint32_t k, uint64_t seed,
+                               uint32_t* out);
+/* n_r (vertices with finite distance) and m_r (sum of their out-degrees) of a distance array. */
+void adds_reached_counts(const adds_host_csr* g, const uint32_t* dist, uint64_t* n_r, uint64_t* m_r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADDS_H */

@@ -1,0 +1,87 @@
+"""ctypes wrapper of oracle/liboracle.so -- the CPU oracle.
+
+Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module, as the checker; the product (adds_amd/) never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_L = None
+
+
+def _lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        p = ctypes.c_void_p
+        u32 = ctypes.c_uint32
+        for name in ("oracle_dijkstra", "oracle_bellman_ford", "oracle_certify"):
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_int, u32, p, p, p, p, u32, p]
+        L.oracle_tree_grad.restype = None
+        L.oracle_tree_grad.argtypes = [u32, p, p, p, u32, p, p, p, p]
+        _L = L
+    return _L
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _prep(row_ptr, col, w):
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    w = np.ascontiguousarray(w)
+    wb = w.view(np.uint32) if w.dtype in (np.float32, np.uint32, np.int32) else w.astype(np.uint32)
+    return row_ptr, col, np.ascontiguousarray(wb)
+
+
+def dijkstra(row_ptr, col, w, srcs, is_float):
+    """Distance words (uint32 view) from the sources; raises on int overflow."""
+    row_ptr, col, wb = _prep(row_ptr, col, w)
+    srcs = np.ascontiguousarray(np.atleast_1d(srcs), dtype=np.uint32)
+    n = len(row_ptr) - 1
+    d = np.empty(n, dtype=np.uint32)
+    if _lib().oracle_dijkstra(int(is_float), n, _p(row_ptr), _p(col), _p(wb), _p(srcs), len(srcs), _p(d)):
+        raise OverflowError("integer distance overflow")
+    return d
+
+
+def bellman_ford(row_ptr, col, w, srcs, is_float):
+    row_ptr, col, wb = _prep(row_ptr, col, w)
+    srcs = np.ascontiguousarray(np.atleast_1d(srcs), dtype=np.uint32)
+    n = len(row_ptr) - 1
+    d = np.empty(n, dtype=np.uint32)
+    if _lib().oracle_bellman_ford(int(is_float), n, _p(row_ptr), _p(col), _p(wb), _p(srcs), len(srcs), _p(d)):
+        raise OverflowError("integer distance overflow")
+    return d
+
+
+def certify(row_ptr, col, w, srcs, dist, is_float):
+    """0 when dist is exactly the shortest-path solution; 1/2/3 name the violated condition."""
+    row_ptr, col, wb = _prep(row_ptr, col, w)
+    srcs = np.ascontiguousarray(np.atleast_1d(srcs), dtype=np.uint32)
+    d = np.ascontiguousarray(dist).view(np.uint32)
+    return int(_lib().oracle_certify(int(is_float), len(row_ptr) - 1, _p(row_ptr), _p(col), _p(wb), _p(srcs),
+                                     len(srcs), _p(d)))
+
+
+def tree_grad(row_ptr, col, w, dist, gdist):
+    """dL/dw (float32[m]) and parent edges ([k][n]) for forward distances dist [k][n]."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    dist = np.ascontiguousarray(np.atleast_2d(dist), dtype=np.float32)
+    gdist = np.ascontiguousarray(np.atleast_2d(gdist), dtype=np.float32)
+    k, n = dist.shape
+    gw = np.empty(len(col), dtype=np.float32)
+    pe = np.empty((k, n), dtype=np.uint32)
+    _lib().oracle_tree_grad(n, _p(row_ptr), _p(col), _p(w), k, _p(dist), _p(gdist), _p(gw), _p(pe))
+    return gw, pe

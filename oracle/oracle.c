@@ -1,0 +1,267 @@
+/* oracle.c -- CPU restatement of the ADDS path, used ONLY as a test/benchmark checker.
+ *
+ * This file is test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it, and never as the thing measured or shipped.  The product (adds_amd/) never links it.
+ *
+ * What it restates (reference file:line):
+ *  - relaxation rule d[v] = min(d[v], d[u] + w) with a strict '>' test:
+ *      ads_int/kernel.cu:112-121 (int32 add), ads_float/kernel.cu:120-130 (float add, min by dist,
+ *      ads_float/csr_graph.h:23-49); unreached = INF (ads_int/csr_graph.h:23, INT32_MAX; float FLT_MAX).
+ *  - the reference's correctness oracle is Galois Dijkstra run on one thread (run_all.sh:93-107,
+ *      cpu_sssp_int.patch:133-159, cpu_sssp_float.patch:227-253): oracle_dijkstra_* is a textbook
+ *      binary-heap Dijkstra with the same arithmetic.  Galois itself is an un-vendored dependency
+ *      (IntelligentSoftwareSystems/Galois @ 38cd91cfb59a30cf0b4f7cf7d19f29d4d7188548, cloned by
+ *      build_all.sh:45-65) and is not available here.
+ *  - Galois .gr v1 parsing (csr_graph.cu:236-281) and the distance text (support.cu:78-96,
+ *      cpu_sssp_float.patch:241-248).
+ *
+ * Distances are 32-bit words: uint32 (INF 0xFFFFFFFF) or IEEE binary32 bits (INF = +inf, 0x7F800000).
+ * Non-negative floats order like their bit patterns, so one heap serves both.
+ * Compile with -ffp-contract=off (see Makefile): every float add is one IEEE binary32 add.
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define INF_U32 0xFFFFFFFFu
+#define INF_F32 0x7F800000u
+
+static float bits_f(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static uint32_t f_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+/* d + w in the variant's arithmetic.  Int: 64-bit sum, *ovf set when it reaches 2^32-1 (the reference's
+ * int32 would already be undefined past 2^31-1). */
+static uint32_t relax_add(int is_float, uint32_t d, uint32_t w, int* ovf) {
+  if (is_float) return f_bits(bits_f(d) + bits_f(w));
+  uint64_t s = (uint64_t)d + (uint64_t)w;
+  if (s >= INF_U32) { *ovf = 1; return INF_U32 - 1; }
+  return (uint32_t)s;
+}
+
+/* ---- binary min-heap of (key, vertex) with lazy deletion ---- */
+typedef struct { uint32_t key, v; } hent;
+typedef struct { hent* a; uint64_t n, cap; } heap;
+
+static void hpush(heap* h, uint32_t key, uint32_t v) {
+  if (h->n == h->cap) {
+    h->cap = h->cap ? 2 * h->cap : 1024;
+    h->a = (hent*)realloc(h->a, h->cap * sizeof(hent));
+  }
+  uint64_t i = h->n++;
+  while (i > 0) {
+    uint64_t p = (i - 1) / 2;
+    if (h->a[p].key <= key) break;
+    h->a[i] = h->a[p];
+    i = p;
+  }
+  h->a[i].key = key;
+  h->a[i].v = v;
+}
+static hent hpop(heap* h) {
+  hent top = h->a[0], last = h->a[--h->n];
+  uint64_t i = 0;
+  for (;;) {
+    uint64_t c = 2 * i + 1;
+    if (c >= h->n) break;
+    if (c + 1 < h->n && h->a[c + 1].key < h->a[c].key) c++;
+    if (h->a[c].key >= last.key) break;
+    h->a[i] = h->a[c];
+    i = c;
+  }
+  if (h->n) h->a[i] = last;
+  return top;
+}
+
+/* Dijkstra from k sources (all at distance 0) over one CSR.  Returns 1 if an int sum overflowed. */
+int oracle_dijkstra(int is_float, uint32_t n, const uint32_t* row, const uint32_t* col, const uint32_t* w,
+                    const uint32_t* srcs, uint32_t k, uint32_t* dist) {
+  const uint32_t inf = is_float ? INF_F32 : INF_U32;
+  int ovf = 0;
+  for (uint32_t v = 0; v < n; v++) dist[v] = inf;
+  heap h = {0, 0, 0};
+  for (uint32_t i = 0; i < k; i++) {
+    dist[srcs[i]] = 0;
+    hpush(&h, 0, srcs[i]);
+  }
+  while (h.n) {
+    hent t = hpop(&h);
+    if (t.key != dist[t.v]) continue; /* stale */
+    for (uint32_t e = row[t.v]; e < row[t.v + 1]; e++) {
+      uint32_t nd = relax_add(is_float, t.key, w[e], &ovf);
+      if (nd < dist[col[e]]) {
+        dist[col[e]] = nd;
+        hpush(&h, nd, col[e]);
+      }
+    }
+  }
+  free(h.a);
+  return ovf;
+}
+
+/* Chaotic Bellman-Ford: sweep all edges until nothing changes (brute force for tiny graphs). */
+int oracle_bellman_ford(int is_float, uint32_t n, const uint32_t* row, const uint32_t* col, const uint32_t* w,
+                        const uint32_t* srcs, uint32_t k, uint32_t* dist) {
+  const uint32_t inf = is_float ? INF_F32 : INF_U32;
+  int ovf = 0, changed = 1;
+  for (uint32_t v = 0; v < n; v++) dist[v] = inf;
+  for (uint32_t i = 0; i < k; i++) dist[srcs[i]] = 0;
+  while (changed) {
+    changed = 0;
+    for (uint32_t u = 0; u < n; u++) {
+      if (dist[u] == inf) continue;
+      for (uint32_t e = row[u]; e < row[u + 1]; e++) {
+        uint32_t nd = relax_add(is_float, dist[u], w[e], &ovf);
+        if (nd < dist[col[e]]) {
+          dist[col[e]] = nd;
+          changed = 1;
+        }
+      }
+    }
+  }
+  return ovf;
+}
+
+/* Certificate that `dist` is the shortest-path solution from the k sources:
+ *   0  ok
+ *   1  a source does not have distance 0
+ *   2  an edge violates dist[v] <= fl(dist[u] + w) (this includes a reached u with an INF neighbour)
+ *   3  a vertex with finite distance is not reached from a source through tight edges
+ *      (dist[v] == fl(dist[u] + w)); this also rejects finite distances on unreachable vertices.
+ * With non-negative weights and a monotone add, (2) gives dist <= the true solution and (3) gives
+ * dist >= it, so together they pin it exactly. */
+int oracle_certify(int is_float, uint32_t n, const uint32_t* row, const uint32_t* col, const uint32_t* w,
+                   const uint32_t* srcs, uint32_t k, const uint32_t* dist) {
+  const uint32_t inf = is_float ? INF_F32 : INF_U32;
+  int ovf = 0;
+  for (uint32_t i = 0; i < k; i++)
+    if (dist[srcs[i]] != 0) return 1;
+  for (uint32_t u = 0; u < n; u++) {
+    if (dist[u] == inf) continue;
+    for (uint32_t e = row[u]; e < row[u + 1]; e++)
+      if (dist[col[e]] > relax_add(is_float, dist[u], w[e], &ovf)) return 2;
+  }
+  uint8_t* seen = (uint8_t*)calloc(n, 1);
+  uint32_t* q = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint64_t qh = 0, qt = 0;
+  for (uint32_t i = 0; i < k; i++)
+    if (!seen[srcs[i]]) { seen[srcs[i]] = 1; q[qt++] = srcs[i]; }
+  while (qh < qt) {
+    uint32_t u = q[qh++];
+    for (uint32_t e = row[u]; e < row[u + 1]; e++) {
+      uint32_t v = col[e];
+      if (!seen[v] && dist[v] == relax_add(is_float, dist[u], w[e], &ovf)) { seen[v] = 1; q[qt++] = v; }
+    }
+  }
+  int rc = 0;
+  for (uint32_t v = 0; v < n && !rc; v++)
+    if (dist[v] != inf && !seen[v]) rc = 3;
+  free(seen);
+  free(q);
+  return rc;
+}
+
+/* ---- backward reference ----
+ * For each source s (dist_s = dist + s*n, g_s = gdist + s*n):
+ *   pe(v) = smallest edge id e = (u -> v) with fl(d[u] + w_e) == d[v] and d[u] < d[v] (none: root);
+ *   adj[v] = g[v] + sum over out-edges e = (v -> c) in ascending e with pe(c) == e of adj[c],
+ *            evaluated left to right in float, vertices in descending d (children before parents);
+ *   gw[pe(v)] += adj[v], sources in ascending s, gw starting at +0.0.
+ * This is the definition the device backward must reproduce bit for bit. */
+static const float* g_sort_d;
+static int cmp_desc(const void* a, const void* b) {
+  uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+  float dx = g_sort_d[x], dy = g_sort_d[y];
+  if (dx > dy) return -1;
+  if (dx < dy) return 1;
+  return (x < y) ? -1 : (x > y);
+}
+
+void oracle_tree_grad(uint32_t n, const uint32_t* row, const uint32_t* col, const float* w, uint32_t k,
+                      const float* dist, const float* gdist, float* gw, uint32_t* pe_out) {
+  const uint64_t m = row[n];
+  uint32_t* rrow = (uint32_t*)calloc((size_t)n + 1, 4);
+  uint32_t* reid = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
+  uint32_t* rsrc = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
+  for (uint64_t e = 0; e < m; e++) rrow[col[e] + 1]++;
+  for (uint32_t v = 0; v < n; v++) rrow[v + 1] += rrow[v];
+  uint32_t* cur = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+  memcpy(cur, rrow, sizeof(uint32_t) * n);
+  for (uint32_t u = 0; u < n; u++)
+    for (uint32_t e = row[u]; e < row[u + 1]; e++) {
+      uint32_t p = cur[col[e]]++;
+      reid[p] = e;
+      rsrc[p] = u;
+    }
+  for (uint64_t e = 0; e < m; e++) gw[e] = 0.0f;
+  uint32_t* pe = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n ? n : 1));
+  float* adj = (float*)malloc(sizeof(float) * ((size_t)n ? n : 1));
+  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n ? n : 1));
+  for (uint32_t s = 0; s < k; s++) {
+    const float* d = dist + (uint64_t)s * n;
+    const float* g = gdist + (uint64_t)s * n;
+    for (uint32_t v = 0; v < n; v++) {
+      pe[v] = 0xFFFFFFFFu;
+      if (f_bits(d[v]) == INF_F32) continue;
+      for (uint32_t p = rrow[v]; p < rrow[v + 1]; p++) {
+        float du = d[rsrc[p]];
+        if (du < d[v] && du + w[reid[p]] == d[v]) { pe[v] = reid[p]; break; }
+      }
+    }
+    if (pe_out) memcpy(pe_out + (uint64_t)s * n, pe, sizeof(uint32_t) * n);
+    for (uint32_t v = 0; v < n; v++) order[v] = v;
+    g_sort_d = d;
+    qsort(order, n, sizeof(uint32_t), cmp_desc);
+    for (uint32_t i = 0; i < n; i++) {
+      uint32_t v = order[i];
+      if (f_bits(d[v]) == INF_F32) continue;
+      float a = g[v];
+      for (uint32_t e = row[v]; e < row[v + 1]; e++)
+        if (pe[col[e]] == e) a += adj[col[e]];
+      adj[v] = a;
+    }
+    for (uint32_t v = 0; v < n; v++)
+      if (pe[v] != 0xFFFFFFFFu) gw[pe[v]] += adj[v];
+  }
+  free(rrow); free(reid); free(rsrc); free(cur); free(pe); free(adj); free(order);
+}
+
+/* ---- .gr v1 reader (csr_graph.cu:236-281) ----
+ * Returns 0 on success; the arrays are malloc'd (free with oracle_free). */
+int oracle_read_gr(const char* path, uint32_t* n_out, uint64_t* m_out, uint32_t** row_out, uint32_t** col_out,
+                   uint32_t** w_out) {
+  FILE* f = fopen(path, "rb");
+  if (!f) return 1;
+  uint64_t hdr[4];
+  if (fread(hdr, 8, 4, f) != 4 || hdr[0] != 1) { fclose(f); return 2; }
+  uint64_t nn = hdr[2], ne = hdr[3], esz = hdr[1];
+  uint32_t* row = (uint32_t*)malloc(sizeof(uint32_t) * (nn + 1));
+  uint32_t* col = (uint32_t*)malloc(sizeof(uint32_t) * (ne ? ne : 1));
+  uint32_t* w = (uint32_t*)calloc(ne ? ne : 1, sizeof(uint32_t));
+  uint64_t* idx = (uint64_t*)malloc(sizeof(uint64_t) * (nn ? nn : 1));
+  int rc = 0;
+  if (fread(idx, 8, nn, f) != nn) rc = 3;
+  if (!rc && fread(col, 4, ne, f) != ne) rc = 3;
+  if (!rc && (ne & 1)) { uint32_t pad; if (fread(&pad, 4, 1, f) != 1) rc = 3; }
+  if (!rc && esz && fread(w, 4, ne, f) != ne) rc = 3;
+  fclose(f);
+  row[0] = 0;
+  for (uint64_t i = 0; i < nn && !rc; i++) row[i + 1] = (uint32_t)idx[i];
+  free(idx);
+  if (rc) { free(row); free(col); free(w); return rc; }
+  *n_out = (uint32_t)nn; *m_out = ne; *row_out = row; *col_out = col; *w_out = w;
+  return 0;
+}
+void oracle_free(void* p) { free(p); }
+
+/* ---- distance text: "%d %u" / "%d %f" / "%d INF" ---- */
+int oracle_write_dist(const char* path, int is_float, uint32_t n, const uint32_t* dist) {
+  FILE* f = fopen(path, "w");
+  if (!f) return 1;
+  for (uint32_t i = 0; i < n; i++) {
+    if (dist[i] == (is_float ? INF_F32 : INF_U32)) fprintf(f, "%d INF\n", (int)i);
+    else if (is_float) fprintf(f, "%d %f\n", (int)i, (double)bits_f(dist[i]));
+    else fprintf(f, "%d %u\n", (int)i, dist[i]);
+  }
+  return fclose(f) ? 1 : 0;
+}

@@ -1,0 +1,105 @@
+"""GPU parity: device distances equal CPU Dijkstra bit for bit (int and float), through the C ABI.
+
+Small cases compare against the oracle's Dijkstra directly; the full-size configurations are checked
+with the oracle's certificate (feasible + tight-reachable + INF exactly on the unreachable set), which
+pins the shortest-path solution uniquely, plus Dijkstra where it finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import adds_amd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TIMEOUT = 60000
+
+
+def _check(g, dist, srcs, is_float):
+    ref = oracle.dijkstra(g.row_ptr, g.col, g.w, srcs, is_float)
+    got = np.ascontiguousarray(dist).view(np.uint32)
+    bad = np.nonzero(got != ref)[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: got {got[bad[:5]]} want {ref[bad[:5]]}"
+
+
+def test_rmat16_int_c1():
+    """Config 1: RMAT scale 16 ef 16, uint32 weights in [1,100], source = highest-degree vertex."""
+    g = adds_amd.HostGraph.rmat(16, 16, seed=1, wkind="u32", lo=1, hi=100)
+    src = g.max_degree_vertex()
+    dg = adds_amd.Graph(g)
+    dist, st = dg.sssp(src, timeout_ms=TIMEOUT)
+    print("c1 stats", st)
+    _check(g, dist, src, False)
+
+
+def test_rmat16_float():
+    g = adds_amd.HostGraph.rmat(16, 16, seed=2, wkind="f32", lo=0.0, hi=1.0)
+    src = g.max_degree_vertex()
+    dist, st = adds_amd.Graph(g).sssp(src, timeout_ms=TIMEOUT)
+    _check(g, dist, src, True)
+
+
+def test_grid_float_small():
+    g = adds_amd.HostGraph.grid(256, 256, 64, seed=5, wkind="f32", lo=1.0, hi=2.0)
+    dist, st = adds_amd.Graph(g).sssp(0, timeout_ms=TIMEOUT)
+    _check(g, dist, 0, True)
+
+
+def test_grid2048_float_c2():
+    """Config 2: 2048x2048 grid + 8385 shortcut pairs, float weights in [1,2), source 0."""
+    g = adds_amd.HostGraph.grid(2048, 2048, 8385, seed=2, wkind="f32", lo=1.0, hi=2.0)
+    dist, st = adds_amd.Graph(g).sssp(0, timeout_ms=TIMEOUT)
+    print("c2 stats", st)
+    _check(g, dist, 0, True)
+
+
+@pytest.mark.parametrize("params", [
+    dict(parts=1, nbuckets=4, chunk=1, wg_per_cu=1),
+    dict(parts=8, nbuckets=256, chunk=64, wg_per_cu=8, concurrency=4),
+    dict(parts=2, nbuckets=16, chunk=17, delta=1.0, hub_degree=64, task_edges=64),
+    dict(parts=4, nbuckets=32, chunk=32, delta=100000.0, concurrency=2),
+])
+def test_param_sweep_identical(params):
+    """Results never depend on the tuning point (delta, buckets, parts, chunk, hub split)."""
+    g = adds_amd.HostGraph.rmat(14, 16, seed=11, wkind="u32", lo=1, hi=1000)
+    src = g.max_degree_vertex()
+    dist, st = adds_amd.Graph(g).sssp(src, timeout_ms=TIMEOUT, **params)
+    _check(g, dist, src, False)
+
+
+def test_multi_source_replicated():
+    g = adds_amd.HostGraph.rmat(13, 8, seed=4, wkind="f32", lo=1.0, hi=10.0)
+    srcs = g.sample_component(g.max_degree_vertex(), 16, seed=9)
+    dist, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT)
+    for i, s in enumerate(srcs):
+        _check(g, dist[i], s, True)
+
+
+def test_batched_union_graphs():
+    gs = [adds_amd.HostGraph.grid(40 + 7 * i, 33, 0, seed=i, wkind="u32", lo=1, hi=1000) if i % 2 == 0 else
+          adds_amd.HostGraph.er(3000 + 500 * i, 8, seed=i, wkind="u32", lo=1, hi=1000) for i in range(6)]
+    dgs = [adds_amd.Graph(g) for g in gs]
+    srcs = [0, 1, 2, 3, 4, 5]
+    dists, st = adds_amd.sssp_batched(dgs, srcs, timeout_ms=TIMEOUT)
+    for g, d, s in zip(gs, dists, srcs):
+        _check(g, d, s, False)
+
+
+def test_overflow_reports_error_not_hang():
+    """A pool too small for the solve returns EOVERFLOW (forced from the host; every wave exits)."""
+    g = adds_amd.HostGraph.rmat(12, 16, seed=1, wkind="u32", lo=1, hi=100)
+    dg = adds_amd.Graph(g)
+    with pytest.raises(adds_amd.AddsError) as ei:
+        dg.sssp(g.max_degree_vertex(), pool_entries=4096, timeout_ms=TIMEOUT)
+    assert ei.value.status == adds_amd._lib.EOVERFLOW
+    # the handle is usable afterwards
+    dist, _ = dg.sssp(g.max_degree_vertex(), timeout_ms=TIMEOUT)
+    _check(g, dist, g.max_degree_vertex(), False)
+
+
+def test_repeated_solves_same_handle():
+    g = adds_amd.HostGraph.rmat(13, 16, seed=3, wkind="u32", lo=1, hi=100)
+    dg = adds_amd.Graph(g)
+    for s in [0, 5, 17, g.max_degree_vertex()]:
+        dist, _ = dg.sssp(s, timeout_ms=TIMEOUT)
+        _check(g, dist, s, False)
