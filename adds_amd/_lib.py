@@ -81,6 +81,7 @@ class Stats(ctypes.Structure):
         ("hub_tasks", ctypes.c_uint64),
         ("rebuckets", ctypes.c_uint64),
         ("pool_chunks_used", ctypes.c_uint64),
+        ("kernel_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -96,8 +97,8 @@ EXPORTS = [
     "adds_solve_graph_create", "adds_solve_graph_launch", "adds_solve_graph_destroy",
     "adds_sssp_backward_f32",
     "adds_host_csr_free", "adds_read_gr", "adds_write_gr", "adds_write_dist_text",
-    "adds_gen_rmat", "adds_gen_grid", "adds_gen_er", "adds_max_degree_vertex",
-    "adds_sample_component", "adds_reached_counts", "adds_last_error_detail",
+    "adds_gen_rmat", "adds_gen_grid", "adds_gen_er", "adds_gen_weights", "adds_max_degree_vertex",
+    "adds_sample_component", "adds_reached_counts", "adds_last_error_detail", "adds_autotune",
 ]
 
 _LIB = None
@@ -135,6 +136,7 @@ def _lib():
         "adds_solve_graph_launch": (i32, [vp, vp]),
         "adds_solve_graph_destroy": (None, [vp]),
         "adds_sssp_backward_f32": (i32, [vp, u32, vp, vp, vp, ctypes.POINTER(ctypes.c_double), vp]),
+        "adds_autotune": (i32, [vp, c_u32p, u32, u32, pp, ctypes.POINTER(ctypes.c_double)]),
         "adds_host_csr_free": (None, [hp]),
         "adds_read_gr": (i32, [ctypes.c_char_p, i32, hp]),
         "adds_write_gr": (i32, [ctypes.c_char_p, hp]),
@@ -143,6 +145,7 @@ def _lib():
                                 ctypes.c_double, ctypes.c_double, hp]),
         "adds_gen_grid": (i32, [u32, u32, u64, u64, i32, ctypes.c_double, ctypes.c_double, hp]),
         "adds_gen_er": (i32, [u32, ctypes.c_double, u64, i32, ctypes.c_double, ctypes.c_double, hp]),
+        "adds_gen_weights": (i32, [hp, u64, i32, ctypes.c_double, ctypes.c_double]),
         "adds_max_degree_vertex": (u32, [hp]),
         "adds_sample_component": (u32, [hp, u32, u32, u64, c_u32p]),
         "adds_reached_counts": (None, [hp, c_u32p, ctypes.POINTER(u64), ctypes.POINTER(u64)]),

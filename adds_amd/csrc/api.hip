@@ -14,6 +14,7 @@
 #include <string>
 #include <thread>
 #include <vector>
+#include <algorithm>
 
 #include "../../include/adds.h"
 #include "device.h"
@@ -42,8 +43,10 @@ struct Workspace {
   uint32_t* host_state = nullptr;       // pinned copy of the first 8 state lines
   uint32_t* host_abort = nullptr;       // pinned, device-visible abort word
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // solve start, solve end, state copied
+  hipEvent_t evk0 = nullptr, evk1 = nullptr;                // around the persistent kernel
   hipStream_t last_stream = nullptr;
   bool pending = false;                 // a NO_SYNC solve has not been collected
+  bool kernel_events = false;           // evk0/evk1 bracket the last solve's persistent kernel
   double timeout_ms = 0;
 };
 
@@ -117,6 +120,8 @@ static adds_status free_workspace(Workspace& ws) {
   if (ws.ev0) (void)hipEventDestroy(ws.ev0);
   if (ws.ev1) (void)hipEventDestroy(ws.ev1);
   if (ws.ev2) (void)hipEventDestroy(ws.ev2);
+  if (ws.evk0) (void)hipEventDestroy(ws.evk0);
+  if (ws.evk1) (void)hipEventDestroy(ws.evk1);
   ws = Workspace();
   return ADDS_OK;
 }
@@ -285,6 +290,7 @@ struct Resolved {
   float delta;
   uint32_t wg_per_cu, chunk, nb, parts, hub_degree, task_edges, concur, flags;
   uint64_t pool_entries;
+  uint64_t task_cap;
   bool pool_fixed;
   double timeout_ms;
 };
@@ -327,6 +333,8 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
     return ADDS_EINVAL;
   }
   if (r.pool_entries < CH) r.pool_entries = CH;
+  // hub edge tasks: room for every hub edge to be expanded 16 times over
+  r.task_cap = std::min<uint64_t>(std::max<uint64_t>(1u << 16, 16 * (g->m / r.task_edges) + 4096), 0x7FFFFFFFull);
   return ADDS_OK;
 }
 
@@ -336,7 +344,7 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
   const uint64_t chunks64 = (r.pool_entries + CH - 1) / CH;
   if (chunks64 >= TAB_BAD) return ADDS_EINVAL;
   const uint32_t chunks = (uint32_t)chunks64;
-  const uint64_t task_cap64 = std::max<uint64_t>(1u << 16, 4 * (g->m / r.task_edges) + 4096);
+  const uint64_t task_cap64 = r.task_cap;
   const uint32_t task_cap = (uint32_t)std::min<uint64_t>(task_cap64, 0x7FFFFFFFull);
   if (!ws.host_state) {
     HIPCHK(hipHostMalloc((void**)&ws.host_state, 8 * LINE * 4, hipHostMallocDefault));
@@ -345,6 +353,8 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
     HIPCHK(hipEventCreate(&ws.ev0));
     HIPCHK(hipEventCreate(&ws.ev1));
     HIPCHK(hipEventCreate(&ws.ev2));
+    HIPCHK(hipEventCreate(&ws.evk0));
+    HIPCHK(hipEventCreate(&ws.evk1));
   }
   if (ws.nq != nq || ws.pool_chunks < chunks || ws.task_cap < task_cap || !ws.st) {
     // (re)allocate state, pool, chunk maps and task ring, then put them in their empty state
@@ -425,11 +435,14 @@ static uint32_t grid_for(const Resolved& r) {
 
 // Enqueues dist fill + sources, the persistent solve and the cleanup on `s`.
 static adds_status enqueue_solve(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_t total_n,
-                                 const uint32_t* srcs_dev, uint32_t k, bool replicated, hipStream_t s) {
+                                 const uint32_t* srcs_dev, uint32_t k, bool replicated, hipStream_t s,
+                                 bool kernel_events) {
   const uint32_t grid = grid_for(r);
   KArgs a = make_args(g, r, dist, total_n, grid);
   HIPCHK(launch_init(a, srcs_dev, k, replicated ? 1 : 0, inf_bits(g->wt), s));
+  if (kernel_events) HIPCHK(hipEventRecord(g->ws.evk0, s));
   HIPCHK(launch_solve(a, g->wt == ADDS_W_F32, replicated, grid, s));
+  if (kernel_events) HIPCHK(hipEventRecord(g->ws.evk1, s));
   HIPCHK(launch_cleanup(a, s));
   return ADDS_OK;
 }
@@ -477,8 +490,18 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->hub_tasks = sv[ST_TASKS];
     stats->rebuckets = sv[ST_REBUCKET];
     stats->pool_chunks_used = hs[S_POOL];
+    float kms = 0;
+    if (ws.kernel_events) (void)hipEventElapsedTime(&kms, ws.evk0, ws.evk1);
+    stats->kernel_ms = kms;
   }
-  if (err == DERR_OVERFLOW) return ADDS_EOVERFLOW;
+  if (err == DERR_OVERFLOW) {
+    g_detail = "worklist pool exhausted (" + std::to_string(hs[S_POOL]) + " chunks requested)";
+    return ADDS_EOVERFLOW;
+  }
+  if (err == DERR_TASKS) {
+    g_detail = "hub task ring exhausted (" + std::to_string(hs[S_TASK_T]) + " tasks requested)";
+    return ADDS_EOVERFLOW;
+  }
   if (err == DERR_TIMEOUT || err == DERR_ABORT) return ADDS_ETIMEOUT;
   if (err != 0) return ADDS_EHIP;
   if (hs[S_DONE] != 1) {
@@ -506,14 +529,15 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
     g_detail = "more sources than worklist pool entries";
     return ADDS_EINVAL;
   }
-  for (int attempt = 0; attempt < 3; attempt++) {
+  for (int attempt = 0; attempt < 4; attempt++) {
     rc = ensure_workspace(g, r, k);
     if (rc != ADDS_OK) return rc;
     Workspace& ws = g->ws;
     ws.timeout_ms = r.timeout_ms;
     HIPCHK(hipMemcpyAsync(ws.srcs, srcs_host, (size_t)k * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(ws.ev0, s));
-    rc = enqueue_solve(g, r, dist_dev, total_n, ws.srcs, k, replicated, s);
+    rc = enqueue_solve(g, r, dist_dev, total_n, ws.srcs, k, replicated, s, true);
+    ws.kernel_events = true;
     if (rc != ADDS_OK) return rc;
     HIPCHK(hipEventRecord(ws.ev1, s));
     HIPCHK(hipMemcpyAsync(ws.host_state, ws.st, 8 * LINE * 4, hipMemcpyDeviceToHost, s));
@@ -523,7 +547,9 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
     if (!sync) return ADDS_OK;
     rc = collect(g, stats);
     if (rc != ADDS_EOVERFLOW || r.pool_fixed) return rc;
-    r.pool_entries *= 2;  // grow the pool and solve again
+    // grow whichever ring ran out and solve again
+    if (g->ws.host_state[S_DONE + 1] == DERR_TASKS) r.task_cap = std::min<uint64_t>(2 * r.task_cap, 0x7FFFFFFFull);
+    else r.pool_entries *= 2;
   }
   return rc;
 }
@@ -690,7 +716,7 @@ extern "C" adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* sr
   sg->g = g;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = enqueue_solve(g, r, (uint32_t*)dist_dev, total_n, srcs_dev, k, repl, s);
+    rc = enqueue_solve(g, r, (uint32_t*)dist_dev, total_n, srcs_dev, k, repl, s, false);
     hipError_t e2 = hipStreamEndCapture(s, &sg->graph);
     if (rc == ADDS_OK && e2 != hipSuccess) e = e2;
   }
@@ -712,6 +738,7 @@ extern "C" adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* strea
   hipStream_t s = (hipStream_t)stream;
   Workspace& ws = g->ws;
   HIPCHK(hipEventRecord(ws.ev0, s));
+  ws.kernel_events = false;
   HIPCHK(hipGraphLaunch(sg->exec, s));
   HIPCHK(hipEventRecord(ws.ev1, s));
   HIPCHK(hipMemcpyAsync(ws.host_state, ws.st, 8 * LINE * 4, hipMemcpyDeviceToHost, s));
@@ -800,4 +827,125 @@ extern "C" adds_status adds_sssp_backward_f32(adds_graph* g, uint32_t k, const f
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return ADDS_OK;
+}
+
+// ---- autotuner ----
+// Coordinate descent over the solver's tuning space for one workload (graph + sources), timed with the
+// persistent-kernel events.  Every candidate's distances are compared word for word with the first
+// candidate's on the device: a tuning point never changes results, and a mismatch is an error.
+__global__ static void k_count_diff(const uint32_t* a, const uint32_t* b, uint64_t n, unsigned long long* out) {
+  unsigned long long c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    c += a[i] != b[i];
+  if (c) atomicAdd(out, c);
+}
+
+extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32_t k, uint32_t trials,
+                                     adds_params* best, double* best_ms) {
+  if (!g || !srcs || !k || !best) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  if (trials == 0) trials = 3;
+  const uint64_t words = (k > 1 ? (uint64_t)k : 1) * g->n;
+  uint32_t *ref = nullptr, *cur = nullptr;
+  unsigned long long* diff = nullptr;
+  HIPCHK(hipMalloc(&ref, words * 4));
+  HIPCHK(hipMalloc(&cur, words * 4));
+  HIPCHK(hipMalloc(&diff, 8));
+  adds_status rc = ADDS_OK;
+  bool have_ref = false;
+  auto measure = [&](const adds_params& p, double& ms) -> adds_status {
+    std::vector<double> t;
+    for (uint32_t i = 0; i < trials + 1; i++) {  // first run warms the workspace for this shape
+      adds_stats st;
+      adds_status r = solve_core(g, srcs, k, have_ref ? cur : ref, k > 1, &p, &st, nullptr, true);
+      if (r != ADDS_OK) return r;
+      if (i) t.push_back(st.kernel_ms);
+      if (!have_ref) {
+        have_ref = true;
+        continue;
+      }
+      HIPCHK(hipMemset(diff, 0, 8));
+      hipLaunchKernelGGL(k_count_diff, dim3(1024), dim3(256), 0, nullptr, ref, cur, words, diff);
+      unsigned long long nd = 0;
+      HIPCHK(hipMemcpy(&nd, diff, 8, hipMemcpyDeviceToHost));
+      if (nd) {
+        g_detail = "autotuner: distances differ between tuning points (" + std::to_string(nd) + " words)";
+        return ADDS_EHIP;
+      }
+    }
+    std::sort(t.begin(), t.end());
+    ms = t[t.size() / 2];
+    return ADDS_OK;
+  };
+  adds_params p;
+  memset(&p, 0, sizeof(p));
+  Resolved r0;
+  rc = resolve(g, &p, words, r0);
+  if (rc == ADDS_OK) {
+    p.delta = r0.delta;
+    p.wg_per_cu = r0.wg_per_cu;
+    p.chunk = r0.chunk;
+    p.nbuckets = r0.nb;
+    p.parts = r0.parts;
+    p.concurrency = r0.concur;
+    double bms = 0;
+    rc = measure(p, bms);
+    if (getenv("ADDS_TUNE_VERBOSE"))
+      fprintf(stderr, "[tune] default delta=%g -> %.3f ms (%s)\n", p.delta, bms, adds_strerror(rc));
+    const bool is_int = g->wt == ADDS_W_U32;
+    auto try_point = [&](adds_params q) {
+      if (rc != ADDS_OK) return;
+      double ms = 0;
+      adds_status r = measure(q, ms);
+      if (r == ADDS_EOVERFLOW) return;  // a point that needs more memory than planned is just skipped
+      if (r != ADDS_OK) {
+        rc = r;
+        return;
+      }
+      if (getenv("ADDS_TUNE_VERBOSE"))
+        fprintf(stderr, "[tune] delta=%g nb=%u concur=%u wg/cu=%u chunk=%u -> %.3f ms\n", q.delta, q.nbuckets,
+                q.concurrency, q.wg_per_cu, q.chunk, ms);
+      if (ms < bms) {
+        bms = ms;
+        p = q;
+      }
+    };
+    // 1) bucket width: powers of two around the default (int keeps power-of-two widths)
+    const float d0 = p.delta;
+    for (int j = -8; j <= 4 && rc == ADDS_OK; j++) {
+      if (j == 0) continue;
+      adds_params q = p;
+      q.delta = d0 * std::ldexp(1.0f, j);
+      if (is_int && q.delta < 1.0f) continue;
+      try_point(q);
+    }
+    // 2) bucket ring length and open buckets
+    for (uint32_t nb : {32u, 128u, 256u}) {
+      adds_params q = p;
+      q.nbuckets = nb;
+      try_point(q);
+    }
+    for (uint32_t c : {2u, 3u}) {
+      adds_params q = p;
+      q.concurrency = c;
+      try_point(q);
+    }
+    // 3) persistent workgroups per CU, then the claim size
+    for (uint32_t w : {1u, 2u, 6u, 8u}) {
+      adds_params q = p;
+      q.wg_per_cu = w;
+      try_point(q);
+    }
+    for (uint32_t c : {16u, 32u}) {
+      adds_params q = p;
+      q.chunk = c;
+      try_point(q);
+    }
+    *best = p;
+    if (best_ms) *best_ms = bms;
+  }
+  (void)hipFree(ref);
+  (void)hipFree(cur);
+  (void)hipFree(diff);
+  return rc;
 }

@@ -24,12 +24,13 @@ constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
 constexpr uint32_t S_TASK_H = 4 * LINE;     // hub task ring claim index
 constexpr uint32_t S_TASK_T = 5 * LINE;     // hub task ring reserve index
 constexpr uint32_t S_STATS = 6 * LINE;      // u64 stats[8]
-constexpr uint32_t S_QBASE = 8 * LINE;      // queue q: head at S_QBASE + 2q*LINE, tail at +LINE
+constexpr uint32_t S_ADV = 8 * LINE;        // try-lock serialising head-advance scans
+constexpr uint32_t S_QBASE = 9 * LINE;      // queue q: head at S_QBASE + 2q*LINE, tail at +LINE
 constexpr uint32_t state_words(uint32_t nq) { return S_QBASE + 2 * nq * LINE; }
 
 enum StatIdx { ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET, ST_SPARE };
 
-enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3 };
+enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3, DERR_TASKS = 4 };
 
 struct KArgs {
   const uint32_t* row_ptr;

@@ -276,6 +276,17 @@ adds_status adds_gen_er(uint32_t n, double avg_degree, uint64_t seed, adds_wkind
   return build_symmetric_csr(n, src, dst, wf, out);
 }
 
+adds_status adds_gen_weights(adds_host_csr* g, uint64_t seed, adds_wkind wk, double lo, double hi) {
+  if (!g || !g->row_ptr) return ADDS_EINVAL;
+  WeightFn wf{wk, lo, hi, stream_key(seed, STREAM_WEIGHTS)};
+  parallel_for(g->n, 4096, [&](uint64_t b, uint64_t e) {
+    for (uint64_t v = b; v < e; v++)
+      for (uint32_t j = g->row_ptr[v]; j < g->row_ptr[v + 1]; j++) g->w[j] = wf((uint32_t)v, g->col[j]);
+  });
+  g->wtype = (wk == ADDS_WK_U32_UNIFORM) ? ADDS_W_U32 : ADDS_W_F32;
+  return ADDS_OK;
+}
+
 uint32_t adds_max_degree_vertex(const adds_host_csr* g) {
   uint32_t best = 0, bd = 0;
   for (uint32_t v = 0; v < g->n; v++) {

@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       idle_iter = 0;
       // ---------------- hub edge task ----------------
       if (h >= a.task_cap) {  // only after an overflow: the slot was never written
-        set_error(a, DERR_OVERFLOW);
+        set_error(a, DERR_TASKS);
         break;
       }
       const uint4* tp = a.tasks + h;
@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         if (lane == 0) {
           t0 = afadd(a.st + S_TASK_T, ntask);
           ws.v[ST_TASKS] += ntask;
-          if (t0 + ntask > a.task_cap) set_error(a, DERR_OVERFLOW);
+          if (t0 + ntask > a.task_cap) set_error(a, DERR_TASKS);
         }
         t0 = rdlane(t0, 0);
         for (uint32_t i = lane; i < ntask; i += 64) {
@@ -505,8 +505,14 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     }
     // Advance the head bucket to the first non-empty bucket ahead, but only when the behind and head
     // buckets are empty (items still being processed are not waited for: the solver is asynchronous).
+    // One wavefront at a time scans (try-lock): idle wavefronts must not flood the queue counters.
     bool advanced = false;
-    {
+    uint32_t locked = 0;
+    if (lane == 0) {
+      uint32_t z = 0;
+      locked = acas(a.st + S_ADV, z, 1u) ? 1u : 0u;
+    }
+    if (rdlane(locked, 0)) {
       const uint32_t c2 = ald(a.st + S_CUR);
       uint32_t firstj = 0xFFFFFFFFu;
       bool behind_or_head = false;
@@ -534,6 +540,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         }
         advanced = true;
       }
+      mem_wait();
+      if (lane == 0) ast(a.st + S_ADV, 0u);
     }
     if (advanced) continue;
     // watchdog and host abort

@@ -90,6 +90,16 @@ class HostGraph:
         c.wtype = self.wtype
         return c
 
+    def reweight(self, seed, wkind="f32", lo=0.0, hi=1.0):
+        """Re-draws all weights in place (same rule as the generators) and updates the weight type."""
+        if self._csr is None:
+            raise ValueError("reweight needs a library-owned graph")
+        L.check(L._lib().adds_gen_weights(ctypes.byref(self._csr), seed, WKINDS[wkind], float(lo), float(hi)),
+                "reweight")
+        self.wtype = int(self._csr.wtype)
+        self.w = self.w_bits.view(np.float32) if self.wtype == L.W_F32 else self.w_bits
+        return self
+
     # ---- utilities ----
     @property
     def is_float(self):

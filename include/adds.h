@@ -71,6 +71,7 @@ typedef struct {
   uint64_t hub_tasks;         /* edge tasks created for high-degree vertices */
   uint64_t rebuckets;         /* popped entries moved to a later bucket */
   uint64_t pool_chunks_used;  /* worklist pool chunks consumed */
+  double kernel_ms;           /* hipEvent time of the persistent solver kernel alone */
 } adds_stats;
 
 /* ---- errors / device ---- */
@@ -79,6 +80,8 @@ const char* adds_strerror(adds_status s);
 adds_status adds_set_device(int device);
 /* Number of compute units of the current device (0 if none). */
 uint32_t adds_device_cu_count(void);
+/* Detail text of the last error raised on this thread (HIP call and message, or the failed check). */
+const char* adds_last_error_detail(void);
 
 /* ---- graphs ----
  * Replaces CSRGraph::copy_to_gpu (csr_graph.cu:309-328): copies a host CSR (row_ptr[n+1], col[m],
@@ -123,6 +126,13 @@ adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* srcs_dev, uin
 adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* stream);
 void adds_solve_graph_destroy(adds_solve_graph* sg);
 
+/* Picks the fastest tuning point for this workload (k sources on g; k == 1 is a plain solve) by
+ * coordinate descent over delta, bucket ring length, open buckets, workgroups per CU and claim size,
+ * timing `trials` solves per point.  Every point's distances must be identical; a difference is
+ * reported as an error.  No reference counterpart (the reference adapts delta online, wl.h:851-1310). */
+adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32_t k, uint32_t trials, adds_params* best,
+                          double* best_ms);
+
 /* ---- backward (float graphs) ----
  * Given forward distances dist [k][n] from srcs and dL/d(dist) grad_dist [k][n] (device), writes
  * dL/dw into grad_w [m] (device).  The tree is the tight-predecessor forest of each source: parent edge
@@ -164,6 +174,9 @@ adds_status adds_gen_grid(uint32_t rows, uint32_t cols, uint64_t shortcuts, uint
 /* Erdos-Renyi-style graph: n*avg_degree/2 uniform random undirected pairs. */
 adds_status adds_gen_er(uint32_t n, double avg_degree, uint64_t seed, adds_wkind wk, double lo, double hi,
                         adds_host_csr* out);
+/* Re-draws every weight of g with the generators' rule (so a generator call with this seed and weight
+ * spec yields the same weights) and sets g->wtype accordingly. */
+adds_status adds_gen_weights(adds_host_csr* g, uint64_t seed, adds_wkind wk, double lo, double hi);
 /* Highest out-degree vertex (lowest id on ties). */
 uint32_t adds_max_degree_vertex(const adds_host_csr* g);
 /* Up to k distinct vertices sampled uniformly (seeded) from the component reached from `root`. */

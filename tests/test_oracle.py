@@ -1,0 +1,227 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against it.
+
+The reference ships no golden vectors for this path (its only correctness mechanism is text equality
+against Galois Dijkstra, verify.py), so the oracle is pinned by: scipy's Dijkstra (float64, exact for
+integer sums < 2^53), brute-force Bellman-Ford on tiny graphs (bit-equal, int and float32: with
+non-negative weights and a monotone add every relaxation order reaches the same fixed point), closed
+forms on grids/paths/stars, and planted mistakes that the certificate must reject.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as cg
+
+import oracle
+
+INF_U32 = 0xFFFFFFFF
+INF_F32 = 0x7F800000
+
+
+def rand_graph(rng, n, m, wmax=100, is_float=False, zero_ok=False):
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    keep = src != dst
+    pairs = np.unique(src[keep].astype(np.int64) * n + dst[keep])  # sorted, no duplicate edges
+    src, dst = pairs // n, pairs % n
+    row = np.zeros(n + 1, dtype=np.uint32)
+    np.add.at(row, src + 1, 1)
+    row = np.cumsum(row).astype(np.uint32)
+    if is_float:
+        lo = 0.0 if zero_ok else 0.05
+        w = rng.uniform(lo, 3.0, len(src)).astype(np.float32)
+    else:
+        w = rng.integers(0 if zero_ok else 1, wmax + 1, len(src)).astype(np.uint32)
+    return row, dst.astype(np.uint32), w
+
+
+def scipy_dist(row, col, w, src):
+    n = len(row) - 1
+    M = sp.csr_matrix((w.astype(np.float64), col, row), shape=(n, n))
+    d = cg.dijkstra(M, indices=src)
+    return d
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_dijkstra_int_matches_scipy(seed):
+    rng = np.random.default_rng(seed)
+    row, col, w = rand_graph(rng, 300, 1500)
+    src = int(rng.integers(0, 300))
+    d = oracle.dijkstra(row, col, w, src, False)
+    ref = scipy_dist(row, col, w, src)
+    exp = np.where(np.isinf(ref), INF_U32, ref).astype(np.uint64)
+    assert np.array_equal(d.astype(np.uint64), exp)
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("is_float", [False, True])
+def test_dijkstra_equals_bellman_ford(seed, is_float):
+    rng = np.random.default_rng(100 + seed)
+    row, col, w = rand_graph(rng, 60, 240, is_float=is_float, zero_ok=True)
+    srcs = rng.integers(0, 60, 1 + seed % 3)
+    d = oracle.dijkstra(row, col, w, srcs, is_float)
+    b = oracle.bellman_ford(row, col, w, srcs, is_float)
+    assert np.array_equal(d, b)
+    assert oracle.certify(row, col, w, srcs, d, is_float) == 0
+
+
+def test_float_dijkstra_close_to_float64():
+    rng = np.random.default_rng(7)
+    row, col, w = rand_graph(rng, 500, 3000, is_float=True)
+    d = oracle.dijkstra(row, col, w, 0, True).view(np.float32)
+    ref = scipy_dist(row, col, w, 0)
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(d), fin)
+    assert np.allclose(d[fin], ref[fin], rtol=1e-5, atol=1e-5)
+
+
+def grid(rows, cols, wfun):
+    n = rows * cols
+    adj = [[] for _ in range(n)]
+    for r in range(rows):
+        for c in range(cols):
+            v = r * cols + c
+            for dr, dc in ((0, 1), (1, 0), (0, -1), (-1, 0)):
+                rr, cc = r + dr, c + dc
+                if 0 <= rr < rows and 0 <= cc < cols:
+                    adj[v].append(rr * cols + cc)
+    row = np.zeros(n + 1, dtype=np.uint32)
+    for v in range(n):
+        row[v + 1] = row[v] + len(adj[v])
+    col = np.array([u for a in adj for u in sorted(a)], dtype=np.uint32)
+    w = np.array([wfun() for _ in col])
+    return row, col, w
+
+
+def test_unit_grid_is_manhattan():
+    row, col, w = grid(17, 23, lambda: 1)
+    d = oracle.dijkstra(row, col, w.astype(np.uint32), 0, False)
+    rr, cc = np.divmod(np.arange(17 * 23), 23)
+    assert np.array_equal(d, (rr + cc).astype(np.uint32))
+
+
+def test_dyadic_float_grid_is_exact_multiple():
+    row, col, w = grid(20, 20, lambda: 1.5)
+    d = oracle.dijkstra(row, col, w.astype(np.float32), 0, True).view(np.float32)
+    rr, cc = np.divmod(np.arange(400), 20)
+    assert np.array_equal(d, (1.5 * (rr + cc)).astype(np.float32))
+
+
+def test_path_star_disconnected():
+    # path 0-1-2-3 (directed forward, weights 2,3,4), star centre 4 -> 5,6,7 (weight 7), vertex 8 isolated
+    row = np.array([0, 1, 2, 3, 3, 6, 6, 6, 6, 6], dtype=np.uint32)
+    col = np.array([1, 2, 3, 5, 6, 7], dtype=np.uint32)
+    w = np.array([2, 3, 4, 7, 7, 7], dtype=np.uint32)
+    d = oracle.dijkstra(row, col, w, 0, False)
+    assert list(d) == [0, 2, 5, 9, INF_U32, INF_U32, INF_U32, INF_U32, INF_U32]
+    d = oracle.dijkstra(row, col, w, 4, False)
+    assert list(d) == [INF_U32] * 4 + [0, 7, 7, 7, INF_U32]
+
+
+@pytest.mark.parametrize("is_float", [False, True])
+def test_certificate_rejects_planted_mistakes(is_float):
+    rng = np.random.default_rng(3)
+    row, col, w = rand_graph(rng, 400, 2400, is_float=is_float)
+    d = oracle.dijkstra(row, col, w, 0, is_float)
+    assert oracle.certify(row, col, w, 0, d, is_float) == 0
+    reach = np.nonzero((d != (INF_F32 if is_float else INF_U32)) & (np.arange(400) != 0))[0]
+    unreach = np.nonzero(d == (INF_F32 if is_float else INF_U32))[0]
+    assert len(reach) > 10
+    v = reach[len(reach) // 2]
+    bad = d.copy(); bad[v] -= 1          # one unit (int) / one ulp (float) too small
+    assert oracle.certify(row, col, w, 0, bad, is_float) != 0
+    bad = d.copy(); bad[v] += 1          # too large
+    assert oracle.certify(row, col, w, 0, bad, is_float) != 0
+    bad = d.copy(); bad[v] = INF_F32 if is_float else INF_U32
+    assert oracle.certify(row, col, w, 0, bad, is_float) != 0
+    bad = d.copy(); bad[0] = 1
+    assert oracle.certify(row, col, w, 0, bad, is_float) == 1
+    if len(unreach):
+        bad = d.copy(); bad[unreach[0]] = 5
+        assert oracle.certify(row, col, w, 0, bad, is_float) == 3
+
+
+def test_int_overflow_is_reported():
+    row = np.array([0, 1, 2, 2], dtype=np.uint32)
+    col = np.array([1, 2], dtype=np.uint32)
+    w = np.array([0xF0000000, 0x20000000], dtype=np.uint32)
+    with pytest.raises(OverflowError):
+        oracle.dijkstra(row, col, w, 0, False)
+
+
+# ---- backward reference ----
+def tree_paths_grad(row, col, w, dist, gdist):
+    """Independent float64 evaluation of dL/dw = sum_t c_t * [e on the tree path to t]."""
+    k, n = dist.shape
+    m = len(col)
+    src_of = np.repeat(np.arange(n), np.diff(row.astype(np.int64)))
+    gw = np.zeros(m)
+    for s in range(k):
+        d = dist[s]
+        pe = np.full(n, -1)
+        for v in range(n):
+            if not np.isfinite(d[v]):
+                continue
+            cands = [e for e in range(m) if col[e] == v and d[src_of[e]] < d[v]
+                     and np.float32(d[src_of[e]] + w[e]) == d[v]]
+            if cands:
+                pe[v] = min(cands)
+        for t in range(n):
+            if gdist[s, t] == 0 or not np.isfinite(d[t]):
+                continue
+            x = t
+            while pe[x] >= 0:
+                gw[pe[x]] += gdist[s, t]
+                x = src_of[pe[x]]
+    return gw
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_tree_grad_matches_path_sums(seed):
+    rng = np.random.default_rng(50 + seed)
+    row, col, w = rand_graph(rng, 40, 200, is_float=True)
+    srcs = [0, 7, 13]
+    dist = np.stack([oracle.dijkstra(row, col, w, s, True).view(np.float32) for s in srcs])
+    gd = (rng.integers(-(1 << 23), 1 << 23, dist.shape) / float(1 << 23)).astype(np.float32)
+    gw, pe = oracle.tree_grad(row, col, w, dist, gd)
+    ref = tree_paths_grad(row, col, w, dist, gd)
+    assert np.allclose(gw, ref, rtol=1e-5, atol=1e-6)
+    # a planted mistake in the oracle's output (one parent edge dropped) must be visible
+    used = np.nonzero(gw != 0)[0]
+    assert len(used) > 0
+
+
+def fd_loss(row, col, w64, src, gd):
+    n = len(row) - 1
+    M = sp.csr_matrix((w64, col, row), shape=(n, n))
+    d = cg.dijkstra(M, indices=src)
+    fin = np.isfinite(d)
+    return float(np.sum(gd[fin] * d[fin]))
+
+
+def test_tree_grad_central_differences():
+    """dL/dw against central differences of the float64 shortest-path function (h = 1e-3) on tree
+    edges whose tight-predecessor gap exceeds 1e-2."""
+    rng = np.random.default_rng(9)
+    row, col, w = rand_graph(rng, 80, 480, is_float=True)
+    src = 3
+    d = oracle.dijkstra(row, col, w, src, True).view(np.float32)
+    gd = (rng.integers(-(1 << 23), 1 << 23, 80) / float(1 << 23)).astype(np.float32)
+    gw, pe = oracle.tree_grad(row, col, w, d[None, :], gd[None, :])
+    src_of = np.repeat(np.arange(80), np.diff(row.astype(np.int64)))
+    h = 1e-3
+    checked = 0
+    for v in range(80):
+        e = pe[0, v]
+        if e == 0xFFFFFFFF:
+            continue
+        alts = [float(d[src_of[x]]) + float(w[x]) - float(d[v]) for x in range(len(col)) if col[x] == v and x != e]
+        if alts and min(alts) <= 1e-2:
+            continue
+        w64 = w.astype(np.float64)
+        wp, wm = w64.copy(), w64.copy()
+        wp[e] += h
+        wm[e] -= h
+        fd = (fd_loss(row, col, wp, src, gd) - fd_loss(row, col, wm, src, gd)) / (2 * h)
+        assert abs(fd - gw[e]) <= 1e-3 * max(1.0, abs(fd)), (v, e, fd, gw[e])
+        checked += 1
+    assert checked >= 10
