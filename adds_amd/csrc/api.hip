@@ -896,8 +896,17 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
     auto try_point = [&](adds_params q) {
       if (rc != ADDS_OK) return;
       double ms = 0;
+      // a point far slower than the best so far is cut off by the device watchdog and skipped
+      q.timeout_ms = (uint32_t)std::min(120000.0, std::max(2000.0, 20.0 * bms));
       adds_status r = measure(q, ms);
-      if (r == ADDS_EOVERFLOW) return;  // a point that needs more memory than planned is just skipped
+      q.timeout_ms = 0;
+      // points that need more memory than planned, or that are cut off, are skipped
+      if (r == ADDS_EOVERFLOW || r == ADDS_ETIMEOUT) {
+        if (getenv("ADDS_TUNE_VERBOSE"))
+          fprintf(stderr, "[tune] delta=%g nb=%u concur=%u wg/cu=%u chunk=%u -> skipped (%s)\n", q.delta,
+                  q.nbuckets, q.concurrency, q.wg_per_cu, q.chunk, adds_strerror(r));
+        return;
+      }
       if (r != ADDS_OK) {
         rc = r;
         return;
@@ -912,7 +921,7 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
     };
     // 1) bucket width: powers of two around the default (int keeps power-of-two widths)
     const float d0 = p.delta;
-    for (int j = -8; j <= 4 && rc == ADDS_OK; j++) {
+    for (int j = -5; j <= 3 && rc == ADDS_OK; j++) {
       if (j == 0) continue;
       adds_params q = p;
       q.delta = d0 * std::ldexp(1.0f, j);
@@ -948,4 +957,27 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
   (void)hipFree(cur);
   (void)hipFree(diff);
   return rc;
+}
+
+// ---- cross-XCD ordering litmus (diagnostic) ----
+extern "C" adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint64_t* forbidden,
+                                   uint64_t* lost) {
+  if (!pairs || !rounds || variant > 2 || !forbidden || !lost) return ADDS_EINVAL;
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return ADDS_ENODEV;
+  uint32_t* mem = nullptr;
+  unsigned long long* out = nullptr;
+  HIPCHK(hipMalloc(&mem, (size_t)pairs * 128 * 4));
+  HIPCHK(hipMalloc(&out, 16));
+  HIPCHK(launch_fill_u32(mem, (uint64_t)pairs * 128, 0, nullptr));
+  for (uint32_t p = 0; p < pairs; p++) HIPCHK(hipMemset(mem + (size_t)p * 128 + 96, 0xFF, 4));
+  HIPCHK(hipMemset(out, 0, 16));
+  HIPCHK(launch_litmus(mem, pairs, rounds, variant, out, nullptr));
+  unsigned long long h[2] = {0, 0};
+  HIPCHK(hipMemcpy(h, out, 16, hipMemcpyDeviceToHost));
+  (void)hipFree(mem);
+  (void)hipFree(out);
+  *forbidden = h[0];
+  *lost = h[1];
+  return ADDS_OK;
 }

@@ -66,6 +66,10 @@ hipError_t launch_init(const KArgs& a, const uint32_t* srcs, uint32_t k, int rep
 hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s);
 hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
 
+// Litmus (litmus.hip)
+hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_t variant, unsigned long long* out,
+                         hipStream_t s);
+
 // Backward (backward_kernels.hip)
 struct BwdArgs {
   const uint32_t* row_ptr;

@@ -1,0 +1,76 @@
+// Cross-XCD message-passing litmus test for the memory-ordering recipe the solver relies on.
+//
+// Workgroup pairs (2p, 2p+1) run on different XCDs (workgroups are dealt round-robin over the 8 XCDs).
+// In every round the producer
+//   1. writes a payload word with a relaxed agent-scope store        (like a worklist entry / adjoint),
+//   2. waits for its completion (workgroup-scope fence = s_waitcnt vmcnt(0)),
+//   3. publishes the round through a flag: a relaxed agent-scope store in variant 0, an agent-scope
+//      fetch_add in variant 1 (like a queue tail / idle counter / child counter).
+// The consumer spins on the flag with relaxed agent-scope loads, waits, then loads the payload with a
+// relaxed agent-scope load.  Seeing the new flag but an old payload is the forbidden outcome.  A third
+// variant checks atomicMin visibility: the producer lowers dist[p] with an agent-scope atomic min, waits,
+// and publishes; the consumer must read a dist no larger than the published value.
+// Every spin is bounded; a bound hit is counted separately (it would mean a lost message).
+#include "device.h"
+
+namespace adds {
+
+__device__ __forceinline__ uint32_t lld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lst(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lwait() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+// words per pair: [0] payload, [32] flag, [64] ack, [96] dist   (each on its own 128-byte line)
+__global__ void k_litmus(uint32_t* mem, uint32_t rounds, uint32_t variant, unsigned long long* out) {
+  if (threadIdx.x != 0) return;
+  const uint32_t pair = blockIdx.x >> 1;
+  const bool producer = (blockIdx.x & 1) == 0;
+  uint32_t* w = mem + (size_t)pair * 128;
+  unsigned long long forbidden = 0, lost = 0;
+  for (uint32_t t = 1; t <= rounds; t++) {
+    if (producer) {
+      if (variant == 2) {
+        __hip_atomic_fetch_min(w + 96, 0xFFFFFFFFu - t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        lst(w, t);
+      }
+      lwait();
+      if (variant == 1) __hip_atomic_fetch_add(w + 32, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else lst(w + 32, t);
+      uint32_t spins = 0;
+      while (lld(w + 64) != t && ++spins < (1u << 22)) __builtin_amdgcn_s_sleep(1);
+      if (spins >= (1u << 22)) {
+        lost++;
+        break;
+      }
+    } else {
+      uint32_t spins = 0;
+      while (lld(w + 32) != t && ++spins < (1u << 22)) __builtin_amdgcn_s_sleep(1);
+      if (spins >= (1u << 22)) {
+        lost++;
+        break;
+      }
+      lwait();
+      if (variant == 2) {
+        if (lld(w + 96) > 0xFFFFFFFFu - t) forbidden++;
+      } else {
+        if (lld(w) < t) forbidden++;
+      }
+      lwait();
+      lst(w + 64, t);
+    }
+  }
+  if (forbidden) atomicAdd(out, forbidden);
+  if (lost) atomicAdd(out + 1, lost);
+}
+
+hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_t variant, unsigned long long* out,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_litmus, dim3(2 * pairs), dim3(64), 0, s, mem, rounds, variant, out);
+  return hipGetLastError();
+}
+
+}  // namespace adds

@@ -210,15 +210,43 @@ __device__ __forceinline__ void push_wave(const KArgs& a, bool f, uint32_t gid, 
   }
 }
 
-// Relaxes one edge per active lane: (tgt, nd) where nd = d[u] + w.  Returns whether this lane pushes.
+// Relaxes UNROLL independent edges per lane (one 64-edge window each) so that UNROLL dependent chains
+// (col/w load -> d[v] load -> atomicMin) are in flight per wavefront, then pushes the improvements.
+constexpr int UNROLL = 4;
+
 template <bool FLT>
-__device__ __forceinline__ bool relax_lane(const KArgs& a, bool act, uint32_t tgt, uint32_t nd, WaveStats& ws) {
-  if (!act) return false;
-  const uint32_t dv = a.dist[tgt];  // plain load: a stale value is larger, so it only costs an atomic
-  if (!(nd < dv)) return false;
-  ws.v[ST_RELAX]++;
-  const uint32_t old = afmin(a.dist + tgt, nd);
-  return old > nd;
+__device__ __forceinline__ void relax_group(const KArgs& a, const bool (&act)[UNROLL], const uint32_t (&e)[UNROLL],
+                                            const uint32_t (&du)[UNROLL], const uint32_t (&gb)[UNROLL], uint32_t part,
+                                            uint32_t c_now, WaveStats& ws) {
+  uint32_t tgt[UNROLL], nd[UNROLL], dv[UNROLL];
+  bool f[UNROLL];
+#pragma unroll
+  for (int j = 0; j < UNROLL; j++) {
+    tgt[j] = 0;
+    nd[j] = 0;
+    if (act[j]) {
+      tgt[j] = gb[j] + a.col[e[j]];
+      nd[j] = dadd<FLT>(du[j], a.w[e[j]]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < UNROLL; j++) dv[j] = act[j] ? a.dist[tgt[j]] : 0u;  // stale values only cost an atomic
+#pragma unroll
+  for (int j = 0; j < UNROLL; j++) {
+    f[j] = act[j] && nd[j] < dv[j];
+    ws.v[ST_EDGES] += act[j] ? 1u : 0u;
+  }
+  uint32_t old[UNROLL];
+#pragma unroll
+  for (int j = 0; j < UNROLL; j++) {
+    old[j] = 0xFFFFFFFFu;
+    if (f[j]) {
+      ws.v[ST_RELAX]++;
+      old[j] = afmin(a.dist + tgt[j], nd[j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < UNROLL; j++) push_wave<FLT>(a, f[j] && old[j] > nd[j], tgt[j], nd[j], part, c_now, ws);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -242,7 +270,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   for (int i = 0; i < 8; i++) ws.v[i] = 0;
   bool busy = false;  // wave-uniform
   const uint64_t t_start = wall_clock64();
-  uint32_t idle_iter = 0;
+  uint32_t idle_iter = 0, work_iter = 0;
 
   for (;;) {
     // ---------------- probe the open queues in parallel, then claim (lane 0) ----------------
@@ -320,6 +348,25 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       }
     }
 
+    // The watchdog and the host abort word are also polled while busy, so that a solve that never
+    // goes idle can still be ended.
+    if (kind != W_NONE && (++work_iter & 255) == 0) {
+      uint32_t quit = 0;
+      if (lane == 0) {
+        if (wall_clock64() - t_start > a.timeout_ticks) {
+          set_error(a, DERR_TIMEOUT);
+          quit = 1;
+        } else if (a.host_abort &&
+                   __hip_atomic_load(a.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
+          set_error(a, DERR_ABORT);
+          quit = 1;
+        } else if (ald(a.st + S_DONE + 1) != 0) {
+          quit = 1;
+        }
+      }
+      if (rdlane(quit, 0)) break;
+    }
+
     if (kind == W_TASK) {
       idle_iter = 0;
       // ---------------- hub edge task ----------------
@@ -351,18 +398,18 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         continue;
       }
       const uint32_t gbase = REPL ? (gid_u / a.n) * a.n : 0u;
-      for (uint32_t base = eb; base < ee; base += 64) {
+      for (uint32_t base = eb; base < ee; base += 64 * UNROLL) {
         const uint32_t c_now = ald(a.st + S_CUR);
-        const uint32_t e = base + lane;
-        const bool act = e < ee;
-        uint32_t tgt = 0, nd = 0;
-        if (act) {
-          tgt = gbase + a.col[e];
-          nd = dadd<FLT>(du, a.w[e]);
-          ws.v[ST_EDGES]++;
+        bool act[UNROLL];
+        uint32_t e[UNROLL], dus[UNROLL], gbs[UNROLL];
+#pragma unroll
+        for (int j = 0; j < UNROLL; j++) {
+          e[j] = base + 64 * j + lane;
+          act[j] = e[j] < ee;
+          dus[j] = du;
+          gbs[j] = gbase;
         }
-        const bool f = relax_lane<FLT>(a, act, tgt, nd, ws);
-        push_wave<FLT>(a, f, tgt, nd, part, c_now, ws);
+        relax_group<FLT>(a, act, e, dus, gbs, part, c_now, ws);
       }
       continue;
     }
@@ -452,26 +499,31 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         if (REPL) s_gb[wv][lane] = gbase;
         if (lane == 0) s_pref[wv][64] = total;
         wave_sync();
-        for (uint32_t base = 0; base < total; base += 64) {
+        for (uint32_t base = 0; base < total; base += 64 * UNROLL) {
           const uint32_t c_now = ald(a.st + S_CUR);
-          const uint32_t i = base + lane;
-          const bool act = i < total;
-          uint32_t tgt = 0, nd = 0;
-          if (act) {
-            // owner = last lane L with s_pref[L] <= i (its degree is then non-zero)
-            uint32_t lo = 0, hi = 64;
-            while (hi - lo > 1) {
-              const uint32_t mid = (lo + hi) >> 1;
-              if (s_pref[wv][mid] <= i) lo = mid;
-              else hi = mid;
+          bool act[UNROLL];
+          uint32_t e[UNROLL], dus[UNROLL], gbs[UNROLL];
+#pragma unroll
+          for (int j = 0; j < UNROLL; j++) {
+            const uint32_t i = base + 64 * j + lane;
+            act[j] = i < total;
+            e[j] = 0;
+            dus[j] = 0;
+            gbs[j] = 0;
+            if (act[j]) {
+              // owner = last lane L with s_pref[L] <= i (its degree is then non-zero)
+              uint32_t lo = 0, hi = 64;
+              while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pref[wv][mid] <= i) lo = mid;
+                else hi = mid;
+              }
+              e[j] = s_rs[wv][lo] + (i - s_pref[wv][lo]);
+              dus[j] = s_du[wv][lo];
+              gbs[j] = REPL ? s_gb[wv][lo] : 0u;
             }
-            const uint32_t e = s_rs[wv][lo] + (i - s_pref[wv][lo]);
-            tgt = (REPL ? s_gb[wv][lo] : 0u) + a.col[e];
-            nd = dadd<FLT>(s_du[wv][lo], a.w[e]);
-            ws.v[ST_EDGES]++;
           }
-          const bool f = relax_lane<FLT>(a, act, tgt, nd, ws);
-          push_wave<FLT>(a, f, tgt, nd, part, c_now, ws);
+          relax_group<FLT>(a, act, e, dus, gbs, part, c_now, ws);
         }
         wave_sync();
       }

@@ -133,6 +133,13 @@ void adds_solve_graph_destroy(adds_solve_graph* sg);
 adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32_t k, uint32_t trials, adds_params* best,
                           double* best_ms);
 
+/* Diagnostic with no reference counterpart: cross-XCD message-passing litmus test of the ordering recipe
+ * the solver uses (relaxed agent-scope atomics ordered by waiting for completion).  `pairs` producer /
+ * consumer workgroup pairs on different XCDs run `rounds` rounds; variant 0 publishes with a store,
+ * 1 with a fetch_add, 2 checks atomicMin visibility.  *forbidden counts stale observations (must be 0),
+ * *lost counts rounds whose bounded wait expired. */
+adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint64_t* forbidden, uint64_t* lost);
+
 /* ---- backward (float graphs) ----
  * Given forward distances dist [k][n] from srcs and dL/d(dist) grad_dist [k][n] (device), writes
  * dL/dw into grad_w [m] (device).  The tree is the tight-predecessor forest of each source: parent edge

@@ -27,6 +27,8 @@ def _lib():
             fn.argtypes = [ctypes.c_int, u32, p, p, p, p, u32, p]
         L.oracle_tree_grad.restype = None
         L.oracle_tree_grad.argtypes = [u32, p, p, p, u32, p, p, p, p]
+        L.oracle_delta_step.restype = ctypes.c_int
+        L.oracle_delta_step.argtypes = [ctypes.c_int, u32, p, p, p, u32, ctypes.c_double, ctypes.c_uint, p]
         _L = L
     return _L
 
@@ -61,6 +63,16 @@ def bellman_ford(row_ptr, col, w, srcs, is_float):
     d = np.empty(n, dtype=np.uint32)
     if _lib().oracle_bellman_ford(int(is_float), n, _p(row_ptr), _p(col), _p(wb), _p(srcs), len(srcs), _p(d)):
         raise OverflowError("integer distance overflow")
+    return d
+
+
+def delta_step(row_ptr, col, w, src, is_float, delta, threads=None):
+    """Parallel delta-stepping (all host threads by default); distance words like dijkstra()."""
+    row_ptr, col, wb = _prep(row_ptr, col, w)
+    n = len(row_ptr) - 1
+    d = np.empty(n, dtype=np.uint32)
+    threads = threads or os.cpu_count() or 1
+    _lib().oracle_delta_step(int(is_float), n, _p(row_ptr), _p(col), _p(wb), int(src), float(delta), threads, _p(d))
     return d
 
 
