@@ -16,7 +16,7 @@ W_U32, W_F32 = 0, 1
 # adds_wkind
 WK_U32_UNIFORM, WK_F32_UNIFORM, WK_F32_LOGNORMAL = 0, 1, 2
 # flags
-F_HOST_OUT, F_NO_SYNC = 0x1, 0x2
+F_HOST_OUT, F_NO_SYNC, F_ATOMIC_PROBE = 0x1, 0x2, 0x4
 
 INF_U32 = 0xFFFFFFFF
 INF_F32_BITS = 0x7F800000
@@ -82,6 +82,9 @@ class Stats(ctypes.Structure):
         ("rebuckets", ctypes.c_uint64),
         ("pool_chunks_used", ctypes.c_uint64),
         ("kernel_ms", ctypes.c_double),
+        ("probes", ctypes.c_uint64),
+        ("cas_failures", ctypes.c_uint64),
+        ("idle_iterations", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -37,9 +37,8 @@ struct Workspace {
   uint32_t* st = nullptr;
   uint64_t* pool = nullptr;
   uint32_t* tab = nullptr;
-  uint4* tasks = nullptr;
   uint32_t* srcs = nullptr;
-  uint32_t nq = 0, maxv = 0, pool_chunks = 0, task_cap = 0, srcs_cap = 0;
+  uint32_t nq = 0, nb = 0, maxv = 0, pool_chunks = 0, srcs_cap = 0;
   uint32_t* host_state = nullptr;       // pinned copy of the first 8 state lines
   uint32_t* host_abort = nullptr;       // pinned, device-visible abort word
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // solve start, solve end, state copied
@@ -113,7 +112,6 @@ static adds_status free_workspace(Workspace& ws) {
   if (ws.st) (void)hipFree(ws.st);
   if (ws.pool) (void)hipFree(ws.pool);
   if (ws.tab) (void)hipFree(ws.tab);
-  if (ws.tasks) (void)hipFree(ws.tasks);
   if (ws.srcs) (void)hipFree(ws.srcs);
   if (ws.host_state) (void)hipHostFree(ws.host_state);
   if (ws.host_abort) (void)hipHostFree(ws.host_abort);
@@ -290,7 +288,6 @@ struct Resolved {
   float delta;
   uint32_t wg_per_cu, chunk, nb, parts, hub_degree, task_edges, concur, flags;
   uint64_t pool_entries;
-  uint64_t task_cap;
   bool pool_fixed;
   double timeout_ms;
 };
@@ -321,20 +318,18 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.parts = p->parts ? p->parts : 8;
   r.hub_degree = p->hub_degree ? p->hub_degree : 512;
   r.task_edges = p->task_edges ? p->task_edges : 1024;
-  r.concur = p->concurrency ? p->concurrency : 1;
+  r.concur = p->concurrency ? p->concurrency : 2;
   r.flags = p->flags;
   r.timeout_ms = p->timeout_ms ? (double)p->timeout_ms : 120000.0;
   r.pool_fixed = p->pool_entries != 0;
   r.pool_entries = p->pool_entries ? p->pool_entries : 3 * total_n + (4u << 20);
   if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 64 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
-      r.parts != 8) || r.concur < 1 || r.concur > 4 || r.task_edges == 0 || r.hub_degree < 64) {
+      r.parts != 8 && r.parts != 16) || (r.concur + 1) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 || r.task_edges == 0 || r.hub_degree < 64) {
     g_detail = "parameter out of range";
     return ADDS_EINVAL;
   }
   if (r.pool_entries < CH) r.pool_entries = CH;
-  // hub edge tasks: room for every hub edge to be expanded 16 times over
-  r.task_cap = std::min<uint64_t>(std::max<uint64_t>(1u << 16, 16 * (g->m / r.task_edges) + 4096), 0x7FFFFFFFull);
   return ADDS_OK;
 }
 
@@ -344,8 +339,6 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
   const uint64_t chunks64 = (r.pool_entries + CH - 1) / CH;
   if (chunks64 >= TAB_BAD) return ADDS_EINVAL;
   const uint32_t chunks = (uint32_t)chunks64;
-  const uint64_t task_cap64 = r.task_cap;
-  const uint32_t task_cap = (uint32_t)std::min<uint64_t>(task_cap64, 0x7FFFFFFFull);
   if (!ws.host_state) {
     HIPCHK(hipHostMalloc((void**)&ws.host_state, 8 * LINE * 4, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&ws.host_abort, 256, hipHostMallocCoherent | hipHostMallocMapped));
@@ -356,32 +349,27 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
     HIPCHK(hipEventCreate(&ws.evk0));
     HIPCHK(hipEventCreate(&ws.evk1));
   }
-  if (ws.nq != nq || ws.pool_chunks < chunks || ws.task_cap < task_cap || !ws.st) {
-    // (re)allocate state, pool, chunk maps and task ring, then put them in their empty state
+  if (ws.nq != nq || ws.nb != r.nb || ws.pool_chunks < chunks || !ws.st) {
+    // (re)allocate state, pool and chunk maps, then put them in their empty state
     HIPCHK(hipDeviceSynchronize());
     if (ws.st) (void)hipFree(ws.st);
     if (ws.pool) (void)hipFree(ws.pool);
     if (ws.tab) (void)hipFree(ws.tab);
-    if (ws.tasks) (void)hipFree(ws.tasks);
     ws.st = nullptr;
     ws.pool = nullptr;
     ws.tab = nullptr;
-    ws.tasks = nullptr;
     const uint32_t pc = std::max(chunks, ws.pool_chunks);
-    const uint32_t tc = std::max(task_cap, ws.task_cap);
-    HIPCHK(hipMalloc(&ws.st, (size_t)state_words(nq) * 4));
+    HIPCHK(hipMalloc(&ws.st, (size_t)state_words(nq, r.nb) * 4));
     HIPCHK(hipMalloc(&ws.pool, (size_t)pc * CH * 8));
     HIPCHK(hipMalloc(&ws.tab, (size_t)nq * pc * 4));
-    HIPCHK(hipMalloc(&ws.tasks, (size_t)tc * 16));
-    HIPCHK(launch_fill_u32(ws.st, state_words(nq), 0, nullptr));
+    HIPCHK(launch_fill_u32(ws.st, state_words(nq, r.nb), 0, nullptr));
     HIPCHK(launch_fill_u64(ws.pool, (uint64_t)pc * CH, EMPTY64, nullptr));
     HIPCHK(launch_fill_u32(ws.tab, (uint64_t)nq * pc, TAB_UNSET, nullptr));
-    HIPCHK(launch_fill_u32((uint32_t*)ws.tasks, (uint64_t)tc * 4, EMPTY32, nullptr));
     HIPCHK(hipDeviceSynchronize());
     ws.nq = nq;
+    ws.nb = r.nb;
     ws.pool_chunks = pc;
     ws.maxv = pc;
-    ws.task_cap = tc;
   }
   if (ws.srcs_cap < nsrc) {
     if (ws.srcs) (void)hipFree(ws.srcs);
@@ -405,13 +393,11 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.st = ws.st;
   a.pool = ws.pool;
   a.tab = ws.tab;
-  a.tasks = ws.tasks;
   a.host_abort = ws.host_abort;
   a.n = g->n;
   a.total_n = (uint32_t)total_n;
   a.pool_chunks = (uint32_t)std::min<uint64_t>(ws.pool_chunks, (r.pool_entries + CH - 1) / CH);
   a.maxv = ws.maxv;
-  a.task_cap = ws.task_cap;
   a.nb = r.nb;
   a.parts = r.parts;
   a.inv_delta = 1.0f / r.delta;
@@ -424,6 +410,7 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->device);
   if (rate_khz <= 0) rate_khz = 100000;
   a.timeout_ticks = (uint64_t)(r.timeout_ms * (double)rate_khz);
+  a.cached_probe = (r.flags & ADDS_F_ATOMIC_PROBE) ? 0u : 1u;
   return a;
 }
 
@@ -493,16 +480,19 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     float kms = 0;
     if (ws.kernel_events) (void)hipEventElapsedTime(&kms, ws.evk0, ws.evk1);
     stats->kernel_ms = kms;
+    stats->probes = sv[ST_PROBE];
+    stats->cas_failures = sv[ST_CASFAIL];
+    stats->idle_iterations = sv[ST_IDLE];
   }
   if (err == DERR_OVERFLOW) {
     g_detail = "worklist pool exhausted (" + std::to_string(hs[S_POOL]) + " chunks requested)";
     return ADDS_EOVERFLOW;
   }
-  if (err == DERR_TASKS) {
-    g_detail = "hub task ring exhausted (" + std::to_string(hs[S_TASK_T]) + " tasks requested)";
-    return ADDS_EOVERFLOW;
-  }
   if (err == DERR_TIMEOUT || err == DERR_ABORT) return ADDS_ETIMEOUT;
+  if (err == DERR_PROTOCOL) {
+    g_detail = "hub slot protocol violated";
+    return ADDS_EHIP;
+  }
   if (err != 0) return ADDS_EHIP;
   if (hs[S_DONE] != 1) {
     g_detail = "solver exited without completing";
@@ -547,9 +537,7 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
     if (!sync) return ADDS_OK;
     rc = collect(g, stats);
     if (rc != ADDS_EOVERFLOW || r.pool_fixed) return rc;
-    // grow whichever ring ran out and solve again
-    if (g->ws.host_state[S_DONE + 1] == DERR_TASKS) r.task_cap = std::min<uint64_t>(2 * r.task_cap, 0x7FFFFFFFull);
-    else r.pool_entries *= 2;
+    r.pool_entries *= 2;  // grow the pool and solve again
   }
   return rc;
 }
@@ -962,7 +950,7 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
 // ---- cross-XCD ordering litmus (diagnostic) ----
 extern "C" adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint64_t* forbidden,
                                    uint64_t* lost) {
-  if (!pairs || !rounds || variant > 2 || !forbidden || !lost) return ADDS_EINVAL;
+  if (!pairs || !rounds || variant > 3 || !forbidden || !lost) return ADDS_EINVAL;
   int cnt = 0;
   if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return ADDS_ENODEV;
   uint32_t* mem = nullptr;

@@ -73,7 +73,10 @@ __global__ void k_bwd_accum(BwdArgs b, const uint32_t* pu) {
       __hip_atomic_store((uint32_t*)b.adj + x, __float_as_uint(a), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t u = pu[x];
       if (u == NO_EDGE) break;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // adj[x] has completed before the count drops
+      // adj[x] has completed (s_waitcnt with all counters at zero) before the count drops
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       const uint32_t old =
           __hip_atomic_fetch_sub(b.cnt + base + u, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (old != 1) break;

@@ -9,11 +9,12 @@ namespace adds {
 constexpr uint32_t CH_SHIFT = 11;  // pool chunk = 2048 entries (16 KiB)
 constexpr uint32_t CH = 1u << CH_SHIFT;
 constexpr uint64_t EMPTY64 = ~0ull;      // unwritten pool entry
-constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;  // unwritten task word
 constexpr uint32_t TAB_UNSET = 0xFFFFFFFFu;
 constexpr uint32_t TAB_BAD = 0xFFFFFFFEu;
 constexpr uint32_t WG_THREADS = 256;  // 4 wavefronts of 64
 constexpr uint32_t WAVES_PER_WG = WG_THREADS / 64;
+constexpr uint32_t HUB_SLOTS = 16;    // concurrently published high-degree vertices
+constexpr uint32_t MAX_PROBE_QUEUES = 48;  // (concurrency + 1) * parts must fit the probe lanes
 
 // ---- state words (uint32 indices); every hot counter owns a 128-byte line (32 words) ----
 constexpr uint32_t LINE = 32;
@@ -21,16 +22,30 @@ constexpr uint32_t S_CUR = 0 * LINE;        // absolute index of the head bucket
 constexpr uint32_t S_VER = 1 * LINE;        // u64: (busy transitions << 32) | idle wavefronts
 constexpr uint32_t S_DONE = 2 * LINE;       // [0] done flag, [1] error code
 constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
-constexpr uint32_t S_TASK_H = 4 * LINE;     // hub task ring claim index
-constexpr uint32_t S_TASK_T = 5 * LINE;     // hub task ring reserve index
-constexpr uint32_t S_STATS = 6 * LINE;      // u64 stats[8]
-constexpr uint32_t S_ADV = 8 * LINE;        // try-lock serialising head-advance scans
-constexpr uint32_t S_QBASE = 9 * LINE;      // queue q: head at S_QBASE + 2q*LINE, tail at +LINE
-constexpr uint32_t state_words(uint32_t nq) { return S_QBASE + 2 * nq * LINE; }
+constexpr uint32_t S_STATS = 4 * LINE;      // u64 stats[NSTATS] (two lines)
+constexpr uint32_t S_ADV = 6 * LINE;        // try-lock serialising head-advance scans
+// Hub slot i: cursor line at S_HUB + 2*i*LINE: [0..1] u64 cursor (gen << 48 | total << 28 | next),
+// [2] finished chunks, [3] free flag; fields line at +LINE: [0] vertex-instance, [1] dist, [2] first
+// edge, [3] edges per chunk, [4] end edge, [5] generation.
+constexpr uint32_t S_HUB = 7 * LINE;
+// Queue q: head (claimed) at S_QBASE + 3q*LINE, tail (reserved) at +LINE, published-unclaimed count
+// at +2*LINE.  Then one in-flight counter line per bucket slot.
+constexpr uint32_t S_QBASE = S_HUB + 2 * HUB_SLOTS * LINE;
+constexpr uint32_t QLINES = 3;
+constexpr uint32_t s_inflight(uint32_t nq) { return S_QBASE + QLINES * nq * LINE; }
+constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_inflight(nq) + nb * LINE; }
+// The next-chunk field is wide enough that over-claims (at most one per probing wavefront per stale
+// view) never reach the chunk-count field.
+constexpr uint32_t HUB_NEXT_BITS = 28;
+constexpr uint32_t HUB_TOTAL_BITS = 20;
+constexpr uint32_t HUB_MAX_CHUNKS = 1u << 19;
 
-enum StatIdx { ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET, ST_SPARE };
+enum StatIdx {
+  ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET,
+  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_HUBMISS, NSTATS
+};
 
-enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3, DERR_TASKS = 4 };
+enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3, DERR_PROTOCOL = 4 };
 
 struct KArgs {
   const uint32_t* row_ptr;
@@ -40,13 +55,11 @@ struct KArgs {
   uint32_t* st;       // state words
   uint64_t* pool;     // pool_chunks * CH entries: (dist << 32) | vertex-instance id
   uint32_t* tab;      // per queue: maxv virtual-chunk -> physical-chunk map
-  uint4* tasks;       // hub edge tasks {gid, dist, edge_begin, edge_end}
   const uint32_t* host_abort;  // host-visible abort word (may be null)
   uint32_t n;         // vertices of the graph (per replica)
   uint32_t total_n;   // replicas * n
   uint32_t pool_chunks;
   uint32_t maxv;      // virtual chunks per queue
-  uint32_t task_cap;
   uint32_t nb;        // bucket ring length (power of two)
   uint32_t parts;     // sub-queues per bucket
   float inv_delta;    // 1 / bucket width
@@ -56,6 +69,7 @@ struct KArgs {
   uint32_t concur;
   uint32_t total_waves;
   uint64_t timeout_ticks;  // wall_clock64 ticks
+  uint32_t cached_probe;   // probe queue counters with L2-cached loads (claims stay atomic)
 };
 
 // Host launchers (sssp_kernels.hip)

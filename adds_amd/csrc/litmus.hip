@@ -21,7 +21,12 @@ __device__ __forceinline__ uint32_t lld(const uint32_t* p) {
 __device__ __forceinline__ void lst(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void lwait() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+// The solver's completion wait: s_waitcnt with all counters at zero, fenced against compiler motion.
+__device__ __forceinline__ void lwait() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 // words per pair: [0] payload, [32] flag, [64] ack, [96] dist   (each on its own 128-byte line)
 __global__ void k_litmus(uint32_t* mem, uint32_t rounds, uint32_t variant, unsigned long long* out) {
@@ -37,7 +42,7 @@ __global__ void k_litmus(uint32_t* mem, uint32_t rounds, uint32_t variant, unsig
       } else {
         lst(w, t);
       }
-      lwait();
+      if (variant != 3) lwait();  // variant 3 omits the wait: it shows whether reordering is observable
       if (variant == 1) __hip_atomic_fetch_add(w + 32, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       else lst(w + 32, t);
       uint32_t spins = 0;

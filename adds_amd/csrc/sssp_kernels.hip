@@ -6,9 +6,16 @@
 // wl.h:1958-1972).  How it is organised is new:
 //
 //  * No manager kernel.  The reference's one-block scheduler (wl.h:192-1527) is replaced by a ring of
-//    `nb` delta buckets, each split into `parts` sub-queues (one per XCD: workgroup i runs on XCD
-//    i % 8), with agent-scope atomic head (claim) / tail (reserve) counters.  Every wavefront claims up
-//    to `chunk` items from the open buckets with one compare-and-swap.
+//    `nb` delta buckets, each split into `parts` sub-queues (workgroup i pushes to and first claims from
+//    sub-queue i % parts; workgroups are dealt round-robin over the 8 XCDs).  Each queue has three
+//    agent-scope counters: tail (positions reserved by producers), published (entries written and not
+//    yet claimed) and head (positions claimed).  A consumer takes up to `chunk` from the published
+//    count with one fetch_sub (a shortfall is given back) and then advances the head by what it took:
+//    no compare-and-swap retry loops.
+//  * Per-bucket in-flight counters gate the head bucket: like the reference's manager, which advances
+//    the head bag only when it is done (wl.h:714-723), the head moves on when the behind and head
+//    buckets have nothing queued and nothing in flight; `concur` buckets are open for claiming at once
+//    (the reference's concurrent bags, wl.h MIN/MAX_CON_BAG).
 //  * Queue storage is a pool of 2048-entry chunks handed out by an atomic bump pointer.  A queue maps
 //    its virtual chunk j to a physical chunk through `tab`; the producer that reserves a chunk's first
 //    slot allocates it.  Nothing is reused inside a solve, so no slot can be overwritten before it is
@@ -17,22 +24,25 @@
 //    smaller distance is stale (a later push covers it) and is dropped; otherwise the vertex is relaxed
 //    with the carried distance, which is a real path length, so correctness never depends on when the
 //    consumer observes the producer's atomicMin.
-//  * High-degree vertices (the reference's TB-cooperative path, wl.h:1608-1696) are cut into edge tasks
-//    of `task_edges` edges on a device-wide task ring that every wavefront serves first.
-//  * Edges of the claimed vertices are flattened with a wavefront prefix sum and processed 64 at a
-//    time; each lane finds its owning vertex by binary search in LDS (replacing CUB's WarpScan and the
-//    byte owner table, kernel.cu:89-102).
+//  * High-degree vertices (the reference's TB-cooperative path, wl.h:1608-1696) are published in one of
+//    HUB_SLOTS hub slots whose 64-bit cursor packs (generation, chunk count, next chunk).  Any wavefront
+//    claims the next `task_edges`-edge chunk with one fetch_add -- no retry loop, an over-claim past the
+//    end is simply void -- and the wavefront that finishes the last chunk frees the slot.
+//  * Edges of the claimed vertices are flattened with a wavefront prefix sum and processed in windows
+//    of 64 (four in flight per lane); each lane finds its owning vertex by binary search in LDS
+//    (replacing CUB's WarpScan and the byte owner table, kernel.cu:89-102).
 //  * Termination: a 64-bit word counts (busy transitions << 32 | idle wavefronts).  A wavefront goes
 //    busy before it claims and idle after its pushes have completed.  The solve is over when a
-//    wavefront reads "all idle", finds every queue empty, and re-reads the very same word: no claim can
-//    have happened in between, and idle wavefronts do not push.
+//    wavefront reads "all idle", finds every queue and hub slot empty, and re-reads the very same word:
+//    no claim can have happened in between, and idle wavefronts do not push.
 //
-// Memory model (gfx950, one agent, 8 XCDs with private L2s): every cross-wavefront word is accessed
-// with relaxed agent-scope atomics (RMW atomics are coherent at agent scope; atomic loads/stores are
-// issued with sc1 so they bypass the non-coherent part of the hierarchy).  Ordering between different
-// words is obtained by waiting for completion of the earlier access (s_waitcnt vmcnt(0), emitted by a
-// workgroup-scope fence) before issuing the later one.  Plain loads are used only for CSR arrays and
-// for the d[v] pre-filter, where a stale (larger) value only costs an extra atomic.
+// Memory model (gfx950, one agent, 8 XCDs with private L2s): every word that decides ownership is
+// accessed with relaxed agent-scope atomics (RMW atomics are coherent at agent scope; atomic
+// loads/stores carry sc1).  Ordering between different words is obtained by waiting for completion of
+// the earlier access (an explicit s_waitcnt with all counters at zero, mem_wait below) before issuing
+// the later one.  Plain (L2-cached) loads are used for the CSR, the d[v] pre-filter (a stale, larger value only
+// costs an extra atomic) and, as hints only, for probing queue and hub counters; every eighth idle
+// probe and every termination test use atomic loads.
 #include "device.h"
 
 namespace adds {
@@ -65,9 +75,21 @@ __device__ __forceinline__ bool acas(uint32_t* p, uint32_t& expected, uint32_t d
   return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
 }
-// Wait until every memory access issued so far by this wavefront has completed (and stop the compiler
-// from moving accesses across).
-__device__ __forceinline__ void mem_wait() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+// Hint loads: through the L2 (may be stale), never used to decide ownership.
+__device__ __forceinline__ uint32_t hld(const uint32_t* p, bool cached) {
+  return cached ? *(const volatile uint32_t*)p : ald(p);
+}
+__device__ __forceinline__ uint64_t hld64(const uint64_t* p, bool cached) {
+  return cached ? *(const volatile uint64_t*)p : ald64(p);
+}
+// Wait until every memory access issued so far by this wavefront has completed (s_waitcnt with all
+// counters at zero), with compiler fences so that no access moves across.  A workgroup-scope fence
+// would not do: outside tgsplit mode it emits no vmcnt wait at all on gfx950.
+__device__ __forceinline__ void mem_wait() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -76,7 +98,10 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l) { return __builtin_amdgcn_readlane(v, l); }
-__device__ __forceinline__ uint32_t first(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
+  return ((uint64_t)rdlane((uint32_t)(v >> 32), l) << 32) | rdlane((uint32_t)v, l);
+}
+__device__ __forceinline__ uint32_t ffs64(uint64_t m) { return (uint32_t)__ffsll((long long)m) - 1; }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   const uint32_t lane = lane_id();
@@ -93,10 +118,30 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t x) {
   return x;
 }
 
-__device__ __forceinline__ uint32_t* qhead(const KArgs& a, uint32_t q) { return a.st + S_QBASE + 2 * q * LINE; }
-__device__ __forceinline__ uint32_t* qtail(const KArgs& a, uint32_t q) {
-  return a.st + S_QBASE + 2 * q * LINE + LINE;
+__device__ __forceinline__ uint32_t* qhead(const KArgs& a, uint32_t q) {
+  return a.st + S_QBASE + QLINES * q * LINE;
 }
+__device__ __forceinline__ uint32_t* qtail(const KArgs& a, uint32_t q) {
+  return a.st + S_QBASE + QLINES * q * LINE + LINE;
+}
+__device__ __forceinline__ uint32_t* qavail(const KArgs& a, uint32_t q) {
+  return a.st + S_QBASE + QLINES * q * LINE + 2 * LINE;
+}
+__device__ __forceinline__ uint32_t* inflight(const KArgs& a, uint32_t slot) {
+  return a.st + s_inflight(a.nb * a.parts) + slot * LINE;
+}
+__device__ __forceinline__ uint64_t* hcursor(const KArgs& a, uint32_t i) {
+  return (uint64_t*)(a.st + S_HUB + 2 * i * LINE);
+}
+__device__ __forceinline__ uint32_t* hdone(const KArgs& a, uint32_t i) { return a.st + S_HUB + 2 * i * LINE + 2; }
+__device__ __forceinline__ uint32_t* hfree(const KArgs& a, uint32_t i) { return a.st + S_HUB + 2 * i * LINE + 3; }
+__device__ __forceinline__ uint32_t* hfield(const KArgs& a, uint32_t i) { return a.st + S_HUB + 2 * i * LINE + LINE; }
+
+__device__ __forceinline__ uint32_t cur_next(uint64_t c) { return (uint32_t)c & ((1u << HUB_NEXT_BITS) - 1); }
+__device__ __forceinline__ uint32_t cur_total(uint64_t c) {
+  return (uint32_t)(c >> HUB_NEXT_BITS) & ((1u << HUB_TOTAL_BITS) - 1);
+}
+__device__ __forceinline__ uint32_t cur_gen(uint64_t c) { return (uint32_t)(c >> (HUB_NEXT_BITS + HUB_TOTAL_BITS)); }
 
 __device__ __forceinline__ void set_error(const KArgs& a, uint32_t code) {
   uint32_t z = 0;
@@ -124,7 +169,7 @@ __device__ __forceinline__ uint32_t bucket_key(uint32_t d, float inv_delta) {
 }
 
 struct WaveStats {
-  uint64_t v[8];
+  uint64_t v[NSTATS];
 };
 
 // Spin until a pool-chunk mapping or entry is written.  Every wait here is on a wavefront that has
@@ -198,16 +243,24 @@ __device__ __forceinline__ void push_wave(const KArgs& a, bool f, uint32_t gid, 
   b = min(b, c_now + a.nb - 2);
   const uint32_t q = (b & (a.nb - 1)) * a.parts + part;
   if (lane == 0) ws.v[ST_PUSH] += __popcll(m);
+  // reserve and write one group (target queue) at a time; the group's leader lane remembers its count
+  uint32_t pub = 0;
   while (m) {
-    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+    const uint32_t leader = ffs64(m);
     const uint32_t lq = rdlane(q, leader);
     const uint64_t peers = ballot(f && q == lq);
     uint32_t base = 0;
-    if (lane == leader) base = afadd(qtail(a, lq), (uint32_t)__popcll(peers));
+    if (lane == leader) {
+      pub = (uint32_t)__popcll(peers);
+      base = afadd(qtail(a, lq), pub);
+    }
     base = rdlane(base, leader);
     if (f && q == lq) write_entry(a, lq, base + mbcnt(peers), ((uint64_t)nd << 32) | gid);
     m &= ~peers;
   }
+  // publish: the entries have completed before consumers may count them as claimable
+  mem_wait();
+  if (pub) afadd(qavail(a, q), pub);
 }
 
 // Relaxes UNROLL independent edges per lane (one 64-edge window each) so that UNROLL dependent chains
@@ -249,10 +302,75 @@ __device__ __forceinline__ void relax_group(const KArgs& a, const bool (&act)[UN
   for (int j = 0; j < UNROLL; j++) push_wave<FLT>(a, f[j] && old[j] > nd[j], tgt[j], nd[j], part, c_now, ws);
 }
 
+// Relaxes the contiguous edge range [eb, ee) of one vertex (distance du, replica base gbase).
+template <bool FLT>
+__device__ __forceinline__ void relax_range(const KArgs& a, uint32_t eb, uint32_t ee, uint32_t du, uint32_t gbase,
+                                            uint32_t part, WaveStats& ws) {
+  const uint32_t lane = lane_id();
+  for (uint32_t base = eb; base < ee; base += 64 * UNROLL) {
+    const uint32_t c_now = ald(a.st + S_CUR);
+    bool act[UNROLL];
+    uint32_t e[UNROLL], dus[UNROLL], gbs[UNROLL];
+#pragma unroll
+    for (int j = 0; j < UNROLL; j++) {
+      e[j] = base + 64 * j + lane;
+      act[j] = e[j] < ee;
+      dus[j] = du;
+      gbs[j] = gbase;
+    }
+    relax_group<FLT>(a, act, e, dus, gbs, part, c_now, ws);
+  }
+}
+
+// Publishes a high-degree vertex in a free hub slot.  Returns false when every slot is busy (the
+// caller then relaxes the vertex itself).
+__device__ __forceinline__ bool publish_hub(const KArgs& a, uint32_t gid, uint32_t du, uint32_t rs, uint32_t deg,
+                                            uint32_t start, WaveStats& ws) {
+  const uint32_t lane = lane_id();
+  bool fr = false;
+  uint32_t slot_l = (start + lane) & (HUB_SLOTS - 1);
+  if (lane < HUB_SLOTS) fr = ald(hfree(a, slot_l)) == 1u;
+  uint64_t fm = ballot(fr);
+  uint32_t got = 0xFFFFFFFFu;
+  while (fm) {
+    const uint32_t L = ffs64(fm);
+    const uint32_t s = rdlane(slot_l, L);
+    uint32_t ok = 0;
+    if (lane == 0) {
+      uint32_t one = 1u;
+      ok = acas(hfree(a, s), one, 0u) ? 1u : 0u;
+    }
+    if (rdlane(ok, 0)) {
+      got = s;
+      break;
+    }
+    fm &= fm - 1;
+  }
+  if (got == 0xFFFFFFFFu) return false;
+  uint32_t te = a.task_edges;
+  if ((deg + te - 1) / te >= HUB_MAX_CHUNKS) te = (deg + HUB_MAX_CHUNKS - 2) / (HUB_MAX_CHUNKS - 1);
+  const uint32_t total = (deg + te - 1) / te;
+  if (lane == 0) {
+    uint32_t* f = hfield(a, got);
+    const uint32_t gen = (ald(f + 5) + 1) & 0xFFFFu;
+    ast(f + 0, gid);
+    ast(f + 1, du);
+    ast(f + 2, rs);
+    ast(f + 3, te);
+    ast(f + 4, rs + deg);
+    ast(f + 5, gen);
+    ast(hdone(a, got), 0u);
+    mem_wait();  // the fields are visible before the cursor opens the slot
+    ast64(hcursor(a, got), ((uint64_t)gen << (HUB_NEXT_BITS + HUB_TOTAL_BITS)) | ((uint64_t)total << HUB_NEXT_BITS));
+    ws.v[ST_TASKS] += total;
+  }
+  return true;
+}
+
 // ------------------------------------------------------------------------------------------------
 // the persistent solver
 // ------------------------------------------------------------------------------------------------
-enum : uint32_t { W_NONE = 0, W_TASK = 1, W_ITEMS = 2, W_STOP = 3 };
+enum : uint32_t { W_NONE = 0, W_HUB = 1, W_ITEMS = 2 };
 
 template <bool FLT, bool REPL>
 __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
@@ -264,87 +382,99 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
   const uint32_t part = blockIdx.x % a.parts;
+  const uint32_t wave_gid = blockIdx.x * WAVES_PER_WG + wv;
   const uint32_t nq = a.nb * a.parts;
   WaveStats ws;
 #pragma unroll
-  for (int i = 0; i < 8; i++) ws.v[i] = 0;
+  for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
   bool busy = false;  // wave-uniform
   const uint64_t t_start = wall_clock64();
   uint32_t idle_iter = 0, work_iter = 0;
+  bool missed = false;  // the last claim attempt lost: probe with atomic loads next time
 
   for (;;) {
-    // ---------------- probe the open queues in parallel, then claim (lane 0) ----------------
-    // lane l < (concur+1)*parts probes bucket c-1+l/parts, sub-queue (part + l%parts) % parts, so the
-    // ballot's bit order is the claim priority; lane 63 probes the hub task ring (served first).
+    // ---------------- probe hub slots and open queues in parallel ----------------
+    // lanes [0, HUB_SLOTS) probe hub slots (rotated per wavefront); lane 16 + l probes bucket
+    // c-1+l/parts, sub-queue (part + l%parts) % parts, so the ballot's bit order is the claim priority.
     const uint32_t c = ald(a.st + S_CUR);
-    uint32_t ph_ = 0, pt_ = 0, pq = 0;
+    const bool cached = a.cached_probe && (idle_iter & 7) != 7 && !missed;
+    missed = false;
+    uint32_t pq = 0;
+    uint64_t hc = 0;
     bool pv = false;
-    {
-      const uint32_t nprobe = (a.concur + 1) * a.parts;
-      if (lane < nprobe) {
-        const uint32_t j = lane / a.parts;  // 0 = behind bucket c-1
+    if (lane < HUB_SLOTS) {
+      pq = (wave_gid + lane) & (HUB_SLOTS - 1);
+      hc = hld64(hcursor(a, pq), cached);
+      pv = cur_next(hc) < cur_total(hc);
+    } else {
+      const uint32_t l = lane - HUB_SLOTS;
+      if (l < (a.concur + 1) * a.parts) {
+        const uint32_t j = l / a.parts;  // 0 = behind bucket c-1
         if (c > 0 || j > 0) {
           const uint32_t slot = (c - 1 + j) & (a.nb - 1);
-          pq = slot * a.parts + ((part + lane % a.parts) % a.parts);
-          ph_ = ald(qhead(a, pq));
-          pt_ = ald(qtail(a, pq));
-          pv = ph_ < pt_;
+          pq = slot * a.parts + ((part + l % a.parts) % a.parts);
+          pv = (int32_t)hld(qavail(a, pq), cached) > 0;
         }
-      } else if (lane == 63) {
-        ph_ = ald(a.st + S_TASK_H);
-        pt_ = ald(a.st + S_TASK_T);
-        pv = ph_ < pt_;
       }
     }
     uint64_t pm = ballot(pv);
+    if (lane == 0) ws.v[ST_PROBE]++;
     uint32_t kind = W_NONE, q = 0, h = 0, cnt = 0, cb = 0;
+    uint64_t claimed = 0;
     if (pm) {
       if (!busy) {
         if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 0xFFFFFFFFull);  // +1 transition, -1 idle
         busy = true;
         mem_wait();
       }
-      // task ring first
-      if (pm >> 63) {
-        uint32_t th = rdlane(ph_, 63), tt = rdlane(pt_, 63);
-        if (lane == 0) {
-          for (uint32_t tries = 0; th < tt && tries < 16; tries++) {
-            if (acas(a.st + S_TASK_H, th, th + 1)) {
-              kind = W_TASK;
-              h = th;
-              break;
-            }
-            tt = ald(a.st + S_TASK_T);
-          }
+      // hub chunks first: one fetch_add per attempt, at most two slots per probe
+      uint64_t hm = pm & ((1ull << HUB_SLOTS) - 1);
+      for (int tries = 0; tries < 2 && hm && kind == W_NONE; tries++) {
+        const uint32_t L = ffs64(hm);
+        const uint32_t s = rdlane(pq, L);
+        uint64_t old = 0;
+        if (lane == 0) old = afadd64(hcursor(a, s), 1ull);
+        old = rdlane64(old, 0);
+        if (cur_next(old) < cur_total(old)) {
+          kind = W_HUB;
+          q = s;
+          claimed = old;
+        } else {
+          missed = true;
+          if (lane == 0) ws.v[ST_HUBMISS]++;
         }
-        kind = rdlane(kind, 0);
-        h = rdlane(h, 0);
-        pm &= ~(1ull << 63);
+        hm &= hm - 1;
       }
-      while (kind == W_NONE && pm) {
-        const uint32_t L = (uint32_t)__ffsll((long long)pm) - 1;
-        uint32_t hh = rdlane(ph_, L), tt2 = rdlane(pt_, L);
+      // Queue items: take up to `chunk` from the published count (fetch_sub; a shortfall is given
+      // back), then fetch_add the head by what was taken.  Published items are at positions the
+      // head has not passed, so every claimed position has been reserved by a producer.
+      uint64_t qm = pm >> HUB_SLOTS;
+      for (int tries = 0; tries < 3 && kind == W_NONE && qm; tries++) {
+        const uint32_t L = ffs64(qm) + HUB_SLOTS;
         const uint32_t qq = rdlane(pq, L);
+        const uint32_t bb = c - 1 + (L - HUB_SLOTS) / a.parts;
         if (lane == 0) {
-          for (uint32_t tries = 0; hh < tt2 && tries < 16; tries++) {
-            const uint32_t nn = min(a.chunk, tt2 - hh);
-            if (acas(qhead(a, qq), hh, hh + nn)) {
-              kind = W_ITEMS;
-              h = hh;
-              cnt = nn;
-              break;
-            }
-            tt2 = ald(qtail(a, qq));
+          const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - a.chunk);
+          const int32_t got = min(max(old, 0), (int32_t)a.chunk);
+          if (got < (int32_t)a.chunk) afadd(qavail(a, qq), (uint32_t)((int32_t)a.chunk - got));
+          if (got > 0) {
+            h = afadd(qhead(a, qq), (uint32_t)got);
+            afadd(inflight(a, bb & (a.nb - 1)), (uint32_t)got);
+            cnt = (uint32_t)got;
+            kind = W_ITEMS;
+          } else {
+            ws.v[ST_CASFAIL]++;
           }
         }
         kind = rdlane(kind, 0);
+        if (kind != W_ITEMS) missed = true;
         if (kind == W_ITEMS) {
           q = qq;
           h = rdlane(h, 0);
           cnt = rdlane(cnt, 0);
-          cb = c - 1 + L / a.parts;
+          cb = bb;
         }
-        pm &= pm - 1;
+        qm &= qm - 1;
       }
     }
 
@@ -367,49 +497,28 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       if (rdlane(quit, 0)) break;
     }
 
-    if (kind == W_TASK) {
+    if (kind == W_HUB) {
       idle_iter = 0;
-      // ---------------- hub edge task ----------------
-      if (h >= a.task_cap) {  // only after an overflow: the slot was never written
-        set_error(a, DERR_TASKS);
+      // ---------------- one chunk of a published hub vertex ----------------
+      mem_wait();  // the claim has completed before the fields are read
+      const uint32_t* f = hfield(a, q);
+      const uint32_t gid_u = ald(f + 0), du = ald(f + 1), rs = ald(f + 2), te = ald(f + 3), re = ald(f + 4);
+      if (ald(f + 5) != cur_gen(claimed)) {  // a slot cannot be recycled while a chunk is outstanding
+        set_error(a, DERR_PROTOCOL);
         break;
       }
-      const uint4* tp = a.tasks + h;
-      uint4 t;
-      uint32_t spins = 0;
-      for (;;) {
-        t.x = ald(&tp->x);
-        t.y = ald(&tp->y);
-        t.z = ald(&tp->z);
-        t.w = ald(&tp->w);
-        if (t.x != EMPTY32 && t.y != EMPTY32 && t.z != EMPTY32 && t.w != EMPTY32) break;
-        __builtin_amdgcn_s_sleep(1);
-        if ((++spins & 1023) == 0 && (stopped(a) || spins > SPIN_CAP)) {
-          if (spins > SPIN_CAP) set_error(a, DERR_TIMEOUT);
-          t.x = EMPTY32;
-          break;
-        }
-      }
-      if (t.x == EMPTY32) break;
-      const uint32_t gid_u = first(t.x), du = first(t.y), eb = first(t.z), ee = first(t.w);
-      const uint32_t dnow = ald(a.dist + gid_u);
-      if (dnow < du) {
+      const uint32_t k = cur_next(claimed);
+      const uint32_t eb = rs + k * te;
+      const uint32_t ee = min(eb + te, re);
+      if (ald(a.dist + gid_u) < du) {
         if (lane == 0) ws.v[ST_STALE]++;
-        continue;
+      } else {
+        relax_range<FLT>(a, eb, ee, du, REPL ? (gid_u / a.n) * a.n : 0u, part, ws);
       }
-      const uint32_t gbase = REPL ? (gid_u / a.n) * a.n : 0u;
-      for (uint32_t base = eb; base < ee; base += 64 * UNROLL) {
-        const uint32_t c_now = ald(a.st + S_CUR);
-        bool act[UNROLL];
-        uint32_t e[UNROLL], dus[UNROLL], gbs[UNROLL];
-#pragma unroll
-        for (int j = 0; j < UNROLL; j++) {
-          e[j] = base + 64 * j + lane;
-          act[j] = e[j] < ee;
-          dus[j] = du;
-          gbs[j] = gbase;
-        }
-        relax_group<FLT>(a, act, e, dus, gbs, part, c_now, ws);
+      mem_wait();  // this chunk's pushes have completed before it is counted as finished
+      if (lane == 0) {
+        const uint32_t d = afadd(hdone(a, q), 1u);
+        if (d + 1 == cur_total(claimed)) ast(hfree(a, q), 1u);
       }
       continue;
     }
@@ -460,32 +569,14 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         deg = a.row_ptr[v + 1] - rs;
         ws.v[ST_EXPANDED]++;
       }
-      // ---------------- hub vertices become edge tasks ----------------
-      const bool hub = ok && deg > a.hub_degree;
-      uint64_t hm = ballot(hub);
+      // ---------------- high-degree vertices go to hub slots ----------------
+      bool hub = false;
+      uint64_t hm = ballot(ok && deg > a.hub_degree);
       while (hm) {
-        const uint32_t L = (uint32_t)__ffsll((long long)hm) - 1;
-        const uint32_t hg = rdlane(gid, L), hd = rdlane(du, L), hrs = rdlane(rs, L), hdeg = rdlane(deg, L);
-        const uint32_t ntask = (hdeg + a.task_edges - 1) / a.task_edges;
-        uint32_t t0 = 0;
-        if (lane == 0) {
-          t0 = afadd(a.st + S_TASK_T, ntask);
-          ws.v[ST_TASKS] += ntask;
-          if (t0 + ntask > a.task_cap) set_error(a, DERR_TASKS);
-        }
-        t0 = rdlane(t0, 0);
-        for (uint32_t i = lane; i < ntask; i += 64) {
-          const uint32_t ti = t0 + i;
-          if (ti < a.task_cap) {
-            const uint32_t e0 = hrs + i * a.task_edges;
-            const uint32_t e1 = min(e0 + a.task_edges, hrs + hdeg);
-            uint4* tp = a.tasks + ti;
-            ast(&tp->x, hg);
-            ast(&tp->y, hd);
-            ast(&tp->z, e0);
-            ast(&tp->w, e1);
-          }
-        }
+        const uint32_t L = ffs64(hm);
+        const bool pub = publish_hub(a, rdlane(gid, L), rdlane(du, L), rdlane(rs, L), rdlane(deg, L),
+                                     wave_gid + L, ws);
+        if (lane == L) hub = pub;
         hm &= hm - 1;
       }
       // ---------------- flattened edge windows ----------------
@@ -527,6 +618,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         }
         wave_sync();
       }
+      mem_wait();  // this chunk's pushes have completed before it stops counting as in flight
+      if (lane == 0) afadd(inflight(a, cb & (a.nb - 1)), 0u - cnt);
       if (ald(a.st + S_DONE + 1) != 0) break;
       continue;
     }
@@ -545,7 +638,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         mem_wait();
         bool nonempty = false;
         for (uint32_t qq = lane; qq < nq; qq += 64) nonempty |= ald(qhead(a, qq)) != ald(qtail(a, qq));
-        if (lane == 0) nonempty |= ald(a.st + S_TASK_H) != ald(a.st + S_TASK_T);
+        if (lane < HUB_SLOTS) {
+          const uint64_t cc = ald64(hcursor(a, lane));
+          nonempty |= cur_next(cc) < cur_total(cc);
+        }
         const bool any = ballot(nonempty) != 0;
         mem_wait();
         const uint64_t w2 = ald64((const uint64_t*)(a.st + S_VER));
@@ -583,14 +679,20 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           behind_or_head = (m & (c2 > 0 ? 3ull : 2ull)) != 0;
           m &= ~3ull;
         }
-        if (m && firstj == 0xFFFFFFFFu) firstj = j0 + (uint32_t)__ffsll((long long)m) - 1;
+        if (m && firstj == 0xFFFFFFFFu) firstj = j0 + ffs64(m);
       }
+      // like the reference, the head moves only when its bucket is done (wl.h:714-723): nothing
+      // queued and nothing still being processed in the behind and head buckets
       if (!behind_or_head && firstj != 0xFFFFFFFFu) {
-        if (lane == 0) {
-          uint32_t cc = c2;
-          acas(a.st + S_CUR, cc, c2 - 1 + firstj);
+        const uint32_t busy_h = ald(inflight(a, c2 & (a.nb - 1)));
+        const uint32_t busy_b = c2 > 0 ? ald(inflight(a, (c2 - 1) & (a.nb - 1))) : 0u;
+        if (busy_h == 0 && busy_b == 0) {
+          if (lane == 0) {
+            uint32_t cc = c2;
+            acas(a.st + S_CUR, cc, c2 - 1 + firstj);
+          }
+          advanced = true;
         }
-        advanced = true;
       }
       mem_wait();
       if (lane == 0) ast(a.st + S_ADV, 0u);
@@ -598,6 +700,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     if (advanced) continue;
     // watchdog and host abort
     ++idle_iter;
+    if (lane == 0) ws.v[ST_IDLE]++;
     if ((idle_iter & 63) == 0) {
       uint32_t quit = 0;
       if (lane == 0) {
@@ -624,7 +727,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 1ull);
   }
 #pragma unroll
-  for (int i = 0; i < 7; i++) {
+  for (int i = 0; i < NSTATS; i++) {
     const uint64_t s = wave_sum64(ws.v[i]);
     if (lane == 0 && s) afadd64((uint64_t*)(a.st + S_STATS) + i, s);
   }
@@ -644,14 +747,16 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t count, uint64_t v) {
 
 // Resets the state words and enqueues the sources into bucket 0, sub-queue 0 (single block).
 __global__ void k_init_state(KArgs a, const uint32_t* srcs, uint32_t k, int replicated) {
-  const uint32_t nw = state_words(a.nb * a.parts);
+  const uint32_t nw = state_words(a.nb * a.parts, a.nb);
   for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) a.st[i] = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
     *(uint64_t*)(a.st + S_VER) = (uint64_t)a.total_waves;
-    a.st[S_QBASE + LINE] = k;  // tail of queue 0
+    a.st[S_QBASE + LINE] = k;      // tail of queue 0
+    a.st[S_QBASE + 2 * LINE] = k;  // all of it published
     a.st[S_POOL] = (k + CH - 1) >> CH_SHIFT;
   }
+  if (threadIdx.x < HUB_SLOTS) a.st[S_HUB + 2 * threadIdx.x * LINE + 3] = 1u;  // every hub slot free
   for (uint32_t j = threadIdx.x; j < ((k + CH - 1) >> CH_SHIFT); j += blockDim.x) a.tab[j] = j;
   for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
     const uint32_t gid = replicated ? i * a.n + srcs[i] : srcs[i];
@@ -660,19 +765,17 @@ __global__ void k_init_state(KArgs a, const uint32_t* srcs, uint32_t k, int repl
   }
 }
 
-// Restores the pool, chunk maps and task ring to their empty state, using the previous solve's
-// counters (which the next k_init_state clears).
+// Restores the pool and the chunk maps to their empty state, using the previous solve's counters
+// (which the next k_init_state clears).
 __global__ void k_cleanup(KArgs a) {
   const uint32_t nq = a.nb * a.parts;
   const uint64_t used_chunks = min((uint64_t)a.st[S_POOL], (uint64_t)a.pool_chunks);
   const uint64_t pool_used = used_chunks << CH_SHIFT;
-  const uint64_t ntask = min((uint64_t)a.st[S_TASK_T], (uint64_t)a.task_cap);
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nth = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = tid; i < pool_used; i += nth) a.pool[i] = EMPTY64;
-  for (uint64_t i = tid; i < ntask; i += nth) a.tasks[i] = make_uint4(EMPTY32, EMPTY32, EMPTY32, EMPTY32);
   for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-    const uint32_t t = a.st[S_QBASE + 2 * q * LINE + LINE];
+    const uint32_t t = a.st[S_QBASE + QLINES * q * LINE + LINE];
     const uint32_t used = min((t + CH - 1) >> CH_SHIFT, a.maxv);
     for (uint32_t j = threadIdx.x; j < used; j += blockDim.x) a.tab[(size_t)q * a.maxv + j] = TAB_UNSET;
   }

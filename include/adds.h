@@ -60,6 +60,7 @@ typedef struct {
 /* flags */
 #define ADDS_F_HOST_OUT 0x1u  /* dist output buffers are host memory (copied back, then synchronised) */
 #define ADDS_F_NO_SYNC 0x2u   /* do not synchronise; status of the solve is read by adds_last_status */
+#define ADDS_F_ATOMIC_PROBE 0x4u /* probe queue counters with atomic (cache-bypassing) loads only */
 
 typedef struct {
   double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */
@@ -72,6 +73,9 @@ typedef struct {
   uint64_t rebuckets;         /* popped entries moved to a later bucket */
   uint64_t pool_chunks_used;  /* worklist pool chunks consumed */
   double kernel_ms;           /* hipEvent time of the persistent solver kernel alone */
+  uint64_t probes;            /* queue probes issued by wavefronts looking for work */
+  uint64_t cas_failures;      /* lost claim races */
+  uint64_t idle_iterations;   /* idle-loop iterations of all wavefronts */
 } adds_stats;
 
 /* ---- errors / device ---- */
@@ -136,7 +140,8 @@ adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32_t k, uint3
 /* Diagnostic with no reference counterpart: cross-XCD message-passing litmus test of the ordering recipe
  * the solver uses (relaxed agent-scope atomics ordered by waiting for completion).  `pairs` producer /
  * consumer workgroup pairs on different XCDs run `rounds` rounds; variant 0 publishes with a store,
- * 1 with a fetch_add, 2 checks atomicMin visibility.  *forbidden counts stale observations (must be 0),
+ * 1 with a fetch_add, 2 checks atomicMin visibility, 3 is variant 0 without the wait (a control: it may
+ * show reordering).  *forbidden counts stale observations (must be 0 for variants 0-2),
  * *lost counts rounds whose bounded wait expired. */
 adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint64_t* forbidden, uint64_t* lost);
 

@@ -72,12 +72,32 @@ def test_backward_finite_differences():
     h = 1e-3
     rng = np.random.default_rng(3)
     verts = [v for v in rng.permutation(n) if pe[0, v] != 0xFFFFFFFF]
+    parent = np.full(n, -1)
+    has = pe[0] != 0xFFFFFFFF
+    parent[has] = src_of[pe[0][has]]
+    children = [[] for _ in range(n)]
+    for x in np.nonzero(has)[0]:
+        children[parent[x]].append(x)
+    tree_edge = np.zeros(host.m, dtype=bool)
+    tree_edge[pe[0][has]] = True
     checked = 0
     for v in verts:
         e = int(pe[0, v])
         ins = np.nonzero(host.col == v)[0]
         alts = slack[ins[ins != e]]
-        if alts.size and alts.min() <= 1e-2:
+        if alts.size and alts.min() <= 1e-2:  # tight-predecessor gap of the edge itself
+            continue
+        # The whole subtree below e moves by +-h: the tree is stable only if every non-tree edge that
+        # crosses the subtree boundary has slack above the step as well (otherwise the function has a
+        # kink inside [w-h, w+h] and central differences do not measure the derivative).
+        sub = np.zeros(n, dtype=bool)
+        stack = [v]
+        while stack:
+            x = stack.pop()
+            sub[x] = True
+            stack.extend(children[x])
+        cross = (sub[src_of] != sub[host.col]) & ~tree_edge
+        if cross.any() and slack[cross].min() <= 1e-2:
             continue
 
         def loss(wv):

@@ -22,6 +22,15 @@ def test_cross_xcd_litmus(variant):
     assert f.value == 0, f"{f.value} forbidden outcomes"
 
 
+def test_litmus_control_without_wait():
+    """Variant 3 drops the completion wait; it is a control that reports how often a stale payload is
+    seen without it (no assertion: the hardware may or may not reorder in a given run)."""
+    f, lost = ctypes.c_uint64(), ctypes.c_uint64()
+    L.check(L._lib().adds_litmus(256, 2000, 3, ctypes.byref(f), ctypes.byref(lost)), "litmus")
+    assert lost.value == 0
+    print(f"litmus without the wait: {f.value} stale payloads in {256 * 2000} rounds")
+
+
 def test_graph_replays_match_eager():
     host = adds_amd.HostGraph.rmat(14, 16, seed=6, wkind="f32", lo=1.0, hi=10.0)
     srcs = host.sample_component(host.max_degree_vertex(), 8, seed=3)
