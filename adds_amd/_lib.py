@@ -85,6 +85,7 @@ class Stats(ctypes.Structure):
         ("probes", ctypes.c_uint64),
         ("cas_failures", ctypes.c_uint64),
         ("idle_iterations", ctypes.c_uint64),
+        ("late_items", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -58,6 +58,12 @@ struct adds_graph {
   uint32_t* col = nullptr;
   uint32_t* w = nullptr;
   float ave_wt = 1.f;  // mean sampled weight (the reference's profiler statistic)
+  // light-first copy of the rows for the solver (col/w stay in input order for the backward pass and
+  // the device-array accessor); valid for split_delta
+  uint32_t* scol = nullptr;
+  uint32_t* sw = nullptr;
+  uint32_t* lcnt = nullptr;
+  float split_delta = 0.f;
   Workspace ws;
   uint32_t* outbuf = nullptr;  // device staging for host outputs
   uint64_t outbuf_cap = 0;
@@ -133,6 +139,11 @@ static void release_graph_memory(adds_graph* g) {
   if (g->rrow) (void)hipFree(g->rrow);
   if (g->reid) (void)hipFree(g->reid);
   if (g->rsrc) (void)hipFree(g->rsrc);
+  if (g->scol) (void)hipFree(g->scol);
+  if (g->sw) (void)hipFree(g->sw);
+  if (g->lcnt) (void)hipFree(g->lcnt);
+  g->scol = g->sw = g->lcnt = nullptr;
+  g->split_delta = 0.f;
   if (g->union_g) {
     release_graph_memory(g->union_g);
     delete g->union_g;
@@ -247,6 +258,9 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   if (!g || (!w_dev && g->m)) return ADDS_EINVAL;
   HIPCHK(hipSetDevice(g->device));
   if (g->m) HIPCHK(hipMemcpyAsync(g->w, w_dev, g->m * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  if (g->lcnt && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
+    HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scol, g->sw, g->lcnt,
+                        (hipStream_t)stream));
   if (g->union_g) {  // weights changed: the union copy is stale
     release_graph_memory(g->union_g);
     delete g->union_g;
@@ -312,7 +326,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   memset(&z, 0, sizeof(z));
   if (!p) p = &z;
   r.delta = p->delta > 0 ? p->delta : default_delta(g);
-  r.wg_per_cu = p->wg_per_cu ? p->wg_per_cu : 4;
+  r.wg_per_cu = p->wg_per_cu ? p->wg_per_cu : 1;
   r.chunk = p->chunk ? p->chunk : 64;
   r.nb = p->nbuckets ? p->nbuckets : 64;
   r.parts = p->parts ? p->parts : 8;
@@ -325,7 +339,8 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.pool_entries = p->pool_entries ? p->pool_entries : 3 * total_n + (4u << 20);
   if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 64 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
-      r.parts != 8 && r.parts != 16) || (r.concur + 1) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 || r.task_edges == 0 || r.hub_degree < 64) {
+      r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||
+      r.task_edges == 0 || r.hub_degree < 64) {
     g_detail = "parameter out of range";
     return ADDS_EINVAL;
   }
@@ -361,10 +376,10 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
     const uint32_t pc = std::max(chunks, ws.pool_chunks);
     HIPCHK(hipMalloc(&ws.st, (size_t)state_words(nq, r.nb) * 4));
     HIPCHK(hipMalloc(&ws.pool, (size_t)pc * CH * 8));
-    HIPCHK(hipMalloc(&ws.tab, (size_t)nq * pc * 4));
+    HIPCHK(hipMalloc(&ws.tab, (size_t)2 * nq * pc * 4));
     HIPCHK(launch_fill_u32(ws.st, state_words(nq, r.nb), 0, nullptr));
     HIPCHK(launch_fill_u64(ws.pool, (uint64_t)pc * CH, EMPTY64, nullptr));
-    HIPCHK(launch_fill_u32(ws.tab, (uint64_t)nq * pc, TAB_UNSET, nullptr));
+    HIPCHK(launch_fill_u32(ws.tab, (uint64_t)2 * nq * pc, TAB_UNSET, nullptr));
     HIPCHK(hipDeviceSynchronize());
     ws.nq = nq;
     ws.nb = r.nb;
@@ -380,6 +395,22 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
   return ADDS_OK;
 }
 
+// Light-first rows for bucket width `delta` (computed once per graph and width; solves with another
+// width re-split).
+static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
+  if (!g->lcnt) {
+    HIPCHK(hipMalloc(&g->scol, std::max<uint64_t>(g->m, 1) * 4));
+    HIPCHK(hipMalloc(&g->sw, std::max<uint64_t>(g->m, 1) * 4));
+    HIPCHK(hipMalloc(&g->lcnt, std::max<uint64_t>(g->n, 1) * 4));
+    g->split_delta = 0.f;
+  }
+  if (g->split_delta != delta) {
+    if (g->n) HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scol, g->sw, g->lcnt, s));
+    g->split_delta = delta;
+  }
+  return ADDS_OK;
+}
+
 static uint32_t inf_bits(adds_wtype wt) { return wt == ADDS_W_F32 ? ADDS_INF_F32_BITS : ADDS_INF_U32; }
 
 static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_t total_n, uint32_t grid) {
@@ -387,8 +418,9 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   KArgs a;
   memset(&a, 0, sizeof(a));
   a.row_ptr = g->row;
-  a.col = g->col;
-  a.w = g->w;
+  a.col = g->scol;
+  a.w = g->sw;
+  a.lcnt = g->lcnt;
   a.dist = dist;
   a.st = ws.st;
   a.pool = ws.pool;
@@ -483,6 +515,7 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->probes = sv[ST_PROBE];
     stats->cas_failures = sv[ST_CASFAIL];
     stats->idle_iterations = sv[ST_IDLE];
+    stats->late_items = sv[ST_LATE];
   }
   if (err == DERR_OVERFLOW) {
     g_detail = "worklist pool exhausted (" + std::to_string(hs[S_POOL]) + " chunks requested)";
@@ -521,6 +554,7 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
   }
   for (int attempt = 0; attempt < 4; attempt++) {
     rc = ensure_workspace(g, r, k);
+    if (rc == ADDS_OK) rc = ensure_split(g, r.delta, s);
     if (rc != ADDS_OK) return rc;
     Workspace& ws = g->ws;
     ws.timeout_ms = r.timeout_ms;
@@ -696,7 +730,9 @@ extern "C" adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* sr
   adds_status rc = resolve(g, p, total_n, r);
   if (rc != ADDS_OK) return rc;
   rc = ensure_workspace(g, r, k);
+  if (rc == ADDS_OK) rc = ensure_split(g, r.delta, nullptr);
   if (rc != ADDS_OK) return rc;
+  HIPCHK(hipDeviceSynchronize());
   g->ws.timeout_ms = r.timeout_ms;
   hipStream_t s;
   HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -928,7 +964,7 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
       try_point(q);
     }
     // 3) persistent workgroups per CU, then the claim size
-    for (uint32_t w : {1u, 2u, 6u, 8u}) {
+    for (uint32_t w : {2u, 3u, 4u}) {
       adds_params q = p;
       q.wg_per_cu = w;
       try_point(q);

@@ -28,12 +28,16 @@ constexpr uint32_t S_ADV = 6 * LINE;        // try-lock serialising head-advance
 // [2] finished chunks, [3] free flag; fields line at +LINE: [0] vertex-instance, [1] dist, [2] first
 // edge, [3] edges per chunk, [4] end edge, [5] generation.
 constexpr uint32_t S_HUB = 7 * LINE;
-// Queue q: head (claimed) at S_QBASE + 3q*LINE, tail (reserved) at +LINE, published-unclaimed count
-// at +2*LINE.  Then one in-flight counter line per bucket slot.
+// Queues: nq = nb * parts light queues (index slot * parts + part) followed by nq heavy queues (index
+// nq + slot * parts + part).  Queue q: head (claimed) at S_QBASE + 3q*LINE, tail (reserved) at +LINE,
+// published-unclaimed count at +2*LINE.  Then two lines per bucket slot: in-flight counts (words
+// [0, 16) light claims per part, [16, 32) heavy claims per part) and the slot's phase word (1 = the
+// heavy queues of the head bucket are open).
 constexpr uint32_t S_QBASE = S_HUB + 2 * HUB_SLOTS * LINE;
 constexpr uint32_t QLINES = 3;
-constexpr uint32_t s_inflight(uint32_t nq) { return S_QBASE + QLINES * nq * LINE; }
-constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_inflight(nq) + nb * LINE; }
+constexpr uint32_t s_slotlines(uint32_t nq) { return S_QBASE + QLINES * 2 * nq * LINE; }
+constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_slotlines(nq) + 2 * nb * LINE; }
+constexpr uint32_t MAX_PARTS = 16;
 // The next-chunk field is wide enough that over-claims (at most one per probing wavefront per stale
 // view) never reach the chunk-count field.
 constexpr uint32_t HUB_NEXT_BITS = 28;
@@ -42,15 +46,16 @@ constexpr uint32_t HUB_MAX_CHUNKS = 1u << 19;
 
 enum StatIdx {
   ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET,
-  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_HUBMISS, NSTATS
+  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_HUBMISS, ST_LATE, NSTATS
 };
 
 enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3, DERR_PROTOCOL = 4 };
 
 struct KArgs {
   const uint32_t* row_ptr;
-  const uint32_t* col;
-  const uint32_t* w;  // 4-byte weights (uint32 or float bits)
+  const uint32_t* col;   // rows split light-then-heavy when lcnt is set
+  const uint32_t* w;     // 4-byte weights (uint32 or float bits), same order as col
+  const uint32_t* lcnt;  // light edges (w <= delta) at the front of each row; null: every edge light
   uint32_t* dist;     // distance words, total_n of them
   uint32_t* st;       // state words
   uint64_t* pool;     // pool_chunks * CH entries: (dist << 32) | vertex-instance id
@@ -79,6 +84,9 @@ hipError_t launch_init(const KArgs& a, const uint32_t* srcs, uint32_t k, int rep
                        hipStream_t s);
 hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s);
 hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
+// Stable per-row partition of (col, w) into light (w <= delta) then heavy edges, with per-row light counts.
+hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
+                        bool is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, hipStream_t s);
 
 // Litmus (litmus.hip)
 hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_t variant, unsigned long long* out,

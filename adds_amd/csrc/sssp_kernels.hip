@@ -12,10 +12,16 @@
 //    yet claimed) and head (positions claimed).  A consumer takes up to `chunk` from the published
 //    count with one fetch_sub (a shortfall is given back) and then advances the head by what it took:
 //    no compare-and-swap retry loops.
-//  * Per-bucket in-flight counters gate the head bucket: like the reference's manager, which advances
-//    the head bag only when it is done (wl.h:714-723), the head moves on when the behind and head
-//    buckets have nothing queued and nothing in flight; `concur` buckets are open for claiming at once
-//    (the reference's concurrent bags, wl.h MIN/MAX_CON_BAG).
+//  * Light and heavy edges (Meyer & Sanders' delta-stepping).  Each CSR row is stored light-first
+//    (w <= delta) with a per-vertex light count.  Expanding a vertex from bucket b relaxes its light
+//    edges and files the vertex in b's heavy queue.  The head bucket's heavy queues open once its light
+//    phase is done (nothing queued or in flight); heavy edges then land in later buckets only, so each
+//    settled vertex relaxes them once.  Without the split, high-degree vertices would re-relax all
+//    their edges every time a light in-neighbour improved them within their bucket.
+//  * Per-bucket in-flight counters gate the head: like the reference's manager, which advances the
+//    head bag only when it is done (wl.h:714-723), the head moves on when the behind and head buckets
+//    have nothing queued and nothing in flight, light or heavy; `concur` buckets accept light claims at
+//    once (the reference's concurrent bags, wl.h MIN/MAX_CON_BAG).
 //  * Queue storage is a pool of 2048-entry chunks handed out by an atomic bump pointer.  A queue maps
 //    its virtual chunk j to a physical chunk through `tab`; the producer that reserves a chunk's first
 //    slot allocates it.  Nothing is reused inside a solve, so no slot can be overwritten before it is
@@ -23,11 +29,13 @@
 //  * Entries carry the distance they were pushed with.  A popped entry whose vertex already holds a
 //    smaller distance is stale (a later push covers it) and is dropped; otherwise the vertex is relaxed
 //    with the carried distance, which is a real path length, so correctness never depends on when the
-//    consumer observes the producer's atomicMin.
-//  * High-degree vertices (the reference's TB-cooperative path, wl.h:1608-1696) are published in one of
-//    HUB_SLOTS hub slots whose 64-bit cursor packs (generation, chunk count, next chunk).  Any wavefront
-//    claims the next `task_edges`-edge chunk with one fetch_add -- no retry loop, an over-claim past the
-//    end is simply void -- and the wavefront that finishes the last chunk frees the slot.
+//    consumer observes the producer's atomicMin.  Pushes are clamped to the window of the bucket being
+//    processed, so no hint of the global head is needed on the hot path.
+//  * High-degree edge ranges (the reference's TB-cooperative path, wl.h:1608-1696) are published in one
+//    of HUB_SLOTS hub slots whose 64-bit cursor packs (generation, chunk count, next chunk).  Any
+//    wavefront claims the next `task_edges`-edge chunk with one fetch_add -- no retry loop, an
+//    over-claim past the end is simply void -- and the wavefront that finishes the last chunk frees the
+//    slot and releases the source bucket's in-flight count.
 //  * Edges of the claimed vertices are flattened with a wavefront prefix sum and processed in windows
 //    of 64 (four in flight per lane); each lane finds its owning vertex by binary search in LDS
 //    (replacing CUB's WarpScan and the byte owner table, kernel.cu:89-102).
@@ -40,9 +48,9 @@
 // accessed with relaxed agent-scope atomics (RMW atomics are coherent at agent scope; atomic
 // loads/stores carry sc1).  Ordering between different words is obtained by waiting for completion of
 // the earlier access (an explicit s_waitcnt with all counters at zero, mem_wait below) before issuing
-// the later one.  Plain (L2-cached) loads are used for the CSR, the d[v] pre-filter (a stale, larger value only
-// costs an extra atomic) and, as hints only, for probing queue and hub counters; every eighth idle
-// probe and every termination test use atomic loads.
+// the later one.  Plain (L2-cached) loads are used for the CSR, the d[v] pre-filter (a stale, larger
+// value only costs an extra atomic) and, as hints only, for probing queue and hub counters; every
+// eighth idle probe and every termination test use atomic loads.
 #include "device.h"
 
 namespace adds {
@@ -118,17 +126,21 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t x) {
   return x;
 }
 
-__device__ __forceinline__ uint32_t* qhead(const KArgs& a, uint32_t q) {
-  return a.st + S_QBASE + QLINES * q * LINE;
-}
+// ---- state addressing ----
+__device__ __forceinline__ uint32_t nq_of(const KArgs& a) { return a.nb * a.parts; }
+__device__ __forceinline__ uint32_t* qhead(const KArgs& a, uint32_t q) { return a.st + S_QBASE + QLINES * q * LINE; }
 __device__ __forceinline__ uint32_t* qtail(const KArgs& a, uint32_t q) {
   return a.st + S_QBASE + QLINES * q * LINE + LINE;
 }
 __device__ __forceinline__ uint32_t* qavail(const KArgs& a, uint32_t q) {
   return a.st + S_QBASE + QLINES * q * LINE + 2 * LINE;
 }
+// in-flight counters of bucket slot `slot`: light claims at [part], heavy claims at [16 + part]
 __device__ __forceinline__ uint32_t* inflight(const KArgs& a, uint32_t slot) {
-  return a.st + s_inflight(a.nb * a.parts) + slot * LINE;
+  return a.st + s_slotlines(nq_of(a)) + 2 * slot * LINE;
+}
+__device__ __forceinline__ uint32_t* phase(const KArgs& a, uint32_t slot) {
+  return a.st + s_slotlines(nq_of(a)) + 2 * slot * LINE + LINE;
 }
 __device__ __forceinline__ uint64_t* hcursor(const KArgs& a, uint32_t i) {
   return (uint64_t*)(a.st + S_HUB + 2 * i * LINE);
@@ -210,57 +222,114 @@ __device__ __forceinline__ uint64_t wait_entry(const KArgs& a, const uint64_t* e
   return e;
 }
 
-// Writes entry `e` at position `pos` of queue `q` (allocating the chunk when pos starts one).
-__device__ __forceinline__ void write_entry(const KArgs& a, uint32_t q, uint32_t pos, uint64_t e) {
-  const uint32_t j = pos >> CH_SHIFT, off = pos & (CH - 1);
-  if (j >= a.maxv) {
+// Allocates a pool chunk for the chunk-map word `tp` (called by the reserver of the chunk's first slot).
+__device__ __forceinline__ void alloc_chunk(const KArgs& a, uint32_t* tp) {
+  uint32_t ph = afadd(a.st + S_POOL, 1u);
+  if (ph >= a.pool_chunks) {
     set_error(a, DERR_OVERFLOW);
-    return;
+    ph = TAB_BAD;
   }
-  uint32_t* tp = a.tab + (size_t)q * a.maxv + j;
-  if (off == 0) {
-    uint32_t ph = afadd(a.st + S_POOL, 1u);
-    if (ph >= a.pool_chunks) {
-      set_error(a, DERR_OVERFLOW);
-      ph = TAB_BAD;
-    }
-    ast(tp, ph);
-  }
-  const uint32_t ph = wait_tab(a, tp);
-  if (ph == TAB_BAD) return;
-  ast64(a.pool + ((size_t)ph << CH_SHIFT) + off, e);
+  ast(tp, ph);
 }
 
-// Pushes (gid, nd) for every lane with f set, one tail reservation per distinct target queue.
-template <bool FLT>
-__device__ __forceinline__ void push_wave(const KArgs& a, bool f, uint32_t gid, uint32_t nd, uint32_t part,
-                                          uint32_t c_now, WaveStats& ws) {
-  uint64_t m = ballot(f);
-  if (m == 0) return;
-  const uint32_t lane = lane_id();
-  uint32_t b = bucket_key<FLT>(nd, a.inv_delta);
-  b = max(b, c_now);
-  b = min(b, c_now + a.nb - 2);
-  const uint32_t q = (b & (a.nb - 1)) * a.parts + part;
-  if (lane == 0) ws.v[ST_PUSH] += __popcll(m);
-  // reserve and write one group (target queue) at a time; the group's leader lane remembers its count
-  uint32_t pub = 0;
-  while (m) {
-    const uint32_t leader = ffs64(m);
-    const uint32_t lq = rdlane(q, leader);
-    const uint64_t peers = ballot(f && q == lq);
-    uint32_t base = 0;
-    if (lane == leader) {
-      pub = (uint32_t)__popcll(peers);
-      base = afadd(qtail(a, lq), pub);
+// Lane 0 reserves `cnt` (<= CH) positions of queue `lq` and resolves the (at most two) pool chunks they
+// fall in; the result is broadcast.  ph0/ph1 are TAB_BAD after an overflow.
+__device__ __forceinline__ void reserve_positions(const KArgs& a, uint32_t lq, uint32_t cnt, uint32_t& base,
+                                                  uint32_t& ph0, uint32_t& ph1) {
+  uint32_t b = 0, p0 = TAB_BAD, p1 = TAB_BAD;
+  if (lane_id() == 0) {
+    b = afadd(qtail(a, lq), cnt);
+    const uint32_t j0 = b >> CH_SHIFT, j1 = (b + cnt - 1) >> CH_SHIFT;
+    if (j1 < a.maxv) {
+      uint32_t* t0 = a.tab + (size_t)lq * a.maxv + j0;
+      uint32_t* t1 = a.tab + (size_t)lq * a.maxv + j1;
+      if ((b & (CH - 1)) == 0) alloc_chunk(a, t0);
+      if (j1 != j0) alloc_chunk(a, t1);
+      p0 = wait_tab(a, t0);
+      p1 = (j1 != j0) ? wait_tab(a, t1) : p0;
+    } else {
+      set_error(a, DERR_OVERFLOW);
     }
-    base = rdlane(base, leader);
-    if (f && q == lq) write_entry(a, lq, base + mbcnt(peers), ((uint64_t)nd << 32) | gid);
-    m &= ~peers;
+  }
+  base = rdlane(b, 0);
+  ph0 = rdlane(p0, 0);
+  ph1 = rdlane(p1, 0);
+}
+
+// Pushes (gid[j], nd[j]) for every lane and window j with f[j] set: one tail reservation per distinct
+// target queue across all U windows, entry stores, one completion wait, then one publish per queue.
+// HEAVY: every entry goes to the heavy queue of bucket cb; otherwise to the light queue of its bucket,
+// clamped to the window of the bucket being processed (cb): never before it (such items are simply
+// early work) and never so far ahead that the ring slot would alias the behind bucket.
+template <bool FLT, bool HEAVY, int U>
+__device__ __forceinline__ void push_multi(const KArgs& a, const bool (&f)[U], const uint32_t (&gid)[U],
+                                           const uint32_t (&nd)[U], uint32_t part, uint32_t cb, WaveStats& ws) {
+  const uint32_t lane = lane_id();
+  uint64_t any = 0;
+  uint32_t q[U];
+  bool rem[U];
+#pragma unroll
+  for (int j = 0; j < U; j++) {
+    const uint64_t m = ballot(f[j]);
+    any |= m;
+    if (lane == 0) ws.v[ST_PUSH] += __popcll(m);
+    if (HEAVY) {
+      q[j] = nq_of(a) + (cb & (a.nb - 1)) * a.parts + part;
+    } else {
+      uint32_t b = bucket_key<FLT>(nd[j], a.inv_delta);
+      b = max(b, cb);
+      b = min(b, cb + a.nb - 3);
+      q[j] = (b & (a.nb - 1)) * a.parts + part;
+    }
+    rem[j] = f[j];
+  }
+  if (any == 0) return;
+  uint32_t pubq = 0, pub = 0;  // group g's queue and count live in lane g
+  uint32_t groups = 0;
+  for (;;) {
+    uint32_t lq = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const uint64_t m = ballot(rem[j]);
+      if (lq == 0xFFFFFFFFu && m) lq = rdlane(q[j], ffs64(m));
+    }
+    if (lq == 0xFFFFFFFFu) break;
+    if (groups == 64) {  // publish what is pending before reusing the lanes
+      mem_wait();
+      if (pub) afadd(qavail(a, pubq), pub);
+      pub = 0;
+      groups = 0;
+    }
+    uint64_t pm[U];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      pm[j] = ballot(rem[j] && q[j] == lq);
+      cnt += (uint32_t)__popcll(pm[j]);
+    }
+    uint32_t base, ph0, ph1;
+    reserve_positions(a, lq, cnt, base, ph0, ph1);
+    const uint32_t j0 = base >> CH_SHIFT;
+    uint32_t off = base;
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      if (rem[j] && q[j] == lq) {
+        const uint32_t pos = off + mbcnt(pm[j]);
+        const uint32_t ph = (pos >> CH_SHIFT) == j0 ? ph0 : ph1;
+        if (ph != TAB_BAD) ast64(a.pool + ((size_t)ph << CH_SHIFT) + (pos & (CH - 1)), ((uint64_t)nd[j] << 32) | gid[j]);
+        rem[j] = false;
+      }
+      off += (uint32_t)__popcll(pm[j]);
+    }
+    if (lane == groups) {
+      pubq = lq;
+      pub = cnt;
+    }
+    groups++;
   }
   // publish: the entries have completed before consumers may count them as claimable
   mem_wait();
-  if (pub) afadd(qavail(a, q), pub);
+  if (pub) afadd(qavail(a, pubq), pub);
 }
 
 // Relaxes UNROLL independent edges per lane (one 64-edge window each) so that UNROLL dependent chains
@@ -270,7 +339,7 @@ constexpr int UNROLL = 4;
 template <bool FLT>
 __device__ __forceinline__ void relax_group(const KArgs& a, const bool (&act)[UNROLL], const uint32_t (&e)[UNROLL],
                                             const uint32_t (&du)[UNROLL], const uint32_t (&gb)[UNROLL], uint32_t part,
-                                            uint32_t c_now, WaveStats& ws) {
+                                            uint32_t cb, WaveStats& ws) {
   uint32_t tgt[UNROLL], nd[UNROLL], dv[UNROLL];
   bool f[UNROLL];
 #pragma unroll
@@ -298,17 +367,18 @@ __device__ __forceinline__ void relax_group(const KArgs& a, const bool (&act)[UN
       old[j] = afmin(a.dist + tgt[j], nd[j]);
     }
   }
+  bool pf[UNROLL];
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++) push_wave<FLT>(a, f[j] && old[j] > nd[j], tgt[j], nd[j], part, c_now, ws);
+  for (int j = 0; j < UNROLL; j++) pf[j] = f[j] && old[j] > nd[j];
+  push_multi<FLT, false, UNROLL>(a, pf, tgt, nd, part, cb, ws);
 }
 
 // Relaxes the contiguous edge range [eb, ee) of one vertex (distance du, replica base gbase).
 template <bool FLT>
 __device__ __forceinline__ void relax_range(const KArgs& a, uint32_t eb, uint32_t ee, uint32_t du, uint32_t gbase,
-                                            uint32_t part, WaveStats& ws) {
+                                            uint32_t part, uint32_t cb, WaveStats& ws) {
   const uint32_t lane = lane_id();
   for (uint32_t base = eb; base < ee; base += 64 * UNROLL) {
-    const uint32_t c_now = ald(a.st + S_CUR);
     bool act[UNROLL];
     uint32_t e[UNROLL], dus[UNROLL], gbs[UNROLL];
 #pragma unroll
@@ -318,14 +388,15 @@ __device__ __forceinline__ void relax_range(const KArgs& a, uint32_t eb, uint32_
       dus[j] = du;
       gbs[j] = gbase;
     }
-    relax_group<FLT>(a, act, e, dus, gbs, part, c_now, ws);
+    relax_group<FLT>(a, act, e, dus, gbs, part, cb, ws);
   }
 }
 
-// Publishes a high-degree vertex in a free hub slot.  Returns false when every slot is busy (the
-// caller then relaxes the vertex itself).
+// Publishes the edge range [rs, rs+deg) of a high-degree vertex in a free hub slot; the slot holds the
+// source bucket's in-flight count (light or heavy, `heavy`) until its last chunk is done.  Returns
+// false when every slot is busy (the caller then relaxes the range itself).
 __device__ __forceinline__ bool publish_hub(const KArgs& a, uint32_t gid, uint32_t du, uint32_t rs, uint32_t deg,
-                                            uint32_t start, WaveStats& ws) {
+                                            uint32_t start, uint32_t cb, uint32_t part, bool heavy, WaveStats& ws) {
   const uint32_t lane = lane_id();
   bool fr = false;
   uint32_t slot_l = (start + lane) & (HUB_SLOTS - 1);
@@ -353,12 +424,16 @@ __device__ __forceinline__ bool publish_hub(const KArgs& a, uint32_t gid, uint32
   if (lane == 0) {
     uint32_t* f = hfield(a, got);
     const uint32_t gen = (ald(f + 5) + 1) & 0xFFFFu;
+    const uint32_t icounter = part + (heavy ? MAX_PARTS : 0u);
     ast(f + 0, gid);
     ast(f + 1, du);
     ast(f + 2, rs);
     ast(f + 3, te);
     ast(f + 4, rs + deg);
     ast(f + 5, gen);
+    ast(f + 6, cb);        // source bucket: chunks clamp their pushes to it
+    ast(f + 7, icounter);  // which in-flight counter of the source bucket the slot holds
+    afadd(inflight(a, cb & (a.nb - 1)) + icounter, 1u);
     ast(hdone(a, got), 0u);
     mem_wait();  // the fields are visible before the cursor opens the slot
     ast64(hcursor(a, got), ((uint64_t)gen << (HUB_NEXT_BITS + HUB_TOTAL_BITS)) | ((uint64_t)total << HUB_NEXT_BITS));
@@ -383,7 +458,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   const uint32_t wv = threadIdx.x >> 6;
   const uint32_t part = blockIdx.x % a.parts;
   const uint32_t wave_gid = blockIdx.x * WAVES_PER_WG + wv;
-  const uint32_t nq = a.nb * a.parts;
+  const uint32_t nq = nq_of(a);
+  const uint32_t nlight = (a.concur + 1) * a.parts;  // light probe lanes: behind, head, concur-1 ahead
   WaveStats ws;
 #pragma unroll
   for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
@@ -394,12 +470,14 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 
   for (;;) {
     // ---------------- probe hub slots and open queues in parallel ----------------
-    // lanes [0, HUB_SLOTS) probe hub slots (rotated per wavefront); lane 16 + l probes bucket
-    // c-1+l/parts, sub-queue (part + l%parts) % parts, so the ballot's bit order is the claim priority.
-    const uint32_t c = ald(a.st + S_CUR);
+    // lanes [0, 16): hub slots (rotated per wavefront).  Then, `parts` lanes each: light behind, light
+    // head, heavy behind, heavy head (only once the head's phase word is set), light of the concur-1
+    // buckets ahead.  Bit order is the claim priority.
     const bool cached = a.cached_probe && (idle_iter & 7) != 7 && !missed;
     missed = false;
-    uint32_t pq = 0;
+    const uint32_t c = hld(a.st + S_CUR, cached);
+    const bool head_heavy_open = hld(phase(a, c & (a.nb - 1)), cached) != 0;
+    uint32_t pq = 0, pb = 0;
     uint64_t hc = 0;
     bool pv = false;
     if (lane < HUB_SLOTS) {
@@ -408,13 +486,22 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       pv = cur_next(hc) < cur_total(hc);
     } else {
       const uint32_t l = lane - HUB_SLOTS;
-      if (l < (a.concur + 1) * a.parts) {
-        const uint32_t j = l / a.parts;  // 0 = behind bucket c-1
-        if (c > 0 || j > 0) {
-          const uint32_t slot = (c - 1 + j) & (a.nb - 1);
-          pq = slot * a.parts + ((part + l % a.parts) % a.parts);
-          pv = (int32_t)hld(qavail(a, pq), cached) > 0;
-        }
+      const uint32_t grp = l / a.parts;
+      uint32_t j = 0xFFFFFFFFu;  // bucket c-1+j
+      bool heavy = false;
+      if (grp < 2) {
+        j = grp;
+      } else if (grp < 4) {
+        j = grp - 2;
+        heavy = true;
+        if (j == 1 && !head_heavy_open) j = 0xFFFFFFFFu;
+      } else if (l < nlight + 2 * a.parts) {
+        j = grp - 2;
+      }
+      if (j != 0xFFFFFFFFu && (c > 0 || j > 0)) {
+        pb = c - 1 + j;
+        pq = (heavy ? nq : 0u) + (pb & (a.nb - 1)) * a.parts + ((part + l % a.parts) % a.parts);
+        pv = (int32_t)hld(qavail(a, pq), cached) > 0;
       }
     }
     uint64_t pm = ballot(pv);
@@ -452,14 +539,15 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       for (int tries = 0; tries < 3 && kind == W_NONE && qm; tries++) {
         const uint32_t L = ffs64(qm) + HUB_SLOTS;
         const uint32_t qq = rdlane(pq, L);
-        const uint32_t bb = c - 1 + (L - HUB_SLOTS) / a.parts;
+        const uint32_t bb = rdlane(pb, L);
         if (lane == 0) {
           const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - a.chunk);
           const int32_t got = min(max(old, 0), (int32_t)a.chunk);
           if (got < (int32_t)a.chunk) afadd(qavail(a, qq), (uint32_t)((int32_t)a.chunk - got));
           if (got > 0) {
             h = afadd(qhead(a, qq), (uint32_t)got);
-            afadd(inflight(a, bb & (a.nb - 1)), (uint32_t)got);
+            const uint32_t ic = (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u);
+            afadd(inflight(a, bb & (a.nb - 1)) + ic, (uint32_t)got);
             cnt = (uint32_t)got;
             kind = W_ITEMS;
           } else {
@@ -499,10 +587,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 
     if (kind == W_HUB) {
       idle_iter = 0;
-      // ---------------- one chunk of a published hub vertex ----------------
+      // ---------------- one chunk of a published edge range ----------------
       mem_wait();  // the claim has completed before the fields are read
       const uint32_t* f = hfield(a, q);
       const uint32_t gid_u = ald(f + 0), du = ald(f + 1), rs = ald(f + 2), te = ald(f + 3), re = ald(f + 4);
+      const uint32_t hcb = ald(f + 6), hic = ald(f + 7);
       if (ald(f + 5) != cur_gen(claimed)) {  // a slot cannot be recycled while a chunk is outstanding
         set_error(a, DERR_PROTOCOL);
         break;
@@ -513,18 +602,22 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       if (ald(a.dist + gid_u) < du) {
         if (lane == 0) ws.v[ST_STALE]++;
       } else {
-        relax_range<FLT>(a, eb, ee, du, REPL ? (gid_u / a.n) * a.n : 0u, part, ws);
+        relax_range<FLT>(a, eb, ee, du, REPL ? (gid_u / a.n) * a.n : 0u, part, hcb, ws);
       }
       mem_wait();  // this chunk's pushes have completed before it is counted as finished
       if (lane == 0) {
         const uint32_t d = afadd(hdone(a, q), 1u);
-        if (d + 1 == cur_total(claimed)) ast(hfree(a, q), 1u);
+        if (d + 1 == cur_total(claimed)) {
+          afadd(inflight(a, hcb & (a.nb - 1)) + hic, 0u - 1u);
+          ast(hfree(a, q), 1u);
+        }
       }
       continue;
     }
 
     if (kind == W_ITEMS) {
       idle_iter = 0;
+      const bool heavy_claim = q >= nq;
       // ---------------- load the claimed entries ----------------
       bool have = lane < cnt;
       uint32_t gid = 0, du = 0;
@@ -542,22 +635,29 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         du = (uint32_t)(e >> 32);
       }
       if (ballot(lane < cnt && !have)) break;  // error path: leave the solve
-      const uint32_t c_now0 = ald(a.st + S_CUR);
       // stale / early filtering
       bool ok = false, early = false;
       if (have) {
-        const uint32_t dnow = a.dist[gid];
+        const uint32_t dnow = ald(a.dist + gid);  // coherent: a stale (larger) value would defeat the filter
         if (dnow < du) {
           ws.v[ST_STALE]++;
-        } else if (bucket_key<FLT>(du, a.inv_delta) > cb) {
+        } else if (!heavy_claim && bucket_key<FLT>(du, a.inv_delta) > cb) {
           early = true;
           ws.v[ST_REBUCKET]++;
         } else {
           ok = true;
+          if (bucket_key<FLT>(du, a.inv_delta) + 1 < cb) ws.v[ST_LATE]++;
         }
       }
-      push_wave<FLT>(a, early, gid, du, part, c_now0, ws);
+      {
+        const bool ef[1] = {early};
+        const uint32_t eg[1] = {gid}, ed[1] = {du};
+        push_multi<FLT, false, 1>(a, ef, eg, ed, part, cb, ws);
+      }
+      // edge range: light edges of a light claim (then the vertex is filed for its heavy edges), the
+      // heavy edges of a heavy claim
       uint32_t rs = 0, deg = 0, gbase = 0;
+      bool file_heavy = false;
       if (ok) {
         uint32_t v = gid;
         if (REPL) {
@@ -565,17 +665,30 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           gbase = inst * a.n;
           v = gid - gbase;
         }
-        rs = a.row_ptr[v];
-        deg = a.row_ptr[v + 1] - rs;
-        ws.v[ST_EXPANDED]++;
+        const uint32_t r0 = a.row_ptr[v], r1 = a.row_ptr[v + 1];
+        const uint32_t lc = a.lcnt ? a.lcnt[v] : (r1 - r0);
+        if (heavy_claim) {
+          rs = r0 + lc;
+          deg = r1 - rs;
+        } else {
+          rs = r0;
+          deg = lc;
+          file_heavy = r1 - r0 > lc;
+        }
+        if (!heavy_claim) ws.v[ST_EXPANDED]++;
       }
-      // ---------------- high-degree vertices go to hub slots ----------------
+      {
+        const bool hf[1] = {file_heavy};
+        const uint32_t hg[1] = {gid}, hd[1] = {du};
+        push_multi<FLT, true, 1>(a, hf, hg, hd, part, cb, ws);
+      }
+      // ---------------- high-degree ranges go to hub slots ----------------
       bool hub = false;
       uint64_t hm = ballot(ok && deg > a.hub_degree);
       while (hm) {
         const uint32_t L = ffs64(hm);
         const bool pub = publish_hub(a, rdlane(gid, L), rdlane(du, L), rdlane(rs, L), rdlane(deg, L),
-                                     wave_gid + L, ws);
+                                     wave_gid + L, cb, part, heavy_claim, ws);
         if (lane == L) hub = pub;
         hm &= hm - 1;
       }
@@ -591,7 +704,6 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         if (lane == 0) s_pref[wv][64] = total;
         wave_sync();
         for (uint32_t base = 0; base < total; base += 64 * UNROLL) {
-          const uint32_t c_now = ald(a.st + S_CUR);
           bool act[UNROLL];
           uint32_t e[UNROLL], dus[UNROLL], gbs[UNROLL];
 #pragma unroll
@@ -614,17 +726,17 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
               gbs[j] = REPL ? s_gb[wv][lo] : 0u;
             }
           }
-          relax_group<FLT>(a, act, e, dus, gbs, part, c_now, ws);
+          relax_group<FLT>(a, act, e, dus, gbs, part, cb, ws);
         }
         wave_sync();
       }
       mem_wait();  // this chunk's pushes have completed before it stops counting as in flight
-      if (lane == 0) afadd(inflight(a, cb & (a.nb - 1)), 0u - cnt);
-      if (ald(a.st + S_DONE + 1) != 0) break;
+      if (lane == 0) afadd(inflight(a, cb & (a.nb - 1)) + (q % a.parts) + (heavy_claim ? MAX_PARTS : 0u), 0u - cnt);
+      if (hld(a.st + S_DONE + 1, true) != 0) break;
       continue;
     }
 
-    // ---------------- no work visible: go idle, test termination, advance the head ----------------
+    // ---------------- no work visible: go idle, test termination, move the head ----------------
     if (busy) {
       mem_wait();  // every push of this wavefront has completed
       if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 1ull);
@@ -637,7 +749,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       if ((uint32_t)w1 == a.total_waves) {
         mem_wait();
         bool nonempty = false;
-        for (uint32_t qq = lane; qq < nq; qq += 64) nonempty |= ald(qhead(a, qq)) != ald(qtail(a, qq));
+        for (uint32_t qq = lane; qq < 2 * nq; qq += 64) nonempty |= ald(qhead(a, qq)) != ald(qtail(a, qq));
         if (lane < HUB_SLOTS) {
           const uint64_t cc = ald64(hcursor(a, lane));
           nonempty |= cur_next(cc) < cur_total(cc);
@@ -651,9 +763,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         }
       }
     }
-    // Advance the head bucket to the first non-empty bucket ahead, but only when the behind and head
-    // buckets are empty (items still being processed are not waited for: the solver is asynchronous).
-    // One wavefront at a time scans (try-lock): idle wavefronts must not flood the queue counters.
+    // One wavefront at a time (try-lock) inspects the head: idle wavefronts must not flood the counters.
+    // (1) When the head bucket's light phase is done (light queues empty, no light claim in flight),
+    //     its heavy queues open.  (2) When the behind and head buckets are done -- nothing queued and
+    //     nothing in flight, light or heavy -- the head moves to the first non-empty bucket ahead, like
+    //     the reference's manager (wl.h:714-723).  Phase words of slots leaving the window are cleared.
     bool advanced = false;
     uint32_t locked = 0;
     if (lane == 0) {
@@ -662,34 +776,49 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     }
     if (rdlane(locked, 0)) {
       const uint32_t c2 = ald(a.st + S_CUR);
-      uint32_t firstj = 0xFFFFFFFFu;
-      bool behind_or_head = false;
-      for (uint32_t j0 = 0; j0 < a.nb; j0 += 64) {
-        const uint32_t jj = j0 + lane;  // bucket c2-1+jj
-        bool ne = false;
-        if (jj < a.nb) {
-          const uint32_t slot = (c2 - 1 + jj) & (a.nb - 1);
-          for (uint32_t pi = 0; pi < a.parts; pi++) {
-            const uint32_t qq = slot * a.parts + pi;
-            ne |= ald(qhead(a, qq)) < ald(qtail(a, qq));
-          }
-        }
-        uint64_t m = ballot(ne);
-        if (j0 == 0) {
-          behind_or_head = (m & (c2 > 0 ? 3ull : 2ull)) != 0;
-          m &= ~3ull;
-        }
-        if (m && firstj == 0xFFFFFFFFu) firstj = j0 + ffs64(m);
+      const uint32_t hs = c2 & (a.nb - 1), bs = (c2 - 1) & (a.nb - 1);
+      // head/behind state, one lane per part
+      bool head_light = false, head_heavy = false, behind_any = false;
+      if (lane < a.parts) {
+        head_light = ald(qhead(a, hs * a.parts + lane)) < ald(qtail(a, hs * a.parts + lane)) ||
+                     ald(inflight(a, hs) + lane) != 0;
+        head_heavy = ald(qhead(a, nq + hs * a.parts + lane)) < ald(qtail(a, nq + hs * a.parts + lane)) ||
+                     ald(inflight(a, hs) + MAX_PARTS + lane) != 0;
+        if (c2 > 0)
+          behind_any = ald(qhead(a, bs * a.parts + lane)) < ald(qtail(a, bs * a.parts + lane)) ||
+                       ald(qhead(a, nq + bs * a.parts + lane)) < ald(qtail(a, nq + bs * a.parts + lane)) ||
+                       ald(inflight(a, bs) + lane) != 0 || ald(inflight(a, bs) + MAX_PARTS + lane) != 0;
       }
-      // like the reference, the head moves only when its bucket is done (wl.h:714-723): nothing
-      // queued and nothing still being processed in the behind and head buckets
-      if (!behind_or_head && firstj != 0xFFFFFFFFu) {
-        const uint32_t busy_h = ald(inflight(a, c2 & (a.nb - 1)));
-        const uint32_t busy_b = c2 > 0 ? ald(inflight(a, (c2 - 1) & (a.nb - 1))) : 0u;
-        if (busy_h == 0 && busy_b == 0) {
+      const bool hl = ballot(head_light) != 0, hh = ballot(head_heavy) != 0, ba = ballot(behind_any) != 0;
+      if (!hl && lane == 0 && ald(phase(a, hs)) == 0) ast(phase(a, hs), 1u);
+      if (!hl && !hh && !ba) {
+        // first bucket ahead with anything queued or in flight (light or heavy), so that no bucket
+        // that still has, or will get, entries is skipped
+        uint32_t firstj = 0xFFFFFFFFu;
+        for (uint32_t j0 = 0; j0 < a.nb; j0 += 64) {
+          const uint32_t jj = j0 + lane;  // bucket c2-1+jj
+          bool ne = false;
+          if (jj >= 2 && jj < a.nb) {
+            const uint32_t slot = (c2 - 1 + jj) & (a.nb - 1);
+            const uint32_t* fl = inflight(a, slot);
+            for (uint32_t pi = 0; pi < a.parts; pi++) {
+              const uint32_t ql = slot * a.parts + pi, qh = nq + ql;
+              ne |= ald(qhead(a, ql)) < ald(qtail(a, ql)) || ald(qhead(a, qh)) < ald(qtail(a, qh)) ||
+                    ald(fl + pi) != 0 || ald(fl + MAX_PARTS + pi) != 0;
+            }
+          }
+          const uint64_t m = ballot(ne);
+          if (m && firstj == 0xFFFFFFFFu) firstj = j0 + ffs64(m);
+        }
+        if (firstj != 0xFFFFFFFFu) {
+          const uint32_t c3 = c2 - 1 + firstj;
+          // buckets c2-1 .. c3-2 leave the window; their slots come back later as fresh buckets
+          for (uint32_t b = c2 - 1 + lane; b + 1 < c3 + 0u && b - (c2 - 1) < a.nb; b += 64)
+            if (c2 > 0 || b != c2 - 1) ast(phase(a, b & (a.nb - 1)), 0u);
+          mem_wait();
           if (lane == 0) {
             uint32_t cc = c2;
-            acas(a.st + S_CUR, cc, c2 - 1 + firstj);
+            acas(a.st + S_CUR, cc, c3);
           }
           advanced = true;
         }
@@ -734,7 +863,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// init / cleanup / fills
+// init / cleanup / fills / light-heavy split
 // ------------------------------------------------------------------------------------------------
 __global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
@@ -745,7 +874,7 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t count, uint64_t v) {
     p[i] = v;
 }
 
-// Resets the state words and enqueues the sources into bucket 0, sub-queue 0 (single block).
+// Resets the state words and enqueues the sources into bucket 0, light sub-queue 0 (single block).
 __global__ void k_init_state(KArgs a, const uint32_t* srcs, uint32_t k, int replicated) {
   const uint32_t nw = state_words(a.nb * a.parts, a.nb);
   for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) a.st[i] = 0;
@@ -768,16 +897,56 @@ __global__ void k_init_state(KArgs a, const uint32_t* srcs, uint32_t k, int repl
 // Restores the pool and the chunk maps to their empty state, using the previous solve's counters
 // (which the next k_init_state clears).
 __global__ void k_cleanup(KArgs a) {
-  const uint32_t nq = a.nb * a.parts;
+  const uint32_t nqt = 2 * a.nb * a.parts;
   const uint64_t used_chunks = min((uint64_t)a.st[S_POOL], (uint64_t)a.pool_chunks);
   const uint64_t pool_used = used_chunks << CH_SHIFT;
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nth = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = tid; i < pool_used; i += nth) a.pool[i] = EMPTY64;
-  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+  for (uint32_t q = blockIdx.x; q < nqt; q += gridDim.x) {
     const uint32_t t = a.st[S_QBASE + QLINES * q * LINE + LINE];
     const uint32_t used = min((t + CH - 1) >> CH_SHIFT, a.maxv);
     for (uint32_t j = threadIdx.x; j < used; j += blockDim.x) a.tab[(size_t)q * a.maxv + j] = TAB_UNSET;
+  }
+}
+
+// One wavefront per row: a stable partition of the row's (col, w) into light (w <= delta) then heavy.
+__global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
+                        int is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t idelta = delta >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)delta;  // int weights: w <= floor(delta)
+  for (uint32_t v = wave; v < n; v += nwaves) {
+    const uint32_t rs = row_ptr[v], re = row_ptr[v + 1];
+    uint32_t nl = 0;
+    for (uint32_t b = rs; b < re; b += 64) {
+      const uint32_t e = b + lane;
+      bool l = false;
+      if (e < re) l = is_float ? __uint_as_float(w[e]) <= delta : w[e] <= idelta;
+      nl += (uint32_t)__popcll(ballot(l));
+    }
+    if (lane == 0) lcnt[v] = nl;
+    uint32_t lo = rs, hi = rs + nl;
+    for (uint32_t b = rs; b < re; b += 64) {
+      const uint32_t e = b + lane;
+      const bool in = e < re;
+      uint32_t c = 0, x = 0;
+      bool l = false;
+      if (in) {
+        c = col[e];
+        x = w[e];
+        l = is_float ? __uint_as_float(x) <= delta : x <= idelta;
+      }
+      const uint64_t ml = ballot(in && l), mh = ballot(in && !l);
+      if (in) {
+        const uint32_t dst = l ? lo + mbcnt(ml) : hi + mbcnt(mh);
+        col2[dst] = c;
+        w2[dst] = x;
+      }
+      lo += (uint32_t)__popcll(ml);
+      hi += (uint32_t)__popcll(mh);
+    }
   }
 }
 
@@ -812,6 +981,13 @@ hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t
 }
 hipError_t launch_cleanup(const KArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_cleanup, dim3(1024), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
+                        bool is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, hipStream_t s) {
+  const uint32_t blocks = std::min<uint32_t>((n + 3) / 4, 16384);
+  hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, row_ptr, col, w, n, delta, is_float ? 1 : 0, col2, w2,
+                     lcnt);
   return hipGetLastError();
 }
 

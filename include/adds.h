@@ -49,7 +49,7 @@ typedef struct {
   uint32_t chunk;        /* worklist items claimed per wavefront (1..64) */
   uint32_t flags;        /* ADDS_F_* */
   uint32_t nbuckets;     /* bucket ring length (power of two, 4..256) */
-  uint32_t parts;        /* sub-queues per bucket (1 or 8: one per XCD) */
+  uint32_t parts;        /* sub-queues per bucket (1, 2, 4 or 8; 8 = one per XCD); (concurrency + 3) * parts <= 48 */
   uint32_t hub_degree;   /* out-degree above which a vertex is split into edge tasks */
   uint32_t task_edges;   /* edges per hub edge task */
   uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
@@ -76,6 +76,7 @@ typedef struct {
   uint64_t probes;            /* queue probes issued by wavefronts looking for work */
   uint64_t cas_failures;      /* lost claim races */
   uint64_t idle_iterations;   /* idle-loop iterations of all wavefronts */
+  uint64_t late_items;        /* items expanded more than one bucket after their own bucket */
 } adds_stats;
 
 /* ---- errors / device ---- */

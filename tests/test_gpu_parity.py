@@ -55,7 +55,8 @@ def test_grid2048_float_c2():
 
 @pytest.mark.parametrize("params", [
     dict(parts=1, nbuckets=4, chunk=1, wg_per_cu=1),
-    dict(parts=8, nbuckets=256, chunk=64, wg_per_cu=8, concurrency=4),
+    dict(parts=8, nbuckets=256, chunk=64, wg_per_cu=8, concurrency=3),
+    dict(parts=4, nbuckets=8, chunk=48, wg_per_cu=2, concurrency=4),
     dict(parts=2, nbuckets=16, chunk=17, delta=1.0, hub_degree=64, task_edges=64),
     dict(parts=4, nbuckets=32, chunk=32, delta=100000.0, concurrency=2),
 ])
