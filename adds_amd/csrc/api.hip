@@ -46,6 +46,8 @@ struct Workspace {
   hipStream_t last_stream = nullptr;
   bool pending = false;                 // a NO_SYNC solve has not been collected
   bool kernel_events = false;           // evk0/evk1 bracket the last solve's persistent kernel
+  bool pool_fixed = false;              // the last solve's pool size was set by the caller
+  uint32_t pool_shift = 0;              // default pool sizes are doubled this often (grown after overflows)
   double timeout_ms = 0;
 };
 
@@ -63,6 +65,7 @@ struct adds_graph {
   uint32_t* scol = nullptr;
   uint32_t* sw = nullptr;
   uint32_t* lcnt = nullptr;
+  uint32_t* split_aux = nullptr;
   float split_delta = 0.f;
   Workspace ws;
   uint32_t* outbuf = nullptr;  // device staging for host outputs
@@ -142,7 +145,8 @@ static void release_graph_memory(adds_graph* g) {
   if (g->scol) (void)hipFree(g->scol);
   if (g->sw) (void)hipFree(g->sw);
   if (g->lcnt) (void)hipFree(g->lcnt);
-  g->scol = g->sw = g->lcnt = nullptr;
+  if (g->split_aux) (void)hipFree(g->split_aux);
+  g->scol = g->sw = g->lcnt = g->split_aux = nullptr;
   g->split_delta = 0.f;
   if (g->union_g) {
     release_graph_memory(g->union_g);
@@ -260,7 +264,7 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   if (g->m) HIPCHK(hipMemcpyAsync(g->w, w_dev, g->m * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   if (g->lcnt && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
     HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scol, g->sw, g->lcnt,
-                        (hipStream_t)stream));
+                        g->split_aux, (hipStream_t)stream));
   if (g->union_g) {  // weights changed: the union copy is stale
     release_graph_memory(g->union_g);
     delete g->union_g;
@@ -336,7 +340,8 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.flags = p->flags;
   r.timeout_ms = p->timeout_ms ? (double)p->timeout_ms : 120000.0;
   r.pool_fixed = p->pool_entries != 0;
-  r.pool_entries = p->pool_entries ? p->pool_entries : 3 * total_n + (4u << 20);
+  // default: 4 entries per vertex-instance + 8M, doubled after each overflow on this graph (remembered)
+  r.pool_entries = p->pool_entries ? p->pool_entries : (4 * total_n + (8u << 20)) << std::min(g->ws.pool_shift, 6u);
   if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 64 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
       r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||
@@ -402,10 +407,11 @@ static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
     HIPCHK(hipMalloc(&g->scol, std::max<uint64_t>(g->m, 1) * 4));
     HIPCHK(hipMalloc(&g->sw, std::max<uint64_t>(g->m, 1) * 4));
     HIPCHK(hipMalloc(&g->lcnt, std::max<uint64_t>(g->n, 1) * 4));
+    HIPCHK(hipMalloc(&g->split_aux, split_aux_words(g->m) * 4));
     g->split_delta = 0.f;
   }
   if (g->split_delta != delta) {
-    if (g->n) HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scol, g->sw, g->lcnt, s));
+    HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scol, g->sw, g->lcnt, g->split_aux, s));
     g->split_delta = delta;
   }
   return ADDS_OK;
@@ -518,6 +524,7 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->late_items = sv[ST_LATE];
   }
   if (err == DERR_OVERFLOW) {
+    if (!ws.pool_fixed && ws.pool_shift < 6) ws.pool_shift++;
     g_detail = "worklist pool exhausted (" + std::to_string(hs[S_POOL]) + " chunks requested)";
     return ADDS_EOVERFLOW;
   }
@@ -558,6 +565,7 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
     if (rc != ADDS_OK) return rc;
     Workspace& ws = g->ws;
     ws.timeout_ms = r.timeout_ms;
+    ws.pool_fixed = r.pool_fixed;
     HIPCHK(hipMemcpyAsync(ws.srcs, srcs_host, (size_t)k * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(ws.ev0, s));
     rc = enqueue_solve(g, r, dist_dev, total_n, ws.srcs, k, replicated, s, true);
@@ -734,6 +742,7 @@ extern "C" adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* sr
   if (rc != ADDS_OK) return rc;
   HIPCHK(hipDeviceSynchronize());
   g->ws.timeout_ms = r.timeout_ms;
+  g->ws.pool_fixed = r.pool_fixed;
   hipStream_t s;
   HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   adds_solve_graph* sg = new adds_solve_graph();

@@ -85,8 +85,10 @@ hipError_t launch_init(const KArgs& a, const uint32_t* srcs, uint32_t k, int rep
 hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s);
 hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
 // Stable per-row partition of (col, w) into light (w <= delta) then heavy edges, with per-row light counts.
+// aux: split_aux_words(m) words of scratch (the list of long rows).
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        bool is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, hipStream_t s);
+                        bool is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, uint32_t* aux, hipStream_t s);
+constexpr uint64_t split_aux_words(uint64_t m) { return m / 4096 + 2; }
 
 // Litmus (litmus.hip)
 hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_t variant, unsigned long long* out,

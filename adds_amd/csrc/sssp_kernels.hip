@@ -910,21 +910,35 @@ __global__ void k_cleanup(KArgs a) {
   }
 }
 
-// One wavefront per row: a stable partition of the row's (col, w) into light (w <= delta) then heavy.
+// Stable per-row partition of (col, w) into light (w <= delta) then heavy edges.  Rows of up to
+// SPLIT_BIG edges: one wavefront per row.  Longer rows are appended to a list (aux[0] = count,
+// aux[1..] = rows) and partitioned by k_split_big, one 1024-thread workgroup per row, so that the
+// few very long rows of a power-law graph do not serialise on one wavefront.
+constexpr uint32_t SPLIT_BIG = 4096;
+
+__device__ __forceinline__ bool is_light(uint32_t x, float delta, uint32_t idelta, int is_float) {
+  return is_float ? __uint_as_float(x) <= delta : x <= idelta;
+}
+__device__ __forceinline__ uint32_t int_delta(float delta) {
+  return delta >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)delta;  // int weights: w <= floor(delta)
+}
+
 __global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        int is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt) {
+                        int is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, uint32_t* aux) {
   const uint32_t lane = lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t idelta = delta >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)delta;  // int weights: w <= floor(delta)
+  const uint32_t idelta = int_delta(delta);
   for (uint32_t v = wave; v < n; v += nwaves) {
     const uint32_t rs = row_ptr[v], re = row_ptr[v + 1];
+    if (re - rs > SPLIT_BIG) {
+      if (lane == 0) aux[1 + afadd(aux, 1u)] = v;
+      continue;
+    }
     uint32_t nl = 0;
     for (uint32_t b = rs; b < re; b += 64) {
       const uint32_t e = b + lane;
-      bool l = false;
-      if (e < re) l = is_float ? __uint_as_float(w[e]) <= delta : w[e] <= idelta;
-      nl += (uint32_t)__popcll(ballot(l));
+      nl += (uint32_t)__popcll(ballot(e < re && is_light(w[e], delta, idelta, is_float)));
     }
     if (lane == 0) lcnt[v] = nl;
     uint32_t lo = rs, hi = rs + nl;
@@ -936,7 +950,7 @@ __global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint
       if (in) {
         c = col[e];
         x = w[e];
-        l = is_float ? __uint_as_float(x) <= delta : x <= idelta;
+        l = is_light(x, delta, idelta, is_float);
       }
       const uint64_t ml = ballot(in && l), mh = ballot(in && !l);
       if (in) {
@@ -946,6 +960,85 @@ __global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint
       }
       lo += (uint32_t)__popcll(ml);
       hi += (uint32_t)__popcll(mh);
+    }
+  }
+}
+
+// One workgroup of 1024 threads per long row.  Windows of 4096 edges: edge base + j*1024 + t goes to
+// thread t, sub-window j; ranks come from per-(sub-window, wavefront) ballot counts scanned in edge
+// order by one wavefront.
+__global__ void __launch_bounds__(1024) k_split_big(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w,
+                                                    float delta, int is_float, uint32_t* col2, uint32_t* w2,
+                                                    uint32_t* lcnt, const uint32_t* aux) {
+  __shared__ uint32_t s_l[64], s_h[64], s_pl[64], s_ph[64];
+  __shared__ uint32_t s_tl, s_th;
+  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const uint32_t idelta = int_delta(delta);
+  const uint32_t nbig = aux[0];
+  for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
+    const uint32_t v = aux[1 + i];
+    const uint32_t rs = row_ptr[v], re = row_ptr[v + 1];
+    uint32_t c = 0;
+    for (uint32_t e = rs + t; e < re; e += 1024) c += is_light(w[e], delta, idelta, is_float) ? 1u : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0) s_l[wv] = c;
+    __syncthreads();
+    if (t == 0) {
+      uint32_t tot = 0;
+      for (uint32_t k = 0; k < 16; k++) tot += s_l[k];
+      s_tl = tot;
+      lcnt[v] = tot;
+    }
+    __syncthreads();
+    uint32_t lo = rs, hi = rs + s_tl;
+    __syncthreads();
+    for (uint32_t base = rs; base < re; base += 4096) {
+      uint32_t cc[4], xx[4];
+      bool in[4], l[4];
+      uint64_t ml[4], mh[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t e = base + j * 1024 + t;
+        in[j] = e < re;
+        cc[j] = 0;
+        xx[j] = 0;
+        l[j] = false;
+        if (in[j]) {
+          cc[j] = col[e];
+          xx[j] = w[e];
+          l[j] = is_light(xx[j], delta, idelta, is_float);
+        }
+        ml[j] = ballot(in[j] && l[j]);
+        mh[j] = ballot(in[j] && !l[j]);
+        if (lane == 0) {
+          s_l[j * 16 + wv] = (uint32_t)__popcll(ml[j]);
+          s_h[j * 16 + wv] = (uint32_t)__popcll(mh[j]);
+        }
+      }
+      __syncthreads();
+      if (wv == 0) {
+        const uint32_t xl = s_l[lane], xh = s_h[lane];
+        const uint32_t il = wave_incl_scan(xl), ih = wave_incl_scan(xh);
+        s_pl[lane] = il - xl;
+        s_ph[lane] = ih - xh;
+        if (lane == 63) {
+          s_tl = il;
+          s_th = ih;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (in[j]) {
+          const uint32_t dst = l[j] ? lo + s_pl[j * 16 + wv] + mbcnt(ml[j]) : hi + s_ph[j * 16 + wv] + mbcnt(mh[j]);
+          col2[dst] = cc[j];
+          w2[dst] = xx[j];
+        }
+      }
+      lo += s_tl;
+      hi += s_th;
+      __syncthreads();  // the counts are rewritten by the next window
     }
   }
 }
@@ -984,10 +1077,15 @@ hipError_t launch_cleanup(const KArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        bool is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, hipStream_t s) {
+                        bool is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, uint32_t* aux, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipError_t e = launch_fill_u32(aux, 1, 0, s);
+  if (e != hipSuccess) return e;
   const uint32_t blocks = std::min<uint32_t>((n + 3) / 4, 16384);
   hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, row_ptr, col, w, n, delta, is_float ? 1 : 0, col2, w2,
-                     lcnt);
+                     lcnt, aux);
+  hipLaunchKernelGGL(k_split_big, dim3(1024), dim3(1024), 0, s, row_ptr, col, w, delta, is_float ? 1 : 0, col2, w2,
+                     lcnt, (const uint32_t*)aux);
   return hipGetLastError();
 }
 

@@ -54,7 +54,8 @@ typedef struct {
   uint32_t task_edges;   /* edges per hub edge task */
   uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
   uint32_t timeout_ms;   /* device watchdog (default 120000) */
-  uint64_t pool_entries; /* worklist pool capacity in entries (default 3 per vertex-instance + 4M) */
+  uint64_t pool_entries; /* worklist pool capacity in entries (default 4 per vertex-instance + 8M, doubled
+                            and remembered per graph after an overflow; a solve that overflows is rerun) */
 } adds_params;
 
 /* flags */

@@ -104,3 +104,56 @@ def test_repeated_solves_same_handle():
     for s in [0, 5, 17, g.max_degree_vertex()]:
         dist, _ = dg.sssp(s, timeout_ms=TIMEOUT)
         _check(g, dist, s, False)
+
+
+@pytest.mark.parametrize("is_float", [False, True])
+def test_long_rows_light_heavy_partition(is_float):
+    """Two hubs with 20000 and 6000 out-edges (longer than one wavefront handles in the light/heavy
+    split) plus a ring: distances from hub 0 are the direct edge weights wherever the direct edge is
+    shortest, so a misplaced (col, w) pair in the partition shows up as a wrong distance."""
+    n = 30000
+    rng = np.random.default_rng(11)
+    src, dst = [], []
+    for v in range(1, 20001):
+        src.append(0), dst.append(v)
+    for v in range(20001, 26001):
+        src.append(1), dst.append(v)
+    for v in range(n):
+        src.append(v), dst.append((v + 1) % n)
+    src, dst = np.array(src, dtype=np.int64), np.array(dst, dtype=np.int64)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    row_ptr = np.zeros(n + 1, dtype=np.uint32)
+    np.add.at(row_ptr, src + 1, 1)
+    row_ptr = np.cumsum(row_ptr).astype(np.uint32)
+    if is_float:
+        w = rng.uniform(0.0, 1000.0, size=len(dst)).astype(np.float32)
+    else:
+        w = rng.integers(1, 1000, size=len(dst)).astype(np.uint32)
+    g = adds_amd.HostGraph.from_arrays(row_ptr, dst.astype(np.uint32), w, "f32" if is_float else "u32")
+    dg = adds_amd.Graph(g)
+    for delta in (16.0, 500.0, 4096.0):  # mostly heavy, mixed, all light
+        dist, _ = dg.sssp(0, delta=delta, timeout_ms=TIMEOUT)
+        _check(g, dist, 0, is_float)
+
+
+def test_weights_update_resplits():
+    """New weights through the device setter are seen by later solves (the light-first copy of the rows is
+    rebuilt), with the same and with a different bucket width."""
+    import torch
+
+    g = adds_amd.HostGraph.rmat(14, 16, seed=12, wkind="f32", lo=1.0, hi=10.0)
+    src = g.max_degree_vertex()
+    dg = adds_amd.Graph(g)
+    dist, _ = dg.sssp(src, delta=2.0, timeout_ms=TIMEOUT)
+    _check(g, dist, src, True)
+    rng = np.random.default_rng(3)
+    # independent per-edge weights (the two directions of an edge may now differ)
+    w2 = rng.uniform(0.5, 20.0, size=g.m).astype(np.float32)
+    wt = torch.from_numpy(w2).cuda()
+    dg.set_weights_device(wt.data_ptr())
+    torch.cuda.synchronize()
+    g2 = adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, w2, "f32")
+    for delta in (2.0, 8.0):
+        dist, _ = dg.sssp(src, delta=delta, timeout_ms=TIMEOUT)
+        _check(g2, dist, src, True)
