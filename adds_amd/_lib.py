@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libadds.so")
+# ADDS_LIB selects another build of the library (for example a variant built for comparison).
+LIB_PATH = os.environ.get("ADDS_LIB") or os.path.join(_HERE, "lib", "libadds.so")
 
 # adds_status
 OK, EINVAL, ENOMEM, EOVERFLOW, ETIMEOUT, EHIP, EIO, ENODEV = range(8)
@@ -86,6 +87,10 @@ class Stats(ctypes.Structure):
         ("cas_failures", ctypes.c_uint64),
         ("idle_iterations", ctypes.c_uint64),
         ("late_items", ctypes.c_uint64),
+        ("ticks_probe", ctypes.c_uint64),
+        ("ticks_items", ctypes.c_uint64),
+        ("ticks_edges", ctypes.c_uint64),
+        ("ticks_idle", ctypes.c_uint64),
     ]
 
     def as_dict(self):

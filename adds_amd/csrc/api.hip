@@ -33,10 +33,14 @@ static thread_local std::string g_detail;
     }                                                                                       \
   } while (0)
 
+constexpr uint32_t TRACE_CAP = 1u << 16;
+
 struct Workspace {
   uint32_t* st = nullptr;
   uint64_t* pool = nullptr;
   uint32_t* tab = nullptr;
+  uint32_t* hrec = nullptr;  // edge-range records
+  uint32_t hrec_cap = 0;
   uint32_t* srcs = nullptr;
   uint32_t nq = 0, nb = 0, maxv = 0, pool_chunks = 0, srcs_cap = 0;
   uint32_t* host_state = nullptr;       // pinned copy of the first 8 state lines
@@ -48,6 +52,8 @@ struct Workspace {
   bool kernel_events = false;           // evk0/evk1 bracket the last solve's persistent kernel
   bool pool_fixed = false;              // the last solve's pool size was set by the caller
   uint32_t pool_shift = 0;              // default pool sizes are doubled this often (grown after overflows)
+  uint64_t* trace = nullptr;            // bucket timeline (ADDS_TRACE=<file>), TRACE_CAP records
+  uint64_t trace_t0 = 0;
   double timeout_ms = 0;
 };
 
@@ -121,7 +127,9 @@ static adds_status free_workspace(Workspace& ws) {
   if (ws.st) (void)hipFree(ws.st);
   if (ws.pool) (void)hipFree(ws.pool);
   if (ws.tab) (void)hipFree(ws.tab);
+  if (ws.hrec) (void)hipFree(ws.hrec);
   if (ws.srcs) (void)hipFree(ws.srcs);
+  if (ws.trace) (void)hipFree(ws.trace);
   if (ws.host_state) (void)hipHostFree(ws.host_state);
   if (ws.host_abort) (void)hipHostFree(ws.host_abort);
   if (ws.ev0) (void)hipEventDestroy(ws.ev0);
@@ -334,8 +342,8 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.chunk = p->chunk ? p->chunk : 64;
   r.nb = p->nbuckets ? p->nbuckets : 64;
   r.parts = p->parts ? p->parts : 8;
-  r.hub_degree = p->hub_degree ? p->hub_degree : 512;
-  r.task_edges = p->task_edges ? p->task_edges : 1024;
+  r.hub_degree = p->hub_degree ? p->hub_degree : 1024;
+  r.task_edges = p->task_edges ? p->task_edges : 512;
   r.concur = p->concurrency ? p->concurrency : 2;
   r.flags = p->flags;
   r.timeout_ms = p->timeout_ms ? (double)p->timeout_ms : 120000.0;
@@ -375,13 +383,18 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
     if (ws.st) (void)hipFree(ws.st);
     if (ws.pool) (void)hipFree(ws.pool);
     if (ws.tab) (void)hipFree(ws.tab);
+    if (ws.hrec) (void)hipFree(ws.hrec);
     ws.st = nullptr;
     ws.pool = nullptr;
     ws.tab = nullptr;
+    ws.hrec = nullptr;
     const uint32_t pc = std::max(chunks, ws.pool_chunks);
     HIPCHK(hipMalloc(&ws.st, (size_t)state_words(nq, r.nb) * 4));
     HIPCHK(hipMalloc(&ws.pool, (size_t)pc * CH * 8));
     HIPCHK(hipMalloc(&ws.tab, (size_t)2 * nq * pc * 4));
+    // one range record per 16 pool entries (a record is only written for a range of > hub_degree edges)
+    ws.hrec_cap = (uint32_t)std::min<uint64_t>((uint64_t)pc * CH / 16, REF_MAX_RECORDS);
+    HIPCHK(hipMalloc(&ws.hrec, (size_t)ws.hrec_cap * HREC_WORDS * 4));
     HIPCHK(launch_fill_u32(ws.st, state_words(nq, r.nb), 0, nullptr));
     HIPCHK(launch_fill_u64(ws.pool, (uint64_t)pc * CH, EMPTY64, nullptr));
     HIPCHK(launch_fill_u32(ws.tab, (uint64_t)2 * nq * pc, TAB_UNSET, nullptr));
@@ -431,6 +444,8 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.st = ws.st;
   a.pool = ws.pool;
   a.tab = ws.tab;
+  a.hrec = ws.hrec;
+  a.hrec_cap = ws.hrec_cap;
   a.host_abort = ws.host_abort;
   a.n = g->n;
   a.total_n = (uint32_t)total_n;
@@ -449,6 +464,8 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   if (rate_khz <= 0) rate_khz = 100000;
   a.timeout_ticks = (uint64_t)(r.timeout_ms * (double)rate_khz);
   a.cached_probe = (r.flags & ADDS_F_ATOMIC_PROBE) ? 0u : 1u;
+  a.trace = ws.trace;
+  a.trace_cap = TRACE_CAP;
   return a;
 }
 
@@ -463,7 +480,9 @@ static adds_status enqueue_solve(adds_graph* g, const Resolved& r, uint32_t* dis
                                  const uint32_t* srcs_dev, uint32_t k, bool replicated, hipStream_t s,
                                  bool kernel_events) {
   const uint32_t grid = grid_for(r);
+  if (!g->ws.trace && getenv("ADDS_TRACE")) HIPCHK(hipMalloc(&g->ws.trace, (4 + 4 * (size_t)TRACE_CAP) * 8));
   KArgs a = make_args(g, r, dist, total_n, grid);
+  if (a.trace) HIPCHK(launch_fill_u64(a.trace, 1, 0, s));
   HIPCHK(launch_init(a, srcs_dev, k, replicated ? 1 : 0, inf_bits(g->wt), s));
   if (kernel_events) HIPCHK(hipEventRecord(g->ws.evk0, s));
   HIPCHK(launch_solve(a, g->wt == ADDS_W_F32, replicated, grid, s));
@@ -495,6 +514,33 @@ static adds_status wait_solve(Workspace& ws) {
   return ADDS_OK;
 }
 
+// Appends the last solve's bucket timeline to the file named by ADDS_TRACE: one line per record,
+// "t_us kind bucket prev_head pool_chunks" (kind 0 = start, 1 = heavy phase opened, 2 = head moved,
+// 3 = done), after a
+// header line per solve.  A diagnostic aid; solves are unaffected apart from the records' writes.
+static void dump_trace(adds_graph* g) {
+  Workspace& ws = g->ws;
+  const char* path = getenv("ADDS_TRACE");
+  if (!path) return;
+  uint64_t cnt = 0;
+  if (hipMemcpy(&cnt, ws.trace, 8, hipMemcpyDeviceToHost) != hipSuccess) return;
+  cnt = std::min<uint64_t>(cnt, TRACE_CAP);
+  std::vector<uint64_t> rec(4 * cnt);
+  if (cnt && hipMemcpy(rec.data(), ws.trace + 4, cnt * 32, hipMemcpyDeviceToHost) != hipSuccess) return;
+  FILE* f = fopen(path, "a");
+  if (!f) return;
+  int rate_khz = 100000;
+  (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->device);
+  const double us_per_tick = 1000.0 / (rate_khz > 0 ? rate_khz : 100000);
+  fprintf(f, "# solve n=%u records=%llu\n", g->n, (unsigned long long)cnt);
+  const uint64_t t0 = cnt ? rec[0] : 0;
+  for (uint64_t i = 0; i < cnt; i++)
+    fprintf(f, "%.2f %llu %llu %llu %llu\n", (double)(rec[4 * i] - t0) * us_per_tick,
+            (unsigned long long)(rec[4 * i + 1] >> 56), (unsigned long long)(rec[4 * i + 1] & 0xFFFFFFFFull),
+            (unsigned long long)rec[4 * i + 2], (unsigned long long)rec[4 * i + 3]);
+  fclose(f);
+}
+
 static adds_status collect(adds_graph* g, adds_stats* stats) {
   Workspace& ws = g->ws;
   adds_status rc = wait_solve(ws);
@@ -522,17 +568,18 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->cas_failures = sv[ST_CASFAIL];
     stats->idle_iterations = sv[ST_IDLE];
     stats->late_items = sv[ST_LATE];
+    stats->ticks_probe = sv[ST_T_PROBE];
+    stats->ticks_items = sv[ST_T_ITEMS];
+    stats->ticks_edges = sv[ST_T_EDGES];
+    stats->ticks_idle = sv[ST_T_IDLE];
   }
+  if (ws.trace) dump_trace(g);
   if (err == DERR_OVERFLOW) {
     if (!ws.pool_fixed && ws.pool_shift < 6) ws.pool_shift++;
     g_detail = "worklist pool exhausted (" + std::to_string(hs[S_POOL]) + " chunks requested)";
     return ADDS_EOVERFLOW;
   }
   if (err == DERR_TIMEOUT || err == DERR_ABORT) return ADDS_ETIMEOUT;
-  if (err == DERR_PROTOCOL) {
-    g_detail = "hub slot protocol violated";
-    return ADDS_EHIP;
-  }
   if (err != 0) return ADDS_EHIP;
   if (hs[S_DONE] != 1) {
     g_detail = "solver exited without completing";

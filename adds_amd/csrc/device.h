@@ -13,8 +13,17 @@ constexpr uint32_t TAB_UNSET = 0xFFFFFFFFu;
 constexpr uint32_t TAB_BAD = 0xFFFFFFFEu;
 constexpr uint32_t WG_THREADS = 256;  // 4 wavefronts of 64
 constexpr uint32_t WAVES_PER_WG = WG_THREADS / 64;
-constexpr uint32_t HUB_SLOTS = 16;    // concurrently published high-degree vertices
-constexpr uint32_t MAX_PROBE_QUEUES = 48;  // (concurrency + 1) * parts must fit the probe lanes
+constexpr uint32_t MAX_PROBE_QUEUES = 64;  // (concurrency + 3) * parts queues are probed, one per lane
+// Edge-range references: a vertex whose edge range (light or heavy part) is longer than hub_degree is
+// expanded as range records + queue entries that each name one chunk of the range.  A reference entry
+// has REF_TAG in its distance word (no pushed distance is ever 0xFFFFFFFF) and (record << REF_CHUNK_BITS
+// | chunk) in its vertex word.  Record r: hrec[8r .. 8r+4] = vertex-instance, distance, first edge, edges
+// per chunk, end edge.
+constexpr uint32_t REF_TAG = 0xFFFFFFFFu;
+constexpr uint32_t REF_CHUNK_BITS = 9;
+constexpr uint32_t REF_MAX_CHUNKS = 1u << REF_CHUNK_BITS;
+constexpr uint32_t REF_MAX_RECORDS = (1u << (32 - REF_CHUNK_BITS)) - 1;  // keeps every entry != EMPTY64
+constexpr uint32_t HREC_WORDS = 8;
 
 // ---- state words (uint32 indices); every hot counter owns a 128-byte line (32 words) ----
 constexpr uint32_t LINE = 32;
@@ -24,32 +33,29 @@ constexpr uint32_t S_DONE = 2 * LINE;       // [0] done flag, [1] error code
 constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
 constexpr uint32_t S_STATS = 4 * LINE;      // u64 stats[NSTATS] (two lines)
 constexpr uint32_t S_ADV = 6 * LINE;        // try-lock serialising head-advance scans
-// Hub slot i: cursor line at S_HUB + 2*i*LINE: [0..1] u64 cursor (gen << 48 | total << 28 | next),
-// [2] finished chunks, [3] free flag; fields line at +LINE: [0] vertex-instance, [1] dist, [2] first
-// edge, [3] edges per chunk, [4] end edge, [5] generation.
-constexpr uint32_t S_HUB = 7 * LINE;
+constexpr uint32_t S_HREC = 7 * LINE;       // next free edge-range record
 // Queues: nq = nb * parts light queues (index slot * parts + part) followed by nq heavy queues (index
 // nq + slot * parts + part).  Queue q: head (claimed) at S_QBASE + 3q*LINE, tail (reserved) at +LINE,
 // published-unclaimed count at +2*LINE.  Then two lines per bucket slot: in-flight counts (words
 // [0, 16) light claims per part, [16, 32) heavy claims per part) and the slot's phase word (1 = the
 // heavy queues of the head bucket are open).
-constexpr uint32_t S_QBASE = S_HUB + 2 * HUB_SLOTS * LINE;
+constexpr uint32_t S_QBASE = 8 * LINE;
 constexpr uint32_t QLINES = 3;
 constexpr uint32_t s_slotlines(uint32_t nq) { return S_QBASE + QLINES * 2 * nq * LINE; }
 constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_slotlines(nq) + 2 * nb * LINE; }
 constexpr uint32_t MAX_PARTS = 16;
-// The next-chunk field is wide enough that over-claims (at most one per probing wavefront per stale
-// view) never reach the chunk-count field.
-constexpr uint32_t HUB_NEXT_BITS = 28;
-constexpr uint32_t HUB_TOTAL_BITS = 20;
-constexpr uint32_t HUB_MAX_CHUNKS = 1u << 19;
 
 enum StatIdx {
   ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET,
-  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_HUBMISS, ST_LATE, NSTATS
+  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_LATE,
+  // wavefront time by activity (wall-clock ticks): probing/claiming, item handling, edge windows,
+  // idle (termination test, head advance, back-off)
+  ST_T_PROBE, ST_T_ITEMS, ST_T_EDGES, ST_T_IDLE, NSTATS
 };
 
-enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3, DERR_PROTOCOL = 4 };
+static_assert(NSTATS <= 32, "the stats block is two state lines of u64");
+
+enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3 };
 
 struct KArgs {
   const uint32_t* row_ptr;
@@ -60,6 +66,7 @@ struct KArgs {
   uint32_t* st;       // state words
   uint64_t* pool;     // pool_chunks * CH entries: (dist << 32) | vertex-instance id
   uint32_t* tab;      // per queue: maxv virtual-chunk -> physical-chunk map
+  uint32_t* hrec;     // edge-range records, hrec_cap of them
   const uint32_t* host_abort;  // host-visible abort word (may be null)
   uint32_t n;         // vertices of the graph (per replica)
   uint32_t total_n;   // replicas * n
@@ -69,12 +76,18 @@ struct KArgs {
   uint32_t parts;     // sub-queues per bucket
   float inv_delta;    // 1 / bucket width
   uint32_t chunk;     // items claimed per wavefront
-  uint32_t hub_degree;
-  uint32_t task_edges;
+  uint32_t hrec_cap;
+  uint32_t hub_degree;  // ranges longer than this are expanded through range references
+  uint32_t task_edges;  // edges per reference chunk (at least)
   uint32_t concur;
   uint32_t total_waves;
   uint64_t timeout_ticks;  // wall_clock64 ticks
   uint32_t cached_probe;   // probe queue counters with L2-cached loads (claims stay atomic)
+  // Optional bucket timeline (null = off): trace[0] = record count, then records of 4 words from
+  // trace + 4: wall clock, (kind << 56 | bucket), previous head, pool chunks used.  Kinds: 1 = the
+  // head bucket's heavy phase opened, 2 = the head moved to `bucket`, 0 = start, 3 = done.
+  uint64_t* trace;
+  uint32_t trace_cap;  // records
 };
 
 // Host launchers (sssp_kernels.hip)

@@ -31,17 +31,17 @@
 //    with the carried distance, which is a real path length, so correctness never depends on when the
 //    consumer observes the producer's atomicMin.  Pushes are clamped to the window of the bucket being
 //    processed, so no hint of the global head is needed on the hot path.
-//  * High-degree edge ranges (the reference's TB-cooperative path, wl.h:1608-1696) are published in one
-//    of HUB_SLOTS hub slots whose 64-bit cursor packs (generation, chunk count, next chunk).  Any
-//    wavefront claims the next `task_edges`-edge chunk with one fetch_add -- no retry loop, an
-//    over-claim past the end is simply void -- and the wavefront that finishes the last chunk frees the
-//    slot and releases the source bucket's in-flight count.
+//  * Long edge ranges (the reference's TB-cooperative path for high-degree vertices, wl.h:1608-1696)
+//    are not relaxed by the wavefront that meets them: it writes a range record and pushes one
+//    reference item per `task_edges`-edge chunk into its own queue, so the chunks spread over every
+//    wavefront through the ordinary claims, phases and in-flight accounting of the bucket.
 //  * Edges of the claimed vertices are flattened with a wavefront prefix sum and processed in windows
 //    of 64 (four in flight per lane); each lane finds its owning vertex by binary search in LDS
-//    (replacing CUB's WarpScan and the byte owner table, kernel.cu:89-102).
+//    (replacing CUB's WarpScan and the byte owner table, kernel.cu:89-102).  Improvements are
+//    collected per wavefront in LDS and written to the queues in one flush per claim.
 //  * Termination: a 64-bit word counts (busy transitions << 32 | idle wavefronts).  A wavefront goes
 //    busy before it claims and idle after its pushes have completed.  The solve is over when a
-//    wavefront reads "all idle", finds every queue and hub slot empty, and re-reads the very same word:
+//    wavefront reads "all idle", finds every queue empty, and re-reads the very same word:
 //    no claim can have happened in between, and idle wavefronts do not push.
 //
 // Memory model (gfx950, one agent, 8 XCDs with private L2s): every word that decides ownership is
@@ -49,7 +49,7 @@
 // loads/stores carry sc1).  Ordering between different words is obtained by waiting for completion of
 // the earlier access (an explicit s_waitcnt with all counters at zero, mem_wait below) before issuing
 // the later one.  Plain (L2-cached) loads are used for the CSR, the d[v] pre-filter (a stale, larger
-// value only costs an extra atomic) and, as hints only, for probing queue and hub counters; every
+// value only costs an extra atomic) and, as hints only, for probing queue counters; every
 // eighth idle probe and every termination test use atomic loads.
 #include "device.h"
 
@@ -142,19 +142,6 @@ __device__ __forceinline__ uint32_t* inflight(const KArgs& a, uint32_t slot) {
 __device__ __forceinline__ uint32_t* phase(const KArgs& a, uint32_t slot) {
   return a.st + s_slotlines(nq_of(a)) + 2 * slot * LINE + LINE;
 }
-__device__ __forceinline__ uint64_t* hcursor(const KArgs& a, uint32_t i) {
-  return (uint64_t*)(a.st + S_HUB + 2 * i * LINE);
-}
-__device__ __forceinline__ uint32_t* hdone(const KArgs& a, uint32_t i) { return a.st + S_HUB + 2 * i * LINE + 2; }
-__device__ __forceinline__ uint32_t* hfree(const KArgs& a, uint32_t i) { return a.st + S_HUB + 2 * i * LINE + 3; }
-__device__ __forceinline__ uint32_t* hfield(const KArgs& a, uint32_t i) { return a.st + S_HUB + 2 * i * LINE + LINE; }
-
-__device__ __forceinline__ uint32_t cur_next(uint64_t c) { return (uint32_t)c & ((1u << HUB_NEXT_BITS) - 1); }
-__device__ __forceinline__ uint32_t cur_total(uint64_t c) {
-  return (uint32_t)(c >> HUB_NEXT_BITS) & ((1u << HUB_TOTAL_BITS) - 1);
-}
-__device__ __forceinline__ uint32_t cur_gen(uint64_t c) { return (uint32_t)(c >> (HUB_NEXT_BITS + HUB_TOTAL_BITS)); }
-
 __device__ __forceinline__ void set_error(const KArgs& a, uint32_t code) {
   uint32_t z = 0;
   acas(a.st + S_DONE + 1, z, code);
@@ -232,114 +219,144 @@ __device__ __forceinline__ void alloc_chunk(const KArgs& a, uint32_t* tp) {
   ast(tp, ph);
 }
 
-// Lane 0 reserves `cnt` (<= CH) positions of queue `lq` and resolves the (at most two) pool chunks they
-// fall in; the result is broadcast.  ph0/ph1 are TAB_BAD after an overflow.
-__device__ __forceinline__ void reserve_positions(const KArgs& a, uint32_t lq, uint32_t cnt, uint32_t& base,
-                                                  uint32_t& ph0, uint32_t& ph1) {
-  uint32_t b = 0, p0 = TAB_BAD, p1 = TAB_BAD;
-  if (lane_id() == 0) {
-    b = afadd(qtail(a, lq), cnt);
-    const uint32_t j0 = b >> CH_SHIFT, j1 = (b + cnt - 1) >> CH_SHIFT;
-    if (j1 < a.maxv) {
-      uint32_t* t0 = a.tab + (size_t)lq * a.maxv + j0;
-      uint32_t* t1 = a.tab + (size_t)lq * a.maxv + j1;
-      if ((b & (CH - 1)) == 0) alloc_chunk(a, t0);
-      if (j1 != j0) alloc_chunk(a, t1);
-      p0 = wait_tab(a, t0);
-      p1 = (j1 != j0) ? wait_tab(a, t1) : p0;
-    } else {
-      set_error(a, DERR_OVERFLOW);
-    }
-  }
-  base = rdlane(b, 0);
-  ph0 = rdlane(p0, 0);
-  ph1 = rdlane(p1, 0);
+// ---- per-wavefront push buffer ----
+// Improvements found while relaxing a claim are collected in LDS (entry + target queue) and written
+// to the queues in one flush at the end of the claim, or earlier when the buffer is full.  A flush
+// groups the entries by queue, reserves every group's positions at once (one lane per group, so the
+// reservations' round trips overlap), writes the entries, waits for their completion and then
+// publishes every group.
+constexpr uint32_t PB_CAP = 256;  // entries per wavefront: 4 per lane in one flush round
+
+struct PushBuf {
+  uint64_t* e;  // LDS: entries (dist << 32 | vertex-instance)
+  uint32_t* q;  // LDS: target queue
+  uint32_t n;   // wave-uniform fill
+};
+
+// Target queue of a light-queue push: the bucket of nd, clamped to the window of the bucket being
+// processed (cb): never before it (such items are simply early work) and never so far ahead that the
+// ring slot would alias the behind bucket.
+template <bool FLT>
+__device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uint32_t part, uint32_t cb) {
+  uint32_t b = bucket_key<FLT>(nd, a.inv_delta);
+  b = max(b, cb);
+  b = min(b, cb + a.nb - 3);
+  return (b & (a.nb - 1)) * a.parts + part;
 }
 
-// Pushes (gid[j], nd[j]) for every lane and window j with f[j] set: one tail reservation per distinct
-// target queue across all U windows, entry stores, one completion wait, then one publish per queue.
-// HEAVY: every entry goes to the heavy queue of bucket cb; otherwise to the light queue of its bucket,
-// clamped to the window of the bucket being processed (cb): never before it (such items are simply
-// early work) and never so far ahead that the ring slot would alias the behind bucket.
-template <bool FLT, bool HEAVY, int U>
-__device__ __forceinline__ void push_multi(const KArgs& a, const bool (&f)[U], const uint32_t (&gid)[U],
-                                           const uint32_t (&nd)[U], uint32_t part, uint32_t cb, WaveStats& ws) {
+__device__ void pb_flush(const KArgs& a, PushBuf& pb, WaveStats& ws) {
+  if (pb.n == 0) return;
+  wave_sync();
   const uint32_t lane = lane_id();
-  uint64_t any = 0;
-  uint32_t q[U];
-  bool rem[U];
+  uint64_t e[4];
+  uint32_t q[4], g[4], off[4];
+  bool rem[4];
 #pragma unroll
-  for (int j = 0; j < U; j++) {
-    const uint64_t m = ballot(f[j]);
-    any |= m;
-    if (lane == 0) ws.v[ST_PUSH] += __popcll(m);
-    if (HEAVY) {
-      q[j] = nq_of(a) + (cb & (a.nb - 1)) * a.parts + part;
-    } else {
-      uint32_t b = bucket_key<FLT>(nd[j], a.inv_delta);
-      b = max(b, cb);
-      b = min(b, cb + a.nb - 3);
-      q[j] = (b & (a.nb - 1)) * a.parts + part;
-    }
-    rem[j] = f[j];
+  for (int j = 0; j < 4; j++) {
+    const uint32_t i = j * 64 + lane;
+    rem[j] = i < pb.n;
+    e[j] = rem[j] ? pb.e[i] : 0ull;
+    q[j] = rem[j] ? pb.q[i] : 0u;
+    g[j] = 0;
+    off[j] = 0;
   }
-  if (any == 0) return;
-  uint32_t pubq = 0, pub = 0;  // group g's queue and count live in lane g
-  uint32_t groups = 0;
+  if (lane == 0) ws.v[ST_PUSH] += pb.n;
   for (;;) {
-    uint32_t lq = 0xFFFFFFFFu;
+    // group the remaining entries by queue, at most 64 groups per pass (group k lives in lane k)
+    bool pend[4];
+    uint32_t gq = 0, gc = 0, ng = 0;
 #pragma unroll
-    for (int j = 0; j < U; j++) {
-      const uint64_t m = ballot(rem[j]);
-      if (lq == 0xFFFFFFFFu && m) lq = rdlane(q[j], ffs64(m));
-    }
-    if (lq == 0xFFFFFFFFu) break;
-    if (groups == 64) {  // publish what is pending before reusing the lanes
-      mem_wait();
-      if (pub) afadd(qavail(a, pubq), pub);
-      pub = 0;
-      groups = 0;
-    }
-    uint64_t pm[U];
-    uint32_t cnt = 0;
+    for (int j = 0; j < 4; j++) pend[j] = false;
+    while (ng < 64) {
+      uint32_t lq = 0xFFFFFFFFu;
 #pragma unroll
-    for (int j = 0; j < U; j++) {
-      pm[j] = ballot(rem[j] && q[j] == lq);
-      cnt += (uint32_t)__popcll(pm[j]);
-    }
-    uint32_t base, ph0, ph1;
-    reserve_positions(a, lq, cnt, base, ph0, ph1);
-    const uint32_t j0 = base >> CH_SHIFT;
-    uint32_t off = base;
-#pragma unroll
-    for (int j = 0; j < U; j++) {
-      if (rem[j] && q[j] == lq) {
-        const uint32_t pos = off + mbcnt(pm[j]);
-        const uint32_t ph = (pos >> CH_SHIFT) == j0 ? ph0 : ph1;
-        if (ph != TAB_BAD) ast64(a.pool + ((size_t)ph << CH_SHIFT) + (pos & (CH - 1)), ((uint64_t)nd[j] << 32) | gid[j]);
-        rem[j] = false;
+      for (int j = 0; j < 4; j++) {
+        const uint64_t m = ballot(rem[j]);
+        if (lq == 0xFFFFFFFFu && m) lq = rdlane(q[j], ffs64(m));
       }
-      off += (uint32_t)__popcll(pm[j]);
+      if (lq == 0xFFFFFFFFu) break;
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const bool mine = rem[j] && q[j] == lq;
+        const uint64_t m = ballot(mine);
+        if (mine) {
+          g[j] = ng;
+          off[j] = cnt + mbcnt(m);
+          rem[j] = false;
+          pend[j] = true;
+        }
+        cnt += (uint32_t)__popcll(m);
+      }
+      if (lane == ng) {
+        gq = lq;
+        gc = cnt;
+      }
+      ng++;
     }
-    if (lane == groups) {
-      pubq = lq;
-      pub = cnt;
+    if (ng == 0) break;
+    // reserve every group's positions at once and resolve their (at most two) pool chunks
+    uint32_t base = 0, p0 = TAB_BAD, p1 = TAB_BAD;
+    if (lane < ng) {
+      base = afadd(qtail(a, gq), gc);
+      const uint32_t j0 = base >> CH_SHIFT, j1 = (base + gc - 1) >> CH_SHIFT;
+      if (j1 < a.maxv) {
+        uint32_t* t0 = a.tab + (size_t)gq * a.maxv + j0;
+        uint32_t* t1 = a.tab + (size_t)gq * a.maxv + j1;
+        if ((base & (CH - 1)) == 0) alloc_chunk(a, t0);
+        if (j1 != j0) alloc_chunk(a, t1);
+        p0 = wait_tab(a, t0);
+        p1 = (j1 != j0) ? wait_tab(a, t1) : p0;
+      } else {
+        set_error(a, DERR_OVERFLOW);
+      }
     }
-    groups++;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t gb = __shfl(base, g[j], 64);
+      const uint32_t g0 = __shfl(p0, g[j], 64);
+      const uint32_t g1 = __shfl(p1, g[j], 64);
+      if (pend[j]) {
+        const uint32_t pos = gb + off[j];
+        const uint32_t ph = (pos >> CH_SHIFT) == (gb >> CH_SHIFT) ? g0 : g1;
+        if (ph != TAB_BAD) ast64(a.pool + ((size_t)ph << CH_SHIFT) + (pos & (CH - 1)), e[j]);
+      }
+    }
+    // publish: the entries have completed before consumers may count them as claimable
+    mem_wait();
+    if (lane < ng) afadd(qavail(a, gq), gc);
   }
-  // publish: the entries have completed before consumers may count them as claimable
-  mem_wait();
-  if (pub) afadd(qavail(a, pubq), pub);
+  pb.n = 0;
+  wave_sync();
+}
+
+// Appends (entry, queue) for every lane with f set; flushes first when the buffer would overflow.
+__device__ __forceinline__ void pb_add(const KArgs& a, PushBuf& pb, bool f, uint64_t entry, uint32_t q, WaveStats& ws) {
+  const uint64_t m = ballot(f);
+  if (m == 0) return;
+  const uint32_t c = (uint32_t)__popcll(m);
+  if (pb.n + c > PB_CAP) pb_flush(a, pb, ws);
+  if (f) {
+    const uint32_t i = pb.n + mbcnt(m);
+    pb.e[i] = entry;
+    pb.q[i] = q;
+  }
+  pb.n += c;
 }
 
 // Relaxes UNROLL independent edges per lane (one 64-edge window each) so that UNROLL dependent chains
-// (col/w load -> d[v] load -> atomicMin) are in flight per wavefront, then pushes the improvements.
-constexpr int UNROLL = 4;
+// (col/w load -> d[v] load -> atomicMin) are in flight per wavefront.  The atomicMin's result is not
+// waited for: an improvement over the pre-filter value is pushed, and an entry that another wavefront
+// beat is dropped as stale when it is popped.
+#ifndef ADDS_UNROLL
+#define ADDS_UNROLL 4
+#endif
+constexpr int UNROLL = ADDS_UNROLL;
 
 template <bool FLT>
 __device__ __forceinline__ void relax_group(const KArgs& a, const bool (&act)[UNROLL], const uint32_t (&e)[UNROLL],
                                             const uint32_t (&du)[UNROLL], const uint32_t (&gb)[UNROLL], uint32_t part,
-                                            uint32_t cb, WaveStats& ws) {
+                                            uint32_t cb, PushBuf& pb, WaveStats& ws) {
   uint32_t tgt[UNROLL], nd[UNROLL], dv[UNROLL];
   bool f[UNROLL];
 #pragma unroll
@@ -357,109 +374,47 @@ __device__ __forceinline__ void relax_group(const KArgs& a, const bool (&act)[UN
   for (int j = 0; j < UNROLL; j++) {
     f[j] = act[j] && nd[j] < dv[j];
     ws.v[ST_EDGES] += act[j] ? 1u : 0u;
-  }
-  uint32_t old[UNROLL];
-#pragma unroll
-  for (int j = 0; j < UNROLL; j++) {
-    old[j] = 0xFFFFFFFFu;
     if (f[j]) {
       ws.v[ST_RELAX]++;
-      old[j] = afmin(a.dist + tgt[j], nd[j]);
+      (void)afmin(a.dist + tgt[j], nd[j]);
     }
   }
-  bool pf[UNROLL];
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++) pf[j] = f[j] && old[j] > nd[j];
-  push_multi<FLT, false, UNROLL>(a, pf, tgt, nd, part, cb, ws);
+  for (int j = 0; j < UNROLL; j++)
+    pb_add(a, pb, f[j], ((uint64_t)nd[j] << 32) | tgt[j], light_queue<FLT>(a, nd[j], part, cb), ws);
 }
 
-// Relaxes the contiguous edge range [eb, ee) of one vertex (distance du, replica base gbase).
-template <bool FLT>
-__device__ __forceinline__ void relax_range(const KArgs& a, uint32_t eb, uint32_t ee, uint32_t du, uint32_t gbase,
-                                            uint32_t part, uint32_t cb, WaveStats& ws) {
-  const uint32_t lane = lane_id();
-  for (uint32_t base = eb; base < ee; base += 64 * UNROLL) {
-    bool act[UNROLL];
-    uint32_t e[UNROLL], dus[UNROLL], gbs[UNROLL];
-#pragma unroll
-    for (int j = 0; j < UNROLL; j++) {
-      e[j] = base + 64 * j + lane;
-      act[j] = e[j] < ee;
-      dus[j] = du;
-      gbs[j] = gbase;
-    }
-    relax_group<FLT>(a, act, e, dus, gbs, part, cb, ws);
-  }
-}
-
-// Publishes the edge range [rs, rs+deg) of a high-degree vertex in a free hub slot; the slot holds the
-// source bucket's in-flight count (light or heavy, `heavy`) until its last chunk is done.  Returns
-// false when every slot is busy (the caller then relaxes the range itself).
-__device__ __forceinline__ bool publish_hub(const KArgs& a, uint32_t gid, uint32_t du, uint32_t rs, uint32_t deg,
-                                            uint32_t start, uint32_t cb, uint32_t part, bool heavy, WaveStats& ws) {
-  const uint32_t lane = lane_id();
-  bool fr = false;
-  uint32_t slot_l = (start + lane) & (HUB_SLOTS - 1);
-  if (lane < HUB_SLOTS) fr = ald(hfree(a, slot_l)) == 1u;
-  uint64_t fm = ballot(fr);
-  uint32_t got = 0xFFFFFFFFu;
-  while (fm) {
-    const uint32_t L = ffs64(fm);
-    const uint32_t s = rdlane(slot_l, L);
-    uint32_t ok = 0;
-    if (lane == 0) {
-      uint32_t one = 1u;
-      ok = acas(hfree(a, s), one, 0u) ? 1u : 0u;
-    }
-    if (rdlane(ok, 0)) {
-      got = s;
-      break;
-    }
-    fm &= fm - 1;
-  }
-  if (got == 0xFFFFFFFFu) return false;
-  uint32_t te = a.task_edges;
-  if ((deg + te - 1) / te >= HUB_MAX_CHUNKS) te = (deg + HUB_MAX_CHUNKS - 2) / (HUB_MAX_CHUNKS - 1);
-  const uint32_t total = (deg + te - 1) / te;
-  if (lane == 0) {
-    uint32_t* f = hfield(a, got);
-    const uint32_t gen = (ald(f + 5) + 1) & 0xFFFFu;
-    const uint32_t icounter = part + (heavy ? MAX_PARTS : 0u);
-    ast(f + 0, gid);
-    ast(f + 1, du);
-    ast(f + 2, rs);
-    ast(f + 3, te);
-    ast(f + 4, rs + deg);
-    ast(f + 5, gen);
-    ast(f + 6, cb);        // source bucket: chunks clamp their pushes to it
-    ast(f + 7, icounter);  // which in-flight counter of the source bucket the slot holds
-    afadd(inflight(a, cb & (a.nb - 1)) + icounter, 1u);
-    ast(hdone(a, got), 0u);
-    mem_wait();  // the fields are visible before the cursor opens the slot
-    ast64(hcursor(a, got), ((uint64_t)gen << (HUB_NEXT_BITS + HUB_TOTAL_BITS)) | ((uint64_t)total << HUB_NEXT_BITS));
-    ws.v[ST_TASKS] += total;
-  }
-  return true;
+// Appends a bucket-timeline record when tracing is on (called by one lane).
+__device__ __forceinline__ void trace_event(const KArgs& a, uint64_t kind, uint32_t bucket, uint32_t prev) {
+  if (!a.trace) return;
+  const uint64_t i = afadd64(a.trace, 1ull);
+  if (i >= a.trace_cap) return;
+  uint64_t* r = a.trace + 4 + 4 * i;
+  r[0] = wall_clock64();
+  r[1] = (kind << 56) | bucket;
+  r[2] = prev;
+  r[3] = ald(a.st + S_POOL);
 }
 
 // ------------------------------------------------------------------------------------------------
 // the persistent solver
 // ------------------------------------------------------------------------------------------------
-enum : uint32_t { W_NONE = 0, W_HUB = 1, W_ITEMS = 2 };
-
 template <bool FLT, bool REPL>
 __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   __shared__ uint32_t s_pref[WAVES_PER_WG][65];
   __shared__ uint32_t s_rs[WAVES_PER_WG][64];
   __shared__ uint32_t s_du[WAVES_PER_WG][64];
   __shared__ uint32_t s_gb[WAVES_PER_WG][64];
+  __shared__ uint64_t s_pbe[WAVES_PER_WG][PB_CAP];
+  __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
 
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
+  PushBuf pbuf{s_pbe[wv], s_pbq[wv], 0u};
   const uint32_t part = blockIdx.x % a.parts;
   const uint32_t wave_gid = blockIdx.x * WAVES_PER_WG + wv;
   const uint32_t nq = nq_of(a);
-  const uint32_t nlight = (a.concur + 1) * a.parts;  // light probe lanes: behind, head, concur-1 ahead
+  const uint32_t nprobe = (a.concur + 3) * a.parts;  // probed queues, one per lane
   WaveStats ws;
 #pragma unroll
   for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
@@ -467,26 +422,23 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   const uint64_t t_start = wall_clock64();
   uint32_t idle_iter = 0, work_iter = 0;
   bool missed = false;  // the last claim attempt lost: probe with atomic loads next time
+  uint64_t tk[4] = {0, 0, 0, 0};  // activity ticks: probe, items, edges, idle (wave-uniform)
+  if (wave_gid == 0 && lane == 0) trace_event(a, 0, 0, 0);
 
   for (;;) {
-    // ---------------- probe hub slots and open queues in parallel ----------------
-    // lanes [0, 16): hub slots (rotated per wavefront).  Then, `parts` lanes each: light behind, light
-    // head, heavy behind, heavy head (only once the head's phase word is set), light of the concur-1
-    // buckets ahead.  Bit order is the claim priority.
+    const uint64_t t0 = wall_clock64();
+    // ---------------- probe the open queues in parallel ----------------
+    // `parts` lanes each: light behind, light head, heavy behind, heavy head (only once the head's
+    // phase word is set), light of the concur-1 buckets ahead.  Bit order is the claim priority; each
+    // wavefront starts at its own part.
     const bool cached = a.cached_probe && (idle_iter & 7) != 7 && !missed;
     missed = false;
     const uint32_t c = hld(a.st + S_CUR, cached);
     const bool head_heavy_open = hld(phase(a, c & (a.nb - 1)), cached) != 0;
-    uint32_t pq = 0, pb = 0;
-    uint64_t hc = 0;
+    uint32_t pq = 0, pbk = 0;
     bool pv = false;
-    if (lane < HUB_SLOTS) {
-      pq = (wave_gid + lane) & (HUB_SLOTS - 1);
-      hc = hld64(hcursor(a, pq), cached);
-      pv = cur_next(hc) < cur_total(hc);
-    } else {
-      const uint32_t l = lane - HUB_SLOTS;
-      const uint32_t grp = l / a.parts;
+    {
+      const uint32_t grp = lane / a.parts;
       uint32_t j = 0xFFFFFFFFu;  // bucket c-1+j
       bool heavy = false;
       if (grp < 2) {
@@ -495,128 +447,83 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         j = grp - 2;
         heavy = true;
         if (j == 1 && !head_heavy_open) j = 0xFFFFFFFFu;
-      } else if (l < nlight + 2 * a.parts) {
+      } else if (lane < nprobe) {
         j = grp - 2;
       }
       if (j != 0xFFFFFFFFu && (c > 0 || j > 0)) {
-        pb = c - 1 + j;
-        pq = (heavy ? nq : 0u) + (pb & (a.nb - 1)) * a.parts + ((part + l % a.parts) % a.parts);
+        pbk = c - 1 + j;
+        pq = (heavy ? nq : 0u) + (pbk & (a.nb - 1)) * a.parts + ((part + lane % a.parts) % a.parts);
         pv = (int32_t)hld(qavail(a, pq), cached) > 0;
       }
     }
-    uint64_t pm = ballot(pv);
+    const uint64_t pm = ballot(pv);
     if (lane == 0) ws.v[ST_PROBE]++;
-    uint32_t kind = W_NONE, q = 0, h = 0, cnt = 0, cb = 0;
-    uint64_t claimed = 0;
+    bool got = false;
+    uint32_t q = 0, h = 0, cnt = 0, cb = 0;
     if (pm) {
       if (!busy) {
         if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 0xFFFFFFFFull);  // +1 transition, -1 idle
         busy = true;
         mem_wait();
       }
-      // hub chunks first: one fetch_add per attempt, at most two slots per probe
-      uint64_t hm = pm & ((1ull << HUB_SLOTS) - 1);
-      for (int tries = 0; tries < 2 && hm && kind == W_NONE; tries++) {
-        const uint32_t L = ffs64(hm);
-        const uint32_t s = rdlane(pq, L);
-        uint64_t old = 0;
-        if (lane == 0) old = afadd64(hcursor(a, s), 1ull);
-        old = rdlane64(old, 0);
-        if (cur_next(old) < cur_total(old)) {
-          kind = W_HUB;
-          q = s;
-          claimed = old;
-        } else {
-          missed = true;
-          if (lane == 0) ws.v[ST_HUBMISS]++;
-        }
-        hm &= hm - 1;
-      }
-      // Queue items: take up to `chunk` from the published count (fetch_sub; a shortfall is given
-      // back), then fetch_add the head by what was taken.  Published items are at positions the
-      // head has not passed, so every claimed position has been reserved by a producer.
-      uint64_t qm = pm >> HUB_SLOTS;
-      for (int tries = 0; tries < 3 && kind == W_NONE && qm; tries++) {
-        const uint32_t L = ffs64(qm) + HUB_SLOTS;
+      // Take up to `chunk` from the published count (fetch_sub; a shortfall is given back), then
+      // fetch_add the head by what was taken.  Published items are at positions the head has not
+      // passed, so every claimed position has been reserved by a producer.
+      uint64_t qm = pm;
+      for (int tries = 0; tries < 3 && !got && qm; tries++) {
+        const uint32_t L = ffs64(qm);
         const uint32_t qq = rdlane(pq, L);
-        const uint32_t bb = rdlane(pb, L);
+        const uint32_t bb = rdlane(pbk, L);
+        uint32_t ok = 0;
         if (lane == 0) {
           const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - a.chunk);
-          const int32_t got = min(max(old, 0), (int32_t)a.chunk);
-          if (got < (int32_t)a.chunk) afadd(qavail(a, qq), (uint32_t)((int32_t)a.chunk - got));
-          if (got > 0) {
-            h = afadd(qhead(a, qq), (uint32_t)got);
+          const int32_t take = min(max(old, 0), (int32_t)a.chunk);
+          if (take < (int32_t)a.chunk) afadd(qavail(a, qq), (uint32_t)((int32_t)a.chunk - take));
+          if (take > 0) {
+            h = afadd(qhead(a, qq), (uint32_t)take);
             const uint32_t ic = (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u);
-            afadd(inflight(a, bb & (a.nb - 1)) + ic, (uint32_t)got);
-            cnt = (uint32_t)got;
-            kind = W_ITEMS;
+            afadd(inflight(a, bb & (a.nb - 1)) + ic, (uint32_t)take);
+            cnt = (uint32_t)take;
+            ok = 1;
           } else {
             ws.v[ST_CASFAIL]++;
           }
         }
-        kind = rdlane(kind, 0);
-        if (kind != W_ITEMS) missed = true;
-        if (kind == W_ITEMS) {
+        if (rdlane(ok, 0)) {
+          got = true;
           q = qq;
           h = rdlane(h, 0);
           cnt = rdlane(cnt, 0);
           cb = bb;
+        } else {
+          missed = true;
         }
         qm &= qm - 1;
       }
     }
+    const uint64_t t1 = wall_clock64();
+    tk[0] += t1 - t0;
 
-    // The watchdog and the host abort word are also polled while busy, so that a solve that never
-    // goes idle can still be ended.
-    if (kind != W_NONE && (++work_iter & 255) == 0) {
-      uint32_t quit = 0;
-      if (lane == 0) {
-        if (wall_clock64() - t_start > a.timeout_ticks) {
-          set_error(a, DERR_TIMEOUT);
-          quit = 1;
-        } else if (a.host_abort &&
-                   __hip_atomic_load(a.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
-          set_error(a, DERR_ABORT);
-          quit = 1;
-        } else if (ald(a.st + S_DONE + 1) != 0) {
-          quit = 1;
-        }
-      }
-      if (rdlane(quit, 0)) break;
-    }
-
-    if (kind == W_HUB) {
+    if (got) {
       idle_iter = 0;
-      // ---------------- one chunk of a published edge range ----------------
-      mem_wait();  // the claim has completed before the fields are read
-      const uint32_t* f = hfield(a, q);
-      const uint32_t gid_u = ald(f + 0), du = ald(f + 1), rs = ald(f + 2), te = ald(f + 3), re = ald(f + 4);
-      const uint32_t hcb = ald(f + 6), hic = ald(f + 7);
-      if (ald(f + 5) != cur_gen(claimed)) {  // a slot cannot be recycled while a chunk is outstanding
-        set_error(a, DERR_PROTOCOL);
-        break;
-      }
-      const uint32_t k = cur_next(claimed);
-      const uint32_t eb = rs + k * te;
-      const uint32_t ee = min(eb + te, re);
-      if (ald(a.dist + gid_u) < du) {
-        if (lane == 0) ws.v[ST_STALE]++;
-      } else {
-        relax_range<FLT>(a, eb, ee, du, REPL ? (gid_u / a.n) * a.n : 0u, part, hcb, ws);
-      }
-      mem_wait();  // this chunk's pushes have completed before it is counted as finished
-      if (lane == 0) {
-        const uint32_t d = afadd(hdone(a, q), 1u);
-        if (d + 1 == cur_total(claimed)) {
-          afadd(inflight(a, hcb & (a.nb - 1)) + hic, 0u - 1u);
-          ast(hfree(a, q), 1u);
+      // The watchdog and the host abort word are also polled while busy, so that a solve that never
+      // goes idle can still be ended.
+      if ((++work_iter & 255) == 0) {
+        uint32_t quit = 0;
+        if (lane == 0) {
+          if (wall_clock64() - t_start > a.timeout_ticks) {
+            set_error(a, DERR_TIMEOUT);
+            quit = 1;
+          } else if (a.host_abort &&
+                     __hip_atomic_load(a.host_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
+            set_error(a, DERR_ABORT);
+            quit = 1;
+          } else if (ald(a.st + S_DONE + 1) != 0) {
+            quit = 1;
+          }
         }
+        if (rdlane(quit, 0)) break;
       }
-      continue;
-    }
-
-    if (kind == W_ITEMS) {
-      idle_iter = 0;
       const bool heavy_claim = q >= nq;
       // ---------------- load the claimed entries ----------------
       bool have = lane < cnt;
@@ -635,13 +542,28 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         du = (uint32_t)(e >> 32);
       }
       if (ballot(lane < cnt && !have)) break;  // error path: leave the solve
-      // stale / early filtering
-      bool ok = false, early = false;
+      // ---------------- resolve each item to an edge range ----------------
+      // A vertex item: stale if the vertex already holds a smaller distance, early if its distance
+      // belongs to a later bucket (moved there), otherwise its light range (light claim, and the vertex
+      // is filed in this bucket's heavy queue) or heavy range (heavy claim).  A reference item: one
+      // chunk of a recorded range, stale if the recorded vertex has moved on.
+      uint32_t vgid = gid, rs = 0, deg = 0, gbase = 0;
+      bool ok = false, early = false, file_heavy = false;
+      const bool is_ref = have && du == REF_TAG;
+      if (is_ref) {
+        const uint32_t* r = a.hrec + (size_t)(gid >> REF_CHUNK_BITS) * HREC_WORDS;
+        const uint32_t k = gid & (REF_MAX_CHUNKS - 1);
+        vgid = ald(r + 0);
+        du = ald(r + 1);
+        const uint32_t r0 = ald(r + 2), te = ald(r + 3), r1 = ald(r + 4);
+        rs = r0 + k * te;
+        deg = min(te, r1 - rs);
+      }
       if (have) {
-        const uint32_t dnow = ald(a.dist + gid);  // coherent: a stale (larger) value would defeat the filter
+        const uint32_t dnow = ald(a.dist + vgid);  // coherent: a stale (larger) value would defeat the filter
         if (dnow < du) {
           ws.v[ST_STALE]++;
-        } else if (!heavy_claim && bucket_key<FLT>(du, a.inv_delta) > cb) {
+        } else if (!is_ref && !heavy_claim && bucket_key<FLT>(du, a.inv_delta) > cb) {
           early = true;
           ws.v[ST_REBUCKET]++;
         } else {
@@ -649,22 +571,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           if (bucket_key<FLT>(du, a.inv_delta) + 1 < cb) ws.v[ST_LATE]++;
         }
       }
-      {
-        const bool ef[1] = {early};
-        const uint32_t eg[1] = {gid}, ed[1] = {du};
-        push_multi<FLT, false, 1>(a, ef, eg, ed, part, cb, ws);
-      }
-      // edge range: light edges of a light claim (then the vertex is filed for its heavy edges), the
-      // heavy edges of a heavy claim
-      uint32_t rs = 0, deg = 0, gbase = 0;
-      bool file_heavy = false;
-      if (ok) {
-        uint32_t v = gid;
-        if (REPL) {
-          const uint32_t inst = gid / a.n;
-          gbase = inst * a.n;
-          v = gid - gbase;
-        }
+      if (REPL && ok) gbase = (vgid / a.n) * a.n;
+      if (ok && !is_ref) {
+        const uint32_t v = vgid - gbase;
         const uint32_t r0 = a.row_ptr[v], r1 = a.row_ptr[v + 1];
         const uint32_t lc = a.lcnt ? a.lcnt[v] : (r1 - r0);
         if (heavy_claim) {
@@ -674,26 +583,48 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           rs = r0;
           deg = lc;
           file_heavy = r1 - r0 > lc;
+          ws.v[ST_EXPANDED]++;
         }
-        if (!heavy_claim) ws.v[ST_EXPANDED]++;
+      }
+      pb_add(a, pbuf, early, ((uint64_t)du << 32) | gid, light_queue<FLT>(a, du, part, cb), ws);
+      pb_add(a, pbuf, file_heavy, ((uint64_t)du << 32) | vgid, nq + (cb & (a.nb - 1)) * a.parts + part, ws);
+      // ---------------- long ranges become range records + reference items ----------------
+      // The references go to the claim's own queue (this bucket, same phase), so every wavefront can
+      // take a chunk; the record is complete before any reference is published (the flush waits for
+      // completion of all earlier stores before it publishes).
+      bool split = ok && !is_ref && deg > a.hub_degree;
+      uint32_t rec = 0, nref = 0;
+      if (split) {
+        rec = afadd(a.st + S_HREC, 1u);
+        if (rec >= a.hrec_cap) {
+          set_error(a, DERR_OVERFLOW);
+          split = false;  // relaxed here; the solve reports the overflow and is rerun larger
+        } else {
+          uint32_t te = max(a.task_edges, (deg + REF_MAX_CHUNKS - 1) / REF_MAX_CHUNKS);
+          nref = (deg + te - 1) / te;
+          uint32_t* r = a.hrec + (size_t)rec * HREC_WORDS;
+          ast(r + 0, vgid);
+          ast(r + 1, du);
+          ast(r + 2, rs);
+          ast(r + 3, te);
+          ast(r + 4, rs + deg);
+          ws.v[ST_TASKS] += nref;
+        }
       }
       {
-        const bool hf[1] = {file_heavy};
-        const uint32_t hg[1] = {gid}, hd[1] = {du};
-        push_multi<FLT, true, 1>(a, hf, hg, hd, part, cb, ws);
+        uint64_t sm = ballot(split);
+        while (sm) {
+          const uint32_t L = ffs64(sm);
+          const uint32_t nr = rdlane(nref, L), rc = rdlane(rec, L);
+          for (uint32_t k0 = 0; k0 < nr; k0 += 64)
+            pb_add(a, pbuf, k0 + lane < nr, ((uint64_t)REF_TAG << 32) | (rc << REF_CHUNK_BITS) | (k0 + lane), q, ws);
+          sm &= sm - 1;
+        }
       }
-      // ---------------- high-degree ranges go to hub slots ----------------
-      bool hub = false;
-      uint64_t hm = ballot(ok && deg > a.hub_degree);
-      while (hm) {
-        const uint32_t L = ffs64(hm);
-        const bool pub = publish_hub(a, rdlane(gid, L), rdlane(du, L), rdlane(rs, L), rdlane(deg, L),
-                                     wave_gid + L, cb, part, heavy_claim, ws);
-        if (lane == L) hub = pub;
-        hm &= hm - 1;
-      }
+      const uint64_t t2 = wall_clock64();
+      tk[1] += t2 - t1;
       // ---------------- flattened edge windows ----------------
-      const uint32_t mydeg = hub ? 0u : deg;
+      const uint32_t mydeg = (ok && !split) ? deg : 0u;
       const uint32_t incl = wave_incl_scan(mydeg);
       const uint32_t total = rdlane(incl, 63);
       if (total) {
@@ -726,12 +657,14 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
               gbs[j] = REPL ? s_gb[wv][lo] : 0u;
             }
           }
-          relax_group<FLT>(a, act, e, dus, gbs, part, cb, ws);
+          relax_group<FLT>(a, act, e, dus, gbs, part, cb, pbuf, ws);
         }
         wave_sync();
       }
-      mem_wait();  // this chunk's pushes have completed before it stops counting as in flight
+      pb_flush(a, pbuf, ws);
+      mem_wait();  // this claim's pushes have completed before it stops counting as in flight
       if (lane == 0) afadd(inflight(a, cb & (a.nb - 1)) + (q % a.parts) + (heavy_claim ? MAX_PARTS : 0u), 0u - cnt);
+      tk[2] += wall_clock64() - t2;
       if (hld(a.st + S_DONE + 1, true) != 0) break;
       continue;
     }
@@ -750,15 +683,14 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         mem_wait();
         bool nonempty = false;
         for (uint32_t qq = lane; qq < 2 * nq; qq += 64) nonempty |= ald(qhead(a, qq)) != ald(qtail(a, qq));
-        if (lane < HUB_SLOTS) {
-          const uint64_t cc = ald64(hcursor(a, lane));
-          nonempty |= cur_next(cc) < cur_total(cc);
-        }
         const bool any = ballot(nonempty) != 0;
         mem_wait();
         const uint64_t w2 = ald64((const uint64_t*)(a.st + S_VER));
         if (!any && w2 == w1) {
-          if (lane == 0) ast(a.st + S_DONE, 1u);
+          if (lane == 0) {
+            ast(a.st + S_DONE, 1u);
+            trace_event(a, 3, (uint32_t)ald(a.st + S_CUR), 0);
+          }
           break;
         }
       }
@@ -766,8 +698,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     // One wavefront at a time (try-lock) inspects the head: idle wavefronts must not flood the counters.
     // (1) When the head bucket's light phase is done (light queues empty, no light claim in flight),
     //     its heavy queues open.  (2) When the behind and head buckets are done -- nothing queued and
-    //     nothing in flight, light or heavy -- the head moves to the first non-empty bucket ahead, like
-    //     the reference's manager (wl.h:714-723).  Phase words of slots leaving the window are cleared.
+    //     nothing in flight, light or heavy -- the head moves to the first bucket ahead that has
+    //     anything queued or in flight, like the reference's manager (wl.h:714-723).  Phase words of
+    //     slots leaving the window are cleared.
     bool advanced = false;
     uint32_t locked = 0;
     if (lane == 0) {
@@ -790,10 +723,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
                        ald(inflight(a, bs) + lane) != 0 || ald(inflight(a, bs) + MAX_PARTS + lane) != 0;
       }
       const bool hl = ballot(head_light) != 0, hh = ballot(head_heavy) != 0, ba = ballot(behind_any) != 0;
-      if (!hl && lane == 0 && ald(phase(a, hs)) == 0) ast(phase(a, hs), 1u);
+      if (!hl && lane == 0 && ald(phase(a, hs)) == 0) {
+        ast(phase(a, hs), 1u);
+        trace_event(a, 1, c2, c2);
+      }
       if (!hl && !hh && !ba) {
-        // first bucket ahead with anything queued or in flight (light or heavy), so that no bucket
-        // that still has, or will get, entries is skipped
         uint32_t firstj = 0xFFFFFFFFu;
         for (uint32_t j0 = 0; j0 < a.nb; j0 += 64) {
           const uint32_t jj = j0 + lane;  // bucket c2-1+jj
@@ -813,12 +747,13 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         if (firstj != 0xFFFFFFFFu) {
           const uint32_t c3 = c2 - 1 + firstj;
           // buckets c2-1 .. c3-2 leave the window; their slots come back later as fresh buckets
-          for (uint32_t b = c2 - 1 + lane; b + 1 < c3 + 0u && b - (c2 - 1) < a.nb; b += 64)
+          for (uint32_t b = c2 - 1 + lane; b + 1 < c3 && b - (c2 - 1) < a.nb; b += 64)
             if (c2 > 0 || b != c2 - 1) ast(phase(a, b & (a.nb - 1)), 0u);
           mem_wait();
           if (lane == 0) {
             uint32_t cc = c2;
             acas(a.st + S_CUR, cc, c3);
+            trace_event(a, 2, c3, c2);
           }
           advanced = true;
         }
@@ -826,7 +761,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       mem_wait();
       if (lane == 0) ast(a.st + S_ADV, 0u);
     }
-    if (advanced) continue;
+    if (advanced) {
+      tk[3] += wall_clock64() - t1;
+      continue;
+    }
     // watchdog and host abort
     ++idle_iter;
     if (lane == 0) ws.v[ST_IDLE]++;
@@ -848,6 +786,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     if (idle_iter < 8) __builtin_amdgcn_s_sleep(2);
     else if (idle_iter < 64) __builtin_amdgcn_s_sleep(16);
     else __builtin_amdgcn_s_sleep(64);
+    tk[3] += wall_clock64() - t1;
   }
 
   // ---------------- exit: publish this wavefront's counters ----------------
@@ -860,6 +799,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     const uint64_t s = wave_sum64(ws.v[i]);
     if (lane == 0 && s) afadd64((uint64_t*)(a.st + S_STATS) + i, s);
   }
+  if (lane == 0)
+    for (int i = 0; i < 4; i++) afadd64((uint64_t*)(a.st + S_STATS) + ST_T_PROBE + i, tk[i]);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -885,7 +826,6 @@ __global__ void k_init_state(KArgs a, const uint32_t* srcs, uint32_t k, int repl
     a.st[S_QBASE + 2 * LINE] = k;  // all of it published
     a.st[S_POOL] = (k + CH - 1) >> CH_SHIFT;
   }
-  if (threadIdx.x < HUB_SLOTS) a.st[S_HUB + 2 * threadIdx.x * LINE + 3] = 1u;  // every hub slot free
   for (uint32_t j = threadIdx.x; j < ((k + CH - 1) >> CH_SHIFT); j += blockDim.x) a.tab[j] = j;
   for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
     const uint32_t gid = replicated ? i * a.n + srcs[i] : srcs[i];

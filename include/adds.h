@@ -50,8 +50,8 @@ typedef struct {
   uint32_t flags;        /* ADDS_F_* */
   uint32_t nbuckets;     /* bucket ring length (power of two, 4..256) */
   uint32_t parts;        /* sub-queues per bucket (1, 2, 4 or 8; 8 = one per XCD); (concurrency + 3) * parts <= 48 */
-  uint32_t hub_degree;   /* out-degree above which a vertex is split into edge tasks */
-  uint32_t task_edges;   /* edges per hub edge task */
+  uint32_t hub_degree;   /* edge ranges longer than this are split into chunks any wavefront can take */
+  uint32_t task_edges;   /* edges per chunk of a split range (at least; at most 512 chunks per range) */
   uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
   uint32_t timeout_ms;   /* device watchdog (default 120000) */
   uint64_t pool_entries; /* worklist pool capacity in entries (default 4 per vertex-instance + 8M, doubled
@@ -70,7 +70,7 @@ typedef struct {
   uint64_t relaxations;       /* atomic min operations issued on dist */
   uint64_t pushes;            /* worklist entries written (including re-bucketed ones) */
   uint64_t stale_skips;       /* popped entries superseded by a smaller distance */
-  uint64_t hub_tasks;         /* edge tasks created for high-degree vertices */
+  uint64_t hub_tasks;         /* chunks created by splitting long edge ranges */
   uint64_t rebuckets;         /* popped entries moved to a later bucket */
   uint64_t pool_chunks_used;  /* worklist pool chunks consumed */
   double kernel_ms;           /* hipEvent time of the persistent solver kernel alone */
@@ -78,6 +78,11 @@ typedef struct {
   uint64_t cas_failures;      /* lost claim races */
   uint64_t idle_iterations;   /* idle-loop iterations of all wavefronts */
   uint64_t late_items;        /* items expanded more than one bucket after their own bucket */
+  /* wavefront time by activity, in device wall-clock ticks (100 MHz) summed over all wavefronts */
+  uint64_t ticks_probe;       /* looking for and claiming work */
+  uint64_t ticks_items;       /* loading and filtering claimed items, filing, recording long ranges */
+  uint64_t ticks_edges;       /* relaxing the claimed items' edges and pushing the improvements */
+  uint64_t ticks_idle;        /* termination tests, head advances and back-off while no work is visible */
 } adds_stats;
 
 /* ---- errors / device ---- */
