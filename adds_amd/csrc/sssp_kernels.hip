@@ -357,27 +357,36 @@ template <bool FLT>
 __device__ __forceinline__ void relax_group(const KArgs& a, const bool (&act)[UNROLL], const uint32_t (&e)[UNROLL],
                                             const uint32_t (&du)[UNROLL], const uint32_t (&gb)[UNROLL], uint32_t part,
                                             uint32_t cb, PushBuf& pb, WaveStats& ws) {
-  uint32_t tgt[UNROLL], nd[UNROLL], dv[UNROLL];
-  bool f[UNROLL];
+  // Every load is issued unconditionally (inactive lanes read index 0, which exists whenever there is
+  // an edge to relax), so that all of them are in flight together: branches around loads make the
+  // compiler wait for each window's loads before the next window's are issued.
+  uint32_t cl[UNROLL], wl[UNROLL];
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
-    tgt[j] = 0;
-    nd[j] = 0;
-    if (act[j]) {
-      tgt[j] = gb[j] + a.col[e[j]];
-      nd[j] = dadd<FLT>(du[j], a.w[e[j]]);
-    }
+    const uint32_t es = act[j] ? e[j] : 0u;
+    cl[j] = a.col[es];
+    wl[j] = a.w[es];
+  }
+  uint32_t tgt[UNROLL], nd[UNROLL], dv[UNROLL];
+#pragma unroll
+  for (int j = 0; j < UNROLL; j++) {
+    tgt[j] = act[j] ? gb[j] + cl[j] : 0u;
+    nd[j] = dadd<FLT>(du[j], wl[j]);
   }
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++) dv[j] = act[j] ? a.dist[tgt[j]] : 0u;  // stale values only cost an atomic
+  for (int j = 0; j < UNROLL; j++) dv[j] = a.dist[tgt[j]];  // stale values only cost an atomic
+  bool f[UNROLL];
+  uint32_t nact = 0, nrel = 0;
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
     f[j] = act[j] && nd[j] < dv[j];
-    ws.v[ST_EDGES] += act[j] ? 1u : 0u;
-    if (f[j]) {
-      ws.v[ST_RELAX]++;
-      (void)afmin(a.dist + tgt[j], nd[j]);
-    }
+    nact += (uint32_t)__popcll(ballot(act[j]));
+    nrel += (uint32_t)__popcll(ballot(f[j]));
+    if (f[j]) (void)afmin(a.dist + tgt[j], nd[j]);
+  }
+  if (lane_id() == 0) {
+    ws.v[ST_EDGES] += nact;
+    ws.v[ST_RELAX] += nrel;
   }
 #pragma unroll
   for (int j = 0; j < UNROLL; j++)
@@ -636,26 +645,28 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         wave_sync();
         for (uint32_t base = 0; base < total; base += 64 * UNROLL) {
           bool act[UNROLL];
-          uint32_t e[UNROLL], dus[UNROLL], gbs[UNROLL];
+          uint32_t idx[UNROLL], lo[UNROLL], e[UNROLL], dus[UNROLL], gbs[UNROLL];
 #pragma unroll
           for (int j = 0; j < UNROLL; j++) {
-            const uint32_t i = base + 64 * j + lane;
-            act[j] = i < total;
-            e[j] = 0;
-            dus[j] = 0;
-            gbs[j] = 0;
-            if (act[j]) {
-              // owner = last lane L with s_pref[L] <= i (its degree is then non-zero)
-              uint32_t lo = 0, hi = 64;
-              while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_pref[wv][mid] <= i) lo = mid;
-                else hi = mid;
-              }
-              e[j] = s_rs[wv][lo] + (i - s_pref[wv][lo]);
-              dus[j] = s_du[wv][lo];
-              gbs[j] = REPL ? s_gb[wv][lo] : 0u;
+            idx[j] = base + 64 * j + lane;
+            act[j] = idx[j] < total;
+            lo[j] = 0;
+          }
+          // owner = last lane L with s_pref[L] <= i (its degree is then non-zero): an upward bisection
+          // of fixed depth, interleaved over the windows
+#pragma unroll
+          for (uint32_t step = 32; step >= 1; step >>= 1) {
+#pragma unroll
+            for (int j = 0; j < UNROLL; j++) {
+              const uint32_t mid = lo[j] + step;
+              if (s_pref[wv][mid] <= idx[j]) lo[j] = mid;
             }
+          }
+#pragma unroll
+          for (int j = 0; j < UNROLL; j++) {
+            e[j] = s_rs[wv][lo[j]] + (idx[j] - s_pref[wv][lo[j]]);
+            dus[j] = s_du[wv][lo[j]];
+            gbs[j] = REPL ? s_gb[wv][lo[j]] : 0u;
           }
           relax_group<FLT>(a, act, e, dus, gbs, part, cb, pbuf, ws);
         }
