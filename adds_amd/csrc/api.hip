@@ -339,7 +339,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   if (!p) p = &z;
   r.delta = p->delta > 0 ? p->delta : default_delta(g);
   r.wg_per_cu = p->wg_per_cu ? p->wg_per_cu : 1;
-  r.chunk = p->chunk ? p->chunk : 64;
+  r.chunk = p->chunk ? p->chunk : 256;
   r.nb = p->nbuckets ? p->nbuckets : 64;
   r.parts = p->parts ? p->parts : 8;
   r.hub_degree = p->hub_degree ? p->hub_degree : 1024;
@@ -350,7 +350,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.pool_fixed = p->pool_entries != 0;
   // default: 4 entries per vertex-instance + 8M, doubled after each overflow on this graph (remembered)
   r.pool_entries = p->pool_entries ? p->pool_entries : (4 * total_n + (8u << 20)) << std::min(g->ws.pool_shift, 6u);
-  if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 64 ||
+  if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 256 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
       r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||
       r.task_edges == 0 || r.hub_degree < 64) {
@@ -455,6 +455,13 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.parts = r.parts;
   a.inv_delta = 1.0f / r.delta;
   a.chunk = r.chunk;
+  // guided claims: a wavefront takes its share of what is published among the wavefronts of one
+  // part, at least claim_min items (ADDS_CLAIM_MIN / ADDS_CLAIM_DIV override, for experiments).
+  // Measured on RMAT graphs, full claims (claim_min = chunk) are fastest: fewer claims, less contention.
+  a.claim_min = r.chunk;
+  a.claim_div = std::max<uint32_t>(1, grid * WAVES_PER_WG / r.parts);
+  if (const char* e = getenv("ADDS_CLAIM_MIN")) a.claim_min = std::max(1, atoi(e));
+  if (const char* e = getenv("ADDS_CLAIM_DIV")) a.claim_div = std::max(1, atoi(e));
   a.hub_degree = r.hub_degree;
   a.task_edges = r.task_edges;
   a.concur = std::min(r.concur, r.nb - 2);
@@ -1025,7 +1032,7 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
       q.wg_per_cu = w;
       try_point(q);
     }
-    for (uint32_t c : {16u, 32u}) {
+    for (uint32_t c : {64u, 128u}) {
       adds_params q = p;
       q.chunk = c;
       try_point(q);

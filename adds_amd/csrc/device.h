@@ -75,7 +75,9 @@ struct KArgs {
   uint32_t nb;        // bucket ring length (power of two)
   uint32_t parts;     // sub-queues per bucket
   float inv_delta;    // 1 / bucket width
-  uint32_t chunk;     // items claimed per wavefront
+  uint32_t chunk;     // most items claimed at once by a wavefront (<= 256)
+  uint32_t claim_min; // a claim takes max(claim_min, published / claim_div) items, at most chunk
+  uint32_t claim_div;
   uint32_t hrec_cap;
   uint32_t hub_degree;  // ranges longer than this are expanded through range references
   uint32_t task_edges;  // edges per reference chunk (at least)

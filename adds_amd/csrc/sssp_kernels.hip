@@ -408,12 +408,13 @@ __device__ __forceinline__ void trace_event(const KArgs& a, uint64_t kind, uint3
 // ------------------------------------------------------------------------------------------------
 // the persistent solver
 // ------------------------------------------------------------------------------------------------
+constexpr int IPL = 4;  // claimed items per lane: a claim takes at most 64 * IPL items
 template <bool FLT, bool REPL>
 __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
-  __shared__ uint32_t s_pref[WAVES_PER_WG][65];
-  __shared__ uint32_t s_rs[WAVES_PER_WG][64];
-  __shared__ uint32_t s_du[WAVES_PER_WG][64];
-  __shared__ uint32_t s_gb[WAVES_PER_WG][64];
+  __shared__ uint32_t s_pref[WAVES_PER_WG][64 * IPL + 1];
+  __shared__ uint32_t s_rs[WAVES_PER_WG][64 * IPL];
+  __shared__ uint32_t s_du[WAVES_PER_WG][64 * IPL];
+  __shared__ uint32_t s_gb[WAVES_PER_WG][64 * IPL];
   __shared__ uint64_t s_pbe[WAVES_PER_WG][PB_CAP];
   __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
 
@@ -485,8 +486,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         const uint32_t bb = rdlane(pbk, L);
         uint32_t ok = 0;
         if (lane == 0) {
+          // guided size: a share of what is published (at least claim_min), at most `chunk`
           const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - a.chunk);
-          const int32_t take = min(max(old, 0), (int32_t)a.chunk);
+          const int32_t avail = max(old, 0);
+          const int32_t share = max((int32_t)a.claim_min, avail / (int32_t)a.claim_div);
+          const int32_t take = min(min(avail, (int32_t)a.chunk), share);
           if (take < (int32_t)a.chunk) afadd(qavail(a, qq), (uint32_t)((int32_t)a.chunk - take));
           if (take > 0) {
             h = afadd(qhead(a, qq), (uint32_t)take);
@@ -534,93 +538,138 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         if (rdlane(quit, 0)) break;
       }
       const bool heavy_claim = q >= nq;
-      // ---------------- load the claimed entries ----------------
-      bool have = lane < cnt;
-      uint32_t gid = 0, du = 0;
-      if (have) {
-        const uint32_t pos = h + lane;
-        const uint32_t ph = (pos >> CH_SHIFT) < a.maxv ? wait_tab(a, a.tab + (size_t)q * a.maxv + (pos >> CH_SHIFT))
-                                                       : TAB_BAD;
-        uint64_t e = EMPTY64;
-        if (ph != TAB_BAD) e = wait_entry(a, a.pool + ((size_t)ph << CH_SHIFT) + (pos & (CH - 1)));
-        if (e == EMPTY64) {
-          have = false;
-          set_error(a, DERR_TIMEOUT);
+      // ---------------- load the claimed entries (IPL per lane: item k = j * 64 + lane) ----------------
+      // Positions h .. h+cnt-1 span at most two pool chunks (cnt <= 256 < CH).
+      uint32_t p0 = TAB_BAD, p1 = TAB_BAD;
+      {
+        const uint32_t j0 = h >> CH_SHIFT, j1 = (h + cnt - 1) >> CH_SHIFT;
+        if (lane == 0 && j1 < a.maxv) {
+          p0 = wait_tab(a, a.tab + (size_t)q * a.maxv + j0);
+          p1 = j1 != j0 ? wait_tab(a, a.tab + (size_t)q * a.maxv + j1) : p0;
         }
-        gid = (uint32_t)e;
-        du = (uint32_t)(e >> 32);
+        p0 = rdlane(p0, 0);
+        p1 = rdlane(p1, 0);
+        if (p0 == TAB_BAD || p1 == TAB_BAD) {
+          if (lane == 0) set_error(a, DERR_TIMEOUT);
+          break;  // error path: leave the solve
+        }
       }
-      if (ballot(lane < cnt && !have)) break;  // error path: leave the solve
+      bool have[IPL];
+      uint32_t gid[IPL], du[IPL];
+      {
+        uint64_t ev[IPL];
+        const uint64_t* ep[IPL];
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          const uint32_t k = j * 64 + lane;
+          have[j] = k < cnt;
+          const uint32_t pos = h + (have[j] ? k : 0u);
+          const uint32_t ph = (pos >> CH_SHIFT) == (h >> CH_SHIFT) ? p0 : p1;
+          ep[j] = a.pool + ((size_t)ph << CH_SHIFT) + (pos & (CH - 1));
+          ev[j] = ald64(ep[j]);
+        }
+        bool lost = false;
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          if (have[j] && ev[j] == EMPTY64) {  // the producer's store has not landed yet
+            ev[j] = wait_entry(a, ep[j]);
+            lost |= ev[j] == EMPTY64;
+          }
+          gid[j] = (uint32_t)ev[j];
+          du[j] = (uint32_t)(ev[j] >> 32);
+        }
+        if (ballot(lost)) {
+          if (lane == 0) set_error(a, DERR_TIMEOUT);
+          break;  // error path: leave the solve
+        }
+      }
       // ---------------- resolve each item to an edge range ----------------
       // A vertex item: stale if the vertex already holds a smaller distance, early if its distance
       // belongs to a later bucket (moved there), otherwise its light range (light claim, and the vertex
       // is filed in this bucket's heavy queue) or heavy range (heavy claim).  A reference item: one
       // chunk of a recorded range, stale if the recorded vertex has moved on.
-      uint32_t vgid = gid, rs = 0, deg = 0, gbase = 0;
-      bool ok = false, early = false, file_heavy = false;
-      const bool is_ref = have && du == REF_TAG;
-      if (is_ref) {
-        const uint32_t* r = a.hrec + (size_t)(gid >> REF_CHUNK_BITS) * HREC_WORDS;
-        const uint32_t k = gid & (REF_MAX_CHUNKS - 1);
-        vgid = ald(r + 0);
-        du = ald(r + 1);
-        const uint32_t r0 = ald(r + 2), te = ald(r + 3), r1 = ald(r + 4);
-        rs = r0 + k * te;
-        deg = min(te, r1 - rs);
-      }
-      if (have) {
-        const uint32_t dnow = ald(a.dist + vgid);  // coherent: a stale (larger) value would defeat the filter
-        if (dnow < du) {
-          ws.v[ST_STALE]++;
-        } else if (!is_ref && !heavy_claim && bucket_key<FLT>(du, a.inv_delta) > cb) {
-          early = true;
-          ws.v[ST_REBUCKET]++;
-        } else {
-          ok = true;
-          if (bucket_key<FLT>(du, a.inv_delta) + 1 < cb) ws.v[ST_LATE]++;
+      uint32_t vgid[IPL], rs[IPL], deg[IPL], gbase[IPL];
+      bool ok[IPL], is_ref[IPL];
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        is_ref[j] = have[j] && du[j] == REF_TAG;
+        vgid[j] = have[j] ? gid[j] : 0u;
+        rs[j] = 0;
+        deg[j] = 0;
+        if (is_ref[j]) {
+          const uint32_t* r = a.hrec + (size_t)(gid[j] >> REF_CHUNK_BITS) * HREC_WORDS;
+          const uint32_t k = gid[j] & (REF_MAX_CHUNKS - 1);
+          vgid[j] = ald(r + 0);
+          du[j] = ald(r + 1);
+          const uint32_t r0 = ald(r + 2), te = ald(r + 3), r1 = ald(r + 4);
+          rs[j] = r0 + k * te;
+          deg[j] = min(te, r1 - rs[j]);
         }
       }
-      if (REPL && ok) gbase = (vgid / a.n) * a.n;
-      if (ok && !is_ref) {
-        const uint32_t v = vgid - gbase;
-        const uint32_t r0 = a.row_ptr[v], r1 = a.row_ptr[v + 1];
-        const uint32_t lc = a.lcnt ? a.lcnt[v] : (r1 - r0);
-        if (heavy_claim) {
-          rs = r0 + lc;
-          deg = r1 - rs;
-        } else {
-          rs = r0;
-          deg = lc;
-          file_heavy = r1 - r0 > lc;
-          ws.v[ST_EXPANDED]++;
-        }
+      // coherent loads: a stale (larger) distance would defeat the filter; the row bounds are loaded
+      // alongside (unconditionally, at vertex 0 for empty slots) so that all are in flight together
+      uint32_t dnow[IPL], r0v[IPL], r1v[IPL], lcv[IPL];
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        gbase[j] = REPL ? (vgid[j] / a.n) * a.n : 0u;
+        const uint32_t v = vgid[j] - gbase[j];
+        dnow[j] = ald(a.dist + vgid[j]);
+        r0v[j] = a.row_ptr[v];
+        r1v[j] = a.row_ptr[v + 1];
+        lcv[j] = a.lcnt ? a.lcnt[v] : 0u;
       }
-      pb_add(a, pbuf, early, ((uint64_t)du << 32) | gid, light_queue<FLT>(a, du, part, cb), ws);
-      pb_add(a, pbuf, file_heavy, ((uint64_t)du << 32) | vgid, nq + (cb & (a.nb - 1)) * a.parts + part, ws);
+      uint32_t nstale = 0, nreb = 0, nexp = 0, nlate = 0;
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        const bool stale = have[j] && dnow[j] < du[j];
+        const bool early = have[j] && !stale && !is_ref[j] && !heavy_claim && bucket_key<FLT>(du[j], a.inv_delta) > cb;
+        ok[j] = have[j] && !stale && !early;
+        bool file_heavy = false;
+        if (!is_ref[j]) {
+          const uint32_t lc = a.lcnt ? lcv[j] : (r1v[j] - r0v[j]);
+          rs[j] = heavy_claim ? r0v[j] + lc : r0v[j];
+          deg[j] = ok[j] ? (heavy_claim ? r1v[j] - rs[j] : lc) : 0u;
+          file_heavy = ok[j] && !heavy_claim && r1v[j] - r0v[j] > lc;
+        }
+        if (!ok[j]) deg[j] = 0;
+        nstale += (uint32_t)__popcll(ballot(stale));
+        nreb += (uint32_t)__popcll(ballot(early));
+        nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
+        nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], a.inv_delta) + 1 < cb));
+        pb_add(a, pbuf, early, ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], part, cb), ws);
+        pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + part, ws);
+      }
+      if (lane == 0) {
+        ws.v[ST_STALE] += nstale;
+        ws.v[ST_REBUCKET] += nreb;
+        ws.v[ST_EXPANDED] += nexp;
+        ws.v[ST_LATE] += nlate;
+      }
       // ---------------- long ranges become range records + reference items ----------------
       // The references go to the claim's own queue (this bucket, same phase), so every wavefront can
       // take a chunk; the record is complete before any reference is published (the flush waits for
       // completion of all earlier stores before it publishes).
-      bool split = ok && !is_ref && deg > a.hub_degree;
-      uint32_t rec = 0, nref = 0;
-      if (split) {
-        rec = afadd(a.st + S_HREC, 1u);
-        if (rec >= a.hrec_cap) {
-          set_error(a, DERR_OVERFLOW);
-          split = false;  // relaxed here; the solve reports the overflow and is rerun larger
-        } else {
-          uint32_t te = max(a.task_edges, (deg + REF_MAX_CHUNKS - 1) / REF_MAX_CHUNKS);
-          nref = (deg + te - 1) / te;
-          uint32_t* r = a.hrec + (size_t)rec * HREC_WORDS;
-          ast(r + 0, vgid);
-          ast(r + 1, du);
-          ast(r + 2, rs);
-          ast(r + 3, te);
-          ast(r + 4, rs + deg);
-          ws.v[ST_TASKS] += nref;
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        bool split = ok[j] && !is_ref[j] && deg[j] > a.hub_degree;
+        uint32_t rec = 0, nref = 0;
+        if (split) {
+          rec = afadd(a.st + S_HREC, 1u);
+          if (rec >= a.hrec_cap) {
+            set_error(a, DERR_OVERFLOW);
+            split = false;  // relaxed here; the solve reports the overflow and is rerun larger
+          } else {
+            const uint32_t te = max(a.task_edges, (deg[j] + REF_MAX_CHUNKS - 1) / REF_MAX_CHUNKS);
+            nref = (deg[j] + te - 1) / te;
+            uint32_t* r = a.hrec + (size_t)rec * HREC_WORDS;
+            ast(r + 0, vgid[j]);
+            ast(r + 1, du[j]);
+            ast(r + 2, rs[j]);
+            ast(r + 3, te);
+            ast(r + 4, rs[j] + deg[j]);
+            ws.v[ST_TASKS] += nref;
+          }
         }
-      }
-      {
         uint64_t sm = ballot(split);
         while (sm) {
           const uint32_t L = ffs64(sm);
@@ -629,19 +678,29 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             pb_add(a, pbuf, k0 + lane < nr, ((uint64_t)REF_TAG << 32) | (rc << REF_CHUNK_BITS) | (k0 + lane), q, ws);
           sm &= sm - 1;
         }
+        if (split) deg[j] = 0;
       }
       const uint64_t t2 = wall_clock64();
       tk[1] += t2 - t1;
       // ---------------- flattened edge windows ----------------
-      const uint32_t mydeg = (ok && !split) ? deg : 0u;
-      const uint32_t incl = wave_incl_scan(mydeg);
+      // Range k = lane * IPL + j occupies [s_pref[k], s_pref[k+1]) of the flattened order.
+      uint32_t mysum = 0;
+#pragma unroll
+      for (int j = 0; j < IPL; j++) mysum += deg[j];
+      const uint32_t incl = wave_incl_scan(mysum);
       const uint32_t total = rdlane(incl, 63);
       if (total) {
-        s_pref[wv][lane] = incl - mydeg;
-        s_rs[wv][lane] = rs;
-        s_du[wv][lane] = du;
-        if (REPL) s_gb[wv][lane] = gbase;
-        if (lane == 0) s_pref[wv][64] = total;
+        uint32_t run = incl - mysum;
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          const uint32_t k = lane * IPL + j;
+          s_pref[wv][k] = run;
+          s_rs[wv][k] = rs[j];
+          s_du[wv][k] = du[j];
+          if (REPL) s_gb[wv][k] = gbase[j];
+          run += deg[j];
+        }
+        if (lane == 0) s_pref[wv][64 * IPL] = total;
         wave_sync();
         for (uint32_t base = 0; base < total; base += 64 * UNROLL) {
           bool act[UNROLL];
@@ -652,10 +711,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             act[j] = idx[j] < total;
             lo[j] = 0;
           }
-          // owner = last lane L with s_pref[L] <= i (its degree is then non-zero): an upward bisection
-          // of fixed depth, interleaved over the windows
+          // owner = last range k with s_pref[k] <= i (its length is then non-zero): an upward
+          // bisection of fixed depth, interleaved over the windows
 #pragma unroll
-          for (uint32_t step = 32; step >= 1; step >>= 1) {
+          for (uint32_t step = 32 * IPL; step >= 1; step >>= 1) {
 #pragma unroll
             for (int j = 0; j < UNROLL; j++) {
               const uint32_t mid = lo[j] + step;

@@ -46,7 +46,7 @@ typedef struct adds_graph adds_graph; /* opaque; owns the device-resident CSR */
 typedef struct {
   float delta;           /* bucket width in distance units */
   uint32_t wg_per_cu;    /* persistent workgroups per compute unit */
-  uint32_t chunk;        /* worklist items claimed per wavefront (1..64) */
+  uint32_t chunk;        /* worklist items a wavefront claims at once (1..256; default 256) */
   uint32_t flags;        /* ADDS_F_* */
   uint32_t nbuckets;     /* bucket ring length (power of two, 4..256) */
   uint32_t parts;        /* sub-queues per bucket (1, 2, 4 or 8; 8 = one per XCD); (concurrency + 3) * parts <= 48 */
