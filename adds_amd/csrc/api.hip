@@ -348,8 +348,11 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.flags = p->flags;
   r.timeout_ms = p->timeout_ms ? (double)p->timeout_ms : 120000.0;
   r.pool_fixed = p->pool_entries != 0;
-  // default: 4 entries per vertex-instance + 8M, doubled after each overflow on this graph (remembered)
-  r.pool_entries = p->pool_entries ? p->pool_entries : (4 * total_n + (8u << 20)) << std::min(g->ws.pool_shift, 6u);
+  // default: 4 entries per vertex-instance + 1 per 16 edge-instances + 8M, doubled after each overflow
+  // on this graph (remembered)
+  const uint64_t total_m = g->n ? (total_n / g->n) * g->m : 0;
+  r.pool_entries = p->pool_entries ? p->pool_entries
+                                   : (4 * total_n + total_m / 16 + (8u << 20)) << std::min(g->ws.pool_shift, 6u);
   if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 256 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
       r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||

@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--scale", type=int, default=24, help="RMAT scale of the headline graph")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="per-launch HBM bytes measured by rocprofv3 --pmc (optional)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read MIB, write MIB), 3 times: the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -90,6 +93,14 @@ def run_c3(host, name, steps, warmup, torch, adds_amd, oracle, do_cpu, certify=T
         c = oracle.certify(host.row_ptr, host.col, host.w, src, d, host.is_float)
         res["certificate"] = "ok" if c == 0 else f"FAILED({c})"
         log(f"{name}: certificate {res['certificate']} ({time.perf_counter() - t0:.1f}s)")
+    # the same solve with the distances copied back to host memory (the CLI's path): PCIe-inclusive
+    hw = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        dg.sssp(src, **params)
+        hw.append(time.perf_counter() - t0)
+    res["host_out_ms"] = float(np.median(hw)) * 1e3
+    res["host_out_gteps"] = m_r / np.median(hw) / 1e9
     if do_cpu:
         t0 = time.perf_counter()
         ref = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)
@@ -97,6 +108,15 @@ def run_c3(host, name, steps, warmup, torch, adds_amd, oracle, do_cpu, certify=T
         res["cpu_dijkstra_s"] = el
         res["bit_exact_vs_dijkstra"] = bool(np.array_equal(ref, d))
         log(f"{name}: cpu dijkstra {el:.1f}s, bit-exact {res['bit_exact_vs_dijkstra']}")
+        # the reference's other CPU baseline: parallel delta-stepping on the host's cores
+        threads = min(os.cpu_count() or 1, 16)
+        delta = params.get("delta") or 4096.0
+        t0 = time.perf_counter()
+        ds = oracle.delta_step(host.row_ptr, host.col, host.w, src, host.is_float, delta, threads)
+        el = time.perf_counter() - t0
+        res["cpu_delta_step"] = {"s": el, "threads": threads, "delta": delta, "gteps": m_r / el / 1e9,
+                                 "bit_exact": bool(np.array_equal(ds, ref))}
+        log(f"{name}: cpu delta-step {el:.1f}s on {threads} threads")
     dg.close()
     del dist
     log(f"{name}: {res['ms_per_step']:.3f} ms/step, {res['gteps']:.2f} GTEPS, kernel {kms:.3f} ms")
@@ -243,6 +263,16 @@ def main():
         sys.exit(2)
     import torch
 
+    if args.copy_calibration:
+        n = args.copy_calibration << 18  # int32 words
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
+        return
+
     import adds_amd
     import oracle
 
@@ -307,7 +337,9 @@ def main():
                      "kernel": "k_sssp<false,false>", "kernel_ms": head["kernel_ms_event"],
                      "bytes_alg_per_launch": head["bytes_alg"]},
         "cpu_baseline": cpu,
-        "extras": extras,
+        "extras": dict(extras, c3_int={k: head[k] for k in ("params", "kernel_ms_event", "solve_ms_event",
+                                                             "gteps_kernel", "host_out_ms", "host_out_gteps",
+                                                             "work", "cpu_delta_step") if k in head}),
     }
     print(json.dumps(line))
 

@@ -30,7 +30,7 @@ typedef enum {
   ADDS_OK = 0,
   ADDS_EINVAL = 1,    /* bad argument, malformed graph, source out of range */
   ADDS_ENOMEM = 2,    /* host or device allocation failed */
-  ADDS_EOVERFLOW = 3, /* worklist pool or hub-task ring exhausted during a solve */
+  ADDS_EOVERFLOW = 3, /* worklist pool or range-record store exhausted during a solve */
   ADDS_ETIMEOUT = 4,  /* device watchdog or host abort ended a solve */
   ADDS_EHIP = 5,      /* a HIP runtime call failed */
   ADDS_EIO = 6,       /* file could not be opened, read or written */
@@ -54,7 +54,8 @@ typedef struct {
   uint32_t task_edges;   /* edges per chunk of a split range (at least; at most 512 chunks per range) */
   uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
   uint32_t timeout_ms;   /* device watchdog (default 120000) */
-  uint64_t pool_entries; /* worklist pool capacity in entries (default 4 per vertex-instance + 8M, doubled
+  uint64_t pool_entries; /* worklist pool capacity in entries (default 4 per vertex-instance + 1 per 16
+                            edge-instances + 8M, doubled
                             and remembered per graph after an overflow; a solve that overflows is rerun) */
 } adds_params;
 
