@@ -344,46 +344,58 @@ __device__ __forceinline__ void pb_add(const KArgs& a, PushBuf& pb, bool f, uint
   pb.n += c;
 }
 
-// Relaxes UNROLL independent edges per lane (one 64-edge window each) so that UNROLL dependent chains
-// (col/w load -> d[v] load -> atomicMin) are in flight per wavefront.  The atomicMin's result is not
-// waited for: an improvement over the pre-filter value is pushed, and an entry that another wavefront
-// beat is dropped as stale when it is popped.
+// Edges are relaxed in groups of UNROLL 64-edge windows per wavefront (one edge per lane and window),
+// software-pipelined over the groups: the (col, w) loads of group g+1 are issued before the d[v]
+// gathers of group g are waited for, so the two dependent load rounds of consecutive groups overlap.
+// Every load is issued unconditionally (inactive lanes read index 0, which exists whenever there is an
+// edge to relax): branches around loads make the compiler wait for each window's loads in turn.
 #ifndef ADDS_UNROLL
 #define ADDS_UNROLL 4
 #endif
 constexpr int UNROLL = ADDS_UNROLL;
 
-template <bool FLT>
-__device__ __forceinline__ void relax_group(const KArgs& a, const bool (&act)[UNROLL], const uint32_t (&e)[UNROLL],
-                                            const uint32_t (&du)[UNROLL], const uint32_t (&gb)[UNROLL], uint32_t part,
-                                            uint32_t cb, PushBuf& pb, WaveStats& ws) {
-  // Every load is issued unconditionally (inactive lanes read index 0, which exists whenever there is
-  // an edge to relax), so that all of them are in flight together: branches around loads make the
-  // compiler wait for each window's loads before the next window's are issued.
-  uint32_t cl[UNROLL], wl[UNROLL];
+struct EdgeGroup {
+  bool act[UNROLL];
+  uint32_t e[UNROLL], du[UNROLL], gb[UNROLL], cl[UNROLL], wl[UNROLL];
+};
+
+// Issues the (col, w) loads of a located group.
+__device__ __forceinline__ void load_cw(const KArgs& a, EdgeGroup& g) {
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
-    const uint32_t es = act[j] ? e[j] : 0u;
-    cl[j] = a.col[es];
-    wl[j] = a.w[es];
+    const uint32_t es = g.act[j] ? g.e[j] : 0u;
+    g.cl[j] = a.col[es];
+    g.wl[j] = a.w[es];
   }
+}
+
+// Relaxes a group whose (col, w) loads are in flight; `next` (already located) gets its (col, w)
+// loads issued while this group's d[v] gathers are outstanding.  The atomicMin's result is not waited
+// for: an improvement over the pre-filter value is pushed, and an entry that another wavefront beat is
+// dropped as stale when it is popped.
+template <bool FLT>
+__device__ __forceinline__ void relax_group(const KArgs& a, const EdgeGroup& g, EdgeGroup& next, uint32_t part,
+                                            uint32_t cb, PushBuf& pb, WaveStats& ws) {
   uint32_t tgt[UNROLL], nd[UNROLL], dv[UNROLL];
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
-    tgt[j] = act[j] ? gb[j] + cl[j] : 0u;
-    nd[j] = dadd<FLT>(du[j], wl[j]);
+    tgt[j] = g.act[j] ? g.gb[j] + g.cl[j] : 0u;
+    nd[j] = dadd<FLT>(g.du[j], g.wl[j]);
   }
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) dv[j] = a.dist[tgt[j]];  // stale values only cost an atomic
+  load_cw(a, next);  // unconditional: a branch here would make the compiler wait for these loads too
   bool f[UNROLL];
   uint32_t nact = 0, nrel = 0;
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
-    f[j] = act[j] && nd[j] < dv[j];
-    nact += (uint32_t)__popcll(ballot(act[j]));
+    f[j] = g.act[j] && nd[j] < dv[j];
+    nact += (uint32_t)__popcll(ballot(g.act[j]));
     nrel += (uint32_t)__popcll(ballot(f[j]));
-    if (f[j]) (void)afmin(a.dist + tgt[j], nd[j]);
   }
+#pragma unroll
+  for (int j = 0; j < UNROLL; j++)
+    if (f[j]) (void)afmin(a.dist + tgt[j], nd[j]);
   if (lane_id() == 0) {
     ws.v[ST_EDGES] += nact;
     ws.v[ST_RELAX] += nrel;
@@ -702,17 +714,16 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         }
         if (lane == 0) s_pref[wv][64 * IPL] = total;
         wave_sync();
-        for (uint32_t base = 0; base < total; base += 64 * UNROLL) {
-          bool act[UNROLL];
-          uint32_t idx[UNROLL], lo[UNROLL], e[UNROLL], dus[UNROLL], gbs[UNROLL];
+        // owner of flattened edge i = last range k with s_pref[k] <= i (its length is then non-zero):
+        // an upward bisection of fixed depth, interleaved over the windows
+        auto locate = [&](uint32_t base, EdgeGroup& g) {
+          uint32_t idx[UNROLL], lo[UNROLL];
 #pragma unroll
           for (int j = 0; j < UNROLL; j++) {
             idx[j] = base + 64 * j + lane;
-            act[j] = idx[j] < total;
+            g.act[j] = idx[j] < total;
             lo[j] = 0;
           }
-          // owner = last range k with s_pref[k] <= i (its length is then non-zero): an upward
-          // bisection of fixed depth, interleaved over the windows
 #pragma unroll
           for (uint32_t step = 32 * IPL; step >= 1; step >>= 1) {
 #pragma unroll
@@ -723,11 +734,18 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           }
 #pragma unroll
           for (int j = 0; j < UNROLL; j++) {
-            e[j] = s_rs[wv][lo[j]] + (idx[j] - s_pref[wv][lo[j]]);
-            dus[j] = s_du[wv][lo[j]];
-            gbs[j] = REPL ? s_gb[wv][lo[j]] : 0u;
+            g.e[j] = s_rs[wv][lo[j]] + (idx[j] - s_pref[wv][lo[j]]);
+            g.du[j] = s_du[wv][lo[j]];
+            g.gb[j] = REPL ? s_gb[wv][lo[j]] : 0u;
           }
-          relax_group<FLT>(a, act, e, dus, gbs, part, cb, pbuf, ws);
+        };
+        EdgeGroup ga, gn;
+        locate(0, ga);
+        load_cw(a, ga);
+        for (uint32_t base = 0; base < total; base += 64 * UNROLL) {
+          locate(base + 64 * UNROLL, gn);  // past the end every window is inactive
+          relax_group<FLT>(a, ga, gn, part, cb, pbuf, ws);
+          ga = gn;
         }
         wave_sync();
       }
