@@ -474,6 +474,8 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   if (rate_khz <= 0) rate_khz = 100000;
   a.timeout_ticks = (uint64_t)(r.timeout_ms * (double)rate_khz);
   a.cached_probe = (r.flags & ADDS_F_ATOMIC_PROBE) ? 0u : 1u;
+  a.sticky = 1;
+  if (const char* e = getenv("ADDS_STICKY")) a.sticky = atoi(e) != 0;
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
   return a;

@@ -85,6 +85,7 @@ struct KArgs {
   uint32_t total_waves;
   uint64_t timeout_ticks;  // wall_clock64 ticks
   uint32_t cached_probe;   // probe queue counters with L2-cached loads (claims stay atomic)
+  uint32_t sticky;         // after a full claim from the behind/head bucket, claim the same queue again
   // Optional bucket timeline (null = off): trace[0] = record count, then records of 4 words from
   // trace + 4: wall clock, (kind << 56 | bucket), previous head, pool chunks used.  Kinds: 1 = the
   // head bucket's heavy phase opened, 2 = the head moved to `bucket`, 0 = start, 3 = done.

@@ -83,12 +83,11 @@ __device__ __forceinline__ bool acas(uint32_t* p, uint32_t& expected, uint32_t d
   return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
 }
-// Hint loads: through the L2 (may be stale), never used to decide ownership.
+// Hint loads: plain cached loads (a workgroup-scope atomic load carries no cache-bypass bits; it
+// may return a stale value), never used to decide ownership.  `volatile` would not do: on gfx950 it
+// bypasses the caches like a system-scope load.
 __device__ __forceinline__ uint32_t hld(const uint32_t* p, bool cached) {
-  return cached ? *(const volatile uint32_t*)p : ald(p);
-}
-__device__ __forceinline__ uint64_t hld64(const uint64_t* p, bool cached) {
-  return cached ? *(const volatile uint64_t*)p : ald64(p);
+  return cached ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : ald(p);
 }
 // Wait until every memory access issued so far by this wavefront has completed (s_waitcnt with all
 // counters at zero), with compiler fences so that no access moves across.  A workgroup-scope fence
@@ -417,6 +416,36 @@ __device__ __forceinline__ void trace_event(const KArgs& a, uint64_t kind, uint3
   r[3] = ald(a.st + S_POOL);
 }
 
+// Claims up to `chunk` items of queue qq (bucket bb): take from the published count (fetch_sub; a
+// shortfall is given back), then fetch_add the head by what was taken, and count the claim in flight
+// for the bucket.  Published items are at positions the head has not passed, so every claimed position
+// has been reserved by a producer.  Lane 0 works; the outcome is broadcast.
+__device__ __forceinline__ bool try_claim(const KArgs& a, uint32_t qq, uint32_t bb, uint32_t nq, uint32_t& h,
+                                          uint32_t& cnt, WaveStats& ws) {
+  uint32_t ok = 0, hh = 0, cc = 0;
+  if (lane_id() == 0) {
+    // guided size: a share of what is published (at least claim_min), at most `chunk`
+    const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - a.chunk);
+    const int32_t avail = max(old, 0);
+    const int32_t share = max((int32_t)a.claim_min, avail / (int32_t)a.claim_div);
+    const int32_t take = min(min(avail, (int32_t)a.chunk), share);
+    if (take < (int32_t)a.chunk) afadd(qavail(a, qq), (uint32_t)((int32_t)a.chunk - take));
+    if (take > 0) {
+      hh = afadd(qhead(a, qq), (uint32_t)take);
+      const uint32_t ic = (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u);
+      afadd(inflight(a, bb & (a.nb - 1)) + ic, (uint32_t)take);
+      cc = (uint32_t)take;
+      ok = 1;
+    } else {
+      ws.v[ST_CASFAIL]++;
+    }
+  }
+  if (!rdlane(ok, 0)) return false;
+  h = rdlane(hh, 0);
+  cnt = rdlane(cc, 0);
+  return true;
+}
+
 // ------------------------------------------------------------------------------------------------
 // the persistent solver
 // ------------------------------------------------------------------------------------------------
@@ -445,6 +474,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   uint32_t idle_iter = 0, work_iter = 0;
   bool missed = false;  // the last claim attempt lost: probe with atomic loads next time
   uint64_t tk[4] = {0, 0, 0, 0};  // activity ticks: probe, items, edges, idle (wave-uniform)
+  uint32_t sticky_q = 0xFFFFFFFFu, sticky_b = 0;  // queue and bucket of the last full claim (see below)
   if (wave_gid == 0 && lane == 0) trace_event(a, 0, 0, 0);
 
   for (;;) {
@@ -453,10 +483,25 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     // `parts` lanes each: light behind, light head, heavy behind, heavy head (only once the head's
     // phase word is set), light of the concur-1 buckets ahead.  Bit order is the claim priority; each
     // wavefront starts at its own part.
+    bool got = false;
+    uint32_t q = 0, h = 0, cnt = 0, cb = 0;
+    // Sticky claim: after a full claim from the behind or head bucket, the same queue is claimed again
+    // directly (no probe) while that bucket is still behind or head.
+    if (sticky_q != 0xFFFFFFFFu) {
+      const uint32_t c0 = ald(a.st + S_CUR);
+      if (sticky_b + 1 >= c0 && sticky_b <= c0) {
+        got = try_claim(a, sticky_q, sticky_b, nq, h, cnt, ws);
+        if (got) {
+          q = sticky_q;
+          cb = sticky_b;
+        }
+      }
+      if (!got) sticky_q = 0xFFFFFFFFu;
+    }
     const bool cached = a.cached_probe && (idle_iter & 7) != 7 && !missed;
     missed = false;
-    const uint32_t c = hld(a.st + S_CUR, cached);
-    const bool head_heavy_open = hld(phase(a, c & (a.nb - 1)), cached) != 0;
+    const uint32_t c = got ? 0u : hld(a.st + S_CUR, cached);
+    const bool head_heavy_open = !got && hld(phase(a, c & (a.nb - 1)), cached) != 0;
     uint32_t pq = 0, pbk = 0;
     bool pv = false;
     {
@@ -472,16 +517,14 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       } else if (lane < nprobe) {
         j = grp - 2;
       }
-      if (j != 0xFFFFFFFFu && (c > 0 || j > 0)) {
+      if (!got && j != 0xFFFFFFFFu && (c > 0 || j > 0)) {
         pbk = c - 1 + j;
         pq = (heavy ? nq : 0u) + (pbk & (a.nb - 1)) * a.parts + ((part + lane % a.parts) % a.parts);
         pv = (int32_t)hld(qavail(a, pq), cached) > 0;
       }
     }
-    const uint64_t pm = ballot(pv);
-    if (lane == 0) ws.v[ST_PROBE]++;
-    bool got = false;
-    uint32_t q = 0, h = 0, cnt = 0, cb = 0;
+    const uint64_t pm = got ? 0ull : ballot(pv);
+    if (lane == 0 && !got) ws.v[ST_PROBE]++;
     if (pm) {
       if (!busy) {
         if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 0xFFFFFFFFull);  // +1 transition, -1 idle
@@ -496,30 +539,14 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         const uint32_t L = ffs64(qm);
         const uint32_t qq = rdlane(pq, L);
         const uint32_t bb = rdlane(pbk, L);
-        uint32_t ok = 0;
-        if (lane == 0) {
-          // guided size: a share of what is published (at least claim_min), at most `chunk`
-          const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - a.chunk);
-          const int32_t avail = max(old, 0);
-          const int32_t share = max((int32_t)a.claim_min, avail / (int32_t)a.claim_div);
-          const int32_t take = min(min(avail, (int32_t)a.chunk), share);
-          if (take < (int32_t)a.chunk) afadd(qavail(a, qq), (uint32_t)((int32_t)a.chunk - take));
-          if (take > 0) {
-            h = afadd(qhead(a, qq), (uint32_t)take);
-            const uint32_t ic = (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u);
-            afadd(inflight(a, bb & (a.nb - 1)) + ic, (uint32_t)take);
-            cnt = (uint32_t)take;
-            ok = 1;
-          } else {
-            ws.v[ST_CASFAIL]++;
-          }
-        }
-        if (rdlane(ok, 0)) {
+        if (try_claim(a, qq, bb, nq, h, cnt, ws)) {
           got = true;
           q = qq;
-          h = rdlane(h, 0);
-          cnt = rdlane(cnt, 0);
           cb = bb;
+          if (cnt == a.chunk && bb <= c && a.sticky) {
+            sticky_q = qq;
+            sticky_b = bb;
+          }
         } else {
           missed = true;
         }
