@@ -343,7 +343,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.nb = p->nbuckets ? p->nbuckets : 64;
   r.parts = p->parts ? p->parts : 8;
   r.hub_degree = p->hub_degree ? p->hub_degree : 1024;
-  r.task_edges = p->task_edges ? p->task_edges : 512;
+  r.task_edges = p->task_edges ? p->task_edges : 64;
   r.concur = p->concurrency ? p->concurrency : 2;
   r.flags = p->flags;
   r.timeout_ms = p->timeout_ms ? (double)p->timeout_ms : 120000.0;
@@ -1004,8 +1004,8 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
         return;
       }
       if (getenv("ADDS_TUNE_VERBOSE"))
-        fprintf(stderr, "[tune] delta=%g nb=%u concur=%u wg/cu=%u chunk=%u -> %.3f ms\n", q.delta, q.nbuckets,
-                q.concurrency, q.wg_per_cu, q.chunk, ms);
+        fprintf(stderr, "[tune] delta=%g nb=%u concur=%u wg/cu=%u chunk=%u task_edges=%u -> %.3f ms\n", q.delta,
+                q.nbuckets, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges, ms);
       if (ms < bms) {
         bms = ms;
         p = q;
@@ -1040,6 +1040,12 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
     for (uint32_t c : {64u, 128u}) {
       adds_params q = p;
       q.chunk = c;
+      try_point(q);
+    }
+    // 4) edges per chunk of a split range
+    for (uint32_t te : {128u, 256u}) {
+      adds_params q = p;
+      q.task_edges = te;
       try_point(q);
     }
     *best = p;

@@ -222,8 +222,9 @@ __device__ __forceinline__ void alloc_chunk(const KArgs& a, uint32_t* tp) {
 // Improvements found while relaxing a claim are collected in LDS (entry + target queue) and written
 // to the queues in one flush at the end of the claim, or earlier when the buffer is full.  A flush
 // groups the entries by queue, reserves every group's positions at once (one lane per group, so the
-// reservations' round trips overlap), writes the entries, waits for their completion and then
-// publishes every group.
+// reservations' round trips overlap), writes the entries and publishes every group without waiting
+// for the stores: a consumer that claims a position whose entry has not landed yet spins on it
+// (wait_entry), and every published position has been reserved by a producer that is writing it.
 constexpr uint32_t PB_CAP = 256;  // entries per wavefront: 4 per lane in one flush round
 
 struct PushBuf {
@@ -321,9 +322,7 @@ __device__ void pb_flush(const KArgs& a, PushBuf& pb, WaveStats& ws) {
         if (ph != TAB_BAD) ast64(a.pool + ((size_t)ph << CH_SHIFT) + (pos & (CH - 1)), e[j]);
       }
     }
-    // publish: the entries have completed before consumers may count them as claimable
-    mem_wait();
-    if (lane < ng) afadd(qavail(a, gq), gc);
+    if (lane < ng) afadd(qavail(a, gq), gc);  // publish
   }
   pb.n = 0;
   wave_sync();
@@ -686,38 +685,49 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       }
       // ---------------- long ranges become range records + reference items ----------------
       // The references go to the claim's own queue (this bucket, same phase), so every wavefront can
-      // take a chunk; the record is complete before any reference is published (the flush waits for
-      // completion of all earlier stores before it publishes).
+      // take a chunk.  The records have completed before any reference is written (a consumer reads
+      // the record as soon as it sees the reference).
+      bool split[IPL];
+      uint32_t rec[IPL], nref[IPL];
+      bool any_split = false;
 #pragma unroll
       for (int j = 0; j < IPL; j++) {
-        bool split = ok[j] && !is_ref[j] && deg[j] > a.hub_degree;
-        uint32_t rec = 0, nref = 0;
-        if (split) {
-          rec = afadd(a.st + S_HREC, 1u);
-          if (rec >= a.hrec_cap) {
+        split[j] = ok[j] && !is_ref[j] && deg[j] > a.hub_degree;
+        rec[j] = 0;
+        nref[j] = 0;
+        if (split[j]) {
+          rec[j] = afadd(a.st + S_HREC, 1u);
+          if (rec[j] >= a.hrec_cap) {
             set_error(a, DERR_OVERFLOW);
-            split = false;  // relaxed here; the solve reports the overflow and is rerun larger
+            split[j] = false;  // relaxed here; the solve reports the overflow and is rerun larger
           } else {
             const uint32_t te = max(a.task_edges, (deg[j] + REF_MAX_CHUNKS - 1) / REF_MAX_CHUNKS);
-            nref = (deg[j] + te - 1) / te;
-            uint32_t* r = a.hrec + (size_t)rec * HREC_WORDS;
+            nref[j] = (deg[j] + te - 1) / te;
+            uint32_t* r = a.hrec + (size_t)rec[j] * HREC_WORDS;
             ast(r + 0, vgid[j]);
             ast(r + 1, du[j]);
             ast(r + 2, rs[j]);
             ast(r + 3, te);
             ast(r + 4, rs[j] + deg[j]);
-            ws.v[ST_TASKS] += nref;
+            ws.v[ST_TASKS] += nref[j];
           }
         }
-        uint64_t sm = ballot(split);
-        while (sm) {
-          const uint32_t L = ffs64(sm);
-          const uint32_t nr = rdlane(nref, L), rc = rdlane(rec, L);
-          for (uint32_t k0 = 0; k0 < nr; k0 += 64)
-            pb_add(a, pbuf, k0 + lane < nr, ((uint64_t)REF_TAG << 32) | (rc << REF_CHUNK_BITS) | (k0 + lane), q, ws);
-          sm &= sm - 1;
+        any_split |= ballot(split[j]) != 0;
+      }
+      if (any_split) {
+        mem_wait();
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          uint64_t sm = ballot(split[j]);
+          while (sm) {
+            const uint32_t L = ffs64(sm);
+            const uint32_t nr = rdlane(nref[j], L), rc = rdlane(rec[j], L);
+            for (uint32_t k0 = 0; k0 < nr; k0 += 64)
+              pb_add(a, pbuf, k0 + lane < nr, ((uint64_t)REF_TAG << 32) | (rc << REF_CHUNK_BITS) | (k0 + lane), q, ws);
+            sm &= sm - 1;
+          }
+          if (split[j]) deg[j] = 0;
         }
-        if (split) deg[j] = 0;
       }
       const uint64_t t2 = wall_clock64();
       tk[1] += t2 - t1;
@@ -776,8 +786,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         }
         wave_sync();
       }
+      // The claim stops counting as in flight.  Its pushes need not have completed: their tail
+      // reservations have (atomics with a result), so the queues already read as non-empty to the head
+      // advance and to the termination test.
       pb_flush(a, pbuf, ws);
-      mem_wait();  // this claim's pushes have completed before it stops counting as in flight
       if (lane == 0) afadd(inflight(a, cb & (a.nb - 1)) + (q % a.parts) + (heavy_claim ? MAX_PARTS : 0u), 0u - cnt);
       tk[2] += wall_clock64() - t2;
       if (hld(a.st + S_DONE + 1, true) != 0) break;
