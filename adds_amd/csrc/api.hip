@@ -68,8 +68,7 @@ struct adds_graph {
   float ave_wt = 1.f;  // mean sampled weight (the reference's profiler statistic)
   // light-first copy of the rows for the solver (col/w stay in input order for the backward pass and
   // the device-array accessor); valid for split_delta
-  uint32_t* scol = nullptr;
-  uint32_t* sw = nullptr;
+  uint2* scw = nullptr;
   uint32_t* lcnt = nullptr;
   uint32_t* split_aux = nullptr;
   float split_delta = 0.f;
@@ -150,11 +149,11 @@ static void release_graph_memory(adds_graph* g) {
   if (g->rrow) (void)hipFree(g->rrow);
   if (g->reid) (void)hipFree(g->reid);
   if (g->rsrc) (void)hipFree(g->rsrc);
-  if (g->scol) (void)hipFree(g->scol);
-  if (g->sw) (void)hipFree(g->sw);
+  if (g->scw) (void)hipFree(g->scw);
   if (g->lcnt) (void)hipFree(g->lcnt);
   if (g->split_aux) (void)hipFree(g->split_aux);
-  g->scol = g->sw = g->lcnt = g->split_aux = nullptr;
+  g->scw = nullptr;
+  g->lcnt = g->split_aux = nullptr;
   g->split_delta = 0.f;
   if (g->union_g) {
     release_graph_memory(g->union_g);
@@ -271,7 +270,7 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   HIPCHK(hipSetDevice(g->device));
   if (g->m) HIPCHK(hipMemcpyAsync(g->w, w_dev, g->m * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   if (g->lcnt && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
-    HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scol, g->sw, g->lcnt,
+    HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scw, g->lcnt,
                         g->split_aux, (hipStream_t)stream));
   if (g->union_g) {  // weights changed: the union copy is stale
     release_graph_memory(g->union_g);
@@ -420,14 +419,13 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
 // width re-split).
 static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
   if (!g->lcnt) {
-    HIPCHK(hipMalloc(&g->scol, std::max<uint64_t>(g->m, 1) * 4));
-    HIPCHK(hipMalloc(&g->sw, std::max<uint64_t>(g->m, 1) * 4));
+    HIPCHK(hipMalloc(&g->scw, std::max<uint64_t>(g->m, 1) * 8));
     HIPCHK(hipMalloc(&g->lcnt, std::max<uint64_t>(g->n, 1) * 4));
     HIPCHK(hipMalloc(&g->split_aux, split_aux_words(g->m) * 4));
     g->split_delta = 0.f;
   }
   if (g->split_delta != delta) {
-    HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scol, g->sw, g->lcnt, g->split_aux, s));
+    HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scw, g->lcnt, g->split_aux, s));
     g->split_delta = delta;
   }
   return ADDS_OK;
@@ -440,8 +438,7 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   KArgs a;
   memset(&a, 0, sizeof(a));
   a.row_ptr = g->row;
-  a.col = g->scol;
-  a.w = g->sw;
+  a.cw = g->scw;
   a.lcnt = g->lcnt;
   a.dist = dist;
   a.st = ws.st;

@@ -59,9 +59,8 @@ enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DER
 
 struct KArgs {
   const uint32_t* row_ptr;
-  const uint32_t* col;   // rows split light-then-heavy when lcnt is set
-  const uint32_t* w;     // 4-byte weights (uint32 or float bits), same order as col
-  const uint32_t* lcnt;  // light edges (w <= delta) at the front of each row; null: every edge light
+  const uint2* cw;       // per edge (destination, weight bits), each row light-first (w <= delta)
+  const uint32_t* lcnt;  // light edges at the front of each row
   uint32_t* dist;     // distance words, total_n of them
   uint32_t* st;       // state words
   uint64_t* pool;     // pool_chunks * CH entries: (dist << 32) | vertex-instance id
@@ -100,10 +99,10 @@ hipError_t launch_init(const KArgs& a, const uint32_t* srcs, uint32_t k, int rep
                        hipStream_t s);
 hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s);
 hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
-// Stable per-row partition of (col, w) into light (w <= delta) then heavy edges, with per-row light counts.
-// aux: split_aux_words(m) words of scratch (the list of long rows).
+// Stable per-row partition of (col, w) into light (w <= delta) then heavy edges, written interleaved as
+// (col, w) pairs, with per-row light counts.  aux: split_aux_words(m) words of scratch (long rows).
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        bool is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, uint32_t* aux, hipStream_t s);
+                        bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, hipStream_t s);
 constexpr uint64_t split_aux_words(uint64_t m) { return m / 4096 + 2; }
 
 // Litmus (litmus.hip)

@@ -362,8 +362,9 @@ __device__ __forceinline__ void load_cw(const KArgs& a, EdgeGroup& g) {
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
     const uint32_t es = g.act[j] ? g.e[j] : 0u;
-    g.cl[j] = a.col[es];
-    g.wl[j] = a.w[es];
+    const uint2 x = a.cw[es];
+    g.cl[j] = x.x;
+    g.wl[j] = x.y;
   }
 }
 
@@ -654,7 +655,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         dnow[j] = ald(a.dist + vgid[j]);
         r0v[j] = a.row_ptr[v];
         r1v[j] = a.row_ptr[v + 1];
-        lcv[j] = a.lcnt ? a.lcnt[v] : 0u;
+        lcv[j] = a.lcnt[v];
       }
       uint32_t nstale = 0, nreb = 0, nexp = 0, nlate = 0;
 #pragma unroll
@@ -664,7 +665,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         ok[j] = have[j] && !stale && !early;
         bool file_heavy = false;
         if (!is_ref[j]) {
-          const uint32_t lc = a.lcnt ? lcv[j] : (r1v[j] - r0v[j]);
+          const uint32_t lc = lcv[j];
           rs[j] = heavy_claim ? r0v[j] + lc : r0v[j];
           deg[j] = ok[j] ? (heavy_claim ? r1v[j] - rs[j] : lc) : 0u;
           file_heavy = ok[j] && !heavy_claim && r1v[j] - r0v[j] > lc;
@@ -991,7 +992,7 @@ __device__ __forceinline__ uint32_t int_delta(float delta) {
 }
 
 __global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        int is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, uint32_t* aux) {
+                        int is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux) {
   const uint32_t lane = lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -1022,8 +1023,7 @@ __global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint
       const uint64_t ml = ballot(in && l), mh = ballot(in && !l);
       if (in) {
         const uint32_t dst = l ? lo + mbcnt(ml) : hi + mbcnt(mh);
-        col2[dst] = c;
-        w2[dst] = x;
+        cw2[dst] = make_uint2(c, x);
       }
       lo += (uint32_t)__popcll(ml);
       hi += (uint32_t)__popcll(mh);
@@ -1035,7 +1035,7 @@ __global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint
 // thread t, sub-window j; ranks come from per-(sub-window, wavefront) ballot counts scanned in edge
 // order by one wavefront.
 __global__ void __launch_bounds__(1024) k_split_big(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w,
-                                                    float delta, int is_float, uint32_t* col2, uint32_t* w2,
+                                                    float delta, int is_float, uint2* cw2,
                                                     uint32_t* lcnt, const uint32_t* aux) {
   __shared__ uint32_t s_l[64], s_h[64], s_pl[64], s_ph[64];
   __shared__ uint32_t s_tl, s_th;
@@ -1099,8 +1099,7 @@ __global__ void __launch_bounds__(1024) k_split_big(const uint32_t* row_ptr, con
       for (int j = 0; j < 4; j++) {
         if (in[j]) {
           const uint32_t dst = l[j] ? lo + s_pl[j * 16 + wv] + mbcnt(ml[j]) : hi + s_ph[j * 16 + wv] + mbcnt(mh[j]);
-          col2[dst] = cc[j];
-          w2[dst] = xx[j];
+          cw2[dst] = make_uint2(cc[j], xx[j]);
         }
       }
       lo += s_tl;
@@ -1144,14 +1143,14 @@ hipError_t launch_cleanup(const KArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        bool is_float, uint32_t* col2, uint32_t* w2, uint32_t* lcnt, uint32_t* aux, hipStream_t s) {
+                        bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipError_t e = launch_fill_u32(aux, 1, 0, s);
   if (e != hipSuccess) return e;
   const uint32_t blocks = std::min<uint32_t>((n + 3) / 4, 16384);
-  hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, row_ptr, col, w, n, delta, is_float ? 1 : 0, col2, w2,
+  hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, row_ptr, col, w, n, delta, is_float ? 1 : 0, cw2,
                      lcnt, aux);
-  hipLaunchKernelGGL(k_split_big, dim3(1024), dim3(1024), 0, s, row_ptr, col, w, delta, is_float ? 1 : 0, col2, w2,
+  hipLaunchKernelGGL(k_split_big, dim3(1024), dim3(1024), 0, s, row_ptr, col, w, delta, is_float ? 1 : 0, cw2,
                      lcnt, (const uint32_t*)aux);
   return hipGetLastError();
 }
