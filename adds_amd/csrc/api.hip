@@ -367,7 +367,7 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
   Workspace& ws = g->ws;
   const uint32_t nq = r.nb * r.parts;
   const uint64_t chunks64 = (r.pool_entries + CH - 1) / CH;
-  if (chunks64 >= TAB_BAD) return ADDS_EINVAL;
+  if (chunks64 >= (1u << 24)) return ADDS_EINVAL;  // chunk ids are 24-bit in the kernel's chunk-map cache
   const uint32_t chunks = (uint32_t)chunks64;
   if (!ws.host_state) {
     HIPCHK(hipHostMalloc((void**)&ws.host_state, 8 * LINE * 4, hipHostMallocDefault));
@@ -459,6 +459,8 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   // part, at least claim_min items (ADDS_CLAIM_MIN / ADDS_CLAIM_DIV override, for experiments).
   // Measured on RMAT graphs, full claims (claim_min = chunk) are fastest: fewer claims, less contention.
   a.claim_min = r.chunk;
+  a.chunk_heavy = r.chunk;
+  if (const char* e = getenv("ADDS_HEAVY_CHUNK")) a.chunk_heavy = std::max(1, std::min(256, atoi(e)));
   a.claim_div = std::max<uint32_t>(1, grid * WAVES_PER_WG / r.parts);
   if (const char* e = getenv("ADDS_CLAIM_MIN")) a.claim_min = std::max(1, atoi(e));
   if (const char* e = getenv("ADDS_CLAIM_DIV")) a.claim_div = std::max(1, atoi(e));

@@ -75,6 +75,7 @@ struct KArgs {
   uint32_t parts;     // sub-queues per bucket
   float inv_delta;    // 1 / bucket width
   uint32_t chunk;     // most items claimed at once by a wavefront (<= 256)
+  uint32_t chunk_heavy;  // the same for heavy queues
   uint32_t claim_min; // a claim takes max(claim_min, published / claim_div) items, at most chunk
   uint32_t claim_div;
   uint32_t hrec_cap;

@@ -208,14 +208,29 @@ __device__ __forceinline__ uint64_t wait_entry(const KArgs& a, const uint64_t* e
   return e;
 }
 
-// Allocates a pool chunk for the chunk-map word `tp` (called by the reserver of the chunk's first slot).
-__device__ __forceinline__ void alloc_chunk(const KArgs& a, uint32_t* tp) {
+// Allocates a pool chunk for the chunk-map word `tp` (called by the reserver of the chunk's first slot)
+// and returns it.
+__device__ __forceinline__ uint32_t alloc_chunk(const KArgs& a, uint32_t* tp) {
   uint32_t ph = afadd(a.st + S_POOL, 1u);
   if (ph >= a.pool_chunks) {
     set_error(a, DERR_OVERFLOW);
     ph = TAB_BAD;
   }
   ast(tp, ph);
+  return ph;
+}
+
+// Per-wavefront cache of chunk-map entries in LDS, direct-mapped by queue: (queue << 48 | virtual
+// chunk << 24 | physical chunk).  A map entry never changes within a solve once written, so a hit is
+// always current; the cache is cleared at kernel start.
+constexpr uint32_t TC_SIZE = 256;
+__device__ __forceinline__ uint64_t tc_key(uint32_t q, uint32_t j) { return ((uint64_t)q << 48) | ((uint64_t)j << 24); }
+__device__ __forceinline__ uint32_t chunk_of(const KArgs& a, uint64_t* tc, uint32_t q, uint32_t j) {
+  const uint64_t e = tc[q & (TC_SIZE - 1)];
+  if ((e & ~0xFFFFFFull) == tc_key(q, j)) return (uint32_t)(e & 0xFFFFFFull);
+  const uint32_t ph = wait_tab(a, a.tab + (size_t)q * a.maxv + j);
+  if (ph < TAB_BAD) tc[q & (TC_SIZE - 1)] = tc_key(q, j) | ph;
+  return ph;
 }
 
 // ---- per-wavefront push buffer ----
@@ -228,9 +243,10 @@ __device__ __forceinline__ void alloc_chunk(const KArgs& a, uint32_t* tp) {
 constexpr uint32_t PB_CAP = 256;  // entries per wavefront: 4 per lane in one flush round
 
 struct PushBuf {
-  uint64_t* e;  // LDS: entries (dist << 32 | vertex-instance)
-  uint32_t* q;  // LDS: target queue
-  uint32_t n;   // wave-uniform fill
+  uint64_t* e;   // LDS: entries (dist << 32 | vertex-instance)
+  uint32_t* q;   // LDS: target queue
+  uint64_t* tc;  // LDS: this wavefront's chunk-map cache
+  uint32_t n;    // wave-uniform fill
 };
 
 // Target queue of a light-queue push: the bucket of nd, clamped to the window of the bucket being
@@ -244,7 +260,7 @@ __device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uin
   return (b & (a.nb - 1)) * a.parts + part;
 }
 
-__device__ void pb_flush(const KArgs& a, PushBuf& pb, WaveStats& ws) {
+__device__ void pb_flush(const KArgs& a, PushBuf& pb, uint64_t* tc, WaveStats& ws) {
   if (pb.n == 0) return;
   wave_sync();
   const uint32_t lane = lane_id();
@@ -301,12 +317,19 @@ __device__ void pb_flush(const KArgs& a, PushBuf& pb, WaveStats& ws) {
       base = afadd(qtail(a, gq), gc);
       const uint32_t j0 = base >> CH_SHIFT, j1 = (base + gc - 1) >> CH_SHIFT;
       if (j1 < a.maxv) {
+        // The reserver of a chunk's first slot allocates it; everyone else looks the chunk up.  Every
+        // lane's allocations come before any lane's lookup: a lookup may spin until another
+        // wavefront's allocation lands, and that wavefront may in turn be spinning on a chunk that a
+        // lane of this wavefront allocates, which must therefore not wait behind the spin.
         uint32_t* t0 = a.tab + (size_t)gq * a.maxv + j0;
         uint32_t* t1 = a.tab + (size_t)gq * a.maxv + j1;
-        if ((base & (CH - 1)) == 0) alloc_chunk(a, t0);
-        if (j1 != j0) alloc_chunk(a, t1);
-        p0 = wait_tab(a, t0);
-        p1 = (j1 != j0) ? wait_tab(a, t1) : p0;
+        const bool own0 = (base & (CH - 1)) == 0;
+        if (own0) p0 = alloc_chunk(a, t0);
+        if (j1 != j0) p1 = alloc_chunk(a, t1);
+        if (own0 && p0 < TAB_BAD) tc[gq & (TC_SIZE - 1)] = tc_key(gq, j0) | p0;
+        if (j1 != j0 && p1 < TAB_BAD) tc[gq & (TC_SIZE - 1)] = tc_key(gq, j1) | p1;
+        if (!own0) p0 = chunk_of(a, tc, gq, j0);
+        if (j1 == j0) p1 = p0;
       } else {
         set_error(a, DERR_OVERFLOW);
       }
@@ -333,7 +356,7 @@ __device__ __forceinline__ void pb_add(const KArgs& a, PushBuf& pb, bool f, uint
   const uint64_t m = ballot(f);
   if (m == 0) return;
   const uint32_t c = (uint32_t)__popcll(m);
-  if (pb.n + c > PB_CAP) pb_flush(a, pb, ws);
+  if (pb.n + c > PB_CAP) pb_flush(a, pb, pb.tc, ws);
   if (f) {
     const uint32_t i = pb.n + mbcnt(m);
     pb.e[i] = entry;
@@ -424,12 +447,13 @@ __device__ __forceinline__ bool try_claim(const KArgs& a, uint32_t qq, uint32_t 
                                           uint32_t& cnt, WaveStats& ws) {
   uint32_t ok = 0, hh = 0, cc = 0;
   if (lane_id() == 0) {
-    // guided size: a share of what is published (at least claim_min), at most `chunk`
-    const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - a.chunk);
+    // guided size: a share of what is published (at least claim_min), at most the queue's chunk
+    const int32_t chunk = (int32_t)(qq >= nq ? a.chunk_heavy : a.chunk);
+    const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - (uint32_t)chunk);
     const int32_t avail = max(old, 0);
     const int32_t share = max((int32_t)a.claim_min, avail / (int32_t)a.claim_div);
-    const int32_t take = min(min(avail, (int32_t)a.chunk), share);
-    if (take < (int32_t)a.chunk) afadd(qavail(a, qq), (uint32_t)((int32_t)a.chunk - take));
+    const int32_t take = min(min(avail, chunk), share);
+    if (take < chunk) afadd(qavail(a, qq), (uint32_t)(chunk - take));
     if (take > 0) {
       hh = afadd(qhead(a, qq), (uint32_t)take);
       const uint32_t ic = (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u);
@@ -458,10 +482,13 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   __shared__ uint32_t s_gb[WAVES_PER_WG][64 * IPL];
   __shared__ uint64_t s_pbe[WAVES_PER_WG][PB_CAP];
   __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
+  __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
 
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
-  PushBuf pbuf{s_pbe[wv], s_pbq[wv], 0u};
+  PushBuf pbuf{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
+  for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;  // no queue has id 0xFFFF
+  wave_sync();
   const uint32_t part = blockIdx.x % a.parts;
   const uint32_t wave_gid = blockIdx.x * WAVES_PER_WG + wv;
   const uint32_t nq = nq_of(a);
@@ -543,7 +570,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           got = true;
           q = qq;
           cb = bb;
-          if (cnt == a.chunk && bb <= c && a.sticky) {
+          if (cnt == (qq >= nq ? a.chunk_heavy : a.chunk) && bb <= c && a.sticky) {
             sticky_q = qq;
             sticky_b = bb;
           }
@@ -583,8 +610,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       {
         const uint32_t j0 = h >> CH_SHIFT, j1 = (h + cnt - 1) >> CH_SHIFT;
         if (lane == 0 && j1 < a.maxv) {
-          p0 = wait_tab(a, a.tab + (size_t)q * a.maxv + j0);
-          p1 = j1 != j0 ? wait_tab(a, a.tab + (size_t)q * a.maxv + j1) : p0;
+          p0 = chunk_of(a, pbuf.tc, q, j0);
+          p1 = j1 != j0 ? chunk_of(a, pbuf.tc, q, j1) : p0;
         }
         p0 = rdlane(p0, 0);
         p1 = rdlane(p1, 0);
@@ -790,7 +817,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       // The claim stops counting as in flight.  Its pushes need not have completed: their tail
       // reservations have (atomics with a result), so the queues already read as non-empty to the head
       // advance and to the termination test.
-      pb_flush(a, pbuf, ws);
+      pb_flush(a, pbuf, pbuf.tc, ws);
       if (lane == 0) afadd(inflight(a, cb & (a.nb - 1)) + (q % a.parts) + (heavy_claim ? MAX_PARTS : 0u), 0u - cnt);
       tk[2] += wall_clock64() - t2;
       if (hld(a.st + S_DONE + 1, true) != 0) break;
