@@ -1003,8 +1003,8 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
         return;
       }
       if (getenv("ADDS_TUNE_VERBOSE"))
-        fprintf(stderr, "[tune] delta=%g nb=%u concur=%u wg/cu=%u chunk=%u task_edges=%u -> %.3f ms\n", q.delta,
-                q.nbuckets, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges, ms);
+        fprintf(stderr, "[tune] delta=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u -> %.3f ms\n",
+                q.delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges, ms);
       if (ms < bms) {
         bms = ms;
         p = q;
@@ -1025,13 +1025,18 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
       q.nbuckets = nb;
       try_point(q);
     }
-    for (uint32_t c : {2u, 3u}) {
-      adds_params q = p;
-      q.concurrency = c;
+    // (sub-queues per bucket, open buckets): fewer parts leave probe lanes for more open buckets
+    const uint32_t pc_pairs[][2] = {{8, 1}, {8, 2}, {8, 3}, {4, 2}, {4, 3}, {4, 4}, {2, 4}};
+    const adds_params p2 = p;
+    for (const auto& pc : pc_pairs) {
+      adds_params q = p2;
+      q.parts = pc[0];
+      q.concurrency = pc[1];
+      if (q.parts == p2.parts && q.concurrency == p2.concurrency) continue;
       try_point(q);
     }
-    // 3) persistent workgroups per CU, then the claim size
-    for (uint32_t w : {2u, 3u, 4u}) {
+    // 3) persistent workgroups per CU (more than one has never won on gfx950), then the claim size
+    for (uint32_t w : {2u}) {
       adds_params q = p;
       q.wg_per_cu = w;
       try_point(q);
