@@ -241,6 +241,7 @@ __device__ __forceinline__ uint32_t chunk_of(const KArgs& a, uint64_t* tc, uint3
 // for the stores: a consumer that claims a position whose entry has not landed yet spins on it
 // (wait_entry), and every published position has been reserved by a producer that is writing it.
 constexpr uint32_t PB_CAP = 256;  // entries per wavefront: 4 per lane in one flush round
+constexpr uint32_t PB_RELAX = 0x80000000u;  // queue-word flag: apply the entry's atomicMin at the flush
 
 struct PushBuf {
   uint64_t* e;   // LDS: entries (dist << 32 | vertex-instance)
@@ -276,7 +277,27 @@ __device__ void pb_flush(const KArgs& a, PushBuf& pb, uint64_t* tc, WaveStats& w
     g[j] = 0;
     off[j] = 0;
   }
-  if (lane == 0) ws.v[ST_PUSH] += pb.n;
+  // Improvements found by the edge windows carry PB_RELAX: their atomicMin is issued here, all at once,
+  // and only those that lowered the distance are pushed.
+  uint32_t old[4];
+  bool relax[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    relax[j] = rem[j] && (q[j] & PB_RELAX) != 0;
+    q[j] &= ~PB_RELAX;
+    old[j] = relax[j] ? afmin(a.dist + (uint32_t)e[j], (uint32_t)(e[j] >> 32)) : 0u;
+  }
+  uint32_t nrel = 0, npush = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    nrel += (uint32_t)__popcll(ballot(relax[j]));
+    if (relax[j] && old[j] <= (uint32_t)(e[j] >> 32)) rem[j] = false;  // another path got there first
+    npush += (uint32_t)__popcll(ballot(rem[j]));
+  }
+  if (lane == 0) {
+    ws.v[ST_RELAX] += nrel;
+    ws.v[ST_PUSH] += npush;
+  }
   for (;;) {
     // group the remaining entries by queue, at most 64 groups per pass (group k lives in lane k)
     bool pend[4];
@@ -366,8 +387,8 @@ __device__ __forceinline__ void pb_add(const KArgs& a, PushBuf& pb, bool f, uint
 }
 
 // Edges are relaxed in groups of UNROLL 64-edge windows per wavefront (one edge per lane and window),
-// software-pipelined over the groups: the (col, w) loads of group g+1 are issued before the d[v]
-// gathers of group g are waited for, so the two dependent load rounds of consecutive groups overlap.
+// software-pipelined over the groups (see the edge loop in k_sssp) so that the two dependent load
+// rounds -- (col, w), then d[v] -- of consecutive groups overlap.
 // Every load is issued unconditionally (inactive lanes read index 0, which exists whenever there is an
 // edge to relax): branches around loads make the compiler wait for each window's loads in turn.
 #ifndef ADDS_UNROLL
@@ -391,40 +412,43 @@ __device__ __forceinline__ void load_cw(const KArgs& a, EdgeGroup& g) {
   }
 }
 
-// Relaxes a group whose (col, w) loads are in flight; `next` (already located) gets its (col, w)
-// loads issued while this group's d[v] gathers are outstanding.  The atomicMin's result is not waited
-// for: an improvement over the pre-filter value is pushed, and an entry that another wavefront beat is
-// dropped as stale when it is popped.
-template <bool FLT>
-__device__ __forceinline__ void relax_group(const KArgs& a, const EdgeGroup& g, EdgeGroup& next, uint32_t part,
-                                            uint32_t cb, PushBuf& pb, WaveStats& ws) {
+// A group whose d[v] gathers are in flight.
+struct Gathered {
+  bool act[UNROLL];
   uint32_t tgt[UNROLL], nd[UNROLL], dv[UNROLL];
+};
+
+// Turns a group whose (col, w) loads have been issued into a gathered group: candidate distances and
+// the d[v] pre-filter gathers (plain loads: a stale, larger value only costs an atomic at the flush).
+template <bool FLT>
+__device__ __forceinline__ void gather(const KArgs& a, const EdgeGroup& g, Gathered& r) {
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
-    tgt[j] = g.act[j] ? g.gb[j] + g.cl[j] : 0u;
-    nd[j] = dadd<FLT>(g.du[j], g.wl[j]);
+    r.act[j] = g.act[j];
+    r.tgt[j] = g.act[j] ? g.gb[j] + g.cl[j] : 0u;
+    r.nd[j] = dadd<FLT>(g.du[j], g.wl[j]);
   }
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++) dv[j] = a.dist[tgt[j]];  // stale values only cost an atomic
-  load_cw(a, next);  // unconditional: a branch here would make the compiler wait for these loads too
+  for (int j = 0; j < UNROLL; j++) r.dv[j] = a.dist[r.tgt[j]];
+}
+
+// Consumes a gathered group: improvements over the pre-filter value go to the push buffer, whose
+// flush applies the atomicMin.  No store or atomic is issued in the edge loop, so the compiler's
+// waits there cover loads only and the pipeline keeps two groups' loads in flight.
+template <bool FLT>
+__device__ __forceinline__ void finish(const KArgs& a, const Gathered& r, uint32_t part, uint32_t cb, PushBuf& pb,
+                                       WaveStats& ws) {
   bool f[UNROLL];
-  uint32_t nact = 0, nrel = 0;
+  uint32_t nact = 0;
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
-    f[j] = g.act[j] && nd[j] < dv[j];
-    nact += (uint32_t)__popcll(ballot(g.act[j]));
-    nrel += (uint32_t)__popcll(ballot(f[j]));
+    f[j] = r.act[j] && r.nd[j] < r.dv[j];
+    nact += (uint32_t)__popcll(ballot(r.act[j]));
   }
+  if (lane_id() == 0) ws.v[ST_EDGES] += nact;
 #pragma unroll
   for (int j = 0; j < UNROLL; j++)
-    if (f[j]) (void)afmin(a.dist + tgt[j], nd[j]);
-  if (lane_id() == 0) {
-    ws.v[ST_EDGES] += nact;
-    ws.v[ST_RELAX] += nrel;
-  }
-#pragma unroll
-  for (int j = 0; j < UNROLL; j++)
-    pb_add(a, pb, f[j], ((uint64_t)nd[j] << 32) | tgt[j], light_queue<FLT>(a, nd[j], part, cb), ws);
+    pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j], light_queue<FLT>(a, r.nd[j], part, cb) | PB_RELAX, ws);
 }
 
 // Appends a bucket-timeline record when tracing is on (called by one lane).
@@ -804,13 +828,23 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             g.gb[j] = REPL ? s_gb[wv][lo[j]] : 0u;
           }
         };
-        EdgeGroup ga, gn;
+        // Three-stage pipeline over groups of 64 * UNROLL edges: while group g is consumed, the
+        // gathers of g+1 and the (col, w) loads of g+2 are in flight.  Loads are issued in the order
+        // they are waited for, and past the end every window is inactive (its loads read index 0).
+        EdgeGroup ga, gb;
+        Gathered ra, rb;
         locate(0, ga);
         load_cw(a, ga);
+        locate(64 * UNROLL, gb);
+        load_cw(a, gb);
+        gather<FLT>(a, ga, ra);
         for (uint32_t base = 0; base < total; base += 64 * UNROLL) {
-          locate(base + 64 * UNROLL, gn);  // past the end every window is inactive
-          relax_group<FLT>(a, ga, gn, part, cb, pbuf, ws);
-          ga = gn;
+          locate(base + 128 * UNROLL, ga);
+          load_cw(a, ga);
+          gather<FLT>(a, gb, rb);
+          finish<FLT>(a, ra, part, cb, pbuf, ws);
+          ra = rb;
+          gb = ga;
         }
         wave_sync();
       }
