@@ -406,9 +406,10 @@ __device__ __forceinline__ void load_cw(const KArgs& a, EdgeGroup& g) {
 #pragma unroll
   for (int j = 0; j < UNROLL; j++) {
     const uint32_t es = g.act[j] ? g.e[j] : 0u;
-    const uint2 x = a.cw[es];
-    g.cl[j] = x.x;
-    g.wl[j] = x.y;
+    // streamed once per expansion: a non-temporal load keeps the L2 for the d[v] gathers
+    const uint64_t x = __builtin_nontemporal_load((const uint64_t*)a.cw + es);
+    g.cl[j] = (uint32_t)x;
+    g.wl[j] = (uint32_t)(x >> 32);
   }
 }
 
