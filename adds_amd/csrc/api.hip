@@ -475,6 +475,8 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.cached_probe = (r.flags & ADDS_F_ATOMIC_PROBE) ? 0u : 1u;
   a.sticky = 1;
   if (const char* e = getenv("ADDS_STICKY")) a.sticky = atoi(e) != 0;
+  a.local_cont = (r.flags & ADDS_F_NO_LOCAL) ? 0u : 1u;
+  if (const char* e = getenv("ADDS_LOCAL")) a.local_cont = atoi(e) != 0;
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
   return a;
@@ -583,6 +585,7 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->ticks_items = sv[ST_T_ITEMS];
     stats->ticks_edges = sv[ST_T_EDGES];
     stats->ticks_idle = sv[ST_T_IDLE];
+    stats->local_items = sv[ST_LOCAL];
   }
   if (ws.trace) dump_trace(g);
   if (err == DERR_OVERFLOW) {
@@ -1003,16 +1006,17 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
         return;
       }
       if (getenv("ADDS_TUNE_VERBOSE"))
-        fprintf(stderr, "[tune] delta=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u -> %.3f ms\n",
-                q.delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges, ms);
+        fprintf(stderr, "[tune] delta=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u flags=%u -> %.3f ms\n",
+                q.delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges, q.flags, ms);
       if (ms < bms) {
         bms = ms;
         p = q;
       }
     };
-    // 1) bucket width: powers of two around the default (int keeps power-of-two widths)
+    // 1) bucket width: powers of two around the default (int keeps power-of-two widths); the default
+    // is too narrow on low-degree, high-diameter graphs, hence the wide range upwards
     const float d0 = p.delta;
-    for (int j = -5; j <= 3 && rc == ADDS_OK; j++) {
+    for (int j = -4; j <= 5 && rc == ADDS_OK; j++) {
       if (j == 0) continue;
       adds_params q = p;
       q.delta = d0 * std::ldexp(1.0f, j);
@@ -1050,6 +1054,12 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
     for (uint32_t te : {128u, 256u}) {
       adds_params q = p;
       q.task_edges = te;
+      try_point(q);
+    }
+    // 5) local continuation off
+    {
+      adds_params q = p;
+      q.flags |= ADDS_F_NO_LOCAL;
       try_point(q);
     }
     *best = p;

@@ -47,7 +47,7 @@ constexpr uint32_t MAX_PARTS = 16;
 
 enum StatIdx {
   ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET,
-  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_LATE,
+  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_LATE, ST_LOCAL,
   // wavefront time by activity (wall-clock ticks): probing/claiming, item handling, edge windows,
   // idle (termination test, head advance, back-off)
   ST_T_PROBE, ST_T_ITEMS, ST_T_EDGES, ST_T_IDLE, NSTATS
@@ -86,6 +86,7 @@ struct KArgs {
   uint64_t timeout_ticks;  // wall_clock64 ticks
   uint32_t cached_probe;   // probe queue counters with L2-cached loads (claims stay atomic)
   uint32_t sticky;         // after a full claim from the behind/head bucket, claim the same queue again
+  uint32_t local_cont;     // a light claim keeps up to 64 same-bucket improvements for itself
   // Optional bucket timeline (null = off): trace[0] = record count, then records of 4 words from
   // trace + 4: wall clock, (kind << 56 | bucket), previous head, pool chunks used.  Kinds: 1 = the
   // head bucket's heavy phase opened, 2 = the head moved to `bucket`, 0 = start, 3 = done.

@@ -128,10 +128,12 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t x) {
 // ---- state addressing ----
 __device__ __forceinline__ uint32_t nq_of(const KArgs& a) { return a.nb * a.parts; }
 __device__ __forceinline__ uint32_t* qhead(const KArgs& a, uint32_t q) { return a.st + S_QBASE + QLINES * q * LINE; }
-__device__ __forceinline__ uint32_t* qtail(const KArgs& a, uint32_t q) {
+template <class A>
+__device__ __forceinline__ uint32_t* qtail(const A& a, uint32_t q) {
   return a.st + S_QBASE + QLINES * q * LINE + LINE;
 }
-__device__ __forceinline__ uint32_t* qavail(const KArgs& a, uint32_t q) {
+template <class A>
+__device__ __forceinline__ uint32_t* qavail(const A& a, uint32_t q) {
   return a.st + S_QBASE + QLINES * q * LINE + 2 * LINE;
 }
 // in-flight counters of bucket slot `slot`: light claims at [part], heavy claims at [16 + part]
@@ -141,11 +143,13 @@ __device__ __forceinline__ uint32_t* inflight(const KArgs& a, uint32_t slot) {
 __device__ __forceinline__ uint32_t* phase(const KArgs& a, uint32_t slot) {
   return a.st + s_slotlines(nq_of(a)) + 2 * slot * LINE + LINE;
 }
-__device__ __forceinline__ void set_error(const KArgs& a, uint32_t code) {
+template <class A>
+__device__ __forceinline__ void set_error(const A& a, uint32_t code) {
   uint32_t z = 0;
   acas(a.st + S_DONE + 1, z, code);
 }
-__device__ __forceinline__ bool stopped(const KArgs& a) {
+template <class A>
+__device__ __forceinline__ bool stopped(const A& a) {
   return (ald(a.st + S_DONE) | ald(a.st + S_DONE + 1)) != 0;
 }
 
@@ -175,7 +179,8 @@ struct WaveStats {
 // invariant into an error instead of a hang.
 constexpr uint32_t SPIN_CAP = 1u << 21;
 
-__device__ __forceinline__ uint32_t wait_tab(const KArgs& a, const uint32_t* tp) {
+template <class A>
+__device__ __forceinline__ uint32_t wait_tab(const A& a, const uint32_t* tp) {
   uint32_t ph = ald(tp);
   uint32_t spins = 0;
   while (ph == TAB_UNSET) {
@@ -191,7 +196,8 @@ __device__ __forceinline__ uint32_t wait_tab(const KArgs& a, const uint32_t* tp)
   }
   return ph;
 }
-__device__ __forceinline__ uint64_t wait_entry(const KArgs& a, const uint64_t* ep) {
+template <class A>
+__device__ __forceinline__ uint64_t wait_entry(const A& a, const uint64_t* ep) {
   uint64_t e = ald64(ep);
   uint32_t spins = 0;
   while (e == EMPTY64) {
@@ -210,7 +216,8 @@ __device__ __forceinline__ uint64_t wait_entry(const KArgs& a, const uint64_t* e
 
 // Allocates a pool chunk for the chunk-map word `tp` (called by the reserver of the chunk's first slot)
 // and returns it.
-__device__ __forceinline__ uint32_t alloc_chunk(const KArgs& a, uint32_t* tp) {
+template <class A>
+__device__ __forceinline__ uint32_t alloc_chunk(const A& a, uint32_t* tp) {
   uint32_t ph = afadd(a.st + S_POOL, 1u);
   if (ph >= a.pool_chunks) {
     set_error(a, DERR_OVERFLOW);
@@ -225,7 +232,8 @@ __device__ __forceinline__ uint32_t alloc_chunk(const KArgs& a, uint32_t* tp) {
 // always current; the cache is cleared at kernel start.
 constexpr uint32_t TC_SIZE = 256;
 __device__ __forceinline__ uint64_t tc_key(uint32_t q, uint32_t j) { return ((uint64_t)q << 48) | ((uint64_t)j << 24); }
-__device__ __forceinline__ uint32_t chunk_of(const KArgs& a, uint64_t* tc, uint32_t q, uint32_t j) {
+template <class A>
+__device__ __forceinline__ uint32_t chunk_of(const A& a, uint64_t* tc, uint32_t q, uint32_t j) {
   const uint64_t e = tc[q & (TC_SIZE - 1)];
   if ((e & ~0xFFFFFFull) == tc_key(q, j)) return (uint32_t)(e & 0xFFFFFFull);
   const uint32_t ph = wait_tab(a, a.tab + (size_t)q * a.maxv + j);
@@ -242,6 +250,20 @@ __device__ __forceinline__ uint32_t chunk_of(const KArgs& a, uint64_t* tc, uint3
 // (wait_entry), and every published position has been reserved by a producer that is writing it.
 constexpr uint32_t PB_CAP = 256;  // entries per wavefront: 4 per lane in one flush round
 constexpr uint32_t PB_RELAX = 0x80000000u;  // queue-word flag: apply the entry's atomicMin at the flush
+
+// The fields of the solver arguments that a flush needs, passed by value to the out-of-line flush
+// (passing the kernel's argument block by reference would copy it to scratch memory).
+struct FlushArgs {
+  uint32_t* dist;
+  uint32_t* st;
+  uint64_t* pool;
+  uint32_t* tab;
+  uint32_t maxv;
+  uint32_t pool_chunks;
+};
+__device__ __forceinline__ FlushArgs flush_args(const KArgs& a) {
+  return FlushArgs{a.dist, a.st, a.pool, a.tab, a.maxv, a.pool_chunks};
+}
 
 struct PushBuf {
   uint64_t* e;   // LDS: entries (dist << 32 | vertex-instance)
@@ -261,8 +283,10 @@ __device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uin
   return (b & (a.nb - 1)) * a.parts + part;
 }
 
-__device__ void pb_flush(const KArgs& a, PushBuf& pb, uint64_t* tc, WaveStats& ws) {
-  if (pb.n == 0) return;
+// own_q != ~0: up to 64 distance-lowering entries for queue own_q are kept out of the queues and left
+// in pb.e[0 .. returned count) for the caller to process itself (see the local continuation in k_sssp).
+__device__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, uint64_t* tc, WaveStats& ws, uint32_t own_q = 0xFFFFFFFFu) {
+  if (pb.n == 0) return 0;
   wave_sync();
   const uint32_t lane = lane_id();
   uint64_t e[4];
@@ -287,16 +311,35 @@ __device__ void pb_flush(const KArgs& a, PushBuf& pb, uint64_t* tc, WaveStats& w
     q[j] &= ~PB_RELAX;
     old[j] = relax[j] ? afmin(a.dist + (uint32_t)e[j], (uint32_t)(e[j] >> 32)) : 0u;
   }
-  uint32_t nrel = 0, npush = 0;
+  uint32_t nrel = 0, npush = 0, nl = 0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     nrel += (uint32_t)__popcll(ballot(relax[j]));
     if (relax[j] && old[j] <= (uint32_t)(e[j] >> 32)) rem[j] = false;  // another path got there first
-    npush += (uint32_t)__popcll(ballot(rem[j]));
   }
+  if (own_q != 0xFFFFFFFFu) {
+    // the first 64 improvements for own_q stay with this wavefront (the buffer is free again: its
+    // contents are in registers)
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const bool mine = rem[j] && relax[j] && q[j] == own_q;
+      const uint64_t m = ballot(mine);
+      const uint32_t idx = nl + mbcnt(m);
+      if (mine && idx < 64) {
+        pb.e[idx] = e[j];
+        rem[j] = false;
+      }
+      nl = min(nl + (uint32_t)__popcll(m), 64u);
+    }
+    wave_sync();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) npush += (uint32_t)__popcll(ballot(rem[j]));
   if (lane == 0) {
     ws.v[ST_RELAX] += nrel;
     ws.v[ST_PUSH] += npush;
+    ws.v[ST_LOCAL] += nl;
   }
   for (;;) {
     // group the remaining entries by queue, at most 64 groups per pass (group k lives in lane k)
@@ -370,6 +413,7 @@ __device__ void pb_flush(const KArgs& a, PushBuf& pb, uint64_t* tc, WaveStats& w
   }
   pb.n = 0;
   wave_sync();
+  return nl;
 }
 
 // Appends (entry, queue) for every lane with f set; flushes first when the buffer would overflow.
@@ -377,7 +421,7 @@ __device__ __forceinline__ void pb_add(const KArgs& a, PushBuf& pb, bool f, uint
   const uint64_t m = ballot(f);
   if (m == 0) return;
   const uint32_t c = (uint32_t)__popcll(m);
-  if (pb.n + c > PB_CAP) pb_flush(a, pb, pb.tc, ws);
+  if (pb.n + c > PB_CAP) pb_flush(flush_args(a), pb, pb.tc, ws);
   if (f) {
     const uint32_t i = pb.n + mbcnt(m);
     pb.e[i] = entry;
@@ -527,6 +571,12 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   bool missed = false;  // the last claim attempt lost: probe with atomic loads next time
   uint64_t tk[4] = {0, 0, 0, 0};  // activity ticks: probe, items, edges, idle (wave-uniform)
   uint32_t sticky_q = 0xFFFFFFFFu, sticky_b = 0;  // queue and bucket of the last full claim (see below)
+  // Local continuation: improvements a light claim finds for its own bucket and part (at most 64, one
+  // per lane) are processed by the same wavefront as its next claim instead of going through the
+  // queue: no reservation, publication, probe or claim round trips for a hop inside a bucket.  They stay
+  // counted in flight for that bucket's light phase until this wavefront has processed them.
+  uint32_t nlocal = 0, local_b = 0;
+  uint64_t local_e = 0;
   if (wave_gid == 0 && lane == 0) trace_event(a, 0, 0, 0);
 
   for (;;) {
@@ -535,11 +585,19 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     // `parts` lanes each: light behind, light head, heavy behind, heavy head (only once the head's
     // phase word is set), light of the concur-1 buckets ahead.  Bit order is the claim priority; each
     // wavefront starts at its own part.
-    bool got = false;
+    bool got = false, from_local = false;
     uint32_t q = 0, h = 0, cnt = 0, cb = 0;
+    if (nlocal) {
+      got = true;
+      from_local = true;
+      cb = local_b;
+      q = (cb & (a.nb - 1)) * a.parts + part;
+      cnt = nlocal;
+      nlocal = 0;
+    }
     // Sticky claim: after a full claim from the behind or head bucket, the same queue is claimed again
     // directly (no probe) while that bucket is still behind or head.
-    if (sticky_q != 0xFFFFFFFFu) {
+    if (!got && sticky_q != 0xFFFFFFFFu) {
       const uint32_t c0 = ald(a.st + S_CUR);
       if (sticky_b + 1 >= c0 && sticky_b <= c0) {
         got = try_claim(a, sticky_q, sticky_b, nq, h, cnt, ws);
@@ -632,7 +690,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       // ---------------- load the claimed entries (IPL per lane: item k = j * 64 + lane) ----------------
       // Positions h .. h+cnt-1 span at most two pool chunks (cnt <= 256 < CH).
       uint32_t p0 = TAB_BAD, p1 = TAB_BAD;
-      {
+      if (!from_local) {
         const uint32_t j0 = h >> CH_SHIFT, j1 = (h + cnt - 1) >> CH_SHIFT;
         if (lane == 0 && j1 < a.maxv) {
           p0 = chunk_of(a, pbuf.tc, q, j0);
@@ -647,7 +705,14 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       }
       bool have[IPL];
       uint32_t gid[IPL], du[IPL];
-      {
+      if (from_local) {
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          have[j] = j == 0 && lane < cnt;
+          gid[j] = (uint32_t)local_e;
+          du[j] = (uint32_t)(local_e >> 32);
+        }
+      } else {
         uint64_t ev[IPL];
         const uint64_t* ep[IPL];
 #pragma unroll
@@ -851,9 +916,29 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       }
       // The claim stops counting as in flight.  Its pushes need not have completed: their tail
       // reservations have (atomics with a result), so the queues already read as non-empty to the head
-      // advance and to the termination test.
-      pb_flush(a, pbuf, pbuf.tc, ws);
-      if (lane == 0) afadd(inflight(a, cb & (a.nb - 1)) + (q % a.parts) + (heavy_claim ? MAX_PARTS : 0u), 0u - cnt);
+      // advance and to the termination test.  Improvements kept for local continuation stay counted,
+      // under this wavefront's own part of the bucket.
+      const uint32_t own_q = (a.local_cont && !heavy_claim) ? (cb & (a.nb - 1)) * a.parts + part : 0xFFFFFFFFu;
+      const uint32_t nl = pb_flush(flush_args(a), pbuf, pbuf.tc, ws, own_q);
+      if (nl) {
+        local_e = lane < nl ? pbuf.e[lane] : 0ull;
+        wave_sync();
+        nlocal = nl;
+        local_b = cb;
+      }
+      if (lane == 0) {
+        uint32_t* fl = inflight(a, cb & (a.nb - 1));
+        const uint32_t ic = (q % a.parts) + (heavy_claim ? MAX_PARTS : 0u);
+        if (ic == part) {
+          afadd(fl + ic, nl - cnt);
+        } else {
+          if (nl) {
+            afadd(fl + part, nl);
+            mem_wait();  // counted under the new part before the claim's count is released
+          }
+          afadd(fl + ic, 0u - cnt);
+        }
+      }
       tk[2] += wall_clock64() - t2;
       if (hld(a.st + S_DONE + 1, true) != 0) break;
       continue;

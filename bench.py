@@ -123,14 +123,23 @@ def run_c3(host, name, steps, warmup, torch, adds_amd, oracle, do_cpu, certify=T
     return res
 
 
+def tuned(dg, host, srcs, name):
+    from adds_amd.autotune import tune as autotune
+    t0 = time.perf_counter()
+    params = autotune(dg, host, srcs)
+    log(f"{name}: tuned {params} ({time.perf_counter() - t0:.1f}s)")
+    return params
+
+
 def run_small(host, name, src, steps, warmup, torch, adds_amd, oracle):
     dg = adds_amd.Graph(host)
+    params = tuned(dg, host, [src], name)
     dist = torch.empty(host.n, dtype=torch.int32, device="cuda")
-    wall, stats = time_single(dg, src, dist.data_ptr(), steps, warmup, torch)
+    wall, stats = time_single(dg, src, dist.data_ptr(), steps, warmup, torch, **params)
     d = dist.cpu().numpy().view(np.uint32)
     n_r, m_r = host.reached_counts(d)
     ref = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)
-    res = {"n": host.n, "m": host.m, "n_r": n_r, "m_r": m_r, "ms_per_step": wall * 1e3,
+    res = {"n": host.n, "m": host.m, "n_r": n_r, "m_r": m_r, "params": params, "ms_per_step": wall * 1e3,
            "kernel_ms_event": float(np.mean([s["kernel_ms"] for s in stats])),
            "gteps": m_r / wall / 1e9, "bit_exact_vs_dijkstra": bool(np.array_equal(ref, d))}
     dg.close()
@@ -143,9 +152,10 @@ def run_c4a(steps, torch, adds_amd, oracle):
     srcs = host.sample_component(host.max_degree_vertex(), 256, seed=40)
     k = len(srcs)
     dg = adds_amd.Graph(host)
+    params = tuned(dg, host, srcs, "c4a")
     dist = torch.empty((k, host.n), dtype=torch.int32, device="cuda")
     srcs_dev = torch.from_numpy(srcs.astype(np.int32)).cuda()
-    sg = adds_amd.SolveGraph(dg, srcs_dev.data_ptr(), k, dist.data_ptr())
+    sg = adds_amd.SolveGraph(dg, srcs_dev.data_ptr(), k, dist.data_ptr(), **params)
     sg.launch()
     sg.status()
     torch.cuda.synchronize()
@@ -158,11 +168,12 @@ def run_c4a(steps, torch, adds_amd, oracle):
     dall = dist.cpu().numpy().view(np.uint32)
     # loop of single-source launches into the same buffer rows
     one = torch.empty(host.n, dtype=torch.int32, device="cuda")
-    dg.sssp_device(int(srcs[0]), one.data_ptr())
+    p1 = tuned(dg, host, [int(srcs[0])], "c4a single-source")  # the loop gets its own best parameters
+    dg.sssp_device(int(srcs[0]), one.data_ptr(), **p1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in srcs:
-        dg.sssp_device(int(s), one.data_ptr())
+        dg.sssp_device(int(s), one.data_ptr(), **p1)
     torch.cuda.synchronize()
     loop_s = time.perf_counter() - t0
     check = [0, k // 2, k - 1]
@@ -233,11 +244,12 @@ def run_c5(steps, torch, adds_amd, oracle):
         c = rng.integers(-(1 << 23), 1 << 23, size=4096).astype(np.float32) / np.float32(1 << 23)
         gdist[s, torch.from_numpy(tg).cuda()] = torch.from_numpy(c).cuda()
     gw = torch.empty(host.m, dtype=torch.float32, device="cuda")
+    params = tuned(dg, host, srcs, "c5")
     fwd, bwd = [], []
     for i in range(steps + 1):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        dg.sssp_multi_device(srcs, dist.data_ptr())
+        dg.sssp_multi_device(srcs, dist.data_ptr(), **params)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         bms = dg.backward_device(k, dist.data_ptr(), gdist.data_ptr(), gw.data_ptr())
@@ -246,7 +258,7 @@ def run_c5(steps, torch, adds_amd, oracle):
             fwd.append((t1 - t0) * 1e3)
             bwd.append(bms)
     ref_gw, _ = oracle.tree_grad(host.row_ptr, host.col, host.w, dist.cpu().numpy(), gdist.cpu().numpy())
-    res = {"n": host.n, "m": host.m, "sources": k, "forward_ms": float(np.median(fwd)),
+    res = {"n": host.n, "m": host.m, "sources": k, "params": params, "forward_ms": float(np.median(fwd)),
            "backward_ms": float(np.median(bwd)),
            "grad_bit_exact_vs_cpu_tree": bool(np.array_equal(ref_gw.view(np.uint32),
                                                              gw.cpu().numpy().view(np.uint32)))}

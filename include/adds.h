@@ -63,6 +63,8 @@ typedef struct {
 #define ADDS_F_HOST_OUT 0x1u  /* dist output buffers are host memory (copied back, then synchronised) */
 #define ADDS_F_NO_SYNC 0x2u   /* do not synchronise; status of the solve is read by adds_last_status */
 #define ADDS_F_ATOMIC_PROBE 0x4u /* probe queue counters with atomic (cache-bypassing) loads only */
+#define ADDS_F_NO_LOCAL 0x8u  /* every improvement goes through the queues (no local continuation); a
+                                 tuning choice: local continuation wins on high-diameter graphs */
 
 typedef struct {
   double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */
@@ -84,6 +86,7 @@ typedef struct {
   uint64_t ticks_items;       /* loading and filtering claimed items, filing, recording long ranges */
   uint64_t ticks_edges;       /* relaxing the claimed items' edges and pushing the improvements */
   uint64_t ticks_idle;        /* termination tests, head advances and back-off while no work is visible */
+  uint64_t local_items;       /* improvements processed by the wavefront that found them (no queue trip) */
 } adds_stats;
 
 /* ---- errors / device ---- */
