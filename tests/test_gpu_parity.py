@@ -57,6 +57,7 @@ def test_grid2048_float_c2():
     dict(parts=1, nbuckets=4, chunk=1, wg_per_cu=1),
     dict(parts=8, nbuckets=256, chunk=64, wg_per_cu=8, concurrency=3),
     dict(parts=4, nbuckets=8, chunk=48, wg_per_cu=2, concurrency=4),
+    dict(flags=0x8 | 0x4, delta=64.0),  # no local continuation, uncached probes
     dict(parts=2, nbuckets=16, chunk=17, delta=1.0, hub_degree=64, task_edges=64),
     dict(parts=4, nbuckets=32, chunk=32, delta=100000.0, concurrency=2),
 ])
@@ -157,3 +158,14 @@ def test_weights_update_resplits():
     for delta in (2.0, 8.0):
         dist, _ = dg.sssp(src, delta=delta, timeout_ms=TIMEOUT)
         _check(g2, dist, src, True)
+
+
+@pytest.mark.parametrize("params", [dict(), dict(delta=16.0), dict(delta=16.0, flags=0x8)])
+def test_multi_source_grid(params):
+    """Many sources on a high-diameter graph: replicated instances share the buckets, and hops inside a
+    bucket go through local continuation (on by default)."""
+    g = adds_amd.HostGraph.grid(200, 150, 0, seed=21, wkind="f32", lo=0.5, hi=3.0)
+    srcs = g.sample_component(0, 12, seed=4)
+    dist, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT, **params)
+    for i, s in enumerate(srcs):
+        _check(g, dist[i], s, True)
