@@ -109,6 +109,7 @@ EXPORTS = [
     "adds_host_csr_free", "adds_read_gr", "adds_write_gr", "adds_write_dist_text",
     "adds_gen_rmat", "adds_gen_grid", "adds_gen_er", "adds_gen_weights", "adds_max_degree_vertex",
     "adds_sample_component", "adds_reached_counts", "adds_last_error_detail", "adds_autotune",
+    "adds_autotune_batched",
     "adds_litmus",
 ]
 
@@ -148,6 +149,7 @@ def _lib():
         "adds_solve_graph_destroy": (None, [vp]),
         "adds_sssp_backward_f32": (i32, [vp, u32, vp, vp, vp, ctypes.POINTER(ctypes.c_double), vp]),
         "adds_autotune": (i32, [vp, c_u32p, u32, u32, pp, ctypes.POINTER(ctypes.c_double)]),
+        "adds_autotune_batched": (i32, [vp, c_u32p, u32, u32, pp, ctypes.POINTER(ctypes.c_double)]),
         "adds_litmus": (i32, [u32, u32, u32, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "adds_host_csr_free": (None, [hp]),
         "adds_read_gr": (i32, [ctypes.c_char_p, i32, hp]),

@@ -33,6 +33,16 @@ def _load(path):
         return {}
 
 
+def _store(cache_path, key, params, ms):
+    cache = _load(cache_path)
+    cache[key] = {"params": params, "kernel_ms": ms}
+    try:
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
 def tune(graph, host, srcs, trials=3, cache_path=DEFAULT_CACHE, refresh=False):
     """Returns the tuned parameter dict for solving `srcs` on `graph` (cached by workload key)."""
     srcs = np.ascontiguousarray(np.atleast_1d(srcs), dtype=np.uint32)
@@ -45,11 +55,23 @@ def tune(graph, host, srcs, trials=3, cache_path=DEFAULT_CACHE, refresh=False):
     L.check(L._lib().adds_autotune(graph.handle, srcs.ctypes.data_as(L.c_u32p), len(srcs), trials,
                                    ctypes.byref(best), ctypes.byref(ms)), "autotune")
     params = {f: getattr(best, f) for f in TUNED_FIELDS}
+    _store(cache_path, key, params, ms.value)
+    return params
+
+
+def tune_batched(graphs, hosts, srcs, trials=3, cache_path=DEFAULT_CACHE, refresh=False):
+    """Tuned parameters for adds_sssp_batched over these graphs and sources (the block-diagonal union
+    is what gets tuned); cached under the list of per-graph keys."""
+    srcs = np.ascontiguousarray(srcs, dtype=np.uint32)
+    key = "batched:" + "|".join(workload_key(h, 1) for h in hosts)
     cache = _load(cache_path)
-    cache[key] = {"params": params, "kernel_ms": ms.value}
-    try:
-        with open(cache_path, "w") as f:
-            json.dump(cache, f, indent=1, sort_keys=True)
-    except OSError:
-        pass
+    if not refresh and key in cache:
+        return dict(cache[key]["params"])
+    hs = (ctypes.c_void_p * len(graphs))(*[g.handle.value for g in graphs])
+    best = L.Params()
+    ms = ctypes.c_double()
+    L.check(L._lib().adds_autotune_batched(hs, srcs.ctypes.data_as(L.c_u32p), len(graphs), trials,
+                                           ctypes.byref(best), ctypes.byref(ms)), "autotune_batched")
+    params = {f: getattr(best, f) for f in TUNED_FIELDS}
+    _store(cache_path, key, params, ms.value)
     return params

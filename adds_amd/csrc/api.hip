@@ -319,17 +319,26 @@ struct Resolved {
 
 // Default bucket width.  The reference derives it from the mean weight and degree
 // (wl.h:427-449: nearest power of two to ave_wt/ave_degree/32, at least 4, then adapted online); this
-// solver has no online adaptation and uses a wider default, ave_wt * 8 / ave_degree, chosen by
-// measurement.  For int graphs it is rounded to a power of two like the reference's shift.
+// solver has no online adaptation.  Measured optima on this solver (RMAT scales 16-24, 2-D grids,
+// Erdos-Renyi) sit near ave_wt * 64 / ave_degree up to about 4M vertices and shrink as n^-1.5 beyond
+// (a larger graph fills the GPU with a narrower bucket): ave_wt * 64 / ave_degree * min(1, (2^22/n)^1.5).
+// For int graphs it is rounded to a power of two like the reference's shift.
 static float default_delta(const adds_graph* g) {
   double ave_deg = g->n ? (double)g->m / (double)g->n : 1.0;
   if (ave_deg < 1) ave_deg = 1;
-  double d = (double)g->ave_wt * 8.0 / ave_deg;
+  const double size = g->n ? std::min(1.0, std::pow(4194304.0 / (double)g->n, 1.5)) : 1.0;
+  double d = (double)g->ave_wt * 64.0 / ave_deg * size;
   if (g->wt == ADDS_W_U32) {
     double p = std::exp2(std::round(std::log2(std::max(d, 1.0))));
     return (float)p;
   }
   return (float)std::max(d, 1e-30);
+}
+
+// Default claim size: small graphs need more, smaller claims to spread their few items over the
+// wavefronts; large ones fewer, larger claims (less contention on the queue counters).
+static uint32_t default_chunk(uint64_t total_n) {
+  return total_n <= (256u << 10) ? 64u : total_n <= (8u << 20) ? 128u : 256u;
 }
 
 static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t total_n, Resolved& r) {
@@ -338,7 +347,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   if (!p) p = &z;
   r.delta = p->delta > 0 ? p->delta : default_delta(g);
   r.wg_per_cu = p->wg_per_cu ? p->wg_per_cu : 1;
-  r.chunk = p->chunk ? p->chunk : 256;
+  r.chunk = p->chunk ? p->chunk : default_chunk(total_n);
   r.nb = p->nbuckets ? p->nbuckets : 64;
   r.parts = p->parts ? p->parts : 8;
   r.hub_degree = p->hub_degree ? p->hub_degree : 1024;
@@ -731,6 +740,30 @@ static adds_status build_union(adds_graph* const* graphs, uint32_t k, adds_graph
   return ADDS_OK;
 }
 
+// The block-diagonal union of `graphs` (cached on the first graph's handle) and the sources mapped into it.
+static adds_status get_union(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k, adds_graph** out,
+                             std::vector<uint32_t>& usrc) {
+  adds_graph* g0 = graphs[0];
+  std::vector<const adds_graph*> key(graphs, graphs + k);
+  if (!g0->union_g || g0->union_key != key) {
+    if (g0->union_g) {
+      release_graph_memory(g0->union_g);
+      delete g0->union_g;
+      g0->union_g = nullptr;
+    }
+    adds_status rc = build_union(graphs, k, &g0->union_g, g0->union_off);
+    if (rc != ADDS_OK) return rc;
+    g0->union_key = key;
+  }
+  usrc.assign(k, 0);
+  for (uint32_t i = 0; i < k; i++) {
+    if (srcs[i] >= graphs[i]->n) return ADDS_EINVAL;
+    usrc[i] = g0->union_off[i] + srcs[i];
+  }
+  *out = g0->union_g;
+  return ADDS_OK;
+}
+
 extern "C" adds_status adds_sssp_batched(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k,
                                          void* const* dists, const adds_params* p, adds_stats* stats, void* stream) {
   if (!graphs || !srcs || !dists || k == 0) return ADDS_EINVAL;
@@ -758,24 +791,11 @@ extern "C" adds_status adds_sssp_batched(adds_graph* const* graphs, const uint32
     HIPCHK(hipStreamSynchronize(s));
     return ADDS_OK;
   }
-  std::vector<const adds_graph*> key(graphs, graphs + k);
-  if (!g0->union_g || g0->union_key != key) {
-    if (g0->union_g) {
-      release_graph_memory(g0->union_g);
-      delete g0->union_g;
-      g0->union_g = nullptr;
-    }
-    adds_status rc = build_union(graphs, k, &g0->union_g, g0->union_off);
-    if (rc != ADDS_OK) return rc;
-    g0->union_key = key;
-  }
-  adds_graph* u = g0->union_g;
-  std::vector<uint32_t> usrc(k);
-  for (uint32_t i = 0; i < k; i++) {
-    if (srcs[i] >= graphs[i]->n) return ADDS_EINVAL;
-    usrc[i] = g0->union_off[i] + srcs[i];
-  }
-  adds_status rc = ensure_outbuf(u, u->n);
+  std::vector<uint32_t> usrc;
+  adds_graph* u = nullptr;
+  adds_status rc = get_union(graphs, srcs, k, &u, usrc);
+  if (rc != ADDS_OK) return rc;
+  rc = ensure_outbuf(u, u->n);
   if (rc != ADDS_OK) return rc;
   adds_params q = p ? *p : adds_params{};
   q.flags &= ~(ADDS_F_HOST_OUT | ADDS_F_NO_SYNC);
@@ -934,12 +954,11 @@ __global__ static void k_count_diff(const uint32_t* a, const uint32_t* b, uint64
   if (c) atomicAdd(out, c);
 }
 
-extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32_t k, uint32_t trials,
-                                     adds_params* best, double* best_ms) {
-  if (!g || !srcs || !k || !best) return ADDS_EINVAL;
+static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k, bool repl, uint32_t trials,
+                                 adds_params* best, double* best_ms) {
   HIPCHK(hipSetDevice(g->device));
   if (trials == 0) trials = 3;
-  const uint64_t words = (k > 1 ? (uint64_t)k : 1) * g->n;
+  const uint64_t words = (repl ? (uint64_t)k : 1) * g->n;
   uint32_t *ref = nullptr, *cur = nullptr;
   unsigned long long* diff = nullptr;
   HIPCHK(hipMalloc(&ref, words * 4));
@@ -951,7 +970,7 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
     std::vector<double> t;
     for (uint32_t i = 0; i < trials + 1; i++) {  // first run warms the workspace for this shape
       adds_stats st;
-      adds_status r = solve_core(g, srcs, k, have_ref ? cur : ref, k > 1, &p, &st, nullptr, true);
+      adds_status r = solve_core(g, srcs, k, have_ref ? cur : ref, repl, &p, &st, nullptr, true);
       if (r != ADDS_OK) return r;
       if (i) t.push_back(st.kernel_ms);
       if (!have_ref) {
@@ -974,7 +993,7 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
   adds_params p;
   memset(&p, 0, sizeof(p));
   Resolved r0;
-  rc = resolve(g, &p, words, r0);
+  rc = resolve(g, &p, words, r0);  // words = vertex-instances of the solve
   if (rc == ADDS_OK) {
     p.delta = r0.delta;
     p.wg_per_cu = r0.wg_per_cu;
@@ -1013,10 +1032,9 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
         p = q;
       }
     };
-    // 1) bucket width: powers of two around the default (int keeps power-of-two widths); the default
-    // is too narrow on low-degree, high-diameter graphs, hence the wide range upwards
+    // 1) bucket width: powers of two around the default (int keeps power-of-two widths)
     const float d0 = p.delta;
-    for (int j = -4; j <= 5 && rc == ADDS_OK; j++) {
+    for (int j = -3; j <= 3 && rc == ADDS_OK; j++) {
       if (j == 0) continue;
       adds_params q = p;
       q.delta = d0 * std::ldexp(1.0f, j);
@@ -1069,6 +1087,25 @@ extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32
   (void)hipFree(cur);
   (void)hipFree(diff);
   return rc;
+}
+
+extern "C" adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32_t k, uint32_t trials,
+                                     adds_params* best, double* best_ms) {
+  if (!g || !srcs || !k || !best) return ADDS_EINVAL;
+  return autotune_core(g, srcs, k, k > 1, trials, best, best_ms);
+}
+
+extern "C" adds_status adds_autotune_batched(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k,
+                                             uint32_t trials, adds_params* best, double* best_ms) {
+  if (!graphs || !srcs || !k || !best) return ADDS_EINVAL;
+  for (uint32_t i = 0; i < k; i++)
+    if (!graphs[i]) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(graphs[0]->device));
+  std::vector<uint32_t> usrc;
+  adds_graph* u = nullptr;
+  adds_status rc = get_union(graphs, srcs, k, &u, usrc);
+  if (rc != ADDS_OK) return rc;
+  return autotune_core(u, usrc.data(), k, false, trials, best, best_ms);
 }
 
 // ---- cross-XCD ordering litmus (diagnostic) ----

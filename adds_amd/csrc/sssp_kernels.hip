@@ -37,8 +37,11 @@
 //    wavefront through the ordinary claims, phases and in-flight accounting of the bucket.
 //  * Edges of the claimed vertices are flattened with a wavefront prefix sum and processed in windows
 //    of 64 (four in flight per lane); each lane finds its owning vertex by binary search in LDS
-//    (replacing CUB's WarpScan and the byte owner table, kernel.cu:89-102).  Improvements are
-//    collected per wavefront in LDS and written to the queues in one flush per claim.
+//    (replacing CUB's WarpScan and the byte owner table, kernel.cu:89-102).  Improvements over a
+//    plain-load pre-filter are collected per wavefront in LDS; one flush per claim applies their
+//    atomicMin and queues those that lowered the distance.
+//  * Local continuation: a light claim keeps up to 64 improvements for its own bucket and part and
+//    processes them as its next claim, so a hop inside a bucket costs no queue round trips.
 //  * Termination: a 64-bit word counts (busy transitions << 32 | idle wavefronts).  A wavefront goes
 //    busy before it claims and idle after its pushes have completed.  The solve is over when a
 //    wavefront reads "all idle", finds every queue empty, and re-reads the very same word:

@@ -46,7 +46,7 @@ typedef struct adds_graph adds_graph; /* opaque; owns the device-resident CSR */
 typedef struct {
   float delta;           /* bucket width in distance units */
   uint32_t wg_per_cu;    /* persistent workgroups per compute unit */
-  uint32_t chunk;        /* worklist items a wavefront claims at once (1..256; default 256) */
+  uint32_t chunk;        /* worklist items a wavefront claims at once (1..256; default 64, 128 or 256 by size) */
   uint32_t flags;        /* ADDS_F_* */
   uint32_t nbuckets;     /* bucket ring length (power of two, 4..256) */
   uint32_t parts;        /* sub-queues per bucket (1, 2, 4 or 8; 8 = one per XCD); (concurrency + 3) * parts <= 48 */
@@ -147,6 +147,11 @@ void adds_solve_graph_destroy(adds_solve_graph* sg);
  * reported as an error.  No reference counterpart (the reference adapts delta online, wl.h:851-1310). */
 adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32_t k, uint32_t trials, adds_params* best,
                           double* best_ms);
+/* The same search for adds_sssp_batched over these graphs (tunes the block-diagonal union that
+ * adds_sssp_batched builds and caches); the result is meant for adds_sssp_batched.  No reference
+ * counterpart (batched forms are an addition). */
+adds_status adds_autotune_batched(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k, uint32_t trials,
+                                  adds_params* best, double* best_ms);
 
 /* Diagnostic with no reference counterpart: cross-XCD message-passing litmus test of the ordering recipe
  * the solver uses (relaxed agent-scope atomics ordered by waiting for completion).  `pairs` producer /

@@ -85,3 +85,16 @@ def test_source_out_of_range_is_einval():
     with pytest.raises(adds_amd.AddsError) as ei:
         dg.sssp(host.n + 5)
     assert ei.value.status == L.EINVAL
+
+
+def test_autotune_batched_keeps_results():
+    from adds_amd.autotune import tune_batched
+
+    hosts = [adds_amd.HostGraph.grid(60 + 10 * i, 40, 0, seed=30 + i, wkind="u32", lo=1, hi=500) if i % 2 == 0 else
+             adds_amd.HostGraph.er(4000 + 700 * i, 6, seed=30 + i, wkind="u32", lo=1, hi=500) for i in range(5)]
+    dgs = [adds_amd.Graph(h) for h in hosts]
+    srcs = [0, 3, 1, 7, 2]
+    params = tune_batched(dgs, hosts, srcs, trials=1, cache_path="/tmp/adds_tune_batched_test.json", refresh=True)
+    dists, _ = adds_amd.sssp_batched(dgs, srcs, **params)
+    for h, d, s in zip(hosts, dists, srcs):
+        assert np.array_equal(d.view(np.uint32), oracle.dijkstra(h.row_ptr, h.col, h.w, s, False))
