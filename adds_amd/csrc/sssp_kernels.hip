@@ -287,9 +287,16 @@ __device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uin
 }
 
 // own_q != ~0: up to 64 distance-lowering entries for queue own_q are kept out of the queues and left
-// in pb.e[0 .. returned count) for the caller to process itself (see the local continuation in k_sssp).
-__device__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, uint64_t* tc, WaveStats& ws, uint32_t own_q = 0xFFFFFFFFu) {
-  if (pb.n == 0) return 0;
+// in pbe[0 .. nl) for the caller to process itself (see the local continuation in k_sssp).
+// Everything it touches comes in by value and the counts go out by value: a reference to the caller's
+// locals would put them in scratch memory whenever the compiler keeps the flush out of line.
+struct FlushOut {
+  uint32_t nrel, npush, nl;  // atomicMins issued, entries queued, entries kept for the caller
+};
+__device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq, uint64_t* tc, uint32_t pbn,
+                                 uint32_t own_q) {
+  FlushOut out{0, 0, 0};
+  if (pbn == 0) return out;
   wave_sync();
   const uint32_t lane = lane_id();
   uint64_t e[4];
@@ -298,9 +305,9 @@ __device__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, uint64_t* tc, WaveS
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const uint32_t i = j * 64 + lane;
-    rem[j] = i < pb.n;
-    e[j] = rem[j] ? pb.e[i] : 0ull;
-    q[j] = rem[j] ? pb.q[i] : 0u;
+    rem[j] = i < pbn;
+    e[j] = rem[j] ? pbe[i] : 0ull;
+    q[j] = rem[j] ? pbq[i] : 0u;
     g[j] = 0;
     off[j] = 0;
   }
@@ -330,7 +337,7 @@ __device__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, uint64_t* tc, WaveS
       const uint64_t m = ballot(mine);
       const uint32_t idx = nl + mbcnt(m);
       if (mine && idx < 64) {
-        pb.e[idx] = e[j];
+        pbe[idx] = e[j];
         rem[j] = false;
       }
       nl = min(nl + (uint32_t)__popcll(m), 64u);
@@ -339,11 +346,9 @@ __device__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, uint64_t* tc, WaveS
   }
 #pragma unroll
   for (int j = 0; j < 4; j++) npush += (uint32_t)__popcll(ballot(rem[j]));
-  if (lane == 0) {
-    ws.v[ST_RELAX] += nrel;
-    ws.v[ST_PUSH] += npush;
-    ws.v[ST_LOCAL] += nl;
-  }
+  out.nrel = nrel;
+  out.npush = npush;
+  out.nl = nl;
   for (;;) {
     // group the remaining entries by queue, at most 64 groups per pass (group k lives in lane k)
     bool pend[4];
@@ -414,9 +419,21 @@ __device__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, uint64_t* tc, WaveS
     }
     if (lane < ng) afadd(qavail(a, gq), gc);  // publish
   }
-  pb.n = 0;
   wave_sync();
-  return nl;
+  return out;
+}
+
+// Flushes the push buffer (see flush_buffer); returns the number of entries kept for the caller.
+__device__ __forceinline__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, WaveStats& ws,
+                                             uint32_t own_q = 0xFFFFFFFFu) {
+  const FlushOut o = flush_buffer(a, pb.e, pb.q, pb.tc, pb.n, own_q);
+  pb.n = 0;
+  if (lane_id() == 0) {
+    ws.v[ST_RELAX] += o.nrel;
+    ws.v[ST_PUSH] += o.npush;
+    ws.v[ST_LOCAL] += o.nl;
+  }
+  return o.nl;
 }
 
 // Appends (entry, queue) for every lane with f set; flushes first when the buffer would overflow.
@@ -424,7 +441,7 @@ __device__ __forceinline__ void pb_add(const KArgs& a, PushBuf& pb, bool f, uint
   const uint64_t m = ballot(f);
   if (m == 0) return;
   const uint32_t c = (uint32_t)__popcll(m);
-  if (pb.n + c > PB_CAP) pb_flush(flush_args(a), pb, pb.tc, ws);
+  if (pb.n + c > PB_CAP) pb_flush(flush_args(a), pb, ws);
   if (f) {
     const uint32_t i = pb.n + mbcnt(m);
     pb.e[i] = entry;
@@ -922,7 +939,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       // advance and to the termination test.  Improvements kept for local continuation stay counted,
       // under this wavefront's own part of the bucket.
       const uint32_t own_q = (a.local_cont && !heavy_claim) ? (cb & (a.nb - 1)) * a.parts + part : 0xFFFFFFFFu;
-      const uint32_t nl = pb_flush(flush_args(a), pbuf, pbuf.tc, ws, own_q);
+      const uint32_t nl = pb_flush(flush_args(a), pbuf, ws, own_q);
       if (nl) {
         local_e = lane < nl ? pbuf.e[lane] : 0ull;
         wave_sync();
