@@ -92,6 +92,8 @@ class Stats(ctypes.Structure):
         ("ticks_edges", ctypes.c_uint64),
         ("ticks_idle", ctypes.c_uint64),
         ("local_items", ctypes.c_uint64),
+        ("heavy_edges", ctypes.c_uint64),
+        ("ticks_flush", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -124,6 +126,13 @@ def _lib():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `make -C adds_amd/csrc` "
             "(or __graft_entry__.build()); the ADDS solver has no fallback path")
+    # One HIP runtime per process.  PyTorch for ROCm ships its own HIP and HSA runtimes; once torch is
+    # loaded, libadds.so binds to those (same sonames).  Loaded first, libadds.so would bring in the
+    # system runtime and torch's would then find no GPU.  So torch, when installed, is loaded first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
     hp = ctypes.POINTER(HostCSR)

@@ -364,7 +364,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 256 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
       r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||
-      r.task_edges == 0 || r.hub_degree < 64) {
+      r.task_edges == 0 || r.task_edges > 65536 || r.hub_degree < 64) {
     g_detail = "parameter out of range";
     return ADDS_EINVAL;
   }
@@ -488,6 +488,7 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   if (const char* e = getenv("ADDS_LOCAL")) a.local_cont = atoi(e) != 0;
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
+  a.trace_slice = std::max(1, rate_khz / 20);  // 50 us bins
   return a;
 }
 
@@ -502,9 +503,13 @@ static adds_status enqueue_solve(adds_graph* g, const Resolved& r, uint32_t* dis
                                  const uint32_t* srcs_dev, uint32_t k, bool replicated, hipStream_t s,
                                  bool kernel_events) {
   const uint32_t grid = grid_for(r);
-  if (!g->ws.trace && getenv("ADDS_TRACE")) HIPCHK(hipMalloc(&g->ws.trace, (4 + 4 * (size_t)TRACE_CAP) * 8));
+  if (!g->ws.trace && getenv("ADDS_TRACE"))
+    HIPCHK(hipMalloc(&g->ws.trace, (4 + 4 * (size_t)TRACE_CAP + 8 * (size_t)TRACE_HBINS * TRACE_HCOPIES) * 8));
   KArgs a = make_args(g, r, dist, total_n, grid);
-  if (a.trace) HIPCHK(launch_fill_u64(a.trace, 1, 0, s));
+  if (a.trace) {
+    HIPCHK(launch_fill_u64(a.trace, 1, 0, s));
+    HIPCHK(launch_fill_u64(a.trace + 4 + 4 * (size_t)TRACE_CAP, 8 * (size_t)TRACE_HBINS * TRACE_HCOPIES, 0, s));
+  }
   HIPCHK(launch_init(a, srcs_dev, k, replicated ? 1 : 0, inf_bits(g->wt), s));
   if (kernel_events) HIPCHK(hipEventRecord(g->ws.evk0, s));
   HIPCHK(launch_solve(a, g->wt == ADDS_W_F32, replicated, grid, s));
@@ -560,6 +565,22 @@ static void dump_trace(adds_graph* g) {
     fprintf(f, "%.2f %llu %llu %llu %llu\n", (double)(rec[4 * i] - t0) * us_per_tick,
             (unsigned long long)(rec[4 * i + 1] >> 56), (unsigned long long)(rec[4 * i + 1] & 0xFFFFFFFFull),
             (unsigned long long)rec[4 * i + 2], (unsigned long long)rec[4 * i + 3]);
+  // activity histogram: per 50 us bin of each wavefront's own clock, wavefront-us probing, on items,
+  // on edges and idle, then edges relaxed and items expanded
+  std::vector<uint64_t> hist(8 * (size_t)TRACE_HBINS * TRACE_HCOPIES);
+  if (hipMemcpy(hist.data(), ws.trace + 4 + 4 * (size_t)TRACE_CAP, hist.size() * 8, hipMemcpyDeviceToHost) ==
+      hipSuccess) {
+    fprintf(f, "# hist bin_us=50: bin probe_us items_us edges_us idle_us edges items flush_us\n");
+    for (uint32_t b = 0; b < TRACE_HBINS; b++) {
+      uint64_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (uint32_t c = 0; c < TRACE_HCOPIES; c++)
+        for (int i = 0; i < 8; i++) h[i] += hist[8 * ((size_t)c * TRACE_HBINS + b) + i];
+      if (!(h[0] | h[1] | h[2] | h[3])) continue;
+      fprintf(f, "h %u %.0f %.0f %.0f %.0f %llu %llu %.0f\n", b, h[0] * us_per_tick, h[1] * us_per_tick,
+              h[2] * us_per_tick, h[3] * us_per_tick, (unsigned long long)h[4], (unsigned long long)h[5],
+              h[6] * us_per_tick);
+    }
+  }
   fclose(f);
 }
 
@@ -595,6 +616,8 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->ticks_edges = sv[ST_T_EDGES];
     stats->ticks_idle = sv[ST_T_IDLE];
     stats->local_items = sv[ST_LOCAL];
+    stats->heavy_edges = sv[ST_HEDGES];
+    stats->ticks_flush = sv[ST_T_FLUSH];
   }
   if (ws.trace) dump_trace(g);
   if (err == DERR_OVERFLOW) {

@@ -44,10 +44,13 @@ constexpr uint32_t QLINES = 3;
 constexpr uint32_t s_slotlines(uint32_t nq) { return S_QBASE + QLINES * 2 * nq * LINE; }
 constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_slotlines(nq) + 2 * nb * LINE; }
 constexpr uint32_t MAX_PARTS = 16;
+constexpr uint32_t TRACE_HBINS = 2048;
+constexpr uint32_t TRACE_HCOPIES = 16;  // histogram copies (wavefront id modulo 16), summed by the host
 
 enum StatIdx {
   ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET,
-  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_LATE, ST_LOCAL,
+  ST_PROBE, ST_CASFAIL, ST_IDLE, ST_LATE, ST_LOCAL, ST_HEDGES,
+  ST_T_FLUSH,  // ticks in push-buffer flushes (part of the item and edge time below)
   // wavefront time by activity (wall-clock ticks): probing/claiming, item handling, edge windows,
   // idle (termination test, head advance, back-off)
   ST_T_PROBE, ST_T_ITEMS, ST_T_EDGES, ST_T_IDLE, NSTATS
@@ -80,7 +83,7 @@ struct KArgs {
   uint32_t claim_div;
   uint32_t hrec_cap;
   uint32_t hub_degree;  // ranges longer than this are expanded through range references
-  uint32_t task_edges;  // edges per reference chunk (at least)
+  uint32_t task_edges;  // edges per reference chunk
   uint32_t concur;
   uint32_t total_waves;
   uint64_t timeout_ticks;  // wall_clock64 ticks
@@ -90,8 +93,14 @@ struct KArgs {
   // Optional bucket timeline (null = off): trace[0] = record count, then records of 4 words from
   // trace + 4: wall clock, (kind << 56 | bucket), previous head, pool chunks used.  Kinds: 1 = the
   // head bucket's heavy phase opened, 2 = the head moved to `bucket`, 0 = start, 3 = done.
+  // After the records: an activity histogram of TRACE_HBINS bins of trace_slice ticks, 8 words per bin
+  // (wavefront ticks probing, on items, on edges, idle; edges relaxed; items expanded; ticks in
+  // push-buffer flushes; 1 spare), in
+  // TRACE_HCOPIES copies.  Kind 4 records mark claims with more than 16384 edges (bucket field = edges,
+  // previous-head field = ticks the claim took).
   uint64_t* trace;
-  uint32_t trace_cap;  // records
+  uint32_t trace_cap;    // records
+  uint32_t trace_slice;  // wall-clock ticks per histogram bin
 };
 
 // Host launchers (sssp_kernels.hip)

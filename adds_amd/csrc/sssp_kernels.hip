@@ -426,9 +426,11 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
 // Flushes the push buffer (see flush_buffer); returns the number of entries kept for the caller.
 __device__ __forceinline__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, WaveStats& ws,
                                              uint32_t own_q = 0xFFFFFFFFu) {
+  const uint64_t t0 = wall_clock64();
   const FlushOut o = flush_buffer(a, pb.e, pb.q, pb.tc, pb.n, own_q);
   pb.n = 0;
   if (lane_id() == 0) {
+    ws.v[ST_T_FLUSH] += wall_clock64() - t0;
     ws.v[ST_RELAX] += o.nrel;
     ws.v[ST_PUSH] += o.npush;
     ws.v[ST_LOCAL] += o.nl;
@@ -501,8 +503,8 @@ __device__ __forceinline__ void gather(const KArgs& a, const EdgeGroup& g, Gathe
 // flush applies the atomicMin.  No store or atomic is issued in the edge loop, so the compiler's
 // waits there cover loads only and the pipeline keeps two groups' loads in flight.
 template <bool FLT>
-__device__ __forceinline__ void finish(const KArgs& a, const Gathered& r, uint32_t part, uint32_t cb, PushBuf& pb,
-                                       WaveStats& ws) {
+__device__ __forceinline__ void finish(const KArgs& a, const Gathered& r, uint32_t part, uint32_t cb, bool heavy,
+                                       PushBuf& pb, WaveStats& ws) {
   bool f[UNROLL];
   uint32_t nact = 0;
 #pragma unroll
@@ -510,7 +512,10 @@ __device__ __forceinline__ void finish(const KArgs& a, const Gathered& r, uint32
     f[j] = r.act[j] && r.nd[j] < r.dv[j];
     nact += (uint32_t)__popcll(ballot(r.act[j]));
   }
-  if (lane_id() == 0) ws.v[ST_EDGES] += nact;
+  if (lane_id() == 0) {
+    ws.v[ST_EDGES] += nact;
+    if (heavy) ws.v[ST_HEDGES] += nact;
+  }
 #pragma unroll
   for (int j = 0; j < UNROLL; j++)
     pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j], light_queue<FLT>(a, r.nd[j], part, cb) | PB_RELAX, ws);
@@ -526,6 +531,30 @@ __device__ __forceinline__ void trace_event(const KArgs& a, uint64_t kind, uint3
   r[1] = (kind << 56) | bucket;
   r[2] = prev;
   r[3] = ald(a.st + S_POOL);
+}
+
+// Activity histogram (tracing only): when this wavefront's clock enters a new bin, lane 0 adds what its
+// counters gained since the last flush to the bin it leaves.  hl (LDS, per wavefront): current bin,
+// then the counters at the last flush.
+__device__ __forceinline__ void trace_hist(const KArgs& a, uint64_t* hl, uint64_t rel, const uint64_t* tk,
+                                           uint64_t edges, uint64_t expanded, uint64_t flush, bool last) {
+  const uint64_t bin = min(rel / a.trace_slice, (uint64_t)TRACE_HBINS - 1);
+  if (lane_id() == 0 && (bin != hl[0] || last)) {
+    const uint32_t copy = (blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6)) % TRACE_HCOPIES;
+    uint64_t* h = a.trace + 4 + 4 * (uint64_t)a.trace_cap + (copy * (uint64_t)TRACE_HBINS + hl[0]) * 8;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      afadd64(h + i, tk[i] - hl[1 + i]);
+      hl[1 + i] = tk[i];
+    }
+    afadd64(h + 4, edges - hl[5]);
+    hl[5] = edges;
+    afadd64(h + 5, expanded - hl[6]);
+    hl[6] = expanded;
+    afadd64(h + 6, flush - hl[7]);
+    hl[7] = flush;
+    hl[0] = bin;
+  }
 }
 
 // Claims up to `chunk` items of queue qq (bucket bb): take from the published count (fetch_sub; a
@@ -572,6 +601,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   __shared__ uint64_t s_pbe[WAVES_PER_WG][PB_CAP];
   __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
   __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
+  __shared__ uint64_t s_hl[WAVES_PER_WG][8];
 
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
@@ -598,9 +628,12 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   uint32_t nlocal = 0, local_b = 0;
   uint64_t local_e = 0;
   if (wave_gid == 0 && lane == 0) trace_event(a, 0, 0, 0);
+  if (lane < 8) s_hl[wv][lane] = 0;
+  wave_sync();
 
   for (;;) {
     const uint64_t t0 = wall_clock64();
+    if (a.trace) trace_hist(a, s_hl[wv], t0 - t_start, tk, ws.v[ST_EDGES], ws.v[ST_EXPANDED], ws.v[ST_T_FLUSH], false);
     // ---------------- probe the open queues in parallel ----------------
     // `parts` lanes each: light behind, light head, heavy behind, heavy head (only once the head's
     // phase word is set), light of the concur-1 buckets ahead.  Bit order is the claim priority; each
@@ -795,24 +828,29 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         lcv[j] = a.lcnt[v];
       }
       uint32_t nstale = 0, nreb = 0, nexp = 0, nlate = 0;
+      bool early[IPL], has_heavy[IPL];
 #pragma unroll
       for (int j = 0; j < IPL; j++) {
         const bool stale = have[j] && dnow[j] < du[j];
-        const bool early = have[j] && !stale && !is_ref[j] && !heavy_claim && bucket_key<FLT>(du[j], a.inv_delta) > cb;
-        ok[j] = have[j] && !stale && !early;
-        bool file_heavy = false;
+        early[j] = have[j] && !stale && !is_ref[j] && !heavy_claim && bucket_key<FLT>(du[j], a.inv_delta) > cb;
+        ok[j] = have[j] && !stale && !early[j];
+        has_heavy[j] = false;
         if (!is_ref[j]) {
           const uint32_t lc = lcv[j];
           rs[j] = heavy_claim ? r0v[j] + lc : r0v[j];
           deg[j] = ok[j] ? (heavy_claim ? r1v[j] - rs[j] : lc) : 0u;
-          file_heavy = ok[j] && !heavy_claim && r1v[j] - r0v[j] > lc;
+          has_heavy[j] = r1v[j] - r0v[j] > lc;
         }
         if (!ok[j]) deg[j] = 0;
         nstale += (uint32_t)__popcll(ballot(stale));
-        nreb += (uint32_t)__popcll(ballot(early));
+        nreb += (uint32_t)__popcll(ballot(early[j]));
+      }
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
         nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], a.inv_delta) + 1 < cb));
-        pb_add(a, pbuf, early, ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], part, cb), ws);
+        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], part, cb), ws);
         pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + part, ws);
       }
       if (lane == 0) {
@@ -823,8 +861,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       }
       // ---------------- long ranges become range records + reference items ----------------
       // The references go to the claim's own queue (this bucket, same phase), so every wavefront can
-      // take a chunk.  The records have completed before any reference is written (a consumer reads
-      // the record as soon as it sees the reference).
+      // take a chunk of task_edges edges.  A range of more than REF_MAX_CHUNKS chunks gets consecutive
+      // records of REF_MAX_CHUNKS chunks each, so that reference ids run on across them: id = first
+      // record << REF_CHUNK_BITS + chunk.  The records have completed before any reference is written
+      // (a consumer reads the record as soon as it sees the reference).
       bool split[IPL];
       uint32_t rec[IPL], nref[IPL];
       bool any_split = false;
@@ -834,19 +874,24 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         rec[j] = 0;
         nref[j] = 0;
         if (split[j]) {
-          rec[j] = afadd(a.st + S_HREC, 1u);
-          if (rec[j] >= a.hrec_cap) {
+          const uint32_t te = a.task_edges;
+          nref[j] = (deg[j] + te - 1) / te;
+          const uint32_t nrec = (nref[j] + REF_MAX_CHUNKS - 1) / REF_MAX_CHUNKS;
+          rec[j] = afadd(a.st + S_HREC, nrec);
+          if (rec[j] + nrec > a.hrec_cap) {
             set_error(a, DERR_OVERFLOW);
             split[j] = false;  // relaxed here; the solve reports the overflow and is rerun larger
+            nref[j] = 0;
           } else {
-            const uint32_t te = max(a.task_edges, (deg[j] + REF_MAX_CHUNKS - 1) / REF_MAX_CHUNKS);
-            nref[j] = (deg[j] + te - 1) / te;
-            uint32_t* r = a.hrec + (size_t)rec[j] * HREC_WORDS;
-            ast(r + 0, vgid[j]);
-            ast(r + 1, du[j]);
-            ast(r + 2, rs[j]);
-            ast(r + 3, te);
-            ast(r + 4, rs[j] + deg[j]);
+            for (uint32_t i = 0; i < nrec; i++) {
+              uint32_t* r = a.hrec + (size_t)(rec[j] + i) * HREC_WORDS;
+              const uint32_t s0 = rs[j] + i * (REF_MAX_CHUNKS * te);
+              ast(r + 0, vgid[j]);
+              ast(r + 1, du[j]);
+              ast(r + 2, s0);
+              ast(r + 3, te);
+              ast(r + 4, min(s0 + REF_MAX_CHUNKS * te, rs[j] + deg[j]));
+            }
             ws.v[ST_TASKS] += nref[j];
           }
         }
@@ -861,7 +906,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             const uint32_t L = ffs64(sm);
             const uint32_t nr = rdlane(nref[j], L), rc = rdlane(rec[j], L);
             for (uint32_t k0 = 0; k0 < nr; k0 += 64)
-              pb_add(a, pbuf, k0 + lane < nr, ((uint64_t)REF_TAG << 32) | (rc << REF_CHUNK_BITS) | (k0 + lane), q, ws);
+              pb_add(a, pbuf, k0 + lane < nr, ((uint64_t)REF_TAG << 32) | ((rc << REF_CHUNK_BITS) + k0 + lane), q, ws);
             sm &= sm - 1;
           }
           if (split[j]) deg[j] = 0;
@@ -928,7 +973,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           locate(base + 128 * UNROLL, ga);
           load_cw(a, ga);
           gather<FLT>(a, gb, rb);
-          finish<FLT>(a, ra, part, cb, pbuf, ws);
+          finish<FLT>(a, ra, part, cb, heavy_claim, pbuf, ws);
           ra = rb;
           gb = ga;
         }
@@ -959,7 +1004,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           afadd(fl + ic, 0u - cnt);
         }
       }
-      tk[2] += wall_clock64() - t2;
+      const uint64_t t3 = wall_clock64();
+      tk[2] += t3 - t2;
+      if (a.trace && total > 16384 && lane == 0) trace_event(a, 4, total, (uint32_t)(t3 - t1));
       if (hld(a.st + S_DONE + 1, true) != 0) break;
       continue;
     }
@@ -1085,6 +1132,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   }
 
   // ---------------- exit: publish this wavefront's counters ----------------
+  if (a.trace)
+    trace_hist(a, s_hl[wv], wall_clock64() - t_start, tk, ws.v[ST_EDGES], ws.v[ST_EXPANDED], ws.v[ST_T_FLUSH], true);
   if (busy) {
     mem_wait();
     if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 1ull);

@@ -37,9 +37,13 @@ class HostGraph:
 
     def __del__(self):
         csr = getattr(self, "_csr", None)
-        if csr is not None and csr.row_ptr:
+        if csr is None or not csr.row_ptr:
+            return
+        try:
             L._lib().adds_host_csr_free(ctypes.byref(csr))
-            self._csr = None
+        except (TypeError, AttributeError):  # interpreter shutdown: module globals already cleared
+            return
+        self._csr = None
 
     # ---- construction ----
     @classmethod
@@ -165,7 +169,10 @@ class Graph:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except (TypeError, AttributeError):  # interpreter shutdown: module globals already cleared
+            pass
 
     @property
     def handle(self):
@@ -266,4 +273,7 @@ class SolveGraph:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except (TypeError, AttributeError):  # interpreter shutdown: module globals already cleared
+            pass

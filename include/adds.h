@@ -51,7 +51,7 @@ typedef struct {
   uint32_t nbuckets;     /* bucket ring length (power of two, 4..256) */
   uint32_t parts;        /* sub-queues per bucket (1, 2, 4 or 8; 8 = one per XCD); (concurrency + 3) * parts <= 48 */
   uint32_t hub_degree;   /* edge ranges longer than this are split into chunks any wavefront can take */
-  uint32_t task_edges;   /* edges per chunk of a split range (default 64; at least; at most 512 chunks per range) */
+  uint32_t task_edges;   /* edges per chunk of a split range (1..65536, default 64) */
   uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
   uint32_t timeout_ms;   /* device watchdog (default 120000) */
   uint64_t pool_entries; /* worklist pool capacity in entries (default 4 per vertex-instance + 1 per 16
@@ -87,6 +87,8 @@ typedef struct {
   uint64_t ticks_edges;       /* relaxing the claimed items' edges and pushing the improvements */
   uint64_t ticks_idle;        /* termination tests, head advances and back-off while no work is visible */
   uint64_t local_items;       /* improvements processed by the wavefront that found them (no queue trip) */
+  uint64_t heavy_edges;       /* of edges_scanned, those relaxed in heavy phases */
+  uint64_t ticks_flush;       /* of ticks_items + ticks_edges, the time in push-buffer flushes */
 } adds_stats;
 
 /* ---- errors / device ---- */

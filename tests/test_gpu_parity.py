@@ -60,6 +60,8 @@ def test_grid2048_float_c2():
     dict(flags=0x8 | 0x4, delta=64.0),  # no local continuation, uncached probes
     dict(parts=2, nbuckets=16, chunk=17, delta=1.0, hub_degree=64, task_edges=64),
     dict(parts=4, nbuckets=32, chunk=32, delta=100000.0, concurrency=2),
+    dict(parts=4, nbuckets=8, chunk=32, delta=1000.0, concurrency=4),
+    dict(parts=2, chunk=8, delta=50000.0, flags=0x8),
 ])
 def test_param_sweep_identical(params):
     """Results never depend on the tuning point (delta, buckets, parts, chunk, hub split)."""
@@ -136,6 +138,10 @@ def test_long_rows_light_heavy_partition(is_float):
     for delta in (16.0, 500.0, 4096.0):  # mostly heavy, mixed, all light
         dist, _ = dg.sssp(0, delta=delta, timeout_ms=TIMEOUT)
         _check(g, dist, 0, is_float)
+    # 16-edge chunks: the 20000-edge hub needs 1250 chunks, over three consecutive range records
+    dist, st = dg.sssp(0, delta=16.0, task_edges=16, hub_degree=64, timeout_ms=TIMEOUT)
+    assert st["hub_tasks"] >= 1250
+    _check(g, dist, 0, is_float)
 
 
 def test_weights_update_resplits():
