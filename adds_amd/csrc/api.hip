@@ -489,6 +489,9 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
   a.trace_slice = std::max(1, rate_khz / 20);  // 50 us bins
+#ifdef ADDS_DBG_EXPCOUNT
+  if (const char* e = getenv("ADDS_DBG_PTR")) a.dbg = (uint32_t*)strtoull(e, nullptr, 0);
+#endif
   return a;
 }
 
@@ -618,6 +621,11 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->local_items = sv[ST_LOCAL];
     stats->heavy_edges = sv[ST_HEDGES];
     stats->ticks_flush = sv[ST_T_FLUSH];
+#ifdef ADDS_DBG_FLUSHT
+    fprintf(stderr, "flusht relaxwait %llu group %llu reserve %llu flushes %llu total_flush %llu\n",
+            (unsigned long long)sv[ST_DBG0], (unsigned long long)sv[ST_DBG1], (unsigned long long)sv[ST_DBG2],
+            (unsigned long long)sv[ST_DBG3], (unsigned long long)sv[ST_T_FLUSH]);
+#endif
   }
   if (ws.trace) dump_trace(g);
   if (err == DERR_OVERFLOW) {

@@ -42,7 +42,12 @@ constexpr uint32_t S_HREC = 7 * LINE;       // next free edge-range record
 constexpr uint32_t S_QBASE = 8 * LINE;
 constexpr uint32_t QLINES = 3;
 constexpr uint32_t s_slotlines(uint32_t nq) { return S_QBASE + QLINES * 2 * nq * LINE; }
-constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_slotlines(nq) + 2 * nb * LINE; }
+// Per bucket slot: the in-flight claim counters (light claims of part p at index p, heavy at 16 + p),
+// each on a cache line of its own -- every claim updates one twice, and packed into one line they were
+// a point of contention for the whole GPU -- then the phase word on a line of its own.
+constexpr uint32_t INFL_STRIDE = LINE;
+constexpr uint32_t SLOT_WORDS = (2 * 16 + 1) * LINE;
+constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_slotlines(nq) + nb * SLOT_WORDS; }
 constexpr uint32_t MAX_PARTS = 16;
 constexpr uint32_t TRACE_HBINS = 2048;
 constexpr uint32_t TRACE_HCOPIES = 16;  // histogram copies (wavefront id modulo 16), summed by the host
@@ -51,6 +56,9 @@ enum StatIdx {
   ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET,
   ST_PROBE, ST_CASFAIL, ST_IDLE, ST_LATE, ST_LOCAL, ST_HEDGES,
   ST_T_FLUSH,  // ticks in push-buffer flushes (part of the item and edge time below)
+#ifdef ADDS_DBG_FLUSHT
+  ST_DBG0, ST_DBG1, ST_DBG2, ST_DBG3,
+#endif
   // wavefront time by activity (wall-clock ticks): probing/claiming, item handling, edge windows,
   // idle (termination test, head advance, back-off)
   ST_T_PROBE, ST_T_ITEMS, ST_T_EDGES, ST_T_IDLE, NSTATS
@@ -101,6 +109,9 @@ struct KArgs {
   uint64_t* trace;
   uint32_t trace_cap;    // records
   uint32_t trace_slice;  // wall-clock ticks per histogram bin
+#ifdef ADDS_DBG_EXPCOUNT
+  uint32_t* dbg;
+#endif
 };
 
 // Host launchers (sssp_kernels.hip)

@@ -139,12 +139,12 @@ template <class A>
 __device__ __forceinline__ uint32_t* qavail(const A& a, uint32_t q) {
   return a.st + S_QBASE + QLINES * q * LINE + 2 * LINE;
 }
-// in-flight counters of bucket slot `slot`: light claims at [part], heavy claims at [16 + part]
-__device__ __forceinline__ uint32_t* inflight(const KArgs& a, uint32_t slot) {
-  return a.st + s_slotlines(nq_of(a)) + 2 * slot * LINE;
+// in-flight counter i of bucket slot `slot`: light claims of part p at i = p, heavy claims at 16 + p
+__device__ __forceinline__ uint32_t* inflight(const KArgs& a, uint32_t slot, uint32_t i) {
+  return a.st + s_slotlines(nq_of(a)) + slot * SLOT_WORDS + i * INFL_STRIDE;
 }
 __device__ __forceinline__ uint32_t* phase(const KArgs& a, uint32_t slot) {
-  return a.st + s_slotlines(nq_of(a)) + 2 * slot * LINE + LINE;
+  return a.st + s_slotlines(nq_of(a)) + slot * SLOT_WORDS + SLOT_WORDS - LINE;
 }
 template <class A>
 __device__ __forceinline__ void set_error(const A& a, uint32_t code) {
@@ -292,6 +292,9 @@ __device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uin
 // locals would put them in scratch memory whenever the compiler keeps the flush out of line.
 struct FlushOut {
   uint32_t nrel, npush, nl;  // atomicMins issued, entries queued, entries kept for the caller
+#ifdef ADDS_DBG_FLUSHT
+  uint32_t t[4];
+#endif
 };
 __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq, uint64_t* tc, uint32_t pbn,
                                  uint32_t own_q) {
@@ -322,11 +325,19 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
     old[j] = relax[j] ? afmin(a.dist + (uint32_t)e[j], (uint32_t)(e[j] >> 32)) : 0u;
   }
   uint32_t nrel = 0, npush = 0, nl = 0;
+#ifdef ADDS_DBG_FLUSHT
+  const uint64_t tq0 = wall_clock64();
+#endif
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     nrel += (uint32_t)__popcll(ballot(relax[j]));
     if (relax[j] && old[j] <= (uint32_t)(e[j] >> 32)) rem[j] = false;  // another path got there first
   }
+#ifdef ADDS_DBG_FLUSHT
+  const uint64_t tq1 = wall_clock64();
+  out.t[0] = (uint32_t)(tq1 - tq0);
+  out.t[1] = out.t[2] = out.t[3] = 0;
+#endif
   if (own_q != 0xFFFFFFFFu) {
     // the first 64 improvements for own_q stay with this wavefront (the buffer is free again: its
     // contents are in registers)
@@ -383,6 +394,9 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
       ng++;
     }
     if (ng == 0) break;
+#ifdef ADDS_DBG_FLUSHT
+    const uint64_t tq2 = wall_clock64();
+#endif
     // reserve every group's positions at once and resolve their (at most two) pool chunks
     uint32_t base = 0, p0 = TAB_BAD, p1 = TAB_BAD;
     if (lane < ng) {
@@ -406,6 +420,12 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
         set_error(a, DERR_OVERFLOW);
       }
     }
+#ifdef ADDS_DBG_FLUSHT
+    const uint32_t gb0 = __shfl(base, 0, 64) + __shfl(p0, 0, 64);  // forces the wait here
+    const uint64_t tq3 = wall_clock64() + (gb0 & 0);
+    out.t[1] += (uint32_t)(tq2 - tq1);
+    out.t[2] += (uint32_t)(tq3 - tq2);
+#endif
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const uint32_t gb = __shfl(base, g[j], 64);
@@ -430,6 +450,12 @@ __device__ __forceinline__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, Wav
   const FlushOut o = flush_buffer(a, pb.e, pb.q, pb.tc, pb.n, own_q);
   pb.n = 0;
   if (lane_id() == 0) {
+#ifdef ADDS_DBG_FLUSHT
+    ws.v[ST_DBG0] += o.t[0];
+    ws.v[ST_DBG0 + 1] += o.t[1];
+    ws.v[ST_DBG0 + 2] += o.t[2];
+    ws.v[ST_DBG0 + 3] += 1;
+#endif
     ws.v[ST_T_FLUSH] += wall_clock64() - t0;
     ws.v[ST_RELAX] += o.nrel;
     ws.v[ST_PUSH] += o.npush;
@@ -575,7 +601,7 @@ __device__ __forceinline__ bool try_claim(const KArgs& a, uint32_t qq, uint32_t 
     if (take > 0) {
       hh = afadd(qhead(a, qq), (uint32_t)take);
       const uint32_t ic = (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u);
-      afadd(inflight(a, bb & (a.nb - 1)) + ic, (uint32_t)take);
+      afadd(inflight(a, bb & (a.nb - 1), ic), (uint32_t)take);
       cc = (uint32_t)take;
       ok = 1;
     } else {
@@ -591,6 +617,15 @@ __device__ __forceinline__ bool try_claim(const KArgs& a, uint32_t qq, uint32_t 
 // ------------------------------------------------------------------------------------------------
 // the persistent solver
 // ------------------------------------------------------------------------------------------------
+#ifndef ADDS_PROBE_PERIOD
+#define ADDS_PROBE_PERIOD 8  // an idle wavefront's every 8th probe uses atomic loads
+#endif
+#ifndef ADDS_SLEEP_MID
+#define ADDS_SLEEP_MID 16
+#endif
+#ifndef ADDS_SLEEP_MAX
+#define ADDS_SLEEP_MAX 64
+#endif
 constexpr int IPL = 4;  // claimed items per lane: a claim takes at most 64 * IPL items
 template <bool FLT, bool REPL>
 __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
@@ -661,7 +696,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       }
       if (!got) sticky_q = 0xFFFFFFFFu;
     }
-    const bool cached = a.cached_probe && (idle_iter & 7) != 7 && !missed;
+    const bool cached = a.cached_probe && (idle_iter & (ADDS_PROBE_PERIOD - 1)) != ADDS_PROBE_PERIOD - 1 && !missed;
     missed = false;
     const uint32_t c = got ? 0u : hld(a.st + S_CUR, cached);
     const bool head_heavy_open = !got && hld(phase(a, c & (a.nb - 1)), cached) != 0;
@@ -849,6 +884,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       for (int j = 0; j < IPL; j++) {
         const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
+#ifdef ADDS_DBG_EXPCOUNT
+        if (a.dbg && ok[j] && !is_ref[j] && !heavy_claim) atomicAdd(a.dbg + vgid[j], 1u);
+#endif
         nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], a.inv_delta) + 1 < cb));
         pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], part, cb), ws);
         pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + part, ws);
@@ -992,16 +1030,16 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         local_b = cb;
       }
       if (lane == 0) {
-        uint32_t* fl = inflight(a, cb & (a.nb - 1));
+        const uint32_t sl = cb & (a.nb - 1);
         const uint32_t ic = (q % a.parts) + (heavy_claim ? MAX_PARTS : 0u);
         if (ic == part) {
-          afadd(fl + ic, nl - cnt);
+          afadd(inflight(a, sl, ic), nl - cnt);
         } else {
           if (nl) {
-            afadd(fl + part, nl);
+            afadd(inflight(a, sl, part), nl);
             mem_wait();  // counted under the new part before the claim's count is released
           }
-          afadd(fl + ic, 0u - cnt);
+          afadd(inflight(a, sl, ic), 0u - cnt);
         }
       }
       const uint64_t t3 = wall_clock64();
@@ -1043,7 +1081,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     //     nothing in flight, light or heavy -- the head moves to the first bucket ahead that has
     //     anything queued or in flight, like the reference's manager (wl.h:714-723).  Phase words of
     //     slots leaving the window are cleared.
-    bool advanced = false;
+    bool advanced = false, opened = false;
     uint32_t locked = 0;
     if (lane == 0) {
       uint32_t z = 0;
@@ -1056,18 +1094,23 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       bool head_light = false, head_heavy = false, behind_any = false;
       if (lane < a.parts) {
         head_light = ald(qhead(a, hs * a.parts + lane)) < ald(qtail(a, hs * a.parts + lane)) ||
-                     ald(inflight(a, hs) + lane) != 0;
+                     ald(inflight(a, hs, lane)) != 0;
         head_heavy = ald(qhead(a, nq + hs * a.parts + lane)) < ald(qtail(a, nq + hs * a.parts + lane)) ||
-                     ald(inflight(a, hs) + MAX_PARTS + lane) != 0;
+                     ald(inflight(a, hs, MAX_PARTS + lane)) != 0;
         if (c2 > 0)
           behind_any = ald(qhead(a, bs * a.parts + lane)) < ald(qtail(a, bs * a.parts + lane)) ||
                        ald(qhead(a, nq + bs * a.parts + lane)) < ald(qtail(a, nq + bs * a.parts + lane)) ||
-                       ald(inflight(a, bs) + lane) != 0 || ald(inflight(a, bs) + MAX_PARTS + lane) != 0;
+                       ald(inflight(a, bs, lane)) != 0 || ald(inflight(a, bs, MAX_PARTS + lane)) != 0;
       }
       const bool hl = ballot(head_light) != 0, hh = ballot(head_heavy) != 0, ba = ballot(behind_any) != 0;
-      if (!hl && lane == 0 && ald(phase(a, hs)) == 0) {
-        ast(phase(a, hs), 1u);
-        trace_event(a, 1, c2, c2);
+      if (!hl) {
+        uint32_t op = 0;
+        if (lane == 0 && ald(phase(a, hs)) == 0) {
+          ast(phase(a, hs), 1u);
+          trace_event(a, 1, c2, c2);
+          op = 1;
+        }
+        opened = rdlane(op, 0) != 0;
       }
       if (!hl && !hh && !ba) {
         uint32_t firstj = 0xFFFFFFFFu;
@@ -1076,11 +1119,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           bool ne = false;
           if (jj >= 2 && jj < a.nb) {
             const uint32_t slot = (c2 - 1 + jj) & (a.nb - 1);
-            const uint32_t* fl = inflight(a, slot);
             for (uint32_t pi = 0; pi < a.parts; pi++) {
               const uint32_t ql = slot * a.parts + pi, qh = nq + ql;
               ne |= ald(qhead(a, ql)) < ald(qtail(a, ql)) || ald(qhead(a, qh)) < ald(qtail(a, qh)) ||
-                    ald(fl + pi) != 0 || ald(fl + MAX_PARTS + pi) != 0;
+                    ald(inflight(a, slot, pi)) != 0 || ald(inflight(a, slot, MAX_PARTS + pi)) != 0;
             }
           }
           const uint64_t m = ballot(ne);
@@ -1103,7 +1145,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       mem_wait();
       if (lane == 0) ast(a.st + S_ADV, 0u);
     }
-    if (advanced) {
+    if (advanced || opened) {
+      // this wavefront changed the head's state: look for the work it opened at once, with a fresh
+      // (atomic) probe, without backing off
+      idle_iter = 0;
+      missed = true;
       tk[3] += wall_clock64() - t1;
       continue;
     }
@@ -1126,8 +1172,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     }
     // back off harder the longer this wavefront stays idle, to keep idle probes off the counters
     if (idle_iter < 8) __builtin_amdgcn_s_sleep(2);
-    else if (idle_iter < 64) __builtin_amdgcn_s_sleep(16);
-    else __builtin_amdgcn_s_sleep(64);
+    else if (idle_iter < 64) __builtin_amdgcn_s_sleep(ADDS_SLEEP_MID);
+    else __builtin_amdgcn_s_sleep(ADDS_SLEEP_MAX);
     tk[3] += wall_clock64() - t1;
   }
 
