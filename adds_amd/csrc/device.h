@@ -28,23 +28,26 @@ constexpr uint32_t HREC_WORDS = 8;
 // ---- state words (uint32 indices); every hot counter owns a 128-byte line (32 words) ----
 constexpr uint32_t LINE = 32;
 constexpr uint32_t S_CUR = 0 * LINE;        // absolute index of the head bucket
-constexpr uint32_t S_VER = 1 * LINE;        // u64: (busy transitions << 32) | idle wavefronts
+// lines 1 and 8-15: spare / termination words (below)
 constexpr uint32_t S_DONE = 2 * LINE;       // [0] done flag, [1] error code
 constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
 constexpr uint32_t S_STATS = 4 * LINE;      // u64 stats[NSTATS] (two lines)
 constexpr uint32_t S_ADV = 6 * LINE;        // try-lock serialising head-advance scans
 constexpr uint32_t S_HREC = 7 * LINE;       // next free edge-range record
+// Termination words, one line each for VER_GROUPS groups of workgroups (workgroup b is in group
+// b % VER_GROUPS, which on MI355X is its XCD): u64 (busy transitions << 32) | idle wavefronts.
+constexpr uint32_t VER_GROUPS = 8;
+constexpr uint32_t S_VER = 8 * LINE;
 // Queues: nq = nb * parts light queues (index slot * parts + part) followed by nq heavy queues (index
 // nq + slot * parts + part).  Queue q: head (claimed) at S_QBASE + 3q*LINE, tail (reserved) at +LINE,
-// published-unclaimed count at +2*LINE.  Then two lines per bucket slot: in-flight counts (words
-// [0, 16) light claims per part, [16, 32) heavy claims per part) and the slot's phase word (1 = the
-// heavy queues of the head bucket are open).
-constexpr uint32_t S_QBASE = 8 * LINE;
+// published-unclaimed count at +2*LINE.  Then the bucket slots (SLOT_WORDS below).
+constexpr uint32_t S_QBASE = (8 + VER_GROUPS) * LINE;
 constexpr uint32_t QLINES = 3;
 constexpr uint32_t s_slotlines(uint32_t nq) { return S_QBASE + QLINES * 2 * nq * LINE; }
 // Per bucket slot: the in-flight claim counters (light claims of part p at index p, heavy at 16 + p),
 // each on a cache line of its own -- every claim updates one twice, and packed into one line they were
-// a point of contention for the whole GPU -- then the phase word on a line of its own.
+// a point of contention for the whole GPU -- then the phase word (1 = the heavy queues of the head
+// bucket are open) on a line of its own.
 constexpr uint32_t INFL_STRIDE = LINE;
 constexpr uint32_t SLOT_WORDS = (2 * 16 + 1) * LINE;
 constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_slotlines(nq) + nb * SLOT_WORDS; }

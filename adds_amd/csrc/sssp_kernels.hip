@@ -146,6 +146,11 @@ __device__ __forceinline__ uint32_t* inflight(const KArgs& a, uint32_t slot, uin
 __device__ __forceinline__ uint32_t* phase(const KArgs& a, uint32_t slot) {
   return a.st + s_slotlines(nq_of(a)) + slot * SLOT_WORDS + SLOT_WORDS - LINE;
 }
+// termination word of group g, and the wavefronts in group g (workgroups b < grid with b % VER_GROUPS == g)
+__device__ __forceinline__ uint64_t* verp(const KArgs& a, uint32_t g) { return (uint64_t*)(a.st + S_VER + g * LINE); }
+__device__ __forceinline__ uint32_t group_waves(uint32_t grid, uint32_t g) {
+  return (grid + VER_GROUPS - 1 - g) / VER_GROUPS * WAVES_PER_WG;
+}
 template <class A>
 __device__ __forceinline__ void set_error(const A& a, uint32_t code) {
   uint32_t z = 0;
@@ -251,7 +256,8 @@ __device__ __forceinline__ uint32_t chunk_of(const A& a, uint64_t* tc, uint32_t 
 // reservations' round trips overlap), writes the entries and publishes every group without waiting
 // for the stores: a consumer that claims a position whose entry has not landed yet spins on it
 // (wait_entry), and every published position has been reserved by a producer that is writing it.
-constexpr uint32_t PB_CAP = 256;  // entries per wavefront: 4 per lane in one flush round
+constexpr uint32_t EPL = 4;           // push-buffer entries per lane (measured: 8 or 16 are slower)
+constexpr uint32_t PB_CAP = 64 * EPL;  // entries per wavefront, all handled in one flush round
 constexpr uint32_t PB_RELAX = 0x80000000u;  // queue-word flag: apply the entry's atomicMin at the flush
 
 // The fields of the solver arguments that a flush needs, passed by value to the out-of-line flush
@@ -302,11 +308,11 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
   if (pbn == 0) return out;
   wave_sync();
   const uint32_t lane = lane_id();
-  uint64_t e[4];
-  uint32_t q[4], g[4], off[4];
-  bool rem[4];
+  uint64_t e[EPL];
+  uint32_t q[EPL], g[EPL], off[EPL];
+  bool rem[EPL];
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
+  for (int j = 0; j < (int)EPL; j++) {
     const uint32_t i = j * 64 + lane;
     rem[j] = i < pbn;
     e[j] = rem[j] ? pbe[i] : 0ull;
@@ -316,10 +322,10 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
   }
   // Improvements found by the edge windows carry PB_RELAX: their atomicMin is issued here, all at once,
   // and only those that lowered the distance are pushed.
-  uint32_t old[4];
-  bool relax[4];
+  uint32_t old[EPL];
+  bool relax[EPL];
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
+  for (int j = 0; j < (int)EPL; j++) {
     relax[j] = rem[j] && (q[j] & PB_RELAX) != 0;
     q[j] &= ~PB_RELAX;
     old[j] = relax[j] ? afmin(a.dist + (uint32_t)e[j], (uint32_t)(e[j] >> 32)) : 0u;
@@ -329,7 +335,7 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
   const uint64_t tq0 = wall_clock64();
 #endif
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
+  for (int j = 0; j < (int)EPL; j++) {
     nrel += (uint32_t)__popcll(ballot(relax[j]));
     if (relax[j] && old[j] <= (uint32_t)(e[j] >> 32)) rem[j] = false;  // another path got there first
   }
@@ -343,7 +349,7 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
     // contents are in registers)
     wave_sync();
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < (int)EPL; j++) {
       const bool mine = rem[j] && relax[j] && q[j] == own_q;
       const uint64_t m = ballot(mine);
       const uint32_t idx = nl + mbcnt(m);
@@ -356,27 +362,27 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
     wave_sync();
   }
 #pragma unroll
-  for (int j = 0; j < 4; j++) npush += (uint32_t)__popcll(ballot(rem[j]));
+  for (int j = 0; j < (int)EPL; j++) npush += (uint32_t)__popcll(ballot(rem[j]));
   out.nrel = nrel;
   out.npush = npush;
   out.nl = nl;
   for (;;) {
     // group the remaining entries by queue, at most 64 groups per pass (group k lives in lane k)
-    bool pend[4];
+    bool pend[EPL];
     uint32_t gq = 0, gc = 0, ng = 0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) pend[j] = false;
+    for (int j = 0; j < (int)EPL; j++) pend[j] = false;
     while (ng < 64) {
       uint32_t lq = 0xFFFFFFFFu;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < (int)EPL; j++) {
         const uint64_t m = ballot(rem[j]);
         if (lq == 0xFFFFFFFFu && m) lq = rdlane(q[j], ffs64(m));
       }
       if (lq == 0xFFFFFFFFu) break;
       uint32_t cnt = 0;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < (int)EPL; j++) {
         const bool mine = rem[j] && q[j] == lq;
         const uint64_t m = ballot(mine);
         if (mine) {
@@ -427,7 +433,7 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
     out.t[2] += (uint32_t)(tq3 - tq2);
 #endif
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < (int)EPL; j++) {
       const uint32_t gb = __shfl(base, g[j], 64);
       const uint32_t g0 = __shfl(p0, g[j], 64);
       const uint32_t g1 = __shfl(p1, g[j], 64);
@@ -645,6 +651,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   wave_sync();
   const uint32_t part = blockIdx.x % a.parts;
   const uint32_t wave_gid = blockIdx.x * WAVES_PER_WG + wv;
+  const uint32_t vgrp = blockIdx.x % VER_GROUPS;  // termination group
   const uint32_t nq = nq_of(a);
   const uint32_t nprobe = (a.concur + 3) * a.parts;  // probed queues, one per lane
   WaveStats ws;
@@ -725,7 +732,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     if (lane == 0 && !got) ws.v[ST_PROBE]++;
     if (pm) {
       if (!busy) {
-        if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 0xFFFFFFFFull);  // +1 transition, -1 idle
+        if (lane == 0) afadd64(verp(a, vgrp), 0xFFFFFFFFull);  // +1 transition, -1 idle
         busy = true;
         mem_wait();
       }
@@ -1052,21 +1059,25 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     // ---------------- no work visible: go idle, test termination, move the head ----------------
     if (busy) {
       mem_wait();  // every push of this wavefront has completed
-      if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 1ull);
+      if (lane == 0) afadd64(verp(a, vgrp), 1ull);
       busy = false;
       mem_wait();
     }
     if (stopped(a)) break;
-    {
-      const uint64_t w1 = ald64((const uint64_t*)(a.st + S_VER));
-      if ((uint32_t)w1 == a.total_waves) {
+    // Termination: every group's word reads all idle, every queue is empty, and no group's word has
+    // changed since (a wavefront goes busy before it claims, and idle only after its pushes have
+    // completed).  The own group's word is checked first, so idle wavefronts mostly read one line.
+    if ((uint32_t)ald64(verp(a, vgrp)) == group_waves(gridDim.x, vgrp)) {
+      const uint64_t w1 = lane < VER_GROUPS ? ald64(verp(a, lane)) : 0ull;
+      const uint64_t idle = wave_sum64(w1 & 0xFFFFFFFFull);
+      if (idle == a.total_waves) {
         mem_wait();
         bool nonempty = false;
         for (uint32_t qq = lane; qq < 2 * nq; qq += 64) nonempty |= ald(qhead(a, qq)) != ald(qtail(a, qq));
         const bool any = ballot(nonempty) != 0;
         mem_wait();
-        const uint64_t w2 = ald64((const uint64_t*)(a.st + S_VER));
-        if (!any && w2 == w1) {
+        const uint64_t w2 = lane < VER_GROUPS ? ald64(verp(a, lane)) : 0ull;
+        if (!any && ballot(w2 != w1) == 0) {
           if (lane == 0) {
             ast(a.st + S_DONE, 1u);
             trace_event(a, 3, (uint32_t)ald(a.st + S_CUR), 0);
@@ -1182,7 +1193,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     trace_hist(a, s_hl[wv], wall_clock64() - t_start, tk, ws.v[ST_EDGES], ws.v[ST_EXPANDED], ws.v[ST_T_FLUSH], true);
   if (busy) {
     mem_wait();
-    if (lane == 0) afadd64((uint64_t*)(a.st + S_VER), 1ull);
+    if (lane == 0) afadd64(verp(a, vgrp), 1ull);
   }
 #pragma unroll
   for (int i = 0; i < NSTATS; i++) {
@@ -1211,7 +1222,8 @@ __global__ void k_init_state(KArgs a, const uint32_t* srcs, uint32_t k, int repl
   for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) a.st[i] = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
-    *(uint64_t*)(a.st + S_VER) = (uint64_t)a.total_waves;
+    for (uint32_t g = 0; g < VER_GROUPS; g++)
+      *verp(a, g) = group_waves(a.total_waves / WAVES_PER_WG, g);  // everyone idle
     a.st[S_QBASE + LINE] = k;      // tail of queue 0
     a.st[S_QBASE + 2 * LINE] = k;  // all of it published
     a.st[S_POOL] = (k + CH - 1) >> CH_SHIFT;
