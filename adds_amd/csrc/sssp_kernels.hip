@@ -1063,7 +1063,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       busy = false;
       mem_wait();
     }
-    if (stopped(a)) break;
+    // (the stop words with cached loads except every eighth idle round, like the probes)
+    {
+      const bool fresh = !a.cached_probe || (idle_iter & 7) == 7;
+      if ((hld(a.st + S_DONE, !fresh) | hld(a.st + S_DONE + 1, !fresh)) != 0) break;
+    }
     // Termination: every group's word reads all idle, every queue is empty, and no group's word has
     // changed since (a wavefront goes busy before it claims, and idle only after its pushes have
     // completed).  The own group's word is checked first, so idle wavefronts mostly read one line.
@@ -1094,7 +1098,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     //     slots leaving the window are cleared.
     bool advanced = false, opened = false;
     uint32_t locked = 0;
-    if (lane == 0) {
+    // test (a cached hint, fresh every eighth idle round: a stale copy must not keep the lock looking
+    // taken), then test-and-set
+    if (lane == 0 && hld(a.st + S_ADV, a.cached_probe && (idle_iter & 7) != 7) == 0) {
       uint32_t z = 0;
       locked = acas(a.st + S_ADV, z, 1u) ? 1u : 0u;
     }
