@@ -92,6 +92,9 @@ __device__ __forceinline__ bool acas(uint32_t* p, uint32_t& expected, uint32_t d
 __device__ __forceinline__ uint32_t hld(const uint32_t* p, bool cached) {
   return cached ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : ald(p);
 }
+__device__ __forceinline__ uint64_t hld64(const uint64_t* p, bool cached) {
+  return cached ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : ald64(p);
+}
 // Wait until every memory access issued so far by this wavefront has completed (s_waitcnt with all
 // counters at zero), with compiler fences so that no access moves across.  A workgroup-scope fence
 // would not do: outside tgsplit mode it emits no vmcnt wait at all on gfx950.
@@ -1071,7 +1074,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     // Termination: every group's word reads all idle, every queue is empty, and no group's word has
     // changed since (a wavefront goes busy before it claims, and idle only after its pushes have
     // completed).  The own group's word is checked first, so idle wavefronts mostly read one line.
-    if ((uint32_t)ald64(verp(a, vgrp)) == group_waves(gridDim.x, vgrp)) {
+    if ((uint32_t)hld64(verp(a, vgrp), a.cached_probe && (idle_iter & 7) != 7) == group_waves(gridDim.x, vgrp)) {
       const uint64_t w1 = lane < VER_GROUPS ? ald64(verp(a, lane)) : 0ull;
       const uint64_t idle = wave_sum64(w1 & 0xFFFFFFFFull);
       if (idle == a.total_waves) {
