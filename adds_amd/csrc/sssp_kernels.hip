@@ -592,6 +592,85 @@ __device__ __forceinline__ void trace_hist(const KArgs& a, uint64_t* hl, uint64_
   }
 }
 
+constexpr int IPL = 4;  // claimed items per lane: a claim takes at most 64 * IPL items
+
+// Pushes the reference items of a claim's split ranges into queue q in one go: one reservation for
+// all of them, the pool chunks that the reservation starts allocated in one step (consecutive physical
+// chunks), every lane writing a share, one publication.  (Through the push buffer, a claim holding
+// a few hubs spent tens of flushes on this.)  Item k (= lane * IPL + j) owns reference positions
+// [pref[k], pref[k] + nref) of the reservation; rec is its first range record.  pref and rk are this
+// wavefront's LDS scratch arrays (free at this point).
+__device__ __forceinline__ uint32_t push_refs(const KArgs& a, uint64_t* tc, uint32_t q, uint32_t* pref, uint32_t* rk,
+                                           const bool* split, const uint32_t* rec, const uint32_t* nref) {
+  const uint32_t lane = lane_id();
+  uint32_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < IPL; j++) mine += split[j] ? nref[j] : 0u;
+  const uint32_t incl = wave_incl_scan(mine);
+  const uint32_t total = rdlane(incl, 63);
+  uint32_t run = incl - mine;
+#pragma unroll
+  for (int j = 0; j < IPL; j++) {
+    pref[lane * IPL + j] = run;
+    rk[lane * IPL + j] = split[j] ? rec[j] : 0u;
+    run += split[j] ? nref[j] : 0u;
+  }
+  if (lane == 0) pref[64 * IPL] = total;
+  wave_sync();
+  // reserve, then allocate the chunks whose first slot is ours (before any lookup, as in the flush),
+  // then look up the chunk the reservation starts in if another producer owns it
+  uint32_t base = 0, ph0 = TAB_BAD, pfirst = TAB_BAD, first_owned = 0, j1 = 0;
+  bool okr = false;
+  if (lane == 0) {
+    base = afadd(qtail(a, q), total);
+    const uint32_t j0 = base >> CH_SHIFT;
+    j1 = (base + total - 1) >> CH_SHIFT;
+    first_owned = j0 + ((base & (CH - 1)) != 0 ? 1u : 0u);
+    okr = j1 < a.maxv;
+    if (!okr) set_error(a, DERR_OVERFLOW);
+    if (okr && first_owned <= j1) {
+      const uint32_t n_own = j1 - first_owned + 1;
+      ph0 = afadd(a.st + S_POOL, n_own);
+      if (ph0 + n_own > a.pool_chunks) {
+        set_error(a, DERR_OVERFLOW);
+        okr = false;
+      }
+    }
+  }
+  okr = rdlane(okr ? 1u : 0u, 0) != 0;
+  base = rdlane(base, 0);
+  ph0 = rdlane(ph0, 0);
+  first_owned = rdlane(first_owned, 0);
+  j1 = rdlane(j1, 0);
+  if (okr) {
+    for (uint32_t jj = first_owned + lane; jj <= j1; jj += 64)
+      ast(a.tab + (size_t)q * a.maxv + jj, ph0 + (jj - first_owned));
+    if (lane == 0 && first_owned != (base >> CH_SHIFT)) pfirst = chunk_of(a, tc, q, base >> CH_SHIFT);
+    pfirst = rdlane(pfirst, 0);
+    if (first_owned != (base >> CH_SHIFT) && pfirst == TAB_BAD) okr = false;
+  }
+  if (okr) {
+    for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+      const uint32_t r = r0 + lane;
+      // owning item: the last k with pref[k] <= r (an upward bisection of fixed depth)
+      uint32_t lo = 0;
+#pragma unroll
+      for (uint32_t step = 32 * IPL; step >= 1; step >>= 1)
+        if (pref[lo + step] <= r) lo += step;
+      if (r < total) {
+        const uint64_t e = ((uint64_t)REF_TAG << 32) | ((rk[lo] << REF_CHUNK_BITS) + (r - pref[lo]));
+        const uint32_t pos = base + r, jc = pos >> CH_SHIFT;
+        const uint32_t ph = jc >= first_owned ? ph0 + (jc - first_owned) : pfirst;
+        ast64(a.pool + ((size_t)ph << CH_SHIFT) + (pos & (CH - 1)), e);
+      }
+    }
+  }
+  // publish (also after an error: the positions are reserved; the solve reports the error)
+  if (lane == 0) afadd(qavail(a, q), total);
+  wave_sync();
+  return total;
+}
+
 // Claims up to `chunk` items of queue qq (bucket bb): take from the published count (fetch_sub; a
 // shortfall is given back), then fetch_add the head by what was taken, and count the claim in flight
 // for the bucket.  Published items are at positions the head has not passed, so every claimed position
@@ -635,7 +714,6 @@ __device__ __forceinline__ bool try_claim(const KArgs& a, uint32_t qq, uint32_t 
 #ifndef ADDS_SLEEP_MAX
 #define ADDS_SLEEP_MAX 64
 #endif
-constexpr int IPL = 4;  // claimed items per lane: a claim takes at most 64 * IPL items
 template <bool FLT, bool REPL>
 __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   __shared__ uint32_t s_pref[WAVES_PER_WG][64 * IPL + 1];
@@ -947,18 +1025,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       }
       if (any_split) {
         mem_wait();
+        const uint32_t nr = push_refs(a, pbuf.tc, q, s_pref[wv], s_rs[wv], split, rec, nref);
+        if (lane == 0) ws.v[ST_PUSH] += nr;
 #pragma unroll
-        for (int j = 0; j < IPL; j++) {
-          uint64_t sm = ballot(split[j]);
-          while (sm) {
-            const uint32_t L = ffs64(sm);
-            const uint32_t nr = rdlane(nref[j], L), rc = rdlane(rec[j], L);
-            for (uint32_t k0 = 0; k0 < nr; k0 += 64)
-              pb_add(a, pbuf, k0 + lane < nr, ((uint64_t)REF_TAG << 32) | ((rc << REF_CHUNK_BITS) + k0 + lane), q, ws);
-            sm &= sm - 1;
-          }
+        for (int j = 0; j < IPL; j++)
           if (split[j]) deg[j] = 0;
-        }
       }
       const uint64_t t2 = wall_clock64();
       tk[1] += t2 - t1;
