@@ -310,7 +310,7 @@ extern "C" adds_status adds_graph_device_arrays(const adds_graph* g, const uint3
 // parameters and workspace
 // ------------------------------------------------------------------------------------------------
 struct Resolved {
-  float delta;
+  float delta, light_delta;
   uint32_t wg_per_cu, chunk, nb, parts, hub_degree, task_edges, concur, flags;
   uint64_t pool_entries;
   bool pool_fixed;
@@ -346,6 +346,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   memset(&z, 0, sizeof(z));
   if (!p) p = &z;
   r.delta = p->delta > 0 ? p->delta : default_delta(g);
+  r.light_delta = p->light_delta > 0 ? p->light_delta : r.delta;
   r.wg_per_cu = p->wg_per_cu ? p->wg_per_cu : 1;
   r.chunk = p->chunk ? p->chunk : default_chunk(total_n);
   r.nb = p->nbuckets ? p->nbuckets : 64;
@@ -361,7 +362,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   const uint64_t total_m = g->n ? (total_n / g->n) * g->m : 0;
   r.pool_entries = p->pool_entries ? p->pool_entries
                                    : (4 * total_n + total_m / 16 + (8u << 20)) << std::min(g->ws.pool_shift, 6u);
-  if (!(r.delta > 0) || !std::isfinite(r.delta) || r.wg_per_cu > 64 || r.chunk > 256 ||
+  if (!(r.delta > 0) || !std::isfinite(r.delta) || !(r.light_delta >= r.delta) || r.wg_per_cu > 64 || r.chunk > 256 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
       r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||
       r.task_edges == 0 || r.task_edges > 65536 || r.hub_degree < 64) {
@@ -424,8 +425,8 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
   return ADDS_OK;
 }
 
-// Light-first rows for bucket width `delta` (computed once per graph and width; solves with another
-// width re-split).
+// Light-first rows for light threshold `delta` (computed once per graph and threshold; solves with
+// another threshold re-split).
 static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
   if (!g->lcnt) {
     HIPCHK(hipMalloc(&g->scw, std::max<uint64_t>(g->m, 1) * 8));
@@ -662,7 +663,7 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
   }
   for (int attempt = 0; attempt < 4; attempt++) {
     rc = ensure_workspace(g, r, k);
-    if (rc == ADDS_OK) rc = ensure_split(g, r.delta, s);
+    if (rc == ADDS_OK) rc = ensure_split(g, r.light_delta, s);
     if (rc != ADDS_OK) return rc;
     Workspace& ws = g->ws;
     ws.timeout_ms = r.timeout_ms;
@@ -850,7 +851,7 @@ extern "C" adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* sr
   adds_status rc = resolve(g, p, total_n, r);
   if (rc != ADDS_OK) return rc;
   rc = ensure_workspace(g, r, k);
-  if (rc == ADDS_OK) rc = ensure_split(g, r.delta, nullptr);
+  if (rc == ADDS_OK) rc = ensure_split(g, r.light_delta, nullptr);
   if (rc != ADDS_OK) return rc;
   HIPCHK(hipDeviceSynchronize());
   g->ws.timeout_ms = r.timeout_ms;
@@ -1056,8 +1057,8 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
         return;
       }
       if (getenv("ADDS_TUNE_VERBOSE"))
-        fprintf(stderr, "[tune] delta=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u flags=%u -> %.3f ms\n",
-                q.delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges, q.flags, ms);
+        fprintf(stderr, "[tune] delta=%g light=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u flags=%u -> %.3f ms\n",
+                q.delta, q.light_delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges, q.flags, ms);
       if (ms < bms) {
         bms = ms;
         p = q;
@@ -1070,6 +1071,13 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       adds_params q = p;
       q.delta = d0 * std::ldexp(1.0f, j);
       if (is_int && q.delta < 1.0f) continue;
+      try_point(q);
+    }
+    // light threshold above the bucket width: all edges light (no heavy phases) pays where vertices are
+    // rarely expanded twice (many sources on one graph)
+    for (float ld : {4.0f * p.delta, 1e30f}) {
+      adds_params q = p;
+      q.light_delta = ld;
       try_point(q);
     }
     // 2) bucket ring length and open buckets

@@ -13,7 +13,7 @@
 //    count with one fetch_sub (a shortfall is given back) and then advances the head by what it took:
 //    no compare-and-swap retry loops.
 //  * Light and heavy edges (Meyer & Sanders' delta-stepping).  Each CSR row is stored light-first
-//    (w <= delta) with a per-vertex light count.  Expanding a vertex from bucket b relaxes its light
+//    (w <= light threshold, which is at least delta) with a per-vertex light count.  Expanding a vertex from bucket b relaxes its light
 //    edges and files the vertex in b's heavy queue.  The head bucket's heavy queues open once its light
 //    phase is done (nothing queued or in flight); heavy edges then land in later buckets only, so each
 //    settled vertex relaxes them once.  Without the split, high-degree vertices would re-relax all

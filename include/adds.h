@@ -54,6 +54,10 @@ typedef struct {
   uint32_t task_edges;   /* edges per chunk of a split range (1..65536, default 64) */
   uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
   uint32_t timeout_ms;   /* device watchdog (default 120000) */
+  float light_delta;     /* edges with w <= light_delta are light: relaxed whenever their source is expanded;
+                            the others are heavy: relaxed once, after the source's bucket has settled.  At
+                            least delta (default delta); a value above every weight makes all edges light
+                            (no heavy phases: pays where vertices are rarely expanded twice) */
   uint64_t pool_entries; /* worklist pool capacity in entries (default 4 per vertex-instance + 1 per 16
                             edge-instances + 8M, doubled
                             and remembered per graph after an overflow; a solve that overflows is rerun) */
@@ -144,7 +148,8 @@ adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* stream);
 void adds_solve_graph_destroy(adds_solve_graph* sg);
 
 /* Picks the fastest tuning point for this workload (k sources on g; k == 1 is a plain solve) by
- * coordinate descent over delta, bucket ring length, open buckets, workgroups per CU and claim size,
+ * coordinate descent over delta, light threshold, bucket ring length, sub-queues and open buckets,
+ * workgroups per CU, claim size, reference chunk size and local continuation,
  * timing `trials` solves per point.  Every point's distances must be identical; a difference is
  * reported as an error.  No reference counterpart (the reference adapts delta online, wl.h:851-1310). */
 adds_status adds_autotune(adds_graph* g, const uint32_t* srcs, uint32_t k, uint32_t trials, adds_params* best,
