@@ -938,17 +938,33 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           deg[j] = min(te, r1 - rs[j]);
         }
       }
-      // coherent loads: a stale (larger) distance would defeat the filter; the row bounds are loaded
-      // alongside (unconditionally, at vertex 0 for empty slots) so that all are in flight together
+      // Coherent loads of the distances: a stale (larger) value would defeat the filter.  A partial
+      // claim is latency-bound: the row bounds are loaded alongside (unconditionally, at vertex 0 for
+      // empty slots) so that all are in flight together.  A full claim (its queue had plenty of work)
+      // loads the rows of live items only, after the distances: a second round trip, but a stale item
+      // -- often most of them -- then costs one random load instead of four.
       uint32_t dnow[IPL], r0v[IPL], r1v[IPL], lcv[IPL];
 #pragma unroll
-      for (int j = 0; j < IPL; j++) {
-        gbase[j] = REPL ? (vgid[j] / a.n) * a.n : 0u;
-        const uint32_t v = vgid[j] - gbase[j];
-        dnow[j] = ald(a.dist + vgid[j]);
-        r0v[j] = a.row_ptr[v];
-        r1v[j] = a.row_ptr[v + 1];
-        lcv[j] = a.lcnt[v];
+      for (int j = 0; j < IPL; j++) gbase[j] = REPL ? (vgid[j] / a.n) * a.n : 0u;
+      if (!from_local && cnt == (heavy_claim ? a.chunk_heavy : a.chunk)) {
+#pragma unroll
+        for (int j = 0; j < IPL; j++) dnow[j] = ald(a.dist + vgid[j]);
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          const uint32_t v = (have[j] && dnow[j] >= du[j]) ? vgid[j] - gbase[j] : 0u;
+          r0v[j] = a.row_ptr[v];
+          r1v[j] = a.row_ptr[v + 1];
+          lcv[j] = a.lcnt[v];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          const uint32_t v = vgid[j] - gbase[j];
+          dnow[j] = ald(a.dist + vgid[j]);
+          r0v[j] = a.row_ptr[v];
+          r1v[j] = a.row_ptr[v + 1];
+          lcv[j] = a.lcnt[v];
+        }
       }
       uint32_t nstale = 0, nreb = 0, nexp = 0, nlate = 0;
       bool early[IPL], has_heavy[IPL];
