@@ -311,7 +311,7 @@ extern "C" adds_status adds_graph_device_arrays(const adds_graph* g, const uint3
 // ------------------------------------------------------------------------------------------------
 struct Resolved {
   float delta, light_delta;
-  uint32_t wg_per_cu, chunk, nb, parts, hub_degree, task_edges, concur, flags;
+  uint32_t wg_per_cu, chunk, nb, parts, hub_degree, task_edges, concur, flags, windows;
   uint64_t pool_entries;
   bool pool_fixed;
   double timeout_ms;
@@ -354,6 +354,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.hub_degree = p->hub_degree ? p->hub_degree : 1024;
   r.task_edges = p->task_edges ? p->task_edges : 64;
   r.concur = p->concurrency ? p->concurrency : 2;
+  r.windows = p->edge_windows ? p->edge_windows : 4;
   r.flags = p->flags;
   r.timeout_ms = p->timeout_ms ? (double)p->timeout_ms : 120000.0;
   r.pool_fixed = p->pool_entries != 0;
@@ -365,7 +366,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   if (!(r.delta > 0) || !std::isfinite(r.delta) || !(r.light_delta >= r.delta) || r.wg_per_cu > 64 || r.chunk > 256 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
       r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||
-      r.task_edges == 0 || r.task_edges > 65536 || r.hub_degree < 64) {
+      r.task_edges == 0 || r.task_edges > 65536 || r.hub_degree < 64 || (r.windows != 4 && r.windows != 8)) {
     g_detail = "parameter out of range";
     return ADDS_EINVAL;
   }
@@ -476,6 +477,7 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   if (const char* e = getenv("ADDS_CLAIM_DIV")) a.claim_div = std::max(1, atoi(e));
   a.hub_degree = r.hub_degree;
   a.task_edges = r.task_edges;
+  a.windows = r.windows;
   a.concur = std::min(r.concur, r.nb - 2);
   a.total_waves = grid * WAVES_PER_WG;
   int rate_khz = 100000;
@@ -490,9 +492,6 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
   a.trace_slice = std::max(1, rate_khz / 20);  // 50 us bins
-#ifdef ADDS_DBG_EXPCOUNT
-  if (const char* e = getenv("ADDS_DBG_PTR")) a.dbg = (uint32_t*)strtoull(e, nullptr, 0);
-#endif
   return a;
 }
 
@@ -622,11 +621,6 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->local_items = sv[ST_LOCAL];
     stats->heavy_edges = sv[ST_HEDGES];
     stats->ticks_flush = sv[ST_T_FLUSH];
-#ifdef ADDS_DBG_FLUSHT
-    fprintf(stderr, "flusht relaxwait %llu group %llu reserve %llu flushes %llu total_flush %llu\n",
-            (unsigned long long)sv[ST_DBG0], (unsigned long long)sv[ST_DBG1], (unsigned long long)sv[ST_DBG2],
-            (unsigned long long)sv[ST_DBG3], (unsigned long long)sv[ST_T_FLUSH]);
-#endif
   }
   if (ws.trace) dump_trace(g);
   if (err == DERR_OVERFLOW) {
@@ -1057,8 +1051,9 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
         return;
       }
       if (getenv("ADDS_TUNE_VERBOSE"))
-        fprintf(stderr, "[tune] delta=%g light=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u flags=%u -> %.3f ms\n",
-                q.delta, q.light_delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges, q.flags, ms);
+        fprintf(stderr, "[tune] delta=%g light=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u windows=%u flags=%u -> %.3f ms\n",
+                q.delta, q.light_delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges,
+                q.edge_windows, q.flags, ms);
       if (ms < bms) {
         bms = ms;
         p = q;
@@ -1113,7 +1108,13 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       q.task_edges = te;
       try_point(q);
     }
-    // 5) local continuation off
+    // 5) more edge windows in flight
+    {
+      adds_params q = p;
+      q.edge_windows = 8;
+      try_point(q);
+    }
+    // 6) local continuation off
     {
       adds_params q = p;
       q.flags |= ADDS_F_NO_LOCAL;

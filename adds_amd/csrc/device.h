@@ -59,9 +59,6 @@ enum StatIdx {
   ST_EXPANDED = 0, ST_EDGES, ST_RELAX, ST_PUSH, ST_STALE, ST_TASKS, ST_REBUCKET,
   ST_PROBE, ST_CASFAIL, ST_IDLE, ST_LATE, ST_LOCAL, ST_HEDGES,
   ST_T_FLUSH,  // ticks in push-buffer flushes (part of the item and edge time below)
-#ifdef ADDS_DBG_FLUSHT
-  ST_DBG0, ST_DBG1, ST_DBG2, ST_DBG3,
-#endif
   // wavefront time by activity (wall-clock ticks): probing/claiming, item handling, edge windows,
   // idle (termination test, head advance, back-off)
   ST_T_PROBE, ST_T_ITEMS, ST_T_EDGES, ST_T_IDLE, NSTATS
@@ -95,6 +92,7 @@ struct KArgs {
   uint32_t hrec_cap;
   uint32_t hub_degree;  // ranges longer than this are expanded through range references
   uint32_t task_edges;  // edges per reference chunk
+  uint32_t windows;     // 64-edge windows per group of the edge pipeline (4 or 8)
   uint32_t concur;
   uint32_t total_waves;
   uint64_t timeout_ticks;  // wall_clock64 ticks
@@ -112,9 +110,6 @@ struct KArgs {
   uint64_t* trace;
   uint32_t trace_cap;    // records
   uint32_t trace_slice;  // wall-clock ticks per histogram bin
-#ifdef ADDS_DBG_EXPCOUNT
-  uint32_t* dbg;
-#endif
 };
 
 // Host launchers (sssp_kernels.hip)

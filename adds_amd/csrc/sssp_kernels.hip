@@ -301,9 +301,6 @@ __device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uin
 // locals would put them in scratch memory whenever the compiler keeps the flush out of line.
 struct FlushOut {
   uint32_t nrel, npush, nl;  // atomicMins issued, entries queued, entries kept for the caller
-#ifdef ADDS_DBG_FLUSHT
-  uint32_t t[4];
-#endif
 };
 __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq, uint64_t* tc, uint32_t pbn,
                                  uint32_t own_q) {
@@ -334,19 +331,11 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
     old[j] = relax[j] ? afmin(a.dist + (uint32_t)e[j], (uint32_t)(e[j] >> 32)) : 0u;
   }
   uint32_t nrel = 0, npush = 0, nl = 0;
-#ifdef ADDS_DBG_FLUSHT
-  const uint64_t tq0 = wall_clock64();
-#endif
 #pragma unroll
   for (int j = 0; j < (int)EPL; j++) {
     nrel += (uint32_t)__popcll(ballot(relax[j]));
     if (relax[j] && old[j] <= (uint32_t)(e[j] >> 32)) rem[j] = false;  // another path got there first
   }
-#ifdef ADDS_DBG_FLUSHT
-  const uint64_t tq1 = wall_clock64();
-  out.t[0] = (uint32_t)(tq1 - tq0);
-  out.t[1] = out.t[2] = out.t[3] = 0;
-#endif
   if (own_q != 0xFFFFFFFFu) {
     // the first 64 improvements for own_q stay with this wavefront (the buffer is free again: its
     // contents are in registers)
@@ -403,9 +392,6 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
       ng++;
     }
     if (ng == 0) break;
-#ifdef ADDS_DBG_FLUSHT
-    const uint64_t tq2 = wall_clock64();
-#endif
     // reserve every group's positions at once and resolve their (at most two) pool chunks
     uint32_t base = 0, p0 = TAB_BAD, p1 = TAB_BAD;
     if (lane < ng) {
@@ -429,12 +415,6 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
         set_error(a, DERR_OVERFLOW);
       }
     }
-#ifdef ADDS_DBG_FLUSHT
-    const uint32_t gb0 = __shfl(base, 0, 64) + __shfl(p0, 0, 64);  // forces the wait here
-    const uint64_t tq3 = wall_clock64() + (gb0 & 0);
-    out.t[1] += (uint32_t)(tq2 - tq1);
-    out.t[2] += (uint32_t)(tq3 - tq2);
-#endif
 #pragma unroll
     for (int j = 0; j < (int)EPL; j++) {
       const uint32_t gb = __shfl(base, g[j], 64);
@@ -459,12 +439,6 @@ __device__ __forceinline__ uint32_t pb_flush(const FlushArgs a, PushBuf& pb, Wav
   const FlushOut o = flush_buffer(a, pb.e, pb.q, pb.tc, pb.n, own_q);
   pb.n = 0;
   if (lane_id() == 0) {
-#ifdef ADDS_DBG_FLUSHT
-    ws.v[ST_DBG0] += o.t[0];
-    ws.v[ST_DBG0 + 1] += o.t[1];
-    ws.v[ST_DBG0 + 2] += o.t[2];
-    ws.v[ST_DBG0 + 3] += 1;
-#endif
     ws.v[ST_T_FLUSH] += wall_clock64() - t0;
     ws.v[ST_RELAX] += o.nrel;
     ws.v[ST_PUSH] += o.npush;
@@ -487,25 +461,22 @@ __device__ __forceinline__ void pb_add(const KArgs& a, PushBuf& pb, bool f, uint
   pb.n += c;
 }
 
-// Edges are relaxed in groups of UNROLL 64-edge windows per wavefront (one edge per lane and window),
-// software-pipelined over the groups (see the edge loop in k_sssp) so that the two dependent load
-// rounds -- (col, w), then d[v] -- of consecutive groups overlap.
+// Edges are relaxed in groups of UN 64-edge windows per wavefront (one edge per lane and window; UN is
+// 4 or 8, a tuning choice), software-pipelined over the groups (see the edge loop in k_sssp) so that
+// the two dependent load rounds -- (col, w), then d[v] -- of consecutive groups overlap.
 // Every load is issued unconditionally (inactive lanes read index 0, which exists whenever there is an
 // edge to relax): branches around loads make the compiler wait for each window's loads in turn.
-#ifndef ADDS_UNROLL
-#define ADDS_UNROLL 4
-#endif
-constexpr int UNROLL = ADDS_UNROLL;
-
+template <int UN>
 struct EdgeGroup {
-  bool act[UNROLL];
-  uint32_t e[UNROLL], du[UNROLL], gb[UNROLL], cl[UNROLL], wl[UNROLL];
+  bool act[UN];
+  uint32_t e[UN], du[UN], gb[UN], cl[UN], wl[UN];
 };
 
 // Issues the (col, w) loads of a located group.
-__device__ __forceinline__ void load_cw(const KArgs& a, EdgeGroup& g) {
+template <int UN>
+__device__ __forceinline__ void load_cw(const KArgs& a, EdgeGroup<UN>& g) {
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++) {
+  for (int j = 0; j < UN; j++) {
     const uint32_t es = g.act[j] ? g.e[j] : 0u;
     // streamed once per expansion: a non-temporal load keeps the L2 for the d[v] gathers
     const uint64_t x = __builtin_nontemporal_load((const uint64_t*)a.cw + es);
@@ -515,35 +486,36 @@ __device__ __forceinline__ void load_cw(const KArgs& a, EdgeGroup& g) {
 }
 
 // A group whose d[v] gathers are in flight.
+template <int UN>
 struct Gathered {
-  bool act[UNROLL];
-  uint32_t tgt[UNROLL], nd[UNROLL], dv[UNROLL];
+  bool act[UN];
+  uint32_t tgt[UN], nd[UN], dv[UN];
 };
 
 // Turns a group whose (col, w) loads have been issued into a gathered group: candidate distances and
 // the d[v] pre-filter gathers (plain loads: a stale, larger value only costs an atomic at the flush).
-template <bool FLT>
-__device__ __forceinline__ void gather(const KArgs& a, const EdgeGroup& g, Gathered& r) {
+template <bool FLT, int UN>
+__device__ __forceinline__ void gather(const KArgs& a, const EdgeGroup<UN>& g, Gathered<UN>& r) {
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++) {
+  for (int j = 0; j < UN; j++) {
     r.act[j] = g.act[j];
     r.tgt[j] = g.act[j] ? g.gb[j] + g.cl[j] : 0u;
     r.nd[j] = dadd<FLT>(g.du[j], g.wl[j]);
   }
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++) r.dv[j] = a.dist[r.tgt[j]];
+  for (int j = 0; j < UN; j++) r.dv[j] = a.dist[r.tgt[j]];
 }
 
 // Consumes a gathered group: improvements over the pre-filter value go to the push buffer, whose
 // flush applies the atomicMin.  No store or atomic is issued in the edge loop, so the compiler's
 // waits there cover loads only and the pipeline keeps two groups' loads in flight.
-template <bool FLT>
-__device__ __forceinline__ void finish(const KArgs& a, const Gathered& r, uint32_t part, uint32_t cb, bool heavy,
+template <bool FLT, int UN>
+__device__ __forceinline__ void finish(const KArgs& a, const Gathered<UN>& r, uint32_t part, uint32_t cb, bool heavy,
                                        PushBuf& pb, WaveStats& ws) {
-  bool f[UNROLL];
+  bool f[UN];
   uint32_t nact = 0;
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++) {
+  for (int j = 0; j < UN; j++) {
     f[j] = r.act[j] && r.nd[j] < r.dv[j];
     nact += (uint32_t)__popcll(ballot(r.act[j]));
   }
@@ -552,7 +524,7 @@ __device__ __forceinline__ void finish(const KArgs& a, const Gathered& r, uint32
     if (heavy) ws.v[ST_HEDGES] += nact;
   }
 #pragma unroll
-  for (int j = 0; j < UNROLL; j++)
+  for (int j = 0; j < UN; j++)
     pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j], light_queue<FLT>(a, r.nd[j], part, cb) | PB_RELAX, ws);
 }
 
@@ -714,7 +686,7 @@ __device__ __forceinline__ bool try_claim(const KArgs& a, uint32_t qq, uint32_t 
 #ifndef ADDS_SLEEP_MAX
 #define ADDS_SLEEP_MAX 64
 #endif
-template <bool FLT, bool REPL>
+template <bool FLT, bool REPL, int UN>
 __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   __shared__ uint32_t s_pref[WAVES_PER_WG][64 * IPL + 1];
   __shared__ uint32_t s_rs[WAVES_PER_WG][64 * IPL];
@@ -988,9 +960,6 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       for (int j = 0; j < IPL; j++) {
         const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
-#ifdef ADDS_DBG_EXPCOUNT
-        if (a.dbg && ok[j] && !is_ref[j] && !heavy_claim) atomicAdd(a.dbg + vgid[j], 1u);
-#endif
         nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], a.inv_delta) + 1 < cb));
         pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], part, cb), ws);
         pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + part, ws);
@@ -1071,10 +1040,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         wave_sync();
         // owner of flattened edge i = last range k with s_pref[k] <= i (its length is then non-zero):
         // an upward bisection of fixed depth, interleaved over the windows
-        auto locate = [&](uint32_t base, EdgeGroup& g) {
-          uint32_t idx[UNROLL], lo[UNROLL];
+        auto locate = [&](uint32_t base, EdgeGroup<UN>& g) {
+          uint32_t idx[UN], lo[UN];
 #pragma unroll
-          for (int j = 0; j < UNROLL; j++) {
+          for (int j = 0; j < UN; j++) {
             idx[j] = base + 64 * j + lane;
             g.act[j] = idx[j] < total;
             lo[j] = 0;
@@ -1082,33 +1051,33 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 #pragma unroll
           for (uint32_t step = 32 * IPL; step >= 1; step >>= 1) {
 #pragma unroll
-            for (int j = 0; j < UNROLL; j++) {
+            for (int j = 0; j < UN; j++) {
               const uint32_t mid = lo[j] + step;
               if (s_pref[wv][mid] <= idx[j]) lo[j] = mid;
             }
           }
 #pragma unroll
-          for (int j = 0; j < UNROLL; j++) {
+          for (int j = 0; j < UN; j++) {
             g.e[j] = s_rs[wv][lo[j]] + (idx[j] - s_pref[wv][lo[j]]);
             g.du[j] = s_du[wv][lo[j]];
             g.gb[j] = REPL ? s_gb[wv][lo[j]] : 0u;
           }
         };
-        // Three-stage pipeline over groups of 64 * UNROLL edges: while group g is consumed, the
+        // Three-stage pipeline over groups of 64 * UN edges: while group g is consumed, the
         // gathers of g+1 and the (col, w) loads of g+2 are in flight.  Loads are issued in the order
         // they are waited for, and past the end every window is inactive (its loads read index 0).
-        EdgeGroup ga, gb;
-        Gathered ra, rb;
+        EdgeGroup<UN> ga, gb;
+        Gathered<UN> ra, rb;
         locate(0, ga);
         load_cw(a, ga);
-        locate(64 * UNROLL, gb);
+        locate(64 * UN, gb);
         load_cw(a, gb);
-        gather<FLT>(a, ga, ra);
-        for (uint32_t base = 0; base < total; base += 64 * UNROLL) {
-          locate(base + 128 * UNROLL, ga);
+        gather<FLT, UN>(a, ga, ra);
+        for (uint32_t base = 0; base < total; base += 64 * UN) {
+          locate(base + 128 * UN, ga);
           load_cw(a, ga);
-          gather<FLT>(a, gb, rb);
-          finish<FLT>(a, ra, part, cb, heavy_claim, pbuf, ws);
+          gather<FLT, UN>(a, gb, rb);
+          finish<FLT, UN>(a, ra, part, cb, heavy_claim, pbuf, ws);
           ra = rb;
           gb = ga;
         }
@@ -1498,14 +1467,19 @@ hipError_t launch_init(const KArgs& a, const uint32_t* srcs, uint32_t k, int rep
   hipLaunchKernelGGL(k_init_state, dim3(1), dim3(256), 0, s, a, srcs, k, replicated);
   return hipGetLastError();
 }
-hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s) {
+template <int UN>
+static void launch_solve_un(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s) {
   if (is_float) {
-    if (replicated) hipLaunchKernelGGL((k_sssp<true, true>), dim3(grid), dim3(WG_THREADS), 0, s, a);
-    else hipLaunchKernelGGL((k_sssp<true, false>), dim3(grid), dim3(WG_THREADS), 0, s, a);
+    if (replicated) hipLaunchKernelGGL((k_sssp<true, true, UN>), dim3(grid), dim3(WG_THREADS), 0, s, a);
+    else hipLaunchKernelGGL((k_sssp<true, false, UN>), dim3(grid), dim3(WG_THREADS), 0, s, a);
   } else {
-    if (replicated) hipLaunchKernelGGL((k_sssp<false, true>), dim3(grid), dim3(WG_THREADS), 0, s, a);
-    else hipLaunchKernelGGL((k_sssp<false, false>), dim3(grid), dim3(WG_THREADS), 0, s, a);
+    if (replicated) hipLaunchKernelGGL((k_sssp<false, true, UN>), dim3(grid), dim3(WG_THREADS), 0, s, a);
+    else hipLaunchKernelGGL((k_sssp<false, false, UN>), dim3(grid), dim3(WG_THREADS), 0, s, a);
   }
+}
+hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s) {
+  if (a.windows == 8) launch_solve_un<8>(a, is_float, replicated, grid, s);
+  else launch_solve_un<4>(a, is_float, replicated, grid, s);
   return hipGetLastError();
 }
 hipError_t launch_cleanup(const KArgs& a, hipStream_t s) {

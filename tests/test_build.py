@@ -32,5 +32,5 @@ def test_solver_kernel_uses_no_scratch(tmp_path):
         if m and current:
             scratch[current] = int(m.group(1))
     solvers = {k: v for k, v in scratch.items() if "k_sssp" in k}
-    assert len(solvers) == 4, scratch  # int/float x single/replicated
+    assert len(solvers) == 8, scratch  # int/float x single/replicated x 4/8 edge windows
     assert all(v == 0 for v in solvers.values()), solvers

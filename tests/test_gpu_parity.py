@@ -64,6 +64,7 @@ def test_grid2048_float_c2():
     dict(parts=2, chunk=8, delta=50000.0, flags=0x8),
     dict(delta=100.0, light_delta=1000.0, parts=4),  # light threshold above the bucket width
     dict(delta=64.0, light_delta=1e30),  # every edge light: no heavy phases
+    dict(edge_windows=8, parts=4, concurrency=4),
 ])
 def test_param_sweep_identical(params):
     """Results never depend on the tuning point (delta, buckets, parts, chunk, hub split)."""
@@ -169,7 +170,7 @@ def test_weights_update_resplits():
 
 
 @pytest.mark.parametrize("params", [dict(), dict(delta=16.0), dict(delta=16.0, flags=0x8),
-                                    dict(delta=2.0, light_delta=1e30)])
+                                    dict(delta=2.0, light_delta=1e30, edge_windows=8)])
 def test_multi_source_grid(params):
     """Many sources on a high-diameter graph: replicated instances share the buckets, and hops inside a
     bucket go through local continuation (on by default)."""
