@@ -351,7 +351,8 @@ def main():
                    "bit_exact_vs_dijkstra": head.get("bit_exact_vs_dijkstra")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sssp<false,false>", "kernel_ms": head["kernel_ms_event"],
+                     "kernel": f"k_sssp<false, false, {head['params'].get('edge_windows') or 4}>",
+                     "kernel_ms": head["kernel_ms_event"],
                      "bytes_alg_per_launch": head["bytes_alg"]},
         "cpu_baseline": cpu,
         "extras": dict(extras, c3_int={k: head[k] for k in ("params", "kernel_ms_event", "solve_ms_event",
