@@ -935,7 +935,7 @@ extern "C" adds_status adds_sssp_backward_f32(adds_graph* g, uint32_t k, const f
   adds_status rc = ensure_transpose(g);
   if (rc != ADDS_OK) return rc;
   const uint64_t kn = (uint64_t)k * g->n;
-  rc = ensure_outbuf(g, 4 * kn);
+  rc = ensure_outbuf(g, 5 * kn);
   if (rc != ADDS_OK) return rc;
   hipStream_t s = (hipStream_t)stream;
   BwdArgs b;
@@ -949,7 +949,8 @@ extern "C" adds_status adds_sssp_backward_f32(adds_graph* g, uint32_t k, const f
   b.gdist = grad_dist;
   b.pe = g->outbuf;
   b.cnt = g->outbuf + kn;
-  b.adj = (float*)(g->outbuf + 2 * kn);
+  b.adj = (float*)(g->outbuf + 2 * kn);  // then the parent vertices at 3 * kn (launch_backward)
+  b.cmask = g->outbuf + 4 * kn;
   b.gw = grad_w;
   b.n = g->n;
   b.k = k;

@@ -141,6 +141,7 @@ struct BwdArgs {
   const float* gdist;    // [k][n]
   uint32_t* pe;          // [k][n] parent edge (or ~0)
   uint32_t* cnt;         // [k][n] pending children
+  uint32_t* cmask;       // [k][n] which out-edges (by position in the row) are tree edges
   float* adj;            // [k][n] adjoint
   float* gw;             // [m]
   uint32_t n;
