@@ -985,8 +985,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         rec[j] = 0;
         nref[j] = 0;
         if (split[j]) {
-          const uint32_t te = a.task_edges;
-          nref[j] = (deg[j] + te - 1) / te;
+          // the claim keeps the range's first chunk for itself (so that a claim of long ranges still
+          // has edges to relax at once); the rest goes out as references
+          const uint32_t te = a.task_edges, keep = min(te, a.hub_degree);
+          nref[j] = (deg[j] - keep + te - 1) / te;
           const uint32_t nrec = (nref[j] + REF_MAX_CHUNKS - 1) / REF_MAX_CHUNKS;
           rec[j] = afadd(a.st + S_HREC, nrec);
           if (rec[j] + nrec > a.hrec_cap) {
@@ -996,7 +998,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           } else {
             for (uint32_t i = 0; i < nrec; i++) {
               uint32_t* r = a.hrec + (size_t)(rec[j] + i) * HREC_WORDS;
-              const uint32_t s0 = rs[j] + i * (REF_MAX_CHUNKS * te);
+              const uint32_t s0 = rs[j] + keep + i * (REF_MAX_CHUNKS * te);
               ast(r + 0, vgid[j]);
               ast(r + 1, du[j]);
               ast(r + 2, s0);
@@ -1014,7 +1016,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         if (lane == 0) ws.v[ST_PUSH] += nr;
 #pragma unroll
         for (int j = 0; j < IPL; j++)
-          if (split[j]) deg[j] = 0;
+          if (split[j]) deg[j] = min(a.task_edges, a.hub_degree);
       }
       const uint64_t t2 = wall_clock64();
       tk[1] += t2 - t1;
