@@ -56,6 +56,7 @@ class Params(ctypes.Structure):
         ("task_edges", ctypes.c_uint32),
         ("concurrency", ctypes.c_uint32),
         ("timeout_ms", ctypes.c_uint32),
+        ("tail_items", ctypes.c_uint32),
         ("edge_windows", ctypes.c_uint32),
         ("light_delta", ctypes.c_float),
         ("pool_entries", ctypes.c_uint64),

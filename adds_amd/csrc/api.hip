@@ -311,7 +311,7 @@ extern "C" adds_status adds_graph_device_arrays(const adds_graph* g, const uint3
 // ------------------------------------------------------------------------------------------------
 struct Resolved {
   float delta, light_delta;
-  uint32_t wg_per_cu, chunk, nb, parts, hub_degree, task_edges, concur, flags, windows;
+  uint32_t wg_per_cu, chunk, nb, parts, hub_degree, task_edges, concur, flags, windows, tail_items;
   uint64_t pool_entries;
   bool pool_fixed;
   double timeout_ms;
@@ -355,6 +355,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   r.task_edges = p->task_edges ? p->task_edges : 64;
   r.concur = p->concurrency ? p->concurrency : 2;
   r.windows = p->edge_windows ? p->edge_windows : 4;
+  r.tail_items = p->tail_items;
   r.flags = p->flags;
   r.timeout_ms = p->timeout_ms ? (double)p->timeout_ms : 120000.0;
   r.pool_fixed = p->pool_entries != 0;
@@ -478,6 +479,7 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.hub_degree = r.hub_degree;
   a.task_edges = r.task_edges;
   a.windows = r.windows;
+  a.tail_items = r.tail_items;
   a.concur = std::min(r.concur, r.nb - 2);
   a.total_waves = grid * WAVES_PER_WG;
   int rate_khz = 100000;
@@ -1115,7 +1117,13 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       q.edge_windows = 8;
       try_point(q);
     }
-    // 6) local continuation off
+    // 6) the sparse tail as one bucket
+    for (uint32_t t : {4096u, 65536u}) {
+      adds_params q = p;
+      q.tail_items = t;
+      try_point(q);
+    }
+    // 7) local continuation off
     {
       adds_params q = p;
       q.flags |= ADDS_F_NO_LOCAL;

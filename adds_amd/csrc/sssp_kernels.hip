@@ -288,8 +288,8 @@ struct PushBuf {
 // processed (cb): never before it (such items are simply early work) and never so far ahead that the
 // ring slot would alias the behind bucket.
 template <bool FLT>
-__device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uint32_t part, uint32_t cb) {
-  uint32_t b = bucket_key<FLT>(nd, a.inv_delta);
+__device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uint32_t part, uint32_t cb, bool tail) {
+  uint32_t b = tail ? cb : bucket_key<FLT>(nd, a.inv_delta);  // the tail is processed as one bucket
   b = max(b, cb);
   b = min(b, cb + a.nb - 3);
   return (b & (a.nb - 1)) * a.parts + part;
@@ -511,7 +511,7 @@ __device__ __forceinline__ void gather(const KArgs& a, const EdgeGroup<UN>& g, G
 // waits there cover loads only and the pipeline keeps two groups' loads in flight.
 template <bool FLT, int UN>
 __device__ __forceinline__ void finish(const KArgs& a, const Gathered<UN>& r, uint32_t part, uint32_t cb, bool heavy,
-                                       PushBuf& pb, WaveStats& ws) {
+                                       bool tail, PushBuf& pb, WaveStats& ws) {
   bool f[UN];
   uint32_t nact = 0;
 #pragma unroll
@@ -525,7 +525,7 @@ __device__ __forceinline__ void finish(const KArgs& a, const Gathered<UN>& r, ui
   }
 #pragma unroll
   for (int j = 0; j < UN; j++)
-    pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j], light_queue<FLT>(a, r.nd[j], part, cb) | PB_RELAX, ws);
+    pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j], light_queue<FLT>(a, r.nd[j], part, cb, tail) | PB_RELAX, ws);
 }
 
 // Appends a bucket-timeline record when tracing is on (called by one lane).
@@ -835,6 +835,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         if (rdlane(quit, 0)) break;
       }
       const bool heavy_claim = q >= nq;
+      // Tail: once set (see the head advance), pushes go to the claim's own bucket and nothing is
+      // early, so the few items left are processed as one bucket instead of one bucket -- two phase
+      // changes -- per distance step.  Read as a hint: a wavefront that sees it late only pushes some
+      // items to their own buckets, which is also correct.
+      const bool tail_mode = a.tail_items != 0 && hld(a.st + S_TAIL, true) != 0;
       // ---------------- load the claimed entries (IPL per lane: item k = j * 64 + lane) ----------------
       // Positions h .. h+cnt-1 span at most two pool chunks (cnt <= 256 < CH).
       uint32_t p0 = TAB_BAD, p1 = TAB_BAD;
@@ -943,7 +948,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 #pragma unroll
       for (int j = 0; j < IPL; j++) {
         const bool stale = have[j] && dnow[j] < du[j];
-        early[j] = have[j] && !stale && !is_ref[j] && !heavy_claim && bucket_key<FLT>(du[j], a.inv_delta) > cb;
+        early[j] = have[j] && !stale && !is_ref[j] && !heavy_claim && !tail_mode &&
+                   bucket_key<FLT>(du[j], a.inv_delta) > cb;
         ok[j] = have[j] && !stale && !early[j];
         has_heavy[j] = false;
         if (!is_ref[j]) {
@@ -961,7 +967,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
         nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], a.inv_delta) + 1 < cb));
-        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], part, cb), ws);
+        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], part, cb, tail_mode), ws);
         pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + part, ws);
       }
       if (lane == 0) {
@@ -1079,7 +1085,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           locate(base + 128 * UN, ga);
           load_cw(a, ga);
           gather<FLT, UN>(a, gb, rb);
-          finish<FLT, UN>(a, ra, part, cb, heavy_claim, pbuf, ws);
+          finish<FLT, UN>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
           ra = rb;
           gb = ga;
         }
@@ -1192,6 +1198,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       }
       if (!hl && !hh && !ba) {
         uint32_t firstj = 0xFFFFFFFFu;
+        uint64_t pend = 0;  // items queued or in flight in the window (this lane's buckets)
         for (uint32_t j0 = 0; j0 < a.nb; j0 += 64) {
           const uint32_t jj = j0 + lane;  // bucket c2-1+jj
           bool ne = false;
@@ -1199,18 +1206,29 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             const uint32_t slot = (c2 - 1 + jj) & (a.nb - 1);
             for (uint32_t pi = 0; pi < a.parts; pi++) {
               const uint32_t ql = slot * a.parts + pi, qh = nq + ql;
-              ne |= ald(qhead(a, ql)) < ald(qtail(a, ql)) || ald(qhead(a, qh)) < ald(qtail(a, qh)) ||
-                    ald(inflight(a, slot, pi)) != 0 || ald(inflight(a, slot, MAX_PARTS + pi)) != 0;
+              const uint32_t hl = ald(qhead(a, ql)), tl = ald(qtail(a, ql));  // head first: tl >= hl
+              const uint32_t hh2 = ald(qhead(a, qh)), th = ald(qtail(a, qh));
+              const uint32_t fl = ald(inflight(a, slot, pi)), fh = ald(inflight(a, slot, MAX_PARTS + pi));
+              ne |= hl < tl || hh2 < th || fl != 0 || fh != 0;
+              pend += (uint64_t)(tl - hl) + (th - hh2) + fl + fh;
             }
           }
           const uint64_t m = ballot(ne);
           if (m && firstj == 0xFFFFFFFFu) firstj = j0 + ffs64(m);
         }
+        pend = wave_sum64(pend);
         if (firstj != 0xFFFFFFFFu) {
           const uint32_t c3 = c2 - 1 + firstj;
           // buckets c2-1 .. c3-2 leave the window; their slots come back later as fresh buckets
           for (uint32_t b = c2 - 1 + lane; b + 1 < c3 && b - (c2 - 1) < a.nb; b += 64)
             if (c2 > 0 || b != c2 - 1) ast(phase(a, b & (a.nb - 1)), 0u);
+          // Few items left: from now on the solve runs as one bucket, whose heavy queues are open
+          // at once (a heavy edge relaxed before its vertex has settled is relaxed again later, at
+          // negligible cost this late).
+          if (lane == 0 && a.tail_items != 0 && (pend <= a.tail_items || ald(a.st + S_TAIL) != 0)) {
+            ast(a.st + S_TAIL, 1u);
+            ast(phase(a, c3 & (a.nb - 1)), 1u);
+          }
           mem_wait();
           if (lane == 0) {
             uint32_t cc = c2;

@@ -65,6 +65,8 @@ def test_grid2048_float_c2():
     dict(delta=100.0, light_delta=1000.0, parts=4),  # light threshold above the bucket width
     dict(delta=64.0, light_delta=1e30),  # every edge light: no heavy phases
     dict(edge_windows=8, parts=4, concurrency=4),
+    dict(tail_items=1 << 30, delta=64.0),  # one bucket after the first head move
+    dict(tail_items=4096, delta=16.0, parts=2),
 ])
 def test_param_sweep_identical(params):
     """Results never depend on the tuning point (delta, buckets, parts, chunk, hub split)."""
@@ -170,7 +172,8 @@ def test_weights_update_resplits():
 
 
 @pytest.mark.parametrize("params", [dict(), dict(delta=16.0), dict(delta=16.0, flags=0x8),
-                                    dict(delta=2.0, light_delta=1e30, edge_windows=8)])
+                                    dict(delta=2.0, light_delta=1e30, edge_windows=8),
+                                    dict(delta=2.0, tail_items=20000)])
 def test_multi_source_grid(params):
     """Many sources on a high-diameter graph: replicated instances share the buckets, and hops inside a
     bucket go through local continuation (on by default)."""
