@@ -16,7 +16,7 @@ from . import _lib as L
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_cache.json")
 TUNED_FIELDS = ("delta", "wg_per_cu", "chunk", "nbuckets", "parts", "concurrency", "task_edges", "flags",
-                "light_delta", "edge_windows", "tail_items")
+                "light_delta", "edge_windows", "tail_items", "hub_degree")
 
 
 def workload_key(host, k):

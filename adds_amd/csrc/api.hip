@@ -1054,9 +1054,9 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
         return;
       }
       if (getenv("ADDS_TUNE_VERBOSE"))
-        fprintf(stderr, "[tune] delta=%g light=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u windows=%u flags=%u -> %.3f ms\n",
+        fprintf(stderr, "[tune] delta=%g light=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u hub=%u windows=%u tail=%u flags=%u -> %.3f ms\n",
                 q.delta, q.light_delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges,
-                q.edge_windows, q.flags, ms);
+                q.hub_degree, q.edge_windows, q.tail_items, q.flags, ms);
       if (ms < bms) {
         bms = ms;
         p = q;
@@ -1109,6 +1109,12 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
     for (uint32_t te : {128u, 256u}) {
       adds_params q = p;
       q.task_edges = te;
+      try_point(q);
+    }
+    // longer ranges relaxed by the wavefront that meets them (fewer references)
+    for (uint32_t hd : {4096u, 16384u}) {
+      adds_params q = p;
+      q.hub_degree = hd;
       try_point(q);
     }
     // 5) more edge windows in flight
