@@ -13,8 +13,8 @@
 //    count with one fetch_sub (a shortfall is given back) and then advances the head by what it took:
 //    no compare-and-swap retry loops.
 //  * Light and heavy edges (Meyer & Sanders' delta-stepping).  Each CSR row is stored light-first
-//    (w <= light threshold, which is at least delta) with a per-vertex light count.  Expanding a vertex from bucket b relaxes its light
-//    edges and files the vertex in b's heavy queue.  The head bucket's heavy queues open once its light
+//    (w <= light threshold, which is at least delta) with a per-vertex light count.  Expanding a
+//    vertex from bucket b relaxes its light edges and files the vertex in b's heavy queue.  The head bucket's heavy queues open once its light
 //    phase is done (nothing queued or in flight); heavy edges then land in later buckets only, so each
 //    settled vertex relaxes them once.  Without the split, high-degree vertices would re-relax all
 //    their edges every time a light in-neighbour improved them within their bucket.
@@ -32,28 +32,34 @@
 //    consumer observes the producer's atomicMin.  Pushes are clamped to the window of the bucket being
 //    processed, so no hint of the global head is needed on the hot path.
 //  * Long edge ranges (the reference's TB-cooperative path for high-degree vertices, wl.h:1608-1696)
-//    are not relaxed by the wavefront that meets them: it writes a range record and pushes one
-//    reference item per `task_edges`-edge chunk into its own queue, so the chunks spread over every
-//    wavefront through the ordinary claims, phases and in-flight accounting of the bucket.
+//    are not relaxed by the wavefront that meets them (beyond their first chunk): it writes range
+//    records and pushes one reference item per `task_edges`-edge chunk into its own queue, all in one
+//    reservation, so the chunks spread over every wavefront through the ordinary claims, phases and
+//    in-flight accounting of the bucket.
 //  * Edges of the claimed vertices are flattened with a wavefront prefix sum and processed in windows
-//    of 64 (four in flight per lane); each lane finds its owning vertex by binary search in LDS
+//    of 64 (four or eight in flight per lane); each lane finds its owning vertex by binary search in LDS
 //    (replacing CUB's WarpScan and the byte owner table, kernel.cu:89-102).  Improvements over a
 //    plain-load pre-filter are collected per wavefront in LDS; one flush per claim applies their
 //    atomicMin and queues those that lowered the distance.
 //  * Local continuation: a light claim keeps up to 64 improvements for its own bucket and part and
 //    processes them as its next claim, so a hop inside a bucket costs no queue round trips.
-//  * Termination: a 64-bit word counts (busy transitions << 32 | idle wavefronts).  A wavefront goes
-//    busy before it claims and idle after its pushes have completed.  The solve is over when a
-//    wavefront reads "all idle", finds every queue empty, and re-reads the very same word:
-//    no claim can have happened in between, and idle wavefronts do not push.
+//  * Tail mode (optional): once few items are left when the head moves, the rest of the solve runs as
+//    one bucket, saving the phase changes of a long sparse tail.
+//  * Termination: eight 64-bit words, one per group of workgroups, count (busy transitions << 32 |
+//    idle wavefronts).  A wavefront goes busy before it claims and idle after its pushes have
+//    completed.  The solve is over when a wavefront reads "all idle" in all eight, finds every queue
+//    empty, and re-reads the very same words: no claim can have happened in between, and idle
+//    wavefronts do not push.
+//  * Contention: every word that many wavefronts update lives on a cache line of its own (queue
+//    counters, in-flight counters, termination words), and idle wavefronts poll through cached hints.
 //
 // Memory model (gfx950, one agent, 8 XCDs with private L2s): every word that decides ownership is
 // accessed with relaxed agent-scope atomics (RMW atomics are coherent at agent scope; atomic
 // loads/stores carry sc1).  Ordering between different words is obtained by waiting for completion of
 // the earlier access (an explicit s_waitcnt with all counters at zero, mem_wait below) before issuing
 // the later one.  Plain (L2-cached) loads are used for the CSR, the d[v] pre-filter (a stale, larger
-// value only costs an extra atomic) and, as hints only, for probing queue counters; every
-// eighth idle probe and every termination test use atomic loads.
+// value only costs an extra atomic) and, as hints only, for probing queue counters and idle polls;
+// every eighth idle round and the full termination test use atomic loads.
 #include "device.h"
 
 namespace adds {
