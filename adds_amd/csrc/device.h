@@ -28,7 +28,8 @@ constexpr uint32_t HREC_WORDS = 8;
 // ---- state words (uint32 indices); every hot counter owns a 128-byte line (32 words) ----
 constexpr uint32_t LINE = 32;
 constexpr uint32_t S_CUR = 0 * LINE;        // absolute index of the head bucket
-constexpr uint32_t S_TAIL = 1 * LINE;       // 1 once the solve is in its tail (see tail_items)
+constexpr uint32_t S_TAIL = 1 * LINE;       // [0] 1 once the solve is in its tail (see tail_items),
+                                            // [1] most items pending seen at a head move
 constexpr uint32_t S_DONE = 2 * LINE;       // [0] done flag, [1] error code
 constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
 constexpr uint32_t S_STATS = 4 * LINE;      // u64 stats[NSTATS] (two lines)
@@ -93,8 +94,8 @@ struct KArgs {
   uint32_t hub_degree;  // ranges longer than this are expanded through range references
   uint32_t task_edges;  // edges per reference chunk
   uint32_t windows;     // 64-edge windows per group of the edge pipeline (4 or 8)
-  uint32_t tail_items;  // when the head moves and at most this many items are left in the window, the
-                        // rest of the solve runs as one bucket (0: never)
+  uint32_t tail_items;  // when the head moves and at most this many items are left in the window (and at
+                        // most a quarter of the peak), the rest of the solve runs as one bucket (0: never)
   uint32_t concur;
   uint32_t total_waves;
   uint64_t timeout_ticks;  // wall_clock64 ticks

@@ -1228,12 +1228,20 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           // buckets c2-1 .. c3-2 leave the window; their slots come back later as fresh buckets
           for (uint32_t b = c2 - 1 + lane; b + 1 < c3 && b - (c2 - 1) < a.nb; b += 64)
             if (c2 > 0 || b != c2 - 1) ast(phase(a, b & (a.nb - 1)), 0u);
-          // Few items left: from now on the solve runs as one bucket, whose heavy queues are open
-          // at once (a heavy edge relaxed before its vertex has settled is relaxed again later, at
-          // negligible cost this late).
-          if (lane == 0 && a.tail_items != 0 && (pend <= a.tail_items || ald(a.st + S_TAIL) != 0)) {
-            ast(a.st + S_TAIL, 1u);
-            ast(phase(a, c3 & (a.nb - 1)), 1u);
+          // Few items left, and well past the peak of the solve: from now on the solve runs as one
+          // bucket, whose heavy queues are open at once (a heavy edge relaxed before its vertex has
+          // settled is relaxed again later, at negligible cost this late).  The peak test keeps the
+          // start of a solve, whose frontier is small too, ordered: run as one bucket from there, a
+          // grid solve does many times the work.  Head moves are serialised (S_ADV), so the peak
+          // word is read and written by one wavefront at a time.
+          if (lane == 0 && a.tail_items != 0) {
+            const uint32_t p32 = (uint32_t)min(pend, (uint64_t)0xFFFFFFFFu);
+            const uint32_t peak = max(ald(a.st + S_TAIL + 1), p32);
+            ast(a.st + S_TAIL + 1, peak);
+            if (ald(a.st + S_TAIL) != 0 || (pend <= a.tail_items && pend * 4 <= peak)) {
+              ast(a.st + S_TAIL, 1u);
+              ast(phase(a, c3 & (a.nb - 1)), 1u);
+            }
           }
           mem_wait();
           if (lane == 0) {

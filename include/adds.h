@@ -55,8 +55,9 @@ typedef struct {
   uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
   uint32_t timeout_ms;   /* device watchdog (default 120000) */
   uint32_t tail_items;   /* when the head bucket moves on and at most this many items are left in the bucket
-                            window, the rest of the solve runs as one bucket with its heavy edges relaxed at
-                            once (default 0: never); saves the phase changes of a long sparse tail */
+                            window, no more than a quarter of the most seen at a head move, the rest of the
+                            solve runs as one bucket with its heavy edges relaxed at once (default 0: never);
+                            saves the phase changes of a long sparse tail */
   uint32_t edge_windows; /* 64-edge windows per wavefront per step of the edge pipeline: 4 (default) or 8
                             (more loads in flight; pays on throughput-bound solves) */
   float light_delta;     /* edges with w <= light_delta are light: relaxed whenever their source is expanded;

@@ -65,7 +65,7 @@ def test_grid2048_float_c2():
     dict(delta=100.0, light_delta=1000.0, parts=4),  # light threshold above the bucket width
     dict(delta=64.0, light_delta=1e30),  # every edge light: no heavy phases
     dict(edge_windows=8, parts=4, concurrency=4),
-    dict(tail_items=1 << 30, delta=64.0),  # one bucket after the first head move
+    dict(tail_items=1 << 30, delta=64.0),  # one bucket once the pending work is a quarter of its peak
     dict(tail_items=4096, delta=16.0, parts=2),
 ])
 def test_param_sweep_identical(params):
@@ -182,3 +182,18 @@ def test_multi_source_grid(params):
     dist, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT, **params)
     for i, s in enumerate(srcs):
         _check(g, dist[i], s, True)
+
+
+def test_tail_mode_waits_for_the_peak():
+    """Tail mode (the rest of a solve as one bucket) starts only once the pending work has fallen to a
+    quarter of its peak.  Sources on one edge of a grid start with a small frontier: entering tail mode
+    at the first head move would run the whole solve unordered, with many times the expansions."""
+    g = adds_amd.HostGraph.grid(256, 256, 0, seed=22, wkind="f32", lo=1.0, hi=2.0)
+    srcs = np.arange(0, g.n, g.n // 8, dtype=np.uint32)  # column 0, every 32nd row
+    dg = adds_amd.Graph(g)
+    _, st0 = dg.sssp_multi(srcs, timeout_ms=TIMEOUT, delta=4.0)
+    dist, st = dg.sssp_multi(srcs, timeout_ms=TIMEOUT, delta=4.0, tail_items=4096)
+    for i, s in enumerate(srcs):
+        _check(g, dist[i], s, True)
+    print("expansions without / with tail mode", st0["vertices_expanded"], st["vertices_expanded"])
+    assert st["vertices_expanded"] <= 1.5 * st0["vertices_expanded"]
