@@ -28,7 +28,7 @@ def _lib():
         L.oracle_tree_grad.restype = None
         L.oracle_tree_grad.argtypes = [u32, p, p, p, u32, p, p, p, p]
         L.oracle_delta_step.restype = ctypes.c_int
-        L.oracle_delta_step.argtypes = [ctypes.c_int, u32, p, p, p, u32, ctypes.c_double, ctypes.c_uint, p]
+        L.oracle_delta_step.argtypes = [ctypes.c_int, u32, p, p, p, u32, ctypes.c_double, ctypes.c_uint, p, p]
         _L = L
     return _L
 
@@ -72,7 +72,8 @@ def delta_step(row_ptr, col, w, src, is_float, delta, threads=None):
     n = len(row_ptr) - 1
     d = np.empty(n, dtype=np.uint32)
     threads = threads or os.cpu_count() or 1
-    _lib().oracle_delta_step(int(is_float), n, _p(row_ptr), _p(col), _p(wb), int(src), float(delta), threads, _p(d))
+    _lib().oracle_delta_step(int(is_float), n, _p(row_ptr), _p(col), _p(wb), int(src), float(delta), threads, _p(d),
+                             None)
     return d
 
 

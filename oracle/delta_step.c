@@ -69,7 +69,7 @@ static void relax_min(shared_t* S, vec* mine, uint32_t v, uint32_t nd) {
   }
 }
 
-typedef struct { shared_t* S; unsigned id; } targ;
+typedef struct { shared_t* S; unsigned id; uint64_t work; } targ;
 
 /* Thread 0 merges the slot lists of bucket `cur` into the frontier (entries of other buckets that share
  * the slot stay), picks the next bucket when empty, and every thread relaxes its share. */
@@ -121,6 +121,7 @@ static void* worker(void* p) {
     for (uint64_t i = lo; i < hi; i++) {
       uint32_t u = S->frontier[i].v, du = S->frontier[i].d;
       if (__atomic_load_n(&S->dist[u], __ATOMIC_RELAXED) < du) continue; /* stale */
+      t->work++; /* Galois WORK_COUNT (cpu_sssp_int.patch:144): requests processed, stale ones excluded */
       for (uint32_t e = S->row[u]; e < S->row[u + 1]; e++) {
         uint32_t nd = S->is_float ? fb(bf(du) + bf(S->w[e])) : du + S->w[e];
         relax_min(S, mine, S->col[e], nd);
@@ -130,9 +131,10 @@ static void* worker(void* p) {
   return NULL;
 }
 
-/* Returns 0 on success.  delta > 0 (distance units). */
+/* Returns 0 on success.  delta > 0 (distance units).  *work (may be null) receives the number of
+ * non-stale requests processed, the count Galois reports as total_work. */
 int oracle_delta_step(int is_float, uint32_t n, const uint32_t* row, const uint32_t* col, const uint32_t* w,
-                      uint32_t src, double delta, unsigned nthreads, uint32_t* dist) {
+                      uint32_t src, double delta, unsigned nthreads, uint32_t* dist, uint64_t* work) {
   if (nthreads < 1) nthreads = 1;
   shared_t S;
   memset(&S, 0, sizeof(S));
@@ -155,10 +157,15 @@ int oracle_delta_step(int is_float, uint32_t n, const uint32_t* row, const uint3
   for (unsigned i = 0; i < nthreads; i++) {
     ta[i].S = &S;
     ta[i].id = i;
+    ta[i].work = 0;
     if (i) pthread_create(&th[i], NULL, worker, &ta[i]);
   }
   worker(&ta[0]);
   for (unsigned i = 1; i < nthreads; i++) pthread_join(th[i], NULL);
+  if (work) {
+    *work = 0;
+    for (unsigned i = 0; i < nthreads; i++) *work += ta[i].work;
+  }
   pthread_barrier_destroy(&S.bar);
   for (uint64_t i = 0; i < (uint64_t)nthreads * NSLOT; i++) free(S.slots[i].a);
   free(S.slots);
