@@ -111,7 +111,8 @@ EXPORTS = [
     "adds_sssp", "adds_sssp_multi", "adds_sssp_batched", "adds_last_status",
     "adds_solve_graph_create", "adds_solve_graph_launch", "adds_solve_graph_destroy",
     "adds_sssp_backward_f32",
-    "adds_host_csr_free", "adds_read_gr", "adds_write_gr", "adds_write_dist_text",
+    "adds_host_csr_free", "adds_read_gr", "adds_write_gr", "adds_write_dist_text", "adds_read_mtx",
+    "adds_read_dimacs",
     "adds_gen_rmat", "adds_gen_grid", "adds_gen_er", "adds_gen_weights", "adds_max_degree_vertex",
     "adds_sample_component", "adds_reached_counts", "adds_last_error_detail", "adds_autotune",
     "adds_autotune_batched",
@@ -165,6 +166,8 @@ def _lib():
         "adds_litmus": (i32, [u32, u32, u32, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "adds_host_csr_free": (None, [hp]),
         "adds_read_gr": (i32, [ctypes.c_char_p, i32, hp]),
+        "adds_read_mtx": (i32, [ctypes.c_char_p, i32, hp]),
+        "adds_read_dimacs": (i32, [ctypes.c_char_p, i32, hp]),
         "adds_write_gr": (i32, [ctypes.c_char_p, hp]),
         "adds_write_dist_text": (i32, [ctypes.c_char_p, c_u32p, u32, i32]),
         "adds_gen_rmat": (i32, [u32, u32, ctypes.c_double, ctypes.c_double, ctypes.c_double, u64, i32,

@@ -72,6 +72,16 @@ class HostGraph:
         return cls._from_call(L._lib().adds_read_gr, str(path).encode(), _wtype(wtype), what=str(path))
 
     @classmethod
+    def read_mtx(cls, path, wtype="float"):
+        """MatrixMarket coordinate file (SuiteSparse); weights |value| (see adds_read_mtx)."""
+        return cls._from_call(L._lib().adds_read_mtx, str(path).encode(), _wtype(wtype), what=str(path))
+
+    @classmethod
+    def read_dimacs(cls, path, wtype="int"):
+        """DIMACS shortest-path file ("p sp n m" / "a u v w")."""
+        return cls._from_call(L._lib().adds_read_dimacs, str(path).encode(), _wtype(wtype), what=str(path))
+
+    @classmethod
     def from_arrays(cls, row_ptr, col, w, wtype):
         """Copies numpy arrays into a library-owned host CSR (through a .gr-free path)."""
         wt = _wtype(wtype)

@@ -200,6 +200,21 @@ adds_status adds_write_gr(const char* path, const adds_host_csr* g);
  * stdout. */
 adds_status adds_write_dist_text(const char* path, const uint32_t* dist, uint32_t n, adds_wtype wt);
 
+/* ---- text graph formats (no reference counterpart: the reference's graph lists name SuiteSparse and
+ * DIMACS graphs that were converted to .gr outside it, graph_list_int / graph_list_float) ----
+ * Shortest paths need non-negative weights, so a weight is |value|: the nearest float for ADDS_W_F32,
+ * the nearest integer for ADDS_W_U32 (above 2^31-1, or NaN: ADDS_EOVERFLOW).  Edges keep file order
+ * within a row; duplicates and self loops are kept.  Parse errors give ADDS_EIO with the line in
+ * adds_last_error_detail().
+ *
+ * MatrixMarket coordinate file (SuiteSparse): entry (i, j, v) is the edge i-1 -> j-1; symmetric,
+ * skew-symmetric and hermitian files also give j-1 -> i-1 for i != j; pattern files give weight 1;
+ * complex entries use their real part; n = max(rows, cols). */
+adds_status adds_read_mtx(const char* path, adds_wtype wt, adds_host_csr* out);
+/* DIMACS shortest-path file (9th DIMACS Implementation Challenge): "c" comments, one "p sp <n> <m>",
+ * then m arcs "a <u> <v> <w>", 1-based. */
+adds_status adds_read_dimacs(const char* path, adds_wtype wt, adds_host_csr* out);
+
 /* ---- synthetic graphs (seeded, deterministic; symmetric, no self loops, no duplicate edges) ---- */
 typedef enum {
   ADDS_WK_U32_UNIFORM = 0, /* uint32 uniform in [lo, hi] */
