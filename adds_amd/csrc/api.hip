@@ -108,6 +108,7 @@ extern "C" const char* adds_strerror(adds_status s) {
 extern "C" const char* adds_last_error_detail(void) { return g_detail.c_str(); }
 namespace adds {
 void set_error_detail(const std::string& s) { g_detail = s; }
+float graph_mean_weight(const adds_graph* g) { return g->ave_wt; }
 }  // namespace adds
 
 extern "C" adds_status adds_set_device(int device) {

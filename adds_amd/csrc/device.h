@@ -115,6 +115,12 @@ struct KArgs {
   uint32_t trace_slice;  // wall-clock ticks per histogram bin
 };
 
+// Mean sampled weight of a graph (the statistic the reference's profiler_kernel computes), api.hip.
+}  // namespace adds
+struct adds_graph;
+namespace adds {
+float graph_mean_weight(const adds_graph* g);
+
 // Host launchers (sssp_kernels.hip)
 hipError_t launch_fill_u32(uint32_t* p, uint64_t count, uint32_t v, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t count, uint64_t v, hipStream_t s);

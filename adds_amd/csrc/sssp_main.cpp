@@ -11,6 +11,9 @@
 // vertices expanded (int, kernel.cu:54) or edges scanned (float, ads_float/kernel.cu:59-60).
 // Deliberate differences: failures print a message and exit non-zero instead of asserting, and a closed
 // fd 3 is reported on stderr instead of crashing.
+// Built with ADDS_NF=1 (sssp_nf_int / sssp_nf_float) it stands in for the reference's near-far GPU
+// comparison binaries instead (nf_int.patch / nf_float.patch: same command line, RUN_LOOP 8, fd-3 line
+// with work = worklist items processed) and solves with adds_sssp_nearfar at its default delta.
 #include <errno.h>
 #include <fcntl.h>
 #include <getopt.h>
@@ -26,6 +29,9 @@
 
 #ifndef ADDS_FLOAT
 #define ADDS_FLOAT 0
+#endif
+#ifndef ADDS_NF
+#define ADDS_NF 0
 #endif
 
 extern "C" const char* adds_last_error_detail(void);
@@ -95,7 +101,7 @@ int main(int argc, char** argv) {
   s = adds_graph_create(h.row_ptr, h.col, h.w, h.n, h.m, wt, &g);
   if (s != ADDS_OK) return fail("graph", s);
 
-  const int run_loop = ADDS_FLOAT ? 1 : 8;
+  const int run_loop = (ADDS_FLOAT && !ADDS_NF) ? 1 : 8;
   std::vector<uint32_t> dist(h.n);
   adds_params p;
   memset(&p, 0, sizeof(p));
@@ -104,9 +110,13 @@ int main(int argc, char** argv) {
   unsigned long long agg_work = 0;
   for (int loop = 0; loop < run_loop; loop++) {
     adds_stats st;
+#if ADDS_NF
+    s = adds_sssp_nearfar(g, (uint32_t)start_node, dist.data(), 0.f, ADDS_F_HOST_OUT, &st, nullptr);
+#else
     s = adds_sssp(g, (uint32_t)start_node, dist.data(), &p, &st, nullptr);
+#endif
     if (s != ADDS_OK) return fail("solve", s);
-    unsigned long long work = ADDS_FLOAT ? st.edges_scanned : st.vertices_expanded;
+    unsigned long long work = (ADDS_FLOAT && !ADDS_NF) ? st.edges_scanned : st.vertices_expanded;
     printf("%s Measured time for sample = %.3fs\n", input, st.ms / 1000.0);
     printf("total work is %llu\n", work);
     agg_ms += st.ms;

@@ -203,6 +203,21 @@ class Graph:
                                    ctypes.byref(st), None), "sssp")
         return out.view(self.dist_dtype), st.as_dict()
 
+    def sssp_nearfar(self, src, delta=0.0):
+        """Bulk-synchronous near-far solve (the reference's NF comparison baseline); (dist, stats)."""
+        out = np.empty(self.n, dtype=np.uint32)
+        st = L.Stats()
+        L.check(L._lib().adds_sssp_nearfar(self._h, src, out.ctypes.data_as(ctypes.c_void_p), float(delta),
+                                           L.F_HOST_OUT, ctypes.byref(st), None), "sssp_nearfar")
+        return out.view(self.dist_dtype), st.as_dict()
+
+    def sssp_nearfar_device(self, src, dist_ptr, delta=0.0):
+        """Near-far solve into a device buffer of n words; returns the stats dict."""
+        st = L.Stats()
+        L.check(L._lib().adds_sssp_nearfar(self._h, src, ctypes.c_void_p(dist_ptr), float(delta), 0,
+                                           ctypes.byref(st), None), "sssp_nearfar")
+        return st.as_dict()
+
     def sssp_multi(self, srcs, **params):
         srcs = np.ascontiguousarray(srcs, dtype=np.uint32)
         out = np.empty((len(srcs), self.n), dtype=np.uint32)

@@ -65,6 +65,19 @@ def time_single(dg, src, dist_ptr, steps, warmup, torch, **params):
     return wall, stats
 
 
+def near_far(dg, src, adds_dist, adds_ms, torch, reps=3):
+    """The reference's GPU comparison baseline (bulk-synchronous near-far at its default delta) on the same
+    graph and source: median solve ms, its work, and whether its distances equal the ADDS solve's."""
+    buf = torch.empty_like(adds_dist)
+    sts = [dg.sssp_nearfar_device(src, buf.data_ptr()) for _ in range(reps)]
+    torch.cuda.synchronize()
+    ms = float(np.median([s["ms"] for s in sts]))
+    st = sts[-1]
+    return {"ms": ms, "rounds": st["probes"], "threshold_moves": st["rebuckets"],
+            "items": st["vertices_expanded"], "edges_scanned": st["edges_scanned"],
+            "same_as_adds": bool(torch.equal(buf, adds_dist)), "adds_speedup": ms / adds_ms}
+
+
 def run_c3(host, name, steps, warmup, torch, adds_amd, oracle, do_cpu, certify=True, tune=True):
     src = host.max_degree_vertex()
     dg = adds_amd.Graph(host)
@@ -101,6 +114,8 @@ def run_c3(host, name, steps, warmup, torch, adds_amd, oracle, do_cpu, certify=T
         hw.append(time.perf_counter() - t0)
     res["host_out_ms"] = float(np.median(hw)) * 1e3
     res["host_out_gteps"] = m_r / np.median(hw) / 1e9
+    res["near_far"] = near_far(dg, src, dist, sms, torch)
+    log(f"{name}: near-far baseline {res['near_far']}")
     if do_cpu:
         t0 = time.perf_counter()
         ref = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)
@@ -142,6 +157,7 @@ def run_small(host, name, src, steps, warmup, torch, adds_amd, oracle):
     res = {"n": host.n, "m": host.m, "n_r": n_r, "m_r": m_r, "params": params, "ms_per_step": wall * 1e3,
            "kernel_ms_event": float(np.mean([s["kernel_ms"] for s in stats])),
            "gteps": m_r / wall / 1e9, "bit_exact_vs_dijkstra": bool(np.array_equal(ref, d))}
+    res["near_far"] = near_far(dg, src, dist, float(np.mean([s["ms"] for s in stats])), torch)
     dg.close()
     log(f"{name}: {res}")
     return res

@@ -133,6 +133,19 @@ adds_status adds_graph_device_arrays(const adds_graph* g, const uint32_t** row_p
  * dist is a device buffer of n words, or host memory with ADDS_F_HOST_OUT. */
 adds_status adds_sssp(adds_graph* g, uint32_t src, void* dist, const adds_params* p, adds_stats* stats,
                       void* stream);
+
+/* Bulk-synchronous near-far SSSP, the GPU baseline the reference compares ADDS with (its nf_int /
+ * nf_float runs: nf_int.patch, nf_float.patch over Galois' GPU sssp).  Rounds relax a near worklist of
+ * vertices below a threshold; vertices improved past it wait in a far pile until the threshold moves up
+ * by multiples of delta.  delta <= 0 uses the reference's 32 * mean weight / mean degree
+ * (nf_int.patch:334; truncated to an integer for uint32 weights).  dist and flags as for adds_sssp
+ * (ADDS_F_HOST_OUT only).  stats: ms = the whole solve (rounds synchronise with the host),
+ * vertices_expanded = worklist items processed (the reference's NF work count), edges_scanned,
+ * relaxations = atomicMins issued, probes = rounds, rebuckets = threshold moves.  Same distances as
+ * adds_sssp, bit for bit; integer path sums above 2^32-2 saturate. */
+adds_status adds_sssp_nearfar(adds_graph* g, uint32_t src, void* dist, float delta, uint32_t flags,
+                              adds_stats* stats, void* stream);
+
 /* k sources on one graph in one persistent launch with a shared worklist; dist is [k][n] (device, or
  * host with ADDS_F_HOST_OUT).  No reference counterpart (batched form). */
 adds_status adds_sssp_multi(adds_graph* g, const uint32_t* srcs, uint32_t k, void* dist,

@@ -1,20 +1,22 @@
 """The evaluation driver of the reference (run_all.sh + verify_against_*), restated over this build.
 
     python tests/run_all.py --inputs-int <dir> --inputs-float <dir> [--lists <dir>] [--threads N]
-                            [--legs ads,dj,ds] [--workdir <dir>]
+                            [--legs ads,nf,dj,ds] [--workdir <dir>]
 
 For each weight type it runs, like run_all.sh:
   * ads_{int,float}: adds_amd/bin/sssp_{int,float} -s <src> -o ads_X_final_dist/<g> <inputs>/<g> over
     graph_list_X (run_sssp, run_all.sh:4-18, 20-37);
+  * nf_{int,float}: the near-far GPU baseline, adds_amd/bin/sssp_nf_{int,float}, with the same command line
+    (the nf blocks of run_all.sh:34-38, 59-63);
   * dj_{int,float}: Dijkstra on one thread over cpu_graph_list_X (run_dj, run_all.sh:94-108, 113-135);
   * ds_{int,float}: parallel delta-step on --threads threads with the list's third column as Galois'
     -delta (run_ds, run_all.sh:78-92);
 each program appending "<graph> <ms> <work>" to fd 3, redirected to <impl>_result.  The CPU legs run
 oracle/oracle_sssp_cpu (the CPU oracle behind Galois sssp-cpu's command line), which is why this driver
-lives with the tests.  Then it checks ads against dj and ds with adds_amd/bin/adds_verify (verify.py
-semantics, verify_against_dj / verify_against_ds) and aggregates the fd-3 files into one table per type
-(mean ms and work per graph and program, ads speed-up over each CPU leg), also written to
-run_all_summary.json.  Exit status 1 when any comparison mismatches.
+lives with the tests.  Then it checks ads against dj, ds and nf with adds_amd/bin/adds_verify (verify.py
+semantics; verify_against_dj, verify_against_ds, verify_against_nf) and aggregates the fd-3 files into one
+table per type (mean ms and work per graph and program, ads speed-up over each other leg), also written
+to run_all_summary.json.  Exit status 1 when any comparison mismatches.
 """
 import argparse
 import json
@@ -25,6 +27,7 @@ import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SSSP_BIN = {t: os.path.join(ROOT, "adds_amd", "bin", f"sssp_{t}") for t in ("int", "float")}
+NF_BIN = {t: os.path.join(ROOT, "adds_amd", "bin", f"sssp_nf_{t}") for t in ("int", "float")}
 VERIFY_BIN = os.path.join(ROOT, "adds_amd", "bin", "adds_verify")
 CPU_BIN = os.path.join(ROOT, "oracle", "oracle_sssp_cpu")
 
@@ -64,8 +67,8 @@ def run_leg(impl, wtype, rows, inputs, workdir, threads, log):
     for tok in rows:
         g, src = tok[0], tok[1]
         gpath = os.path.join(os.path.abspath(inputs), g)
-        if kind == "ads":
-            cmd = [SSSP_BIN[wtype], "-s", src, "-o", os.path.join(out_dir, g), gpath]
+        if kind in ("ads", "nf"):
+            cmd = [(SSSP_BIN if kind == "ads" else NF_BIN)[wtype], "-s", src, "-o", os.path.join(out_dir, g), gpath]
         else:
             delta = tok[2] if len(tok) > 2 else ("1" if wtype == "float" else "13")
             cmd = [CPU_BIN, wtype, gpath, "-noverify", "-algo=" + ("deltaStep" if kind == "ds" else "dijkstra"),
@@ -110,7 +113,7 @@ def aggregate(wtype, impls, graphs, workdir):
             if g in res[impl]:
                 row[impl.split("_")[0] + "_ms"], row[impl.split("_")[0] + "_work"] = res[impl][g]
         ads = row.get("ads_ms")
-        for cpu in ("dj", "ds"):
+        for cpu in ("dj", "ds", "nf"):
             if ads and row.get(cpu + "_ms"):
                 row["speedup_vs_" + cpu] = row[cpu + "_ms"] / ads
         table.append(row)
@@ -124,7 +127,7 @@ def main(argv=None):
     ap.add_argument("--lists", default=".", help="directory holding graph_list_{int,float} and "
                                                  "cpu_graph_list_{int,float}")
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1, help="delta-step threads (NUM_THREADS)")
-    ap.add_argument("--legs", default="ads,dj,ds")
+    ap.add_argument("--legs", default="ads,nf,dj,ds")
     ap.add_argument("--workdir", default=".")
     args = ap.parse_args(argv)
     legs = [x for x in args.legs.split(",") if x]
@@ -145,7 +148,7 @@ def main(argv=None):
         rows_cpu = read_list(clist) if os.path.exists(clist) else rows_gpu
         impls = []
         for leg in legs:
-            rows = rows_gpu if leg == "ads" else rows_cpu
+            rows = rows_gpu if leg in ("ads", "nf") else rows_cpu
             if not rows:
                 continue
             impl = f"{leg}_{wtype}"
@@ -155,7 +158,7 @@ def main(argv=None):
             log(f"{impl}: {len(rows) - len(failed)}/{len(rows)} graphs")
         checks = {}
         if "ads" in legs and rows_gpu:
-            for cpu in ("dj", "ds"):
+            for cpu in ("dj", "ds", "nf"):
                 if cpu in legs:
                     match, out = verify(os.path.join(workdir, f"ads_{wtype}_final_dist"),
                                         os.path.join(workdir, f"{cpu}_{wtype}_final_dist"), glist, workdir)
@@ -167,7 +170,8 @@ def main(argv=None):
         graphs = [t[0] for t in (rows_gpu or rows_cpu)]
         table = aggregate(wtype, impls, graphs, workdir)
         summary[wtype] = {"checks": checks, "graphs": table}
-        cols = [c for c in ("ads_ms", "dj_ms", "ds_ms", "speedup_vs_dj", "speedup_vs_ds", "ads_work", "ds_work")
+        cols = [c for c in ("ads_ms", "nf_ms", "dj_ms", "ds_ms", "speedup_vs_nf", "speedup_vs_dj", "speedup_vs_ds",
+                            "ads_work", "nf_work", "ds_work")
                 if any(c in r for r in table)]
         print(f"# {wtype}: " + "  ".join(["graph"] + cols))
         for r in table:

@@ -80,17 +80,19 @@ def test_cpu_leg_failure_is_reported(tmp_path):
 
 @pytest.mark.gpu
 def test_full_driver(tmp_path):
-    """The drop-in binaries, both CPU legs and verify_against_dj / verify_against_ds, as run_all.sh runs them."""
+    """The drop-in binaries, the near-far GPU baseline, both CPU legs and verify_against_{dj,ds,nf}, as
+    run_all.sh runs them."""
     tmp = str(tmp_path)
     fx = _setup(tmp)
-    assert run_all.main(_args(tmp, "ads,dj,ds")) == 0
+    assert run_all.main(_args(tmp, "ads,nf,dj,ds")) == 0
     w = os.path.join(tmp, "w")
     with open(os.path.join(w, "run_all_summary.json")) as f:
         summary = json.load(f)
     for wtype in ("int", "float"):
-        assert summary[wtype]["checks"] == {"ads_vs_dj": True, "ads_vs_ds": True}
+        assert summary[wtype]["checks"] == {"ads_vs_dj": True, "ads_vs_ds": True, "ads_vs_nf": True}
         for r in summary[wtype]["graphs"]:
-            assert r["ads_ms"] >= 0 and "speedup_vs_dj" in r
+            assert r["ads_ms"] >= 0 and r["nf_ms"] >= 0 and "speedup_vs_dj" in r
     for name, meta in fx.items():
-        assert _read(os.path.join(w, f"ads_{meta['wtype']}_final_dist", f"{name}.gr")) == \
-            _read(os.path.join(GOLD, f"{name}.dist"))
+        for leg in ("ads", "nf"):
+            assert _read(os.path.join(w, f"{leg}_{meta['wtype']}_final_dist", f"{name}.gr")) == \
+                _read(os.path.join(GOLD, f"{name}.dist")), (leg, name)
