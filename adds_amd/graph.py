@@ -83,11 +83,12 @@ class HostGraph:
 
     @classmethod
     def from_arrays(cls, row_ptr, col, w, wtype):
-        """Copies numpy arrays into a library-owned host CSR (through a .gr-free path)."""
+        """A host graph holding its own copies of the given arrays (they may be views into another graph,
+        which can then go away)."""
         wt = _wtype(wtype)
-        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
-        col = np.ascontiguousarray(col, dtype=np.uint32)
-        wb = np.ascontiguousarray(w, dtype=np.float32 if wt == L.W_F32 else np.uint32).view(np.uint32)
+        row_ptr = np.array(row_ptr, dtype=np.uint32, copy=True, order="C")
+        col = np.array(col, dtype=np.uint32, copy=True, order="C")
+        wb = np.array(w, dtype=np.float32 if wt == L.W_F32 else np.uint32, copy=True, order="C").view(np.uint32)
         obj = cls.__new__(cls)
         obj._csr = None
         obj.n, obj.m, obj.wtype = len(row_ptr) - 1, len(col), wt

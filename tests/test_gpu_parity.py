@@ -197,3 +197,59 @@ def test_tail_mode_waits_for_the_peak():
         _check(g, dist[i], s, True)
     print("expansions without / with tail mode", st0["vertices_expanded"], st["vertices_expanded"])
     assert st["vertices_expanded"] <= 1.5 * st0["vertices_expanded"]
+
+
+def _csr(n, edges, wtype):
+    """CSR from (u, v, w) triples, rows in the given order."""
+    edges = sorted(edges, key=lambda e: e[0])
+    row = np.zeros(n + 1, dtype=np.uint32)
+    for u, _, _ in edges:
+        row[u + 1] += 1
+    row = np.cumsum(row).astype(np.uint32)
+    col = np.array([v for _, v, _ in edges], dtype=np.uint32)
+    w = np.array([x for _, _, x in edges], dtype=np.float32 if wtype == "float" else np.uint32)
+    return adds_amd.HostGraph.from_arrays(row, col, w, wtype)
+
+
+def _zero_heavy_rmat(is_float):
+    g = adds_amd.HostGraph.rmat(12, 8, seed=31, wkind="f32" if is_float else "u32", lo=0 if is_float else 1,
+                                hi=1.0 if is_float else 50)
+    rng = np.random.default_rng(31)
+    w = g.w.copy()
+    w[rng.random(g.m) < 0.3] = 0  # a third of the edges weigh nothing
+    return adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, w, "float" if is_float else "int")
+
+
+EDGE_CASES = {
+    "single_vertex": lambda: (_csr(1, [], "int"), 0),
+    "zero_weights_self_loops_parallel_edges": lambda: (_csr(5, [(0, 1, 0), (1, 2, 0), (2, 3, 5), (1, 1, 3),
+                                                                (0, 2, 7), (0, 2, 1), (3, 3, 0), (3, 4, 0),
+                                                                (4, 0, 0)], "int"), 0),
+    "source_without_out_edges": lambda: (_csr(4, [(1, 2, 3), (2, 3, 4), (3, 1, 1)], "int"), 0),
+    "extreme_float_weights": lambda: (_csr(6, [(0, 1, 1e-30), (1, 2, 0.0), (2, 3, 1e30), (0, 3, 3e30),
+                                               (3, 4, 1e-38), (0, 4, 2e30), (4, 5, 1.5), (5, 0, 0.0)],
+                                          "float"), 0),
+    "zero_heavy_rmat_int": lambda: (_zero_heavy_rmat(False), 0),
+    "zero_heavy_rmat_float": lambda: (_zero_heavy_rmat(True), 0),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EDGE_CASES))
+def test_edge_cases(case):
+    """Single vertex, zero weights (ties inside a bucket), self loops, parallel edges, a source with no
+    out-edges, float weights from 1e-38 to 3e30: ADDS (default, then a narrow bucket width), several sources
+    at once and the near-far baseline all equal Dijkstra bit for bit."""
+    g, src = EDGE_CASES[case]()
+    dg = adds_amd.Graph(g)
+    # a narrow bucket width too, except where distances span 1e30 (10^33 buckets of 1e-3 would take the
+    # head forever to walk)
+    narrow = [] if case == "extreme_float_weights" else [dict(delta=1.0 if not g.is_float else 1e-3)]
+    for params in [dict()] + narrow:
+        dist, _ = dg.sssp(src, timeout_ms=TIMEOUT, **params)
+        _check(g, dist, src, g.is_float)
+    srcs = np.array([src, g.n - 1], dtype=np.uint32)
+    dist, _ = dg.sssp_multi(srcs, timeout_ms=TIMEOUT)
+    for i, s in enumerate(srcs):
+        _check(g, dist[i], s, g.is_float)
+    dist, _ = dg.sssp_nearfar(src)
+    _check(g, dist, src, g.is_float)
