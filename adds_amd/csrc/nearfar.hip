@@ -5,7 +5,8 @@
 // solver so that the two run on the same hardware and inputs.
 //
 // One round: k_nf_items relaxes the near worklist -- each wavefront takes 64 items, flattens the edges
-// of rows up to NF_BIG long into 64-edge windows, and turns longer rows into NF_BIG-edge tasks that
+// of rows up to NF_BIG long into 64-edge windows (relaxed 16 or 4 at a time, their loads overlapped), and
+// turns longer rows into NF_BIG-edge tasks that
 // k_nf_tasks relaxes one per wavefront, so a hub does not hold one wavefront for its whole row.  An
 // improved vertex joins the next near worklist (once per round, a stamp per vertex) when its distance
 // is below the threshold, else the far pile (once per phase).  When a round leaves the near worklist
@@ -76,26 +77,76 @@ __device__ __forceinline__ void wave_append(bool f, uint32_t val, uint32_t* list
   if (f) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = val;
 }
 
-// Relaxes edge e from a vertex at distance du (act: this lane has an edge); returns 1 if it issued an
-// atomicMin.
-__device__ __forceinline__ uint32_t nf_relax(const NfArgs& a, bool act, uint32_t du, uint32_t e) {
-  uint32_t v = 0, nd = 0;
-  bool imp = false, issued = false;
-  if (act) {
-    v = a.col[e];
-    nd = nf_add(du, a.w[e], a.is_float);
-    if (nd < a.dist[v]) {  // plain load as a pre-filter; the atomic decides
-      issued = true;
-      imp = nd < atomicMin(a.dist + v, nd);
-    }
+// Appends v[j] for every (lane, j) with f[j] set: one reservation per wavefront for all UN windows.
+template <int UN>
+__device__ __forceinline__ void wave_append_n(const bool (&f)[UN], const uint32_t (&v)[UN], uint32_t* list,
+                                              uint32_t* counter) {
+  uint64_t m[UN];
+  uint32_t tot = 0;
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    m[j] = __ballot(f[j]);
+    tot += (uint32_t)__popcll(m[j]);
   }
-  const bool near = imp && nd < a.thr;
-  bool push_near = false, push_far = false;
-  if (near) push_near = atomicExch(a.near_stamp + v, a.round) != a.round;
-  if (imp && !near) push_far = atomicExch(a.far_stamp + v, a.phase) != a.phase;
-  wave_append(push_near, v, a.out, a.cnt + C_NEAR);
-  wave_append(push_far, v, a.far, a.cnt + C_FAR);
-  return issued ? 1u : 0u;
+  if (!tot) return;
+  const uint32_t lane = lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(counter, tot);
+  base = __shfl(base, 0, 64);
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    if (f[j]) list[base + (uint32_t)__popcll(m[j] & ((1ull << lane) - 1))] = v[j];
+    base += (uint32_t)__popcll(m[j]);
+  }
+}
+
+// Relaxes UN windows of edges: edge e[j] from a vertex at distance du[j] where act[j].  Each step is
+// issued for every window before any result is used -- the (col, w) loads, the d[v] pre-filter loads,
+// the atomicMins, the worklist stamps, one reservation per worklist -- so a group of windows costs a few
+// round trips, not a few per window (inactive lanes read edge 0, which exists whenever any edge is
+// relaxed).  Returns the atomicMins issued by this lane.
+// Measured on RMAT-24: 4 windows 166 ms, 8 windows 110 ms, 16 windows 79 ms per solve.
+template <int UN>
+__device__ __forceinline__ uint32_t nf_relax(const NfArgs& a, const bool (&act)[UN], const uint32_t (&du)[UN],
+                                             const uint32_t (&e)[UN]) {
+  uint32_t v[UN], nd[UN], dv[UN], old[UN];
+  bool cand[UN], pn[UN], pf[UN];
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    const uint32_t ej = act[j] ? e[j] : 0u;
+    v[j] = a.col[ej];
+    nd[j] = a.w[ej];
+  }
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    nd[j] = nf_add(du[j], nd[j], a.is_float);
+    dv[j] = a.dist[act[j] ? v[j] : 0u];
+  }
+  uint32_t issued = 0;
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    cand[j] = act[j] && nd[j] < dv[j];  // plain load as a pre-filter; the atomic decides
+    issued += cand[j] ? 1u : 0u;
+    old[j] = cand[j] ? atomicMin(a.dist + v[j], nd[j]) : 0u;
+  }
+  uint32_t sn[UN];
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    const bool imp = cand[j] && nd[j] < old[j];
+    const bool near = imp && nd[j] < a.thr;
+    pn[j] = near;
+    pf[j] = imp && !near;
+    sn[j] = near ? atomicExch(a.near_stamp + v[j], a.round)
+                 : (pf[j] ? atomicExch(a.far_stamp + v[j], a.phase) : 0u);
+  }
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    pn[j] = pn[j] && sn[j] != a.round;
+    pf[j] = pf[j] && sn[j] != a.phase;
+  }
+  wave_append_n<UN>(pn, v, a.out, a.cnt + C_NEAR);
+  wave_append_n<UN>(pf, v, a.far, a.cnt + C_FAR);
+  return issued;
 }
 
 __device__ __forceinline__ void add_stats(const NfArgs& a, uint32_t edges, uint32_t relax) {
@@ -110,10 +161,38 @@ __device__ __forceinline__ void add_stats(const NfArgs& a, uint32_t edges, uint3
   }
 }
 
+// Windows per group: NF_WIDE while that many windows of edges remain (a task is exactly NF_WIDE windows),
+// NF_NARROW for the rest of a short-row chunk.
+constexpr int NF_WIDE = 16;
+constexpr int NF_NARROW = 4;
+static_assert(NF_BIG == 64 * NF_WIDE, "a task is one wide group");
+
+__shared__ uint32_t s_pref[NF_WAVES][65];
+__shared__ uint32_t s_rs[NF_WAVES][64];
+__shared__ uint32_t s_du[NF_WAVES][64];
+
+// One group of UN windows of a k_nf_items chunk, starting at flattened edge k0: owner of each edge by
+// bisection over the chunk's prefix sums, then nf_relax.
+template <int UN>
+__device__ __forceinline__ uint32_t items_group(const NfArgs& a, uint32_t wv, uint32_t lane, uint32_t k0,
+                                                uint32_t total, uint32_t& edges) {
+  bool act[UN];
+  uint32_t du_[UN], e[UN];
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    const uint32_t k = k0 + 64 * j + lane;
+    act[j] = k < total;
+    uint32_t lo = 0;  // last range whose prefix is <= k (its length is then non-zero)
+    for (uint32_t step = 32; step >= 1; step >>= 1)
+      if (s_pref[wv][lo + step] <= k) lo += step;
+    e[j] = act[j] ? s_rs[wv][lo] + (k - s_pref[wv][lo]) : 0u;
+    du_[j] = s_du[wv][lo];
+    edges += act[j] ? 1u : 0u;
+  }
+  return nf_relax<UN>(a, act, du_, e);
+}
+
 __global__ void __launch_bounds__(64 * NF_WAVES) k_nf_items(NfArgs a) {
-  __shared__ uint32_t s_pref[NF_WAVES][65];
-  __shared__ uint32_t s_rs[NF_WAVES][64];
-  __shared__ uint32_t s_du[NF_WAVES][64];
   const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
   const uint32_t gw = blockIdx.x * NF_WAVES + wv, nw = gridDim.x * NF_WAVES;
   uint32_t edges = 0, relax = 0;
@@ -141,16 +220,9 @@ __global__ void __launch_bounds__(64 * NF_WAVES) k_nf_items(NfArgs a) {
     s_du[wv][lane] = du;
     if (lane == 0) s_pref[wv][64] = total;
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t k0 = 0; k0 < total; k0 += 64) {
-      const uint32_t k = k0 + lane;
-      const bool act = k < total;
-      uint32_t lo = 0;  // last range whose prefix is <= k (its length is then non-zero)
-      for (uint32_t step = 32; step >= 1; step >>= 1)
-        if (s_pref[wv][lo + step] <= k) lo += step;
-      const uint32_t e = s_rs[wv][lo] + (k - s_pref[wv][lo]);
-      relax += nf_relax(a, act, s_du[wv][lo], act ? e : 0u);
-      edges += act ? 1u : 0u;
-    }
+    uint32_t k0 = 0;
+    for (; k0 + 64 * NF_WIDE <= total; k0 += 64 * NF_WIDE) relax += items_group<NF_WIDE>(a, wv, lane, k0, total, edges);
+    for (; k0 < total; k0 += 64 * NF_NARROW) relax += items_group<NF_NARROW>(a, wv, lane, k0, total, edges);
     __builtin_amdgcn_wave_barrier();
   }
   add_stats(a, edges, relax);
@@ -166,11 +238,17 @@ __global__ void __launch_bounds__(64 * NF_WAVES) k_nf_tasks(NfArgs a) {
     const uint32_t u = tk.x;
     const uint32_t du = a.dist[u];
     const uint32_t rs = a.row[u] + tk.y * NF_BIG, re = min(rs + NF_BIG, a.row[u + 1]);
-    for (uint32_t e0 = rs; e0 < re; e0 += 64) {
-      const uint32_t e = e0 + lane;
-      const bool act = e < re;
-      relax += nf_relax(a, act, du, act ? e : 0u);
-      edges += act ? 1u : 0u;
+    for (uint32_t e0 = rs; e0 < re; e0 += 64 * NF_WIDE) {
+      bool act[NF_WIDE];
+      uint32_t du_[NF_WIDE], e[NF_WIDE];
+#pragma unroll
+      for (int j = 0; j < NF_WIDE; j++) {
+        e[j] = e0 + 64 * j + lane;
+        act[j] = e[j] < re;
+        du_[j] = du;
+        edges += act[j] ? 1u : 0u;
+      }
+      relax += nf_relax<NF_WIDE>(a, act, du_, e);
     }
   }
   add_stats(a, edges, relax);
@@ -345,9 +423,11 @@ extern "C" adds_status adds_sssp_nearfar(adds_graph* g, uint32_t src, void* dist
       a.phase = phase;
       a.thr = thr;
       hipLaunchKernelGGL(k_nf_round, dim3(1), dim3(64), 0, s, B.cnt);
-      const uint32_t blocks = std::min<uint64_t>(8ull * cu, (n_in + 64 * NF_WAVES - 1) / (64 * NF_WAVES));
+      // Enough wavefronts to keep many loads in flight per CU (both kernels are bound by load latency):
+      // up to 16 workgroups of 4 wavefronts per CU.
+      const uint32_t blocks = std::min<uint64_t>(16ull * cu, (n_in + 64 * NF_WAVES - 1) / (64 * NF_WAVES));
       hipLaunchKernelGGL(k_nf_items, dim3(blocks), dim3(64 * NF_WAVES), 0, s, a);
-      hipLaunchKernelGGL(k_nf_tasks, dim3(4 * cu), dim3(64 * NF_WAVES), 0, s, a);
+      hipLaunchKernelGGL(k_nf_tasks, dim3(16 * cu), dim3(64 * NF_WAVES), 0, s, a);
       NFCHK(hipGetLastError());
       NFCHK(hipMemcpyAsync(B.hcnt, B.cnt, 4 * C_WORDS, hipMemcpyDeviceToHost, s));
       NFCHK(hipStreamSynchronize(s));
