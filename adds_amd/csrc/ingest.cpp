@@ -114,7 +114,7 @@ adds_status read_mtx(const char* path, adds_wtype wt, adds_host_csr* out) {
   if (!line || sscanf(line, "%%%%MatrixMarket %63s %63s %63s %63s", obj, fmt, field, sym) != 4)
     return fail(lr, path, "missing %%MatrixMarket header");
   for (char* s : {obj, fmt, field, sym})
-    for (char* c = s; *c; c++) *c = (char)tolower(*c);
+    for (char* c = s; *c; c++) *c = (char)tolower((unsigned char)*c);
   if (strcmp(obj, "matrix") || strcmp(fmt, "coordinate"))
     return fail(lr, path, "only 'matrix coordinate' files are graphs");
   const bool pattern = !strcmp(field, "pattern");

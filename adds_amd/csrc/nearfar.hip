@@ -315,7 +315,10 @@ struct Bufs {
   uint2* tasks = nullptr;
   unsigned long long* st = nullptr;
   uint32_t* hcnt = nullptr;  // pinned copy of the counters
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   ~Bufs() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     for (void* p : {(void*)dist, (void*)a, (void*)b, (void*)far0, (void*)far1, (void*)ns, (void*)fs, (void*)cnt,
                     (void*)tasks, (void*)st})
       if (p) (void)hipFree(p);
@@ -384,10 +387,9 @@ extern "C" adds_status adds_sssp_nearfar(adds_graph* g, uint32_t src, void* dist
   NFCHK(hipHostMalloc((void**)&B.hcnt, 4 * C_WORDS, hipHostMallocDefault));
   NFCHK(hipMemsetAsync(B.cnt, 0, 4 * C_WORDS, s));
   NFCHK(hipMemsetAsync(B.st, 0, 8 * S_WORDS, s));
-  hipEvent_t e0, e1;
-  NFCHK(hipEventCreate(&e0));
-  NFCHK(hipEventCreate(&e1));
-  NFCHK(hipEventRecord(e0, s));
+  NFCHK(hipEventCreate(&B.e0));
+  NFCHK(hipEventCreate(&B.e1));
+  NFCHK(hipEventRecord(B.e0, s));
   const uint32_t inf = is_float ? 0x7F800000u : INF32;
   hipLaunchKernelGGL(k_nf_init, dim3(1024), dim3(256), 0, s, B.dist, n, src, inf, B.ns, B.fs, B.a);
   NFCHK(hipGetLastError());
@@ -412,7 +414,6 @@ extern "C" adds_status adds_sssp_nearfar(adds_graph* g, uint32_t src, void* dist
   uint32_t thr = threshold_word(k, dl, is_float);
   uint64_t work = 0;
   const uint32_t cu = std::max(1u, adds_device_cu_count());
-  adds_status rc = ADDS_OK;
   for (;;) {
     while (n_in) {
       a.in = in;
@@ -475,12 +476,10 @@ extern "C" adds_status adds_sssp_nearfar(adds_graph* g, uint32_t src, void* dist
     splits++;
     if (n_in == 0 && B.hcnt[C_FAR] == 0) break;
   }
-  NFCHK(hipEventRecord(e1, s));
-  NFCHK(hipEventSynchronize(e1));
+  NFCHK(hipEventRecord(B.e1, s));
+  NFCHK(hipEventSynchronize(B.e1));
   float ms = 0;
-  NFCHK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  NFCHK(hipEventElapsedTime(&ms, B.e0, B.e1));
   unsigned long long hst[S_WORDS];
   NFCHK(hipMemcpy(hst, B.st, 8 * S_WORDS, hipMemcpyDeviceToHost));
   const hipMemcpyKind kind = (flags & ADDS_F_HOST_OUT) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
@@ -496,5 +495,5 @@ extern "C" adds_status adds_sssp_nearfar(adds_graph* g, uint32_t src, void* dist
     stats->rebuckets = splits;
     stats->probes = rounds;
   }
-  return rc;
+  return ADDS_OK;
 }
