@@ -494,6 +494,7 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.sticky = 1;
   if (const char* e = getenv("ADDS_STICKY")) a.sticky = atoi(e) != 0;
   a.local_cont = (r.flags & ADDS_F_NO_LOCAL) ? 0u : 1u;
+  a.nowait = (r.flags & ADDS_F_NO_WAIT) ? 1u : 0u;
   if (const char* e = getenv("ADDS_LOCAL")) a.local_cont = atoi(e) != 0;
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
@@ -1136,10 +1137,15 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       q.tail_items = t;
       try_point(q);
     }
-    // 7) local continuation off
+    // 7) local continuation off; flushes that do not wait for their atomicMins
     {
       adds_params q = p;
       q.flags |= ADDS_F_NO_LOCAL;
+      try_point(q);
+    }
+    {
+      adds_params q = p;
+      q.flags |= ADDS_F_NO_WAIT;
       try_point(q);
     }
     // 8) a second, shorter pass over the choices that interact most (the best bucket width depends on

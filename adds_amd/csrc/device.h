@@ -102,6 +102,7 @@ struct KArgs {
   uint32_t cached_probe;   // probe queue counters with L2-cached loads (claims stay atomic)
   uint32_t sticky;         // after a full claim from the behind/head bucket, claim the same queue again
   uint32_t local_cont;     // a light claim keeps up to 64 same-bucket improvements for itself
+  uint32_t nowait;         // flushes do not wait for their atomicMins (every improvement is queued)
   // Optional bucket timeline (null = off): trace[0] = record count, then records of 4 words from
   // trace + 4: wall clock, (kind << 56 | bucket), previous head, pool chunks used.  Kinds: 1 = the
   // head bucket's heavy phase opened, 2 = the head moved to `bucket`, 0 = start, 3 = done.

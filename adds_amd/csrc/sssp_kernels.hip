@@ -278,9 +278,10 @@ struct FlushArgs {
   uint32_t* tab;
   uint32_t maxv;
   uint32_t pool_chunks;
+  uint32_t nowait;
 };
 __device__ __forceinline__ FlushArgs flush_args(const KArgs& a) {
-  return FlushArgs{a.dist, a.st, a.pool, a.tab, a.maxv, a.pool_chunks};
+  return FlushArgs{a.dist, a.st, a.pool, a.tab, a.maxv, a.pool_chunks, a.nowait};
 }
 
 struct PushBuf {
@@ -327,14 +328,21 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
     off[j] = 0;
   }
   // Improvements found by the edge windows carry PB_RELAX: their atomicMin is issued here, all at once,
-  // and only those that lowered the distance are pushed.
+  // and only those that lowered the distance are pushed.  With nowait the atomics are not waited for and
+  // every improvement is pushed: one that lost to a smaller distance becomes a stale entry, dropped
+  // when claimed (a round trip saved per flush against a few more queue entries).
   uint32_t old[EPL];
   bool relax[EPL];
 #pragma unroll
   for (int j = 0; j < (int)EPL; j++) {
     relax[j] = rem[j] && (q[j] & PB_RELAX) != 0;
     q[j] &= ~PB_RELAX;
-    old[j] = relax[j] ? afmin(a.dist + (uint32_t)e[j], (uint32_t)(e[j] >> 32)) : 0u;
+    if (a.nowait) {
+      if (relax[j]) (void)afmin(a.dist + (uint32_t)e[j], (uint32_t)(e[j] >> 32));
+      old[j] = 0xFFFFFFFFu;  // above every pushed distance: nothing is dropped
+    } else {
+      old[j] = relax[j] ? afmin(a.dist + (uint32_t)e[j], (uint32_t)(e[j] >> 32)) : 0u;
+    }
   }
   uint32_t nrel = 0, npush = 0, nl = 0;
 #pragma unroll

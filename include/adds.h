@@ -75,6 +75,9 @@ typedef struct {
 #define ADDS_F_ATOMIC_PROBE 0x4u /* probe queue counters with atomic (cache-bypassing) loads only */
 #define ADDS_F_NO_LOCAL 0x8u  /* every improvement goes through the queues (no local continuation); a
                                  tuning choice: local continuation wins on high-diameter graphs */
+#define ADDS_F_NO_WAIT 0x10u  /* flushes issue their atomicMins without waiting for the results and queue
+                                 every improvement (one that lost becomes a stale entry, dropped when
+                                 claimed): a round trip less per flush; pays on throughput-bound solves */
 
 typedef struct {
   double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */

@@ -58,6 +58,8 @@ def test_grid2048_float_c2():
     dict(parts=8, nbuckets=256, chunk=64, wg_per_cu=8, concurrency=3),
     dict(parts=4, nbuckets=8, chunk=48, wg_per_cu=2, concurrency=4),
     dict(flags=0x8 | 0x4, delta=64.0),  # no local continuation, uncached probes
+    dict(flags=0x10, delta=64.0),  # flushes do not wait for their atomicMins
+    dict(flags=0x10 | 0x8, edge_windows=8, chunk=256, parts=4),
     dict(parts=2, nbuckets=16, chunk=17, delta=1.0, hub_degree=64, task_edges=64),
     dict(parts=4, nbuckets=32, chunk=32, delta=100000.0, concurrency=2),
     dict(parts=4, nbuckets=8, chunk=32, delta=1000.0, concurrency=4),
@@ -173,6 +175,7 @@ def test_weights_update_resplits():
 
 @pytest.mark.parametrize("params", [dict(), dict(delta=16.0), dict(delta=16.0, flags=0x8),
                                     dict(delta=2.0, light_delta=1e30, edge_windows=8),
+                                    dict(delta=2.0, light_delta=1e30, edge_windows=8, flags=0x10 | 0x8),
                                     dict(delta=2.0, tail_items=20000)])
 def test_multi_source_grid(params):
     """Many sources on a high-diameter graph: replicated instances share the buckets, and hops inside a
