@@ -49,3 +49,39 @@ def test_cli_rejects_out_of_range_source(tmp_path):
     r = subprocess.run([os.path.join(BIN, "sssp_int"), "-s", "999999", os.path.join(GOLDEN, "rmat8_int.gr")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "out of range" in r.stderr
+
+
+def test_convert_then_solve(tmp_path):
+    """A SuiteSparse-style MatrixMarket file (symmetric, negative values) -> adds_convert -> .gr -> the
+    drop-in float binary and the near-far binary: both write the oracle's distance text for the graph the
+    converter made."""
+    import numpy as np
+
+    import adds_amd
+    import oracle
+
+    rng = np.random.default_rng(12)
+    n, nnz = 3000, 12000
+    i = rng.integers(1, n + 1, nnz)
+    j = rng.integers(1, n + 1, nnz)
+    keep = i >= j  # lower triangle, as symmetric files store it
+    v = rng.normal(0.0, 2.0, nnz)
+    mtx = tmp_path / "g.mtx"
+    with open(mtx, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real symmetric\n")
+        f.write(f"{n} {n} {int(keep.sum())}\n")
+        for a, b, x in zip(i[keep], j[keep], v[keep]):
+            f.write(f"{a} {b} {x:.6g}\n")
+    gr = tmp_path / "g.gr"
+    r = subprocess.run([os.path.join(BIN, "adds_convert"), "mtx", "float", str(mtx), str(gr)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    h = adds_amd.HostGraph.read_gr(str(gr), "float")
+    ref = tmp_path / "ref.txt"
+    adds_amd.write_dist_text(str(ref), oracle.dijkstra(h.row_ptr, h.col, h.w, 0, True), "float")
+    for exe in ("sssp_float", "sssp_nf_float"):
+        out = tmp_path / f"{exe}.txt"
+        r = subprocess.run(f"{os.path.join(BIN, exe)} -s 0 -o {out} {gr} 3>/dev/null", shell=True,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert out.read_text() == ref.read_text(), exe

@@ -96,3 +96,23 @@ def test_full_driver(tmp_path):
         for leg in ("ads", "nf"):
             assert _read(os.path.join(w, f"{leg}_{meta['wtype']}_final_dist", f"{name}.gr")) == \
                 _read(os.path.join(GOLD, f"{name}.dist")), (leg, name)
+
+
+def test_cpu_command_contract(tmp_path):
+    """oracle_sssp_cpu: usage errors exit 2; a missing output directory is reported (exit 1); the int -delta is
+    a shift and the float -delta a divisor, and neither changes the distances."""
+    import subprocess
+    g = os.path.join(GOLD, "rmat8_int.gr")
+    r = subprocess.run([run_all.CPU_BIN, "int", g, "-algo=bellman"], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([run_all.CPU_BIN, "int", g, "-algo=dijkstra", "-startNode=0"], cwd=str(tmp_path),
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot write" in r.stderr
+    outs = []
+    for d in ("0", "3", "12"):
+        os.makedirs(tmp_path / "ds_int_final_dist", exist_ok=True)
+        r = subprocess.run([run_all.CPU_BIN, "int", g, "-algo=deltaStep", "-startNode=0", "-delta=" + d, "-t", "3"],
+                           cwd=str(tmp_path), capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        outs.append(_read(os.path.join(tmp_path, "ds_int_final_dist", "rmat8_int.gr")))
+    assert outs[0] == outs[1] == outs[2] == _read(os.path.join(GOLD, "rmat8_int.dist"))
