@@ -373,7 +373,7 @@ def main():
         "cpu_baseline": cpu,
         "extras": dict(extras, c3_int={k: head[k] for k in ("params", "kernel_ms_event", "solve_ms_event",
                                                              "gteps_kernel", "host_out_ms", "host_out_gteps",
-                                                             "work", "cpu_delta_step") if k in head}),
+                                                             "work", "cpu_delta_step", "near_far") if k in head}),
     }
     print(json.dumps(line))
 
