@@ -435,7 +435,7 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
 // another threshold re-split).
 static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
   if (!g->lcnt) {
-    HIPCHK(hipMalloc(&g->scw, std::max<uint64_t>(g->m, 1) * 8));
+    HIPCHK(hipMalloc(&g->scw, (g->m + 1) * 8));  // one pad entry: the last 16-byte pair is in bounds
     HIPCHK(hipMalloc(&g->lcnt, std::max<uint64_t>(g->n, 1) * 4));
     HIPCHK(hipMalloc(&g->split_aux, split_aux_words(g->m) * 4));
     g->split_delta = 0.f;

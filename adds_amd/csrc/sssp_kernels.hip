@@ -520,6 +520,76 @@ __device__ __forceinline__ void gather(const KArgs& a, const EdgeGroup<UN>& g, G
   for (int j = 0; j < UN; j++) r.dv[j] = a.dist[r.tgt[j]];
 }
 
+// Pair windows (the 8-window form): each lane relaxes the two edges of one 16-byte-aligned pair of the
+// light-first rows, loaded with one 16-byte load, so a window covers 128 edges for one (col, w) load
+// instruction and two gathers (64-edge windows need two load instructions per 128 edges): 4 pair windows
+// per group relax as many edges as 8 plain ones with a quarter fewer vector memory instructions
+// (measured: C4a -3%, C3 float -1.5%).  The flattened order counts the pairs that overlap each range; a
+// pair's edges outside the range are masked.  The row array carries one pad entry, so the last pair is
+// always in bounds.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <int UN>
+struct PairGroup {
+  bool act[UN];
+  uint32_t p[UN], du[UN], gb[UN], lo[UN], hi[UN];  // pair index, source distance, instance base, range
+  uint32_t c0[UN], w0[UN], c1[UN], w1[UN];
+};
+template <int UN>
+struct PairGathered {
+  bool a0[UN], a1[UN];
+  uint32_t t0[UN], t1[UN], n0[UN], n1[UN], d0[UN], d1[UN];
+};
+template <int UN>
+__device__ __forceinline__ void load_pairs(const KArgs& a, PairGroup<UN>& g) {
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    const uint32_t ps = g.act[j] ? g.p[j] : 0u;
+    const u32x4 x = __builtin_nontemporal_load((const u32x4*)a.cw + ps);
+    g.c0[j] = x.x;
+    g.w0[j] = x.y;
+    g.c1[j] = x.z;
+    g.w1[j] = x.w;
+  }
+}
+template <bool FLT, int UN>
+__device__ __forceinline__ void gather_pairs(const KArgs& a, const PairGroup<UN>& g, PairGathered<UN>& r) {
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    r.a0[j] = g.act[j] && 2 * g.p[j] >= g.lo[j];
+    r.a1[j] = g.act[j] && 2 * g.p[j] + 1 < g.hi[j];
+    r.t0[j] = r.a0[j] ? g.gb[j] + g.c0[j] : 0u;
+    r.t1[j] = r.a1[j] ? g.gb[j] + g.c1[j] : 0u;
+    r.n0[j] = dadd<FLT>(g.du[j], g.w0[j]);
+    r.n1[j] = dadd<FLT>(g.du[j], g.w1[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    r.d0[j] = a.dist[r.t0[j]];
+    r.d1[j] = a.dist[r.t1[j]];
+  }
+}
+template <bool FLT, int UN>
+__device__ __forceinline__ void finish_pairs(const KArgs& a, const PairGathered<UN>& r, uint32_t part, uint32_t cb,
+                                             bool heavy, bool tail, PushBuf& pb, WaveStats& ws) {
+  bool f0[UN], f1[UN];
+  uint32_t nact = 0;
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    f0[j] = r.a0[j] && r.n0[j] < r.d0[j];
+    f1[j] = r.a1[j] && r.n1[j] < r.d1[j];
+    nact += (uint32_t)__popcll(ballot(r.a0[j])) + (uint32_t)__popcll(ballot(r.a1[j]));
+  }
+  if (lane_id() == 0) {
+    ws.v[ST_EDGES] += nact;
+    if (heavy) ws.v[ST_HEDGES] += nact;
+  }
+#pragma unroll
+  for (int j = 0; j < UN; j++) {
+    pb_add(a, pb, f0[j], ((uint64_t)r.n0[j] << 32) | r.t0[j], light_queue<FLT>(a, r.n0[j], part, cb, tail) | PB_RELAX, ws);
+    pb_add(a, pb, f1[j], ((uint64_t)r.n1[j] << 32) | r.t1[j], light_queue<FLT>(a, r.n1[j], part, cb, tail) | PB_RELAX, ws);
+  }
+}
+
 // Consumes a gathered group: improvements over the pre-filter value go to the push buffer, whose
 // flush applies the atomicMin.  No store or atomic is issued in the edge loop, so the compiler's
 // waits there cover loads only and the pipeline keeps two groups' loads in flight.
@@ -706,6 +776,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   __shared__ uint32_t s_rs[WAVES_PER_WG][64 * IPL];
   __shared__ uint32_t s_du[WAVES_PER_WG][64 * IPL];
   __shared__ uint32_t s_gb[WAVES_PER_WG][64 * IPL];
+  __shared__ uint32_t s_re[WAVES_PER_WG][UN == 8 ? 64 * IPL : 1];
+  constexpr bool PAIRS = UN == 8;  // the 8-window form relaxes 4 pair windows per group
   __shared__ uint64_t s_pbe[WAVES_PER_WG][PB_CAP];
   __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
   __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
@@ -1042,9 +1114,12 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       tk[1] += t2 - t1;
       // ---------------- flattened edge windows ----------------
       // Range k = lane * IPL + j occupies [s_pref[k], s_pref[k+1]) of the flattened order.
-      uint32_t mysum = 0;
+      uint32_t mysum = 0, unit[IPL];  // flattened units of each range: edges, or pairs in pair mode
 #pragma unroll
-      for (int j = 0; j < IPL; j++) mysum += deg[j];
+      for (int j = 0; j < IPL; j++) {
+        unit[j] = PAIRS ? (deg[j] ? ((rs[j] + deg[j] + 1) >> 1) - (rs[j] >> 1) : 0u) : deg[j];
+        mysum += unit[j];
+      }
       const uint32_t incl = wave_incl_scan(mysum);
       const uint32_t total = rdlane(incl, 63);
       if (total) {
@@ -1056,7 +1131,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           s_rs[wv][k] = rs[j];
           s_du[wv][k] = du[j];
           if (REPL) s_gb[wv][k] = gbase[j];
-          run += deg[j];
+          if (PAIRS) s_re[wv][k] = rs[j] + deg[j];
+          run += unit[j];
         }
         if (lane == 0) s_pref[wv][64 * IPL] = total;
         wave_sync();
@@ -1085,6 +1161,50 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             g.gb[j] = REPL ? s_gb[wv][lo[j]] : 0u;
           }
         };
+        if constexpr (PAIRS) {
+          constexpr int UP = UN / 2;
+          auto locate_p = [&](uint32_t base, PairGroup<UP>& g) {
+            uint32_t idx[UP], lo[UP];
+#pragma unroll
+            for (int j = 0; j < UP; j++) {
+              idx[j] = base + 64 * j + lane;
+              g.act[j] = idx[j] < total;
+              lo[j] = 0;
+            }
+#pragma unroll
+            for (uint32_t step = 32 * IPL; step >= 1; step >>= 1) {
+#pragma unroll
+              for (int j = 0; j < UP; j++) {
+                const uint32_t mid = lo[j] + step;
+                if (s_pref[wv][mid] <= idx[j]) lo[j] = mid;
+              }
+            }
+#pragma unroll
+            for (int j = 0; j < UP; j++) {
+              const uint32_t r0 = s_rs[wv][lo[j]];
+              g.p[j] = (r0 >> 1) + (idx[j] - s_pref[wv][lo[j]]);
+              g.lo[j] = r0;
+              g.hi[j] = s_re[wv][lo[j]];
+              g.du[j] = s_du[wv][lo[j]];
+              g.gb[j] = REPL ? s_gb[wv][lo[j]] : 0u;
+            }
+          };
+          PairGroup<UP> ga, gb;
+          PairGathered<UP> ra, rb;
+          locate_p(0, ga);
+          load_pairs(a, ga);
+          locate_p(64 * UP, gb);
+          load_pairs(a, gb);
+          gather_pairs<FLT, UP>(a, ga, ra);
+          for (uint32_t base = 0; base < total; base += 64 * UP) {
+            locate_p(base + 128 * UP, ga);
+            load_pairs(a, ga);
+            gather_pairs<FLT, UP>(a, gb, rb);
+            finish_pairs<FLT, UP>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
+            ra = rb;
+            gb = ga;
+          }
+        } else {
         // Three-stage pipeline over groups of 64 * UN edges: while group g is consumed, the
         // gathers of g+1 and the (col, w) loads of g+2 are in flight.  Loads are issued in the order
         // they are waited for, and past the end every window is inactive (its loads read index 0).
@@ -1102,6 +1222,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           finish<FLT, UN>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
           ra = rb;
           gb = ga;
+        }
         }
         wave_sync();
       }

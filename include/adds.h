@@ -58,8 +58,9 @@ typedef struct {
                             window, no more than a quarter of the most seen at a head move, the rest of the
                             solve runs as one bucket with its heavy edges relaxed at once (default 0: never);
                             saves the phase changes of a long sparse tail */
-  uint32_t edge_windows; /* 64-edge windows per wavefront per step of the edge pipeline: 4 (default) or 8
-                            (more loads in flight; pays on throughput-bound solves) */
+  uint32_t edge_windows; /* edges per wavefront per step of the edge pipeline, in 64-edge windows: 4
+                            (default) or 8 (twice the edges in flight, loaded as 16-byte pairs; pays on
+                            throughput-bound solves) */
   float light_delta;     /* edges with w <= light_delta are light: relaxed whenever their source is expanded;
                             the others are heavy: relaxed once, after the source's bucket has settled.  At
                             least delta (default delta); a value above every weight makes all edges light
