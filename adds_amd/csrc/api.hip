@@ -1070,13 +1070,16 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
         p = q;
       }
     };
-    // 1) bucket width: powers of two around the default (int keeps power-of-two widths)
+    // 1) bucket width: powers of two around the default (int keeps power-of-two widths), each also with
+    //    every edge light: many sources on one graph do best with narrow buckets and no heavy phases, a
+    //    corner that one change at a time from the default does not reliably reach
     const float d0 = p.delta;
     for (int j = -3; j <= 3 && rc == ADDS_OK; j++) {
-      if (j == 0) continue;
       adds_params q = p;
       q.delta = d0 * std::ldexp(1.0f, j);
       if (is_int && q.delta < 1.0f) continue;
+      if (j != 0) try_point(q);
+      q.light_delta = 1e30f;
       try_point(q);
     }
     // light threshold above the bucket width: all edges light (no heavy phases) pays where vertices are
@@ -1172,6 +1175,19 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
     {
       adds_params q = p;
       q.edge_windows = q.edge_windows == 8 ? 4 : 8;
+      try_point(q);
+    }
+    // the light threshold and the two flags again: the first pass can settle on a point from which
+    // all-light many-source solves or the no-wait flush look worse than they are
+    for (float ld : {0.0f, 1e30f}) {
+      adds_params q = p;
+      if (q.light_delta == ld) continue;
+      q.light_delta = ld;
+      try_point(q);
+    }
+    for (uint32_t f : {ADDS_F_NO_LOCAL, ADDS_F_NO_WAIT}) {
+      adds_params q = p;
+      q.flags ^= f;
       try_point(q);
     }
     // 9) playoff: single timings differ by a few percent from run to run, about as much as the best
