@@ -256,3 +256,24 @@ def test_edge_cases(case):
         _check(g, dist[i], s, g.is_float)
     dist, _ = dg.sssp_nearfar(src)
     _check(g, dist, src, g.is_float)
+
+
+@pytest.mark.parametrize("is_float", [False, True])
+def test_pair_windows_odd_ranges(is_float):
+    """The 8-window form loads 16-byte edge pairs: ranges that start or end on an odd edge (odd task sizes,
+    short hub splits, light/heavy boundaries inside rows, the last edge of the array) mask the half that
+    lies outside; single and several sources, against Dijkstra bit for bit."""
+    g = adds_amd.HostGraph.rmat(14, 12, seed=77, wkind="f32" if is_float else "u32", lo=0 if is_float else 1,
+                                hi=1.0 if is_float else 999)
+    dg = adds_amd.Graph(g)
+    src = g.max_degree_vertex()
+    delta = 0.05 if is_float else 64.0
+    for params in (dict(edge_windows=8, task_edges=37, hub_degree=64, delta=delta),
+                   dict(edge_windows=8, task_edges=1, hub_degree=64, delta=delta, chunk=7),
+                   dict(edge_windows=8, light_delta=1e30, delta=delta, flags=0x10)):
+        dist, _ = dg.sssp(src, timeout_ms=TIMEOUT, **params)
+        _check(g, dist, src, is_float)
+        srcs = g.sample_component(src, 5, seed=3)
+        dm, _ = dg.sssp_multi(srcs, timeout_ms=TIMEOUT, **params)
+        for i, s in enumerate(srcs):
+            _check(g, dm[i], s, is_float)
