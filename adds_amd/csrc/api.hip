@@ -472,14 +472,11 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.inv_delta = 1.0f / r.delta;
   a.chunk = r.chunk;
   // guided claims: a wavefront takes its share of what is published among the wavefronts of one
-  // part, at least claim_min items (ADDS_CLAIM_MIN / ADDS_CLAIM_DIV override, for experiments).
-  // Measured on RMAT graphs, full claims (claim_min = chunk) are fastest: fewer claims, less contention.
+  // part, at least claim_min items.  Measured on RMAT graphs, full claims (claim_min = chunk) are
+  // fastest: fewer claims, less contention.
   a.claim_min = r.chunk;
   a.chunk_heavy = r.chunk;
-  if (const char* e = getenv("ADDS_HEAVY_CHUNK")) a.chunk_heavy = std::max(1, std::min(256, atoi(e)));
   a.claim_div = std::max<uint32_t>(1, grid * WAVES_PER_WG / r.parts);
-  if (const char* e = getenv("ADDS_CLAIM_MIN")) a.claim_min = std::max(1, atoi(e));
-  if (const char* e = getenv("ADDS_CLAIM_DIV")) a.claim_div = std::max(1, atoi(e));
   a.hub_degree = r.hub_degree;
   a.task_edges = r.task_edges;
   a.windows = r.windows;
@@ -492,10 +489,8 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.timeout_ticks = (uint64_t)(r.timeout_ms * (double)rate_khz);
   a.cached_probe = (r.flags & ADDS_F_ATOMIC_PROBE) ? 0u : 1u;
   a.sticky = 1;
-  if (const char* e = getenv("ADDS_STICKY")) a.sticky = atoi(e) != 0;
   a.local_cont = (r.flags & ADDS_F_NO_LOCAL) ? 0u : 1u;
   a.nowait = (r.flags & ADDS_F_NO_WAIT) ? 1u : 0u;
-  if (const char* e = getenv("ADDS_LOCAL")) a.local_cont = atoi(e) != 0;
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
   a.trace_slice = std::max(1, rate_khz / 20);  // 50 us bins
