@@ -70,6 +70,7 @@ struct adds_graph {
   // the device-array accessor); valid for split_delta
   uint2* scw = nullptr;
   uint32_t* lcnt = nullptr;
+  uint4* rinfo = nullptr;  // per-vertex row record (row start, row end, light edges)
   uint32_t* split_aux = nullptr;
   float split_delta = 0.f;
   Workspace ws;
@@ -155,6 +156,8 @@ static void release_graph_memory(adds_graph* g) {
   if (g->rsrc) (void)hipFree(g->rsrc);
   if (g->scw) (void)hipFree(g->scw);
   if (g->lcnt) (void)hipFree(g->lcnt);
+  if (g->rinfo) (void)hipFree(g->rinfo);
+  g->rinfo = nullptr;
   if (g->split_aux) (void)hipFree(g->split_aux);
   g->scw = nullptr;
   g->lcnt = g->split_aux = nullptr;
@@ -275,7 +278,7 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   if (g->m) HIPCHK(hipMemcpyAsync(g->w, w_dev, g->m * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   if (g->lcnt && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
     HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scw, g->lcnt,
-                        g->split_aux, (hipStream_t)stream));
+                        g->split_aux, g->rinfo, (hipStream_t)stream));
   if (g->union_g) {  // weights changed: the union copy is stale
     release_graph_memory(g->union_g);
     delete g->union_g;
@@ -437,11 +440,13 @@ static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
   if (!g->lcnt) {
     HIPCHK(hipMalloc(&g->scw, (g->m + 1) * 8));  // one pad entry: the last 16-byte pair is in bounds
     HIPCHK(hipMalloc(&g->lcnt, std::max<uint64_t>(g->n, 1) * 4));
+    HIPCHK(hipMalloc(&g->rinfo, std::max<uint64_t>(g->n, 1) * 16));
     HIPCHK(hipMalloc(&g->split_aux, split_aux_words(g->m) * 4));
     g->split_delta = 0.f;
   }
   if (g->split_delta != delta) {
-    HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scw, g->lcnt, g->split_aux, s));
+    HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scw, g->lcnt, g->split_aux,
+                        g->rinfo, s));
     g->split_delta = delta;
   }
   return ADDS_OK;
@@ -456,6 +461,7 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   a.row_ptr = g->row;
   a.cw = g->scw;
   a.lcnt = g->lcnt;
+  a.rinfo = g->rinfo;
   a.dist = dist;
   a.st = ws.st;
   a.pool = ws.pool;

@@ -73,6 +73,7 @@ struct KArgs {
   const uint32_t* row_ptr;
   const uint2* cw;       // per edge (destination, weight bits), each row light-first (w <= delta)
   const uint32_t* lcnt;  // light edges at the front of each row
+  const uint4* rinfo;    // per vertex (row start, row end, light edges, 0): one 16-byte load per item
   uint32_t* dist;     // distance words, total_n of them
   uint32_t* st;       // state words
   uint64_t* pool;     // pool_chunks * CH entries: (dist << 32) | vertex-instance id
@@ -132,7 +133,7 @@ hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
 // Stable per-row partition of (col, w) into light (w <= delta) then heavy edges, written interleaved as
 // (col, w) pairs, with per-row light counts.  aux: split_aux_words(m) words of scratch (long rows).
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, hipStream_t s);
+                        bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, uint4* rinfo, hipStream_t s);
 constexpr uint64_t split_aux_words(uint64_t m) { return m / 4096 + 2; }
 
 // Litmus (litmus.hip)

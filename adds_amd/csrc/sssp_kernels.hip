@@ -1001,11 +1001,13 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           deg[j] = min(te, r1 - rs[j]);
         }
       }
-      // Coherent loads of the distances: a stale (larger) value would defeat the filter.  A partial
-      // claim is latency-bound: the row bounds are loaded alongside (unconditionally, at vertex 0 for
-      // empty slots) so that all are in flight together.  A full claim (its queue had plenty of work)
-      // loads the rows of live items only, after the distances: a second round trip, but a stale item
-      // -- often most of them -- then costs one random load instead of four.
+      // Coherent loads of the distances: a stale (larger) value would defeat the filter.  Each item's row
+      // (start, end, light edges) is one 16-byte record (rinfo): one random load instead of three
+      // (measured: C3 int -2%, float -4%).  A partial claim is latency-bound: the records are loaded
+      // alongside the distances (unconditionally, at vertex 0 for empty slots) so that all are in
+      // flight together.  A full claim (its queue had plenty of work) loads the records of live items
+      // only, after the distances: a second round trip, but a stale item -- often most of them --
+      // then costs one random load instead of two.
       uint32_t dnow[IPL], r0v[IPL], r1v[IPL], lcv[IPL];
 #pragma unroll
       for (int j = 0; j < IPL; j++) gbase[j] = REPL ? (vgid[j] / a.n) * a.n : 0u;
@@ -1014,19 +1016,19 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         for (int j = 0; j < IPL; j++) dnow[j] = ald(a.dist + vgid[j]);
 #pragma unroll
         for (int j = 0; j < IPL; j++) {
-          const uint32_t v = (have[j] && dnow[j] >= du[j]) ? vgid[j] - gbase[j] : 0u;
-          r0v[j] = a.row_ptr[v];
-          r1v[j] = a.row_ptr[v + 1];
-          lcv[j] = a.lcnt[v];
+          const uint4 ri = a.rinfo[(have[j] && dnow[j] >= du[j]) ? vgid[j] - gbase[j] : 0u];
+          r0v[j] = ri.x;
+          r1v[j] = ri.y;
+          lcv[j] = ri.z;
         }
       } else {
 #pragma unroll
         for (int j = 0; j < IPL; j++) {
-          const uint32_t v = vgid[j] - gbase[j];
           dnow[j] = ald(a.dist + vgid[j]);
-          r0v[j] = a.row_ptr[v];
-          r1v[j] = a.row_ptr[v + 1];
-          lcv[j] = a.lcnt[v];
+          const uint4 ri = a.rinfo[vgid[j] - gbase[j]];
+          r0v[j] = ri.x;
+          r1v[j] = ri.y;
+          lcv[j] = ri.z;
         }
       }
       uint32_t nstale = 0, nreb = 0, nexp = 0, nlate = 0;
@@ -1649,8 +1651,14 @@ hipError_t launch_cleanup(const KArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_cleanup, dim3(1024), dim3(256), 0, s, a);
   return hipGetLastError();
 }
+// Per-vertex row record (row start, row end, light edges, 0) for the solver's item loads.
+__global__ void k_rinfo(const uint32_t* row_ptr, const uint32_t* lcnt, uint32_t n, uint4* rinfo) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+    rinfo[v] = make_uint4(row_ptr[v], row_ptr[v + 1], lcnt[v], 0u);
+}
+
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, hipStream_t s) {
+                        bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, uint4* rinfo, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipError_t e = launch_fill_u32(aux, 1, 0, s);
   if (e != hipSuccess) return e;
@@ -1659,6 +1667,9 @@ hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint
                      lcnt, aux);
   hipLaunchKernelGGL(k_split_big, dim3(1024), dim3(1024), 0, s, row_ptr, col, w, delta, is_float ? 1 : 0, cw2,
                      lcnt, (const uint32_t*)aux);
+  if (rinfo)
+    hipLaunchKernelGGL(k_rinfo, dim3(std::min<uint32_t>((n + 255) / 256, 8192)), dim3(256), 0, s, row_ptr, lcnt, n,
+                       rinfo);
   return hipGetLastError();
 }
 
