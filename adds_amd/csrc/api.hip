@@ -458,9 +458,7 @@ static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_
   Workspace& ws = g->ws;
   KArgs a;
   memset(&a, 0, sizeof(a));
-  a.row_ptr = g->row;
   a.cw = g->scw;
-  a.lcnt = g->lcnt;
   a.rinfo = g->rinfo;
   a.dist = dist;
   a.st = ws.st;

@@ -70,10 +70,8 @@ static_assert(NSTATS <= 32, "the stats block is two state lines of u64");
 enum DevErr : uint32_t { DERR_NONE = 0, DERR_OVERFLOW = 1, DERR_TIMEOUT = 2, DERR_ABORT = 3 };
 
 struct KArgs {
-  const uint32_t* row_ptr;
-  const uint2* cw;       // per edge (destination, weight bits), each row light-first (w <= delta)
-  const uint32_t* lcnt;  // light edges at the front of each row
-  const uint4* rinfo;    // per vertex (row start, row end, light edges, 0): one 16-byte load per item
+  const uint2* cw;       // per edge (destination, weight bits), each row light-first (w <= delta), one pad
+  const uint4* rinfo;    // per vertex (row start, row end, light edges at the front, 0): one load per item
   uint32_t* dist;     // distance words, total_n of them
   uint32_t* st;       // state words
   uint64_t* pool;     // pool_chunks * CH entries: (dist << 32) | vertex-instance id
