@@ -109,7 +109,8 @@ EXPORTS = [
     "adds_graph_create", "adds_graph_create_device", "adds_graph_set_weights_device",
     "adds_graph_destroy", "adds_graph_info", "adds_graph_device_arrays",
     "adds_sssp", "adds_sssp_nearfar", "adds_sssp_multi", "adds_sssp_batched", "adds_last_status",
-    "adds_solve_graph_create", "adds_solve_graph_launch", "adds_solve_graph_destroy",
+    "adds_solve_graph_create", "adds_solve_graph_create_batched", "adds_solve_graph_launch",
+    "adds_solve_graph_status", "adds_solve_graph_destroy",
     "adds_sssp_backward_f32",
     "adds_host_csr_free", "adds_read_gr", "adds_write_gr", "adds_write_dist_text", "adds_read_mtx",
     "adds_read_dimacs",
@@ -159,7 +160,10 @@ def _lib():
         "adds_sssp_batched": (i32, [ctypes.POINTER(vp), c_u32p, u32, ctypes.POINTER(vp), pp, sp, vp]),
         "adds_last_status": (i32, [vp, sp]),
         "adds_solve_graph_create": (i32, [vp, vp, u32, vp, pp, ctypes.POINTER(vp)]),
+        "adds_solve_graph_create_batched": (i32, [ctypes.POINTER(vp), c_u32p, u32, ctypes.POINTER(vp), pp,
+                                                  ctypes.POINTER(vp)]),
         "adds_solve_graph_launch": (i32, [vp, vp]),
+        "adds_solve_graph_status": (i32, [vp, sp]),
         "adds_solve_graph_destroy": (None, [vp]),
         "adds_sssp_backward_f32": (i32, [vp, u32, vp, vp, vp, ctypes.POINTER(ctypes.c_double), vp]),
         "adds_autotune": (i32, [vp, c_u32p, u32, u32, pp, ctypes.POINTER(ctypes.c_double)]),

@@ -27,6 +27,8 @@ def workload_key(host, k):
 
 
 def _load(path):
+    if path is None:  # no cache
+        return {}
     try:
         with open(path) as f:
             return json.load(f)
@@ -35,6 +37,8 @@ def _load(path):
 
 
 def _store(cache_path, key, params, ms):
+    if cache_path is None:
+        return
     cache = _load(cache_path)
     cache[key] = {"params": params, "kernel_ms": ms}
     try:

@@ -5,6 +5,7 @@
 // Reference seams replaced here: CSRGraph::copy_to_gpu / allocOnDevice (csr_graph.cu:86-105, 309-328),
 // gg_main's grid sizing, worklist sizing and timed solve (ads_int/kernel.cu:185-288), worklist::alloc /
 // reinit / set_param (wl.h:203-449) and the weight profiler (kernel.cu:151-183, 238-257).
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -57,8 +58,14 @@ struct Workspace {
   double timeout_ms = 0;
 };
 
+// Every graph handle gets a number no other handle of this process has had, renewed whenever its
+// weights change: a cached union of graphs is keyed by (handle, number), so neither a weight update nor
+// a new handle allocated at a freed handle's address can match a stale union.
+static std::atomic<uint64_t> g_next_gen{1};
+
 struct adds_graph {
   int device = 0;
+  uint64_t gen = 0;
   uint32_t n = 0;
   uint64_t m = 0;
   adds_wtype wt = ADDS_W_U32;
@@ -80,14 +87,22 @@ struct adds_graph {
   uint32_t* rrow = nullptr;
   uint32_t* reid = nullptr;
   uint32_t* rsrc = nullptr;
-  // batched union cache
-  std::vector<const adds_graph*> union_key;
+  // batched union cache: the members' (handle, generation) pairs it was built from
+  std::vector<std::pair<const adds_graph*, uint64_t>> union_key;
   adds_graph* union_g = nullptr;
   std::vector<uint32_t> union_off;
 };
 
+// A solve captured in a HIP graph.  It owns its workspace (queues, pool, chunk maps, range records,
+// status words), so nothing an eager solve on the same graph reallocates can move under it; a batched
+// capture also owns its block-diagonal union of the member graphs (a snapshot taken at capture).
 struct adds_solve_graph {
-  adds_graph* g = nullptr;
+  adds_graph* g = nullptr;       // the graph solved: the caller's, or the owned union
+  adds_graph* owned = nullptr;   // batched capture: the union (destroyed with the solve graph)
+  Workspace ws;
+  float split_light = 0.f;       // light threshold of the light-first rows the capture was made for
+  uint32_t* srcs = nullptr;      // batched capture: sources mapped into the union (device)
+  uint64_t* scatter = nullptr;   // batched capture: per instance (destination pointer, union offset | n)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -198,12 +213,19 @@ static adds_status alloc_graph(uint32_t n, uint64_t m, adds_wtype wt, adds_graph
   HIPCHK(hipGetDevice(&dev));
   adds_graph* g = new adds_graph();
   g->device = dev;
+  g->gen = g_next_gen.fetch_add(1);
   g->n = n;
   g->m = m;
   g->wt = wt;
   hipError_t e = hipMalloc(&g->row, ((size_t)n + 1) * 4);
   if (e == hipSuccess) e = hipMalloc(&g->col, std::max<uint64_t>(m, 1) * 4);
   if (e == hipSuccess) e = hipMalloc(&g->w, std::max<uint64_t>(m, 1) * 4);
+  // the solver's light-first copy of the rows: allocated with the graph (filled by the first solve for
+  // its light threshold), so that no solve needs to allocate it
+  if (e == hipSuccess) e = hipMalloc(&g->scw, (m + 1) * 8);  // one pad entry: the last 16-byte pair is in bounds
+  if (e == hipSuccess) e = hipMalloc(&g->lcnt, std::max<uint64_t>(n, 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->rinfo, std::max<uint64_t>(n, 1) * 16);
+  if (e == hipSuccess) e = hipMalloc(&g->split_aux, split_aux_words(m) * 4);
   if (e != hipSuccess) {
     g_detail = std::string("graph allocation: ") + hipGetErrorString(e);
     release_graph_memory(g);
@@ -276,9 +298,11 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   if (!g || (!w_dev && g->m)) return ADDS_EINVAL;
   HIPCHK(hipSetDevice(g->device));
   if (g->m) HIPCHK(hipMemcpyAsync(g->w, w_dev, g->m * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
-  if (g->lcnt && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
+  if (g->split_delta != 0.f && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
     HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scw, g->lcnt,
                         g->split_aux, g->rinfo, (hipStream_t)stream));
+  // a new generation: every cached union that holds this graph (on any handle) no longer matches
+  g->gen = g_next_gen.fetch_add(1);
   if (g->union_g) {  // weights changed: the union copy is stale
     release_graph_memory(g->union_g);
     delete g->union_g;
@@ -348,7 +372,8 @@ static uint32_t default_chunk(uint64_t total_n) {
   return total_n <= (256u << 10) ? 64u : total_n <= (8u << 20) ? 128u : 256u;
 }
 
-static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t total_n, Resolved& r) {
+static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t total_n, uint32_t pool_shift,
+                           Resolved& r) {
   adds_params z;
   memset(&z, 0, sizeof(z));
   if (!p) p = &z;
@@ -370,7 +395,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   // on this graph (remembered)
   const uint64_t total_m = g->n ? (total_n / g->n) * g->m : 0;
   r.pool_entries = p->pool_entries ? p->pool_entries
-                                   : (4 * total_n + total_m / 16 + (8u << 20)) << std::min(g->ws.pool_shift, 6u);
+                                   : (4 * total_n + total_m / 16 + (8u << 20)) << std::min(pool_shift, 6u);
   if (!(r.delta > 0) || !std::isfinite(r.delta) || !(r.light_delta >= r.delta) || r.wg_per_cu > 64 || r.chunk > 256 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
       r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||
@@ -382,8 +407,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   return ADDS_OK;
 }
 
-static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t nsrc) {
-  Workspace& ws = g->ws;
+static adds_status ensure_workspace(Workspace& ws, const Resolved& r, uint32_t nsrc) {
   const uint32_t nq = r.nb * r.parts;
   const uint64_t chunks64 = (r.pool_entries + CH - 1) / CH;
   if (chunks64 >= (1u << 24)) return ADDS_EINVAL;  // chunk ids are 24-bit in the kernel's chunk-map cache
@@ -436,14 +460,8 @@ static adds_status ensure_workspace(adds_graph* g, const Resolved& r, uint32_t n
 
 // Light-first rows for light threshold `delta` (computed once per graph and threshold; solves with
 // another threshold re-split).
+// The buffers are allocated with the graph; this launches kernels only.
 static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
-  if (!g->lcnt) {
-    HIPCHK(hipMalloc(&g->scw, (g->m + 1) * 8));  // one pad entry: the last 16-byte pair is in bounds
-    HIPCHK(hipMalloc(&g->lcnt, std::max<uint64_t>(g->n, 1) * 4));
-    HIPCHK(hipMalloc(&g->rinfo, std::max<uint64_t>(g->n, 1) * 16));
-    HIPCHK(hipMalloc(&g->split_aux, split_aux_words(g->m) * 4));
-    g->split_delta = 0.f;
-  }
   if (g->split_delta != delta) {
     HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scw, g->lcnt, g->split_aux,
                         g->rinfo, s));
@@ -454,8 +472,8 @@ static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
 
 static uint32_t inf_bits(adds_wtype wt) { return wt == ADDS_W_F32 ? ADDS_INF_F32_BITS : ADDS_INF_U32; }
 
-static KArgs make_args(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_t total_n, uint32_t grid) {
-  Workspace& ws = g->ws;
+static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved& r, uint32_t* dist, uint64_t total_n,
+                       uint32_t grid) {
   KArgs a;
   memset(&a, 0, sizeof(a));
   a.cw = g->scw;
@@ -507,22 +525,27 @@ static uint32_t grid_for(const Resolved& r) {
   return cu * r.wg_per_cu;
 }
 
-// Enqueues dist fill + sources, the persistent solve and the cleanup on `s`.
-static adds_status enqueue_solve(adds_graph* g, const Resolved& r, uint32_t* dist, uint64_t total_n,
-                                 const uint32_t* srcs_dev, uint32_t k, bool replicated, hipStream_t s,
-                                 bool kernel_events) {
+// The bucket timeline buffer when ADDS_TRACE names a file (eager solves only).
+static adds_status ensure_trace(Workspace& ws) {
+  if (!ws.trace && getenv("ADDS_TRACE"))
+    HIPCHK(hipMalloc(&ws.trace, (4 + 4 * (size_t)TRACE_CAP + 8 * (size_t)TRACE_HBINS * TRACE_HCOPIES) * 8));
+  return ADDS_OK;
+}
+
+// Enqueues dist fill + sources, the persistent solve and the cleanup on `s` (kernel launches only).
+static adds_status enqueue_solve(const adds_graph* g, const Workspace& ws, const Resolved& r, uint32_t* dist,
+                                 uint64_t total_n, const uint32_t* srcs_dev, uint32_t k, bool replicated,
+                                 hipStream_t s, bool kernel_events) {
   const uint32_t grid = grid_for(r);
-  if (!g->ws.trace && getenv("ADDS_TRACE"))
-    HIPCHK(hipMalloc(&g->ws.trace, (4 + 4 * (size_t)TRACE_CAP + 8 * (size_t)TRACE_HBINS * TRACE_HCOPIES) * 8));
-  KArgs a = make_args(g, r, dist, total_n, grid);
+  KArgs a = make_args(g, ws, r, dist, total_n, grid);
   if (a.trace) {
     HIPCHK(launch_fill_u64(a.trace, 1, 0, s));
     HIPCHK(launch_fill_u64(a.trace + 4 + 4 * (size_t)TRACE_CAP, 8 * (size_t)TRACE_HBINS * TRACE_HCOPIES, 0, s));
   }
   HIPCHK(launch_init(a, srcs_dev, k, replicated ? 1 : 0, inf_bits(g->wt), s));
-  if (kernel_events) HIPCHK(hipEventRecord(g->ws.evk0, s));
+  if (kernel_events) HIPCHK(hipEventRecord(ws.evk0, s));
   HIPCHK(launch_solve(a, g->wt == ADDS_W_F32, replicated, grid, s));
-  if (kernel_events) HIPCHK(hipEventRecord(g->ws.evk1, s));
+  if (kernel_events) HIPCHK(hipEventRecord(ws.evk1, s));
   HIPCHK(launch_cleanup(a, s));
   return ADDS_OK;
 }
@@ -554,8 +577,7 @@ static adds_status wait_solve(Workspace& ws) {
 // "t_us kind bucket prev_head pool_chunks" (kind 0 = start, 1 = heavy phase opened, 2 = head moved,
 // 3 = done), after a
 // header line per solve.  A diagnostic aid; solves are unaffected apart from the records' writes.
-static void dump_trace(adds_graph* g) {
-  Workspace& ws = g->ws;
+static void dump_trace(const adds_graph* g, const Workspace& ws) {
   const char* path = getenv("ADDS_TRACE");
   if (!path) return;
   uint64_t cnt = 0;
@@ -593,8 +615,7 @@ static void dump_trace(adds_graph* g) {
   fclose(f);
 }
 
-static adds_status collect(adds_graph* g, adds_stats* stats) {
-  Workspace& ws = g->ws;
+static adds_status collect(const adds_graph* g, Workspace& ws, adds_stats* stats) {
   adds_status rc = wait_solve(ws);
   ws.pending = false;
   if (rc != ADDS_OK) return rc;
@@ -628,14 +649,22 @@ static adds_status collect(adds_graph* g, adds_stats* stats) {
     stats->heavy_edges = sv[ST_HEDGES];
     stats->ticks_flush = sv[ST_T_FLUSH];
   }
-  if (ws.trace) dump_trace(g);
+  if (ws.trace) dump_trace(g, ws);
   if (err == DERR_OVERFLOW) {
     if (!ws.pool_fixed && ws.pool_shift < 6) ws.pool_shift++;
     g_detail = "worklist pool exhausted (" + std::to_string(hs[S_POOL]) + " chunks requested)";
     return ADDS_EOVERFLOW;
   }
-  if (err == DERR_TIMEOUT || err == DERR_ABORT) return ADDS_ETIMEOUT;
-  if (err != 0) return ADDS_EHIP;
+  if (err == DERR_TIMEOUT || err == DERR_ABORT) {
+    g_detail = err == DERR_ABORT ? "solve aborted by the host watchdog"
+                                 : "device watchdog ended the solve after " + std::to_string((long)ws.timeout_ms) +
+                                       " ms (or a bounded wait expired)";
+    return ADDS_ETIMEOUT;
+  }
+  if (err != 0) {
+    g_detail = "solver error code " + std::to_string(err);
+    return ADDS_EHIP;
+  }
   if (hs[S_DONE] != 1) {
     g_detail = "solver exited without completing";
     return ADDS_EHIP;
@@ -650,7 +679,7 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
   const uint64_t total_n = replicated ? (uint64_t)k * g->n : g->n;
   if (total_n >= 0xFFFFFFFFull) return ADDS_EINVAL;
   Resolved r;
-  adds_status rc = resolve(g, p, total_n, r);
+  adds_status rc = resolve(g, p, total_n, g->ws.pool_shift, r);
   if (rc != ADDS_OK) return rc;
   for (uint32_t i = 0; i < k; i++)
     if (srcs_host[i] >= g->n) {
@@ -662,15 +691,16 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
     return ADDS_EINVAL;
   }
   for (int attempt = 0; attempt < 4; attempt++) {
-    rc = ensure_workspace(g, r, k);
+    Workspace& ws = g->ws;
+    rc = ensure_workspace(ws, r, k);
+    if (rc == ADDS_OK) rc = ensure_trace(ws);
     if (rc == ADDS_OK) rc = ensure_split(g, r.light_delta, s);
     if (rc != ADDS_OK) return rc;
-    Workspace& ws = g->ws;
     ws.timeout_ms = r.timeout_ms;
     ws.pool_fixed = r.pool_fixed;
     HIPCHK(hipMemcpyAsync(ws.srcs, srcs_host, (size_t)k * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(ws.ev0, s));
-    rc = enqueue_solve(g, r, dist_dev, total_n, ws.srcs, k, replicated, s, true);
+    rc = enqueue_solve(g, ws, r, dist_dev, total_n, ws.srcs, k, replicated, s, true);
     ws.kernel_events = true;
     if (rc != ADDS_OK) return rc;
     HIPCHK(hipEventRecord(ws.ev1, s));
@@ -679,7 +709,7 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
     ws.last_stream = s;
     ws.pending = true;
     if (!sync) return ADDS_OK;
-    rc = collect(g, stats);
+    rc = collect(g, ws, stats);
     if (rc != ADDS_EOVERFLOW || r.pool_fixed) return rc;
     r.pool_entries *= 2;  // grow the pool and solve again
   }
@@ -728,7 +758,7 @@ extern "C" adds_status adds_last_status(adds_graph* g, adds_stats* stats) {
   if (!g) return ADDS_EINVAL;
   if (!g->ws.pending) return ADDS_OK;
   HIPCHK(hipSetDevice(g->device));
-  return collect(g, stats);
+  return collect(g, g->ws, stats);
 }
 
 // ---- batched over different graphs: block-diagonal union CSR ----
@@ -776,8 +806,13 @@ static adds_status build_union(adds_graph* const* graphs, uint32_t k, adds_graph
 static adds_status get_union(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k, adds_graph** out,
                              std::vector<uint32_t>& usrc) {
   adds_graph* g0 = graphs[0];
-  std::vector<const adds_graph*> key(graphs, graphs + k);
-  if (!g0->union_g || g0->union_key != key) {
+  // keyed by (handle, generation): a member whose weights changed, or a new handle at the address of a
+  // destroyed one, does not match
+  std::vector<std::pair<const adds_graph*, uint64_t>> key(k);
+  for (uint32_t i = 0; i < k; i++) key[i] = {graphs[i], graphs[i]->gen};
+  bool hit = g0->union_g && g0->union_key == key && g0->union_off.size() == (size_t)k + 1;
+  for (uint32_t i = 0; hit && i < k; i++) hit = g0->union_off[i + 1] - g0->union_off[i] == graphs[i]->n;
+  if (!hit) {
     if (g0->union_g) {
       release_graph_memory(g0->union_g);
       delete g0->union_g;
@@ -839,39 +874,132 @@ extern "C" adds_status adds_sssp_batched(adds_graph* const* graphs, const uint32
   return ADDS_OK;
 }
 
-// ---- HIP graph capture of a multi-source solve ----
-extern "C" adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* srcs_dev, uint32_t k, void* dist_dev,
-                                               const adds_params* p, adds_solve_graph** out) {
-  if (!g || !srcs_dev || !k || !dist_dev || !out) return ADDS_EINVAL;
-  HIPCHK(hipSetDevice(g->device));
-  const bool repl = k > 1;
+// ---- HIP graph capture ----
+// Copies each instance's rows of a union solve's output into the caller's buffers: tab[2i] is the
+// destination, tab[2i+1] = (offset in the union << 32) | vertices.
+__global__ static void k_scatter_union(const uint32_t* src, const uint64_t* tab, uint32_t k) {
+  for (uint32_t i = blockIdx.y; i < k; i += gridDim.y) {
+    uint32_t* dst = (uint32_t*)tab[2 * i];
+    const uint32_t off = (uint32_t)(tab[2 * i + 1] >> 32), n = (uint32_t)tab[2 * i + 1];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) dst[j] = src[off + j];
+  }
+}
+
+extern "C" void adds_solve_graph_destroy(adds_solve_graph* sg) {
+  if (!sg) return;
+  if (sg->g) (void)hipSetDevice(sg->g->device);
+  (void)hipDeviceSynchronize();
+  if (sg->exec) (void)hipGraphExecDestroy(sg->exec);
+  if (sg->graph) (void)hipGraphDestroy(sg->graph);
+  free_workspace(sg->ws);
+  if (sg->srcs) (void)hipFree(sg->srcs);
+  if (sg->scatter) (void)hipFree(sg->scatter);
+  if (sg->owned) {
+    release_graph_memory(sg->owned);
+    delete sg->owned;
+  }
+  delete sg;
+}
+
+// Captures the solve of k sources (device array) on sg->g into sg's HIP graph, with sg's own workspace:
+// dist fill + sources, the persistent kernel, the worklist cleanup, and, for a batched capture, the copy
+// of every instance's distances out of the union's output.
+static adds_status capture_solve(adds_solve_graph* sg, const adds_params* p, const uint32_t* srcs_dev, uint32_t k,
+                                 bool repl, uint32_t* dist_dev) {
+  adds_graph* g = sg->g;
   const uint64_t total_n = repl ? (uint64_t)k * g->n : g->n;
   if (total_n >= 0xFFFFFFFFull) return ADDS_EINVAL;
   Resolved r;
-  adds_status rc = resolve(g, p, total_n, r);
+  adds_status rc = resolve(g, p, total_n, 0, r);
   if (rc != ADDS_OK) return rc;
-  rc = ensure_workspace(g, r, k);
+  if ((uint64_t)k > r.pool_entries) {
+    g_detail = "more sources than worklist pool entries";
+    return ADDS_EINVAL;
+  }
+  rc = ensure_workspace(sg->ws, r, 0);
   if (rc == ADDS_OK) rc = ensure_split(g, r.light_delta, nullptr);
   if (rc != ADDS_OK) return rc;
   HIPCHK(hipDeviceSynchronize());
-  g->ws.timeout_ms = r.timeout_ms;
-  g->ws.pool_fixed = r.pool_fixed;
+  sg->ws.timeout_ms = r.timeout_ms;
+  sg->ws.pool_fixed = true;  // a replay cannot grow its pool: an overflow is reported by the status call
+  sg->split_light = r.light_delta;
   hipStream_t s;
   HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  adds_solve_graph* sg = new adds_solve_graph();
-  sg->g = g;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = enqueue_solve(g, r, (uint32_t*)dist_dev, total_n, srcs_dev, k, repl, s, false);
+    rc = enqueue_solve(g, sg->ws, r, dist_dev, total_n, srcs_dev, k, repl, s, false);
+    if (rc == ADDS_OK && sg->scatter) {
+      hipLaunchKernelGGL(k_scatter_union, dim3(64, std::min<uint32_t>(k, 1024)), dim3(256), 0, s, dist_dev,
+                         (const uint64_t*)sg->scatter, k);
+      if (hipGetLastError() != hipSuccess) rc = ADDS_EHIP;
+    }
     hipError_t e2 = hipStreamEndCapture(s, &sg->graph);
     if (rc == ADDS_OK && e2 != hipSuccess) e = e2;
   }
   if (e == hipSuccess && rc == ADDS_OK) e = hipGraphInstantiate(&sg->exec, sg->graph, nullptr, nullptr, 0);
   (void)hipStreamDestroy(s);
-  if (e != hipSuccess || rc != ADDS_OK) {
-    if (e != hipSuccess) g_detail = std::string("graph capture: ") + hipGetErrorString(e);
+  if (e != hipSuccess) {
+    g_detail = std::string("graph capture: ") + hipGetErrorString(e);
+    return ADDS_EHIP;
+  }
+  return rc;
+}
+
+extern "C" adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* srcs_dev, uint32_t k, void* dist_dev,
+                                               const adds_params* p, adds_solve_graph** out) {
+  if (!g || !srcs_dev || !k || !dist_dev || !out) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  adds_solve_graph* sg = new adds_solve_graph();
+  sg->g = g;
+  const adds_status rc = capture_solve(sg, p, srcs_dev, k, k > 1, (uint32_t*)dist_dev);
+  if (rc != ADDS_OK) {
     adds_solve_graph_destroy(sg);
-    return rc != ADDS_OK ? rc : ADDS_EHIP;
+    return rc;
+  }
+  *out = sg;
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_solve_graph_create_batched(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k,
+                                                       void* const* dists, const adds_params* p,
+                                                       adds_solve_graph** out) {
+  if (!graphs || !srcs || !dists || !k || !out) return ADDS_EINVAL;
+  for (uint32_t i = 0; i < k; i++) {
+    if (!graphs[i] || !dists[i]) return ADDS_EINVAL;
+    if (srcs[i] >= graphs[i]->n) {
+      g_detail = "source out of range";
+      return ADDS_EINVAL;
+    }
+  }
+  HIPCHK(hipSetDevice(graphs[0]->device));
+  adds_solve_graph* sg = new adds_solve_graph();
+  std::vector<uint32_t> off;
+  adds_status rc = build_union(graphs, k, &sg->owned, off);
+  if (rc == ADDS_OK) {
+    sg->g = sg->owned;
+    rc = ensure_outbuf(sg->owned, sg->owned->n);
+  }
+  if (rc == ADDS_OK) {
+    std::vector<uint32_t> usrc(k);
+    std::vector<uint64_t> tab(2 * (size_t)k);
+    for (uint32_t i = 0; i < k; i++) {
+      usrc[i] = off[i] + srcs[i];
+      tab[2 * i] = (uint64_t)(uintptr_t)dists[i];
+      tab[2 * i + 1] = ((uint64_t)off[i] << 32) | graphs[i]->n;
+    }
+    hipError_t e = hipMalloc(&sg->srcs, (size_t)k * 4);
+    if (e == hipSuccess) e = hipMalloc(&sg->scatter, (size_t)k * 16);
+    if (e == hipSuccess) e = hipMemcpy(sg->srcs, usrc.data(), (size_t)k * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(sg->scatter, tab.data(), (size_t)k * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      g_detail = std::string("batched solve graph: ") + hipGetErrorString(e);
+      rc = e == hipErrorOutOfMemory ? ADDS_ENOMEM : ADDS_EHIP;
+    }
+  }
+  if (rc == ADDS_OK) rc = capture_solve(sg, p, sg->srcs, k, false, sg->owned->outbuf);
+  if (rc != ADDS_OK) {
+    adds_solve_graph_destroy(sg);
+    return rc;
   }
   *out = sg;
   return ADDS_OK;
@@ -882,7 +1010,10 @@ extern "C" adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* strea
   adds_graph* g = sg->g;
   HIPCHK(hipSetDevice(g->device));
   hipStream_t s = (hipStream_t)stream;
-  Workspace& ws = g->ws;
+  Workspace& ws = sg->ws;
+  // an eager solve with another light threshold may have re-split the rows since the capture
+  const adds_status rc = ensure_split(g, sg->split_light, s);
+  if (rc != ADDS_OK) return rc;
   HIPCHK(hipEventRecord(ws.ev0, s));
   ws.kernel_events = false;
   HIPCHK(hipGraphLaunch(sg->exec, s));
@@ -893,11 +1024,11 @@ extern "C" adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* strea
   return ADDS_OK;
 }
 
-extern "C" void adds_solve_graph_destroy(adds_solve_graph* sg) {
-  if (!sg) return;
-  if (sg->exec) (void)hipGraphExecDestroy(sg->exec);
-  if (sg->graph) (void)hipGraphDestroy(sg->graph);
-  delete sg;
+extern "C" adds_status adds_solve_graph_status(adds_solve_graph* sg, adds_stats* stats) {
+  if (!sg) return ADDS_EINVAL;
+  if (!sg->ws.pending) return ADDS_OK;
+  HIPCHK(hipSetDevice(sg->g->device));
+  return collect(sg->g, sg->ws, stats);
 }
 
 // ---- backward ----
@@ -1026,7 +1157,7 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
   adds_params p;
   memset(&p, 0, sizeof(p));
   Resolved r0;
-  rc = resolve(g, &p, words, r0);  // words = vertex-instances of the solve
+  rc = resolve(g, &p, words, g->ws.pool_shift, r0);  // words = vertex-instances of the solve
   if (rc == ADDS_OK) {
     p.delta = r0.delta;
     p.wg_per_cu = r0.wg_per_cu;

@@ -170,14 +170,18 @@ __device__ __forceinline__ bool stopped(const A& a) {
   return (ald(a.st + S_DONE) | ald(a.st + S_DONE + 1)) != 0;
 }
 
-// Distance arithmetic.  Int: uint32 add.  Float: IEEE binary32 add (round to nearest even; the build
-// disables contraction), compared as uint32 bits, which orders non-negative floats.
+// Distance arithmetic.  Int: uint32 add saturating at ADDS_SAT_U32 = 2^32-2, so a path sum that does
+// not fit never wraps to a small distance and never reads as unreached (INF = 2^32-1); saturation is
+// monotone, so the solution is still the unique least fixed point (the near-far solver and the oracle
+// saturate the same way).  Float: IEEE binary32 add (round to nearest even; the build disables
+// contraction), compared as uint32 bits, which orders non-negative floats.
 template <bool FLT>
 __device__ __forceinline__ uint32_t dadd(uint32_t d, uint32_t w) {
   if constexpr (FLT) {
     return __float_as_uint(__uint_as_float(d) + __uint_as_float(w));
   } else {
-    return d + w;
+    const uint32_t s = d + w;
+    return (s < d || s == 0xFFFFFFFFu) ? 0xFFFFFFFEu : s;
   }
 }
 template <bool FLT>
@@ -1036,8 +1040,13 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 #pragma unroll
       for (int j = 0; j < IPL; j++) {
         const bool stale = have[j] && dnow[j] < du[j];
-        early[j] = have[j] && !stale && !is_ref[j] && !heavy_claim && !tail_mode &&
-                   bucket_key<FLT>(du[j], a.inv_delta) > cb;
+        // Early items go to their own bucket -- when it is inside the ring's window.  One beyond it was
+        // clamped to the window's end when pushed and would be clamped there again at every head move (a
+        // distance 2^31 past the head with delta 1 would take tens of millions of moves), so it is
+        // relaxed now: the order of relaxations only affects the work, never the result.
+        const uint32_t key = bucket_key<FLT>(du[j], a.inv_delta);
+        early[j] = have[j] && !stale && !is_ref[j] && !heavy_claim && !tail_mode && key > cb &&
+                   key <= cb + a.nb - 3;
         ok[j] = have[j] && !stale && !early[j];
         has_heavy[j] = false;
         if (!is_ref[j]) {

@@ -275,7 +275,9 @@ def sssp_batched(graphs, srcs, **params):
 
 
 class SolveGraph:
-    """A multi-source solve captured in a HIP graph over fixed device buffers."""
+    """A solve captured in a HIP graph over fixed device buffers: k sources on one graph
+    (``SolveGraph(graph, srcs_ptr, k, dist_ptr)``) or one source on each of several graphs
+    (``SolveGraph.batched(graphs, srcs, dist_ptrs)``, one persistent launch over their union)."""
 
     def __init__(self, graph: Graph, srcs_ptr, k, dist_ptr, **params):
         p = L.Params.make(**params)
@@ -285,12 +287,28 @@ class SolveGraph:
         self._h = h
         self.graph = graph
 
+    @classmethod
+    def batched(cls, graphs, srcs, dist_ptrs, **params):
+        k = len(graphs)
+        srcs = np.ascontiguousarray(srcs, dtype=np.uint32)
+        hs = (ctypes.c_void_p * k)(*[g.handle.value for g in graphs])
+        ds = (ctypes.c_void_p * k)(*[int(d) for d in dist_ptrs])
+        p = L.Params.make(**params)
+        h = ctypes.c_void_p()
+        L.check(L._lib().adds_solve_graph_create_batched(hs, srcs.ctypes.data_as(L.c_u32p), k, ds, ctypes.byref(p),
+                                                         ctypes.byref(h)), "solve_graph_create_batched")
+        obj = cls.__new__(cls)
+        obj._h = h
+        obj.graph = graphs[0]
+        obj.graphs = list(graphs)  # the members stay alive while the capture exists
+        return obj
+
     def launch(self, stream=None):
         L.check(L._lib().adds_solve_graph_launch(self._h, _stream_ptr(stream)), "solve_graph_launch")
 
     def status(self):
         st = L.Stats()
-        L.check(L._lib().adds_last_status(self.graph.handle, ctypes.byref(st)), "solve_graph")
+        L.check(L._lib().adds_solve_graph_status(self._h, ctypes.byref(st)), "solve_graph")
         return st.as_dict()
 
     def close(self):

@@ -11,7 +11,11 @@
  *
  * Distances are 32-bit words: uint32 for ADDS_W_U32 graphs (unreached = ADDS_INF_U32) and IEEE binary32
  * for ADDS_W_F32 graphs (unreached = +infinity, bits ADDS_INF_F32_BITS).  Int distances follow the
- * reference's int32 arithmetic for every finite distance below 2^31.
+ * reference's int32 arithmetic for every finite distance below 2^31 (its INF is INT32_MAX,
+ * ads_int/csr_graph.h:20-23, so larger sums are undefined there).  Here every integer path sum is
+ * computed exactly up to 2^32-3 and saturates at ADDS_SAT_U32 = 2^32-2 beyond: a sum that does not fit
+ * never wraps to a small distance and never reads as unreached.  Every solver (adds_sssp*, near-far)
+ * and the CPU oracle saturate the same way, so their results agree bit for bit on such inputs too.
  */
 #ifndef ADDS_H
 #define ADDS_H
@@ -24,6 +28,7 @@ extern "C" {
 #endif
 
 #define ADDS_INF_U32 0xFFFFFFFFu
+#define ADDS_SAT_U32 0xFFFFFFFEu /* saturated integer path sum (see above) */
 #define ADDS_INF_F32_BITS 0x7F800000u
 
 typedef enum {
@@ -156,18 +161,36 @@ adds_status adds_sssp_multi(adds_graph* g, const uint32_t* srcs, uint32_t k, voi
                             const adds_params* p, adds_stats* stats, void* stream);
 /* k instances on possibly different graphs in one launch: when every handle is the same graph this is
  * adds_sssp_multi; otherwise the graphs are solved as one block-diagonal union CSR (built on first use and
- * cached in graphs[0]) from k sources.  dists[i] receives n_i words.  No reference counterpart. */
+ * cached in graphs[0], keyed by the members' handles and weight versions: a weight update of any member
+ * rebuilds it) from k sources.  dists[i] receives n_i words.  No reference counterpart. */
 adds_status adds_sssp_batched(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k,
                               void* const* dists, const adds_params* p, adds_stats* stats, void* stream);
 /* Status of the last solve issued with ADDS_F_NO_SYNC on this handle (synchronises its stream). */
 adds_status adds_last_status(adds_graph* g, adds_stats* stats);
 
-/* Captures one multi-source solve (dist fill, persistent kernel, worklist cleanup) into a HIP graph
- * bound to the given device buffers; adds_solve_graph_launch replays it.  No reference counterpart. */
+/* Solves captured in a HIP graph (the reference's device-side launch of manager + workers,
+ * ads_int/kernel.cu:130-149, becomes one host-side graph replay).  No reference counterpart as an API.
+ *
+ * adds_solve_graph_create captures one multi-source solve of g (dist fill, persistent kernel, worklist
+ * cleanup) bound to the device buffers srcs_dev [k] and dist_dev ([k][n]).  The solve graph owns its own
+ * worklist workspace, so eager solves, autotuning or other captures on the same handle never move memory
+ * under it.  The handle g must outlive the solve graph; a later weight update of g is seen by replays.
+ *
+ * adds_solve_graph_create_batched captures k instances on different graphs in one persistent launch (the
+ * block-diagonal union of adds_sssp_batched, built for and owned by the solve graph: a snapshot of the
+ * members' CSR at capture, so later weight updates of the members are not seen).  srcs is host memory;
+ * dists[i] is a device buffer of n_i words that every replay fills.
+ *
+ * adds_solve_graph_launch enqueues one replay on `stream`; adds_solve_graph_status waits for the last
+ * replay and returns its status and counters (ms = replay time).  The pool of a capture is fixed: a replay
+ * that exhausts it reports ADDS_EOVERFLOW (capture again with a larger pool_entries). */
 typedef struct adds_solve_graph adds_solve_graph;
 adds_status adds_solve_graph_create(adds_graph* g, const uint32_t* srcs_dev, uint32_t k, void* dist_dev,
                                     const adds_params* p, adds_solve_graph** out);
+adds_status adds_solve_graph_create_batched(adds_graph* const* graphs, const uint32_t* srcs, uint32_t k,
+                                            void* const* dists, const adds_params* p, adds_solve_graph** out);
 adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* stream);
+adds_status adds_solve_graph_status(adds_solve_graph* sg, adds_stats* stats);
 void adds_solve_graph_destroy(adds_solve_graph* sg);
 
 /* Picks the fastest tuning point for this workload (k sources on g; k == 1 is a plain solve) by

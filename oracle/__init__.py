@@ -45,23 +45,27 @@ def _prep(row_ptr, col, w):
     return row_ptr, col, np.ascontiguousarray(wb)
 
 
-def dijkstra(row_ptr, col, w, srcs, is_float):
-    """Distance words (uint32 view) from the sources; raises on int overflow."""
+def dijkstra(row_ptr, col, w, srcs, is_float, saturate=False):
+    """Distance words (uint32 view) from the sources.  An integer path sum at or above 2^32-1 saturates
+    at 2^32-2 (oracle.c relax_add); that raises OverflowError unless saturate=True, which returns the
+    saturated solution (the one the device solvers compute)."""
     row_ptr, col, wb = _prep(row_ptr, col, w)
     srcs = np.ascontiguousarray(np.atleast_1d(srcs), dtype=np.uint32)
     n = len(row_ptr) - 1
     d = np.empty(n, dtype=np.uint32)
-    if _lib().oracle_dijkstra(int(is_float), n, _p(row_ptr), _p(col), _p(wb), _p(srcs), len(srcs), _p(d)):
+    if _lib().oracle_dijkstra(int(is_float), n, _p(row_ptr), _p(col), _p(wb), _p(srcs), len(srcs), _p(d)) \
+            and not saturate:
         raise OverflowError("integer distance overflow")
     return d
 
 
-def bellman_ford(row_ptr, col, w, srcs, is_float):
+def bellman_ford(row_ptr, col, w, srcs, is_float, saturate=False):
     row_ptr, col, wb = _prep(row_ptr, col, w)
     srcs = np.ascontiguousarray(np.atleast_1d(srcs), dtype=np.uint32)
     n = len(row_ptr) - 1
     d = np.empty(n, dtype=np.uint32)
-    if _lib().oracle_bellman_ford(int(is_float), n, _p(row_ptr), _p(col), _p(wb), _p(srcs), len(srcs), _p(d)):
+    if _lib().oracle_bellman_ford(int(is_float), n, _p(row_ptr), _p(col), _p(wb), _p(srcs), len(srcs), _p(d)) \
+            and not saturate:
         raise OverflowError("integer distance overflow")
     return d
 

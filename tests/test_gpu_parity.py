@@ -108,6 +108,27 @@ def test_overflow_reports_error_not_hang():
     _check(g, dist, g.max_degree_vertex(), False)
 
 
+@pytest.mark.parametrize("params", [{}, dict(flags=0x10, edge_windows=8), dict(delta=1.0, light_delta=1e30)])
+def test_int_sums_saturate_like_the_oracle(params):
+    """Integer path sums past 2^32-2 saturate at ADDS_SAT_U32 in adds_sssp, adds_sssp_multi and the
+    near-far solver, exactly as the oracle does; a wrapping add would give vertex 2 the distance 0x10
+    through the detour 0 -> 5 -> 2."""
+    from test_oracle import saturation_case
+
+    row, col, w, want = saturation_case()
+    g = adds_amd.HostGraph.from_arrays(row, col, w, "u32")
+    ref = oracle.dijkstra(row, col, w, 0, False, saturate=True)
+    assert np.array_equal(ref, want)
+    dg = adds_amd.Graph(g)
+    dist, _ = dg.sssp(0, timeout_ms=TIMEOUT, **params)
+    assert np.array_equal(dist.view(np.uint32), want)
+    dm, _ = dg.sssp_multi([0, 1, 5], timeout_ms=TIMEOUT, **params)
+    for i, s in enumerate([0, 1, 5]):
+        assert np.array_equal(dm[i].view(np.uint32), oracle.dijkstra(row, col, w, s, False, saturate=True))
+    dn, _ = dg.sssp_nearfar(0)
+    assert np.array_equal(dn.view(np.uint32), want)
+
+
 def test_repeated_solves_same_handle():
     g = adds_amd.HostGraph.rmat(13, 16, seed=3, wkind="u32", lo=1, hi=100)
     dg = adds_amd.Graph(g)

@@ -55,6 +55,85 @@ def test_graph_replays_match_eager():
                               oracle.dijkstra(host.row_ptr, host.col, host.w, srcs[i], True))
 
 
+def test_graph_replay_survives_workspace_changes():
+    """A capture owns its workspace: eager solves with another bucket ring, sub-queue count, light
+    threshold or a pool that must grow (which reallocate the handle's own workspace and re-split its rows)
+    leave later replays correct."""
+    host = adds_amd.HostGraph.rmat(13, 16, seed=8, wkind="u32", lo=1, hi=1000)
+    srcs = host.sample_component(host.max_degree_vertex(), 4, seed=5)
+    dg = adds_amd.Graph(host)
+    dist = torch.empty((len(srcs), host.n), dtype=torch.int32, device="cuda")
+    sd = torch.from_numpy(srcs.astype(np.int32)).cuda()
+    sg = adds_amd.SolveGraph(dg, sd.data_ptr(), len(srcs), dist.data_ptr(), nbuckets=64, parts=8, delta=64.0)
+    sg.launch()
+    sg.status()
+    ref = dist.clone()
+    for i in range(len(srcs)):
+        assert np.array_equal(ref[i].cpu().numpy().view(np.uint32),
+                              oracle.dijkstra(host.row_ptr, host.col, host.w, srcs[i], False))
+    eager = torch.empty(host.n, dtype=torch.int32, device="cuda")
+    for params in (dict(nbuckets=256, parts=2), dict(light_delta=1e30, delta=8.0), dict(pool_entries=1 << 24),
+                   dict(nbuckets=16, parts=4, delta=512.0)):
+        dg.sssp_device(int(srcs[0]), eager.data_ptr(), **params)
+        dist.fill_(3)
+        sg.launch()
+        sg.status()
+        assert torch.equal(dist, ref), f"replay after an eager solve with {params} differs"
+    from adds_amd.autotune import tune
+    tune(dg, host, [int(srcs[1])], trials=1, cache_path=None, refresh=True)
+    dist.fill_(3)
+    sg.launch()
+    sg.status()
+    assert torch.equal(dist, ref)
+    sg.close()
+
+
+def test_batched_graph_capture_replays():
+    """Several graphs of uneven size in one captured persistent launch: every replay fills every
+    caller buffer with that graph's distances, bit-exact against Dijkstra, with eager work in between."""
+    hosts = [adds_amd.HostGraph.grid(30 + 11 * i, 25, 0, seed=i, wkind="u32", lo=1, hi=1000) if i % 2 == 0 else
+             adds_amd.HostGraph.er(2000 + 700 * i, 8, seed=i, wkind="u32", lo=1, hi=1000) for i in range(7)]
+    dgs = [adds_amd.Graph(h) for h in hosts]
+    srcs = [i % 3 for i in range(len(hosts))]
+    outs = [torch.full((h.n,), 5, dtype=torch.int32, device="cuda") for h in hosts]
+    sg = adds_amd.SolveGraph.batched(dgs, srcs, [o.data_ptr() for o in outs], delta=256.0)
+    refs = [oracle.dijkstra(h.row_ptr, h.col, h.w, s, False) for h, s in zip(hosts, srcs)]
+    for rep in range(3):
+        for o in outs:
+            o.fill_(rep)
+        torch.cuda.synchronize()
+        sg.launch()
+        st = sg.status()
+        assert st["vertices_expanded"] > 0
+        for h, o, r in zip(hosts, outs, refs):
+            assert np.array_equal(o.cpu().numpy().view(np.uint32), r), f"replay {rep} differs (n={h.n})"
+        adds_amd.sssp_batched(dgs[:3], srcs[:3])  # eager batched work on some of the members in between
+    sg.close()
+
+
+def test_union_cache_sees_member_weight_updates():
+    """A batched solve caches the union of its graphs on the first handle; changing the weights of
+    another member, or replacing a member with a new handle, must rebuild it."""
+    a = adds_amd.HostGraph.er(3000, 8, seed=1, wkind="f32", lo=1.0, hi=10.0)
+    b = adds_amd.HostGraph.grid(40, 50, 0, seed=2, wkind="f32", lo=1.0, hi=10.0)
+    da, db = adds_amd.Graph(a), adds_amd.Graph(b)
+    dists, _ = adds_amd.sssp_batched([da, db], [0, 0])
+    assert np.array_equal(dists[1].view(np.uint32), oracle.dijkstra(b.row_ptr, b.col, b.w, 0, True))
+    w2 = np.random.default_rng(4).uniform(0.5, 3.0, size=b.m).astype(np.float32)
+    wt = torch.from_numpy(w2).cuda()
+    db.set_weights_device(wt.data_ptr())
+    torch.cuda.synchronize()
+    dists, _ = adds_amd.sssp_batched([da, db], [0, 0])
+    assert np.array_equal(dists[1].view(np.uint32), oracle.dijkstra(b.row_ptr, b.col, w2, 0, True))
+    assert np.array_equal(dists[0].view(np.uint32), oracle.dijkstra(a.row_ptr, a.col, a.w, 0, True))
+    # a new member handle (possibly at the old one's address) of a larger graph
+    db.close()
+    c = adds_amd.HostGraph.grid(60, 70, 0, seed=3, wkind="f32", lo=1.0, hi=10.0)
+    dc = adds_amd.Graph(c)
+    dists, _ = adds_amd.sssp_batched([da, dc], [0, 5])
+    assert np.array_equal(dists[1].view(np.uint32), oracle.dijkstra(c.row_ptr, c.col, c.w, 5, True))
+
+
 def test_autotune_keeps_results():
     from adds_amd.autotune import tune
 

@@ -148,6 +148,34 @@ def test_int_overflow_is_reported():
         oracle.dijkstra(row, col, w, 0, False)
 
 
+def saturation_case():
+    """A path 0 -> 1 -> 2 -> 3 whose sums pass 2^32 (0xF0000000 + 0x20000000), a branch 1 -> 4 that ends
+    exactly at 2^32 - 3 (the largest exact sum), and a detour 0 -> 5 -> 2 that is shorter than the
+    saturated direct route only by a wrapped sum: the saturated solution is fixed by hand."""
+    # edges in CSR order: 0->1, 0->5, 1->2, 1->4, 2->3, 5->2
+    row = np.array([0, 2, 4, 5, 5, 5, 6], dtype=np.uint32)
+    col = np.array([1, 5, 2, 4, 3, 2], dtype=np.uint32)
+    w = np.array([0xF0000000, 0xFFFFFFF0, 0x20000000, 0x0FFFFFFD, 7, 0x20], dtype=np.uint32)
+    # d1 = 0xF0000000; d4 = 0xF0000000 + 0x0FFFFFFD = 0xFFFFFFFD (exact); d5 = 0xFFFFFFF0;
+    # d2 = min(sat(0xF0000000 + 0x20000000), sat(0xFFFFFFF0 + 0x20)) = 0xFFFFFFFE; d3 = sat(d2 + 7)
+    want = np.array([0, 0xF0000000, 0xFFFFFFFE, 0xFFFFFFFE, 0xFFFFFFFD, 0xFFFFFFF0], dtype=np.uint32)
+    return row, col, w, want
+
+
+def test_int_saturation_closed_form():
+    """With saturate=True the oracle returns the saturating solution (sums >= 2^32-1 become 2^32-2),
+    the contract the device solvers follow; Dijkstra and Bellman-Ford agree on it."""
+    row, col, w, want = saturation_case()
+    assert np.array_equal(oracle.dijkstra(row, col, w, 0, False, saturate=True), want)
+    assert np.array_equal(oracle.bellman_ford(row, col, w, 0, False, saturate=True), want)
+    with pytest.raises(OverflowError):
+        oracle.bellman_ford(row, col, w, 0, False)
+    # a sum of exactly 2^32 - 3 does not count as overflow
+    d = oracle.dijkstra(np.array([0, 1, 2, 2], dtype=np.uint32), np.array([1, 2], dtype=np.uint32),
+                        np.array([0xF0000000, 0x0FFFFFFD], dtype=np.uint32), 0, False)
+    assert np.array_equal(d, [0, 0xF0000000, 0xFFFFFFFD])
+
+
 # ---- backward reference ----
 def tree_paths_grad(row, col, w, dist, gdist):
     """Independent float64 evaluation of dL/dw = sum_t c_t * [e on the tree path to t]."""
