@@ -615,49 +615,40 @@ static void dump_trace(const adds_graph* g, const Workspace& ws) {
   fclose(f);
 }
 
-static adds_status collect(const adds_graph* g, Workspace& ws, adds_stats* stats) {
-  adds_status rc = wait_solve(ws);
-  ws.pending = false;
-  if (rc != ADDS_OK) return rc;
-  const uint32_t* hs = ws.host_state;
+// The solve's counters and outcome from a host copy of the first 8 state lines.  ms / kernel_ms are
+// left to the caller.
+static void read_stats(const uint32_t* hs, adds_stats* stats) {
+  const uint64_t* sv = (const uint64_t*)(hs + S_STATS);
+  stats->vertices_expanded = sv[ST_EXPANDED];
+  stats->edges_scanned = sv[ST_EDGES];
+  stats->relaxations = sv[ST_RELAX];
+  stats->pushes = sv[ST_PUSH];
+  stats->stale_skips = sv[ST_STALE];
+  stats->hub_tasks = sv[ST_TASKS];
+  stats->rebuckets = sv[ST_REBUCKET];
+  stats->pool_chunks_used = hs[S_POOL];
+  stats->probes = sv[ST_PROBE];
+  stats->cas_failures = sv[ST_CASFAIL];
+  stats->idle_iterations = sv[ST_IDLE];
+  stats->late_items = sv[ST_LATE];
+  stats->ticks_probe = sv[ST_T_PROBE];
+  stats->ticks_items = sv[ST_T_ITEMS];
+  stats->ticks_edges = sv[ST_T_EDGES];
+  stats->ticks_idle = sv[ST_T_IDLE];
+  stats->local_items = sv[ST_LOCAL];
+  stats->heavy_edges = sv[ST_HEDGES];
+  stats->ticks_flush = sv[ST_T_FLUSH];
+}
+
+static adds_status solve_outcome(const uint32_t* hs, double timeout_ms) {
   const uint32_t err = hs[S_DONE + 1];
-  if (stats) {
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, ws.ev0, ws.ev1);
-    const uint64_t* sv = (const uint64_t*)(hs + S_STATS);
-    stats->ms = ms;
-    stats->vertices_expanded = sv[ST_EXPANDED];
-    stats->edges_scanned = sv[ST_EDGES];
-    stats->relaxations = sv[ST_RELAX];
-    stats->pushes = sv[ST_PUSH];
-    stats->stale_skips = sv[ST_STALE];
-    stats->hub_tasks = sv[ST_TASKS];
-    stats->rebuckets = sv[ST_REBUCKET];
-    stats->pool_chunks_used = hs[S_POOL];
-    float kms = 0;
-    if (ws.kernel_events) (void)hipEventElapsedTime(&kms, ws.evk0, ws.evk1);
-    stats->kernel_ms = kms;
-    stats->probes = sv[ST_PROBE];
-    stats->cas_failures = sv[ST_CASFAIL];
-    stats->idle_iterations = sv[ST_IDLE];
-    stats->late_items = sv[ST_LATE];
-    stats->ticks_probe = sv[ST_T_PROBE];
-    stats->ticks_items = sv[ST_T_ITEMS];
-    stats->ticks_edges = sv[ST_T_EDGES];
-    stats->ticks_idle = sv[ST_T_IDLE];
-    stats->local_items = sv[ST_LOCAL];
-    stats->heavy_edges = sv[ST_HEDGES];
-    stats->ticks_flush = sv[ST_T_FLUSH];
-  }
-  if (ws.trace) dump_trace(g, ws);
   if (err == DERR_OVERFLOW) {
-    if (!ws.pool_fixed && ws.pool_shift < 6) ws.pool_shift++;
     g_detail = "worklist pool exhausted (" + std::to_string(hs[S_POOL]) + " chunks requested)";
     return ADDS_EOVERFLOW;
   }
   if (err == DERR_TIMEOUT || err == DERR_ABORT) {
     g_detail = err == DERR_ABORT ? "solve aborted by the host watchdog"
-                                 : "device watchdog ended the solve after " + std::to_string((long)ws.timeout_ms) +
+                                 : "device watchdog ended the solve after " + std::to_string((long)timeout_ms) +
                                        " ms (or a bounded wait expired)";
     return ADDS_ETIMEOUT;
   }
@@ -670,6 +661,25 @@ static adds_status collect(const adds_graph* g, Workspace& ws, adds_stats* stats
     return ADDS_EHIP;
   }
   return ADDS_OK;
+}
+
+static adds_status collect(const adds_graph* g, Workspace& ws, adds_stats* stats) {
+  adds_status rc = wait_solve(ws);
+  ws.pending = false;
+  if (rc != ADDS_OK) return rc;
+  const uint32_t* hs = ws.host_state;
+  if (stats) {
+    read_stats(hs, stats);
+    float ms = 0, kms = 0;
+    (void)hipEventElapsedTime(&ms, ws.ev0, ws.ev1);
+    if (ws.kernel_events) (void)hipEventElapsedTime(&kms, ws.evk0, ws.evk1);
+    stats->ms = ms;
+    stats->kernel_ms = kms;
+  }
+  if (ws.trace) dump_trace(g, ws);
+  rc = solve_outcome(hs, ws.timeout_ms);
+  if (rc == ADDS_EOVERFLOW && !ws.pool_fixed && ws.pool_shift < 6) ws.pool_shift++;
+  return rc;
 }
 
 // One synchronous-or-not solve of k sources into dist (device, [k][n] when replicated).
@@ -1377,6 +1387,130 @@ extern "C" adds_status adds_autotune_batched(adds_graph* const* graphs, const ui
   adds_status rc = get_union(graphs, srcs, k, &u, usrc);
   if (rc != ADDS_OK) return rc;
   return autotune_core(u, usrc.data(), k, false, trials, best, best_ms);
+}
+
+// ---- allocation-free single-source solve ----
+// adds_sssp_noalloc runs adds_sssp's kernels with their workspace -- state words, queue pool, chunk maps,
+// range records, the source word -- in `scratch`, a file-scope device array of a fixed size that every
+// GPU the library runs on holds until the process ends.  In host code `scratch` names a host-side stand-in,
+// not device memory: its device address is taken once per device with hipGetSymbolAddress (with that
+// device current) and only that address is used afterwards.  Every call on a device shares its array, so
+// calls that overlap on one device race; each call leaves the pool and chunk maps empty again (the cleanup
+// kernel) and clears the state words at its start (the init kernel).
+constexpr uint64_t NA_POOL_ENTRIES = 8ull << 20;                 // 64 MiB of queue entries
+constexpr uint32_t NA_CHUNKS = (uint32_t)(NA_POOL_ENTRIES / CH);  // chunk-map entries per queue
+constexpr uint32_t NA_MAX_QUEUES = 512;                           // nbuckets * parts
+constexpr uint64_t NA_STATE_WORDS = 1ull << 19;                   // 2 MiB
+constexpr uint32_t NA_HREC = NA_CHUNKS * CH / 16;                 // range records (one per 16 entries)
+constexpr uint64_t NA_OFF_POOL = 0;
+constexpr uint64_t NA_OFF_TAB = NA_OFF_POOL + NA_POOL_ENTRIES * 8;
+constexpr uint64_t NA_OFF_HREC = NA_OFF_TAB + (uint64_t)2 * NA_MAX_QUEUES * NA_CHUNKS * 4;
+constexpr uint64_t NA_OFF_STATE = NA_OFF_HREC + (uint64_t)NA_HREC * HREC_WORDS * 4;
+constexpr uint64_t NA_OFF_SRC = NA_OFF_STATE + NA_STATE_WORDS * 4;
+constexpr uint64_t NA_BYTES = NA_OFF_SRC + 256;  // 98 MiB + 256 bytes
+static_assert(NA_OFF_STATE % 256 == 0 && NA_OFF_SRC % 256 == 0, "scratch regions are 256-byte aligned");
+
+static __device__ __align__(256) uint64_t scratch[NA_BYTES / 8];
+
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex g_na_mu;
+static uint8_t* g_na_ptr[NA_MAX_DEVICES];  // device address of `scratch` per device, once set up
+static bool g_na_clean[NA_MAX_DEVICES];    // the pool and chunk maps are in their empty state
+
+// Called with `dev` current.  Takes the device address of `scratch` (first call on the device) and puts
+// the pool and chunk maps in their empty state when a previous call may have left them otherwise (first
+// call, or a call that failed before its cleanup kernel was enqueued).  Fill kernels, no memset nodes.
+static adds_status noalloc_setup(int dev, hipStream_t s, uint8_t** base) {
+  if (dev < 0 || dev >= NA_MAX_DEVICES) {
+    g_detail = "allocation-free solve: device index out of range";
+    return ADDS_EINVAL;
+  }
+  if (!g_na_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      g_detail = std::string("allocation-free solve: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return ADDS_EHIP;
+    }
+    g_na_ptr[dev] = (uint8_t*)ptr;
+    g_na_clean[dev] = false;
+  }
+  uint8_t* b = g_na_ptr[dev];
+  if (!g_na_clean[dev]) {
+    HIPCHK(launch_fill_u64((uint64_t*)(b + NA_OFF_POOL), NA_POOL_ENTRIES, EMPTY64, s));
+    HIPCHK(launch_fill_u32((uint32_t*)(b + NA_OFF_TAB), (uint64_t)2 * NA_MAX_QUEUES * NA_CHUNKS, TAB_UNSET, s));
+    g_na_clean[dev] = true;
+  }
+  *base = b;
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_sssp_noalloc(adds_graph* g, uint32_t src, void* dist, const adds_params* p,
+                                         adds_stats* stats, void* stream) {
+  if (!g || !dist) return ADDS_EINVAL;
+  if (p && (p->flags & (ADDS_F_HOST_OUT | ADDS_F_NO_SYNC))) {
+    g_detail = "allocation-free solve: dist must be device memory and the call is synchronous";
+    return ADDS_EINVAL;
+  }
+  if (src >= g->n) {
+    g_detail = "source out of range";
+    return ADDS_EINVAL;
+  }
+  HIPCHK(hipSetDevice(g->device));
+  Resolved r;
+  adds_status rc = resolve(g, p, g->n, 0, r);
+  if (rc != ADDS_OK) return rc;
+  // the sizes the scratch holds: a worklist of 4 entries per vertex + 1 per 16 edges (the default rule
+  // without its slack, or the caller's pool_entries) and at most NA_MAX_QUEUES queues per phase
+  const uint64_t need = p && p->pool_entries ? p->pool_entries : 4ull * g->n + g->m / 16;
+  const uint32_t nq = r.nb * r.parts;
+  if (need > NA_POOL_ENTRIES || nq > NA_MAX_QUEUES || state_words(nq, r.nb) > NA_STATE_WORDS) {
+    g_detail = "graph or parameters too large for the allocation-free solve: it needs " + std::to_string(need) +
+               " worklist entries and " + std::to_string(nq) + " queues per phase; the scratch holds " +
+               std::to_string(NA_POOL_ENTRIES) + " and " + std::to_string(NA_MAX_QUEUES) + " (use adds_sssp)";
+    return ADDS_EINVAL;
+  }
+  r.pool_entries = NA_POOL_ENTRIES;
+  hipStream_t s = (hipStream_t)stream;
+  int dev = g->device;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  uint8_t* b = nullptr;
+  rc = noalloc_setup(dev, s, &b);
+  if (rc != ADDS_OK) return rc;
+  Workspace ws;  // views into the scratch; nothing here is allocated
+  ws.st = (uint32_t*)(b + NA_OFF_STATE);
+  ws.pool = (uint64_t*)(b + NA_OFF_POOL);
+  ws.tab = (uint32_t*)(b + NA_OFF_TAB);
+  ws.hrec = (uint32_t*)(b + NA_OFF_HREC);
+  ws.hrec_cap = NA_HREC;
+  ws.nq = nq;
+  ws.nb = r.nb;
+  ws.pool_chunks = NA_CHUNKS;
+  ws.maxv = NA_CHUNKS;
+  ws.timeout_ms = r.timeout_ms;
+  uint32_t* srcw = (uint32_t*)(b + NA_OFF_SRC);
+  const auto t0 = std::chrono::steady_clock::now();
+  g_na_clean[dev] = false;  // until the cleanup kernel is enqueued
+  rc = ensure_split(g, r.light_delta, s);  // kernels only: the graph owns its row buffers
+  if (rc == ADDS_OK) {
+    const hipError_t e = launch_fill_u32(srcw, 1, src, s);
+    if (e != hipSuccess) {
+      g_detail = std::string("allocation-free solve: ") + hipGetErrorString(e);
+      rc = ADDS_EHIP;
+    }
+  }
+  if (rc == ADDS_OK) rc = enqueue_solve(g, ws, r, (uint32_t*)dist, g->n, srcw, 1, false, s, false);
+  if (rc != ADDS_OK) return rc;
+  g_na_clean[dev] = true;
+  uint32_t hs[8 * LINE];
+  HIPCHK(hipMemcpyAsync(hs, ws.st, sizeof(hs), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    read_stats(hs, stats);
+    stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return solve_outcome(hs, r.timeout_ms);
 }
 
 // ---- cross-XCD ordering litmus (diagnostic) ----

@@ -143,6 +143,19 @@ adds_status adds_graph_device_arrays(const adds_graph* g, const uint32_t** row_p
 adds_status adds_sssp(adds_graph* g, uint32_t src, void* dist, const adds_params* p, adds_stats* stats,
                       void* stream);
 
+/* adds_sssp that allocates nothing: no reference counterpart; it is for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid allocation).  The solver workspace lives in a
+ * fixed 98 MiB device array of the library, which each GPU the library runs on holds until the process
+ * ends; the graph handle already owns its row buffers.  Limits: the graph's worklist estimate
+ * 4 n + m / 16 (or p->pool_entries) must be at most 8,388,608 entries (about 2M vertices) and
+ * nbuckets * parts at most 512; larger inputs return ADDS_EINVAL with the reason in
+ * adds_last_error_detail().  dist must be a device buffer of n words; the call is synchronous
+ * (ADDS_F_HOST_OUT and ADDS_F_NO_SYNC return ADDS_EINVAL) and cannot grow its pool (a solve that needs
+ * more returns ADDS_EOVERFLOW).  Calls share the device's array: calls that overlap on one device race
+ * (calls from threads of one process are serialised).  stats->ms is host wall time, kernel_ms is 0. */
+adds_status adds_sssp_noalloc(adds_graph* g, uint32_t src, void* dist, const adds_params* p, adds_stats* stats,
+                              void* stream);
+
 /* Bulk-synchronous near-far SSSP, the GPU baseline the reference compares ADDS with (its nf_int /
  * nf_float runs: nf_int.patch, nf_float.patch over Galois' GPU sssp).  Rounds relax a near worklist of
  * vertices below a threshold; vertices improved past it wait in a far pile until the threshold moves up
