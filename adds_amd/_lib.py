@@ -166,7 +166,7 @@ def _lib():
         "adds_solve_graph_launch": (i32, [vp, vp]),
         "adds_solve_graph_status": (i32, [vp, sp]),
         "adds_solve_graph_destroy": (None, [vp]),
-        "adds_sssp_backward_f32": (i32, [vp, u32, vp, vp, vp, ctypes.POINTER(ctypes.c_double), vp]),
+        "adds_sssp_backward_f32": (i32, [vp, c_u32p, u32, vp, vp, vp, ctypes.POINTER(ctypes.c_double), vp]),
         "adds_autotune": (i32, [vp, c_u32p, u32, u32, pp, ctypes.POINTER(ctypes.c_double)]),
         "adds_autotune_batched": (i32, [vp, c_u32p, u32, u32, pp, ctypes.POINTER(ctypes.c_double)]),
         "adds_litmus": (i32, [u32, u32, u32, ctypes.POINTER(u64), ctypes.POINTER(u64)]),

@@ -6,9 +6,11 @@ torch's current stream; torch only provides device memory and streams.
     dist = sssp(w, graph, srcs)      # w: float32 CUDA tensor [m] (requires_grad), dist: [k, n]
     loss = (c * dist).sum(); loss.backward()   # w.grad = dL/dw through each source's tree
 
-The tree of a source is its tight-predecessor forest: the parent edge of v is the smallest edge id
-(u -> v) with fl(d[u] + w) == d[v] and d[u] < d[v].  Unreached vertices (dist = +inf) receive no
-gradient.  The gradient is deterministic: every floating-point sum runs in a fixed order.
+The tree of a source is rooted at the source: the parent edge of v is the smallest edge id (u -> v) with
+d[u] < d[v] and fl(d[u] + w) == d[v]; vertices reached only through zero weights take a tight edge from an
+equal-distance vertex resolved earlier (adds_sssp_backward_f32).  Unreached vertices (dist = +inf)
+receive and pass no gradient.  The gradient is the exact sum of the fixed-point-scaled inputs rounded once
+to float32: bit-identical run to run, whatever the order of the device's additions.
 """
 import numpy as np
 import torch
@@ -34,6 +36,7 @@ class SSSPFunction(torch.autograd.Function):
         dist = torch.empty((k, graph.n), dtype=torch.float32, device=w.device)
         graph.sssp_multi_device(srcs, dist.data_ptr(), stream=_stream())
         ctx.graph = graph
+        ctx.srcs = srcs
         ctx.save_for_backward(w, dist)
         ctx.mark_non_differentiable()
         return dist
@@ -46,7 +49,7 @@ class SSSPFunction(torch.autograd.Function):
         # the handle's weights may have been replaced by a later forward: restore this call's weights
         graph.set_weights_device(w.data_ptr(), stream=_stream())
         gw = torch.empty(graph.m, dtype=torch.float32, device=w.device)
-        graph.backward_device(dist.shape[0], dist.data_ptr(), g.data_ptr(), gw.data_ptr(), stream=_stream())
+        graph.backward_device(ctx.srcs, dist.data_ptr(), g.data_ptr(), gw.data_ptr(), stream=_stream())
         return gw, None, None
 
 

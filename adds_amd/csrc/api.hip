@@ -87,6 +87,11 @@ struct adds_graph {
   uint32_t* rrow = nullptr;
   uint32_t* reid = nullptr;
   uint32_t* rsrc = nullptr;
+  float* rw = nullptr;          // weights in transposed order, valid for weight generation rw_gen
+  uint64_t rw_gen = 0;
+  uint8_t* bwd = nullptr;       // backward workspace (per vertex-instance arrays, gradient sums, counters)
+  uint64_t bwd_bytes = 0;
+  unsigned long long* bwd_host = nullptr;  // pinned copy of the counters
   // batched union cache: the members' (handle, generation) pairs it was built from
   std::vector<std::pair<const adds_graph*, uint64_t>> union_key;
   adds_graph* union_g = nullptr;
@@ -169,6 +174,13 @@ static void release_graph_memory(adds_graph* g) {
   if (g->rrow) (void)hipFree(g->rrow);
   if (g->reid) (void)hipFree(g->reid);
   if (g->rsrc) (void)hipFree(g->rsrc);
+  if (g->rw) (void)hipFree(g->rw);
+  if (g->bwd) (void)hipFree(g->bwd);
+  if (g->bwd_host) (void)hipHostFree(g->bwd_host);
+  g->rw = nullptr;
+  g->bwd = nullptr;
+  g->bwd_bytes = 0;
+  g->bwd_host = nullptr;
   if (g->scw) (void)hipFree(g->scw);
   if (g->lcnt) (void)hipFree(g->lcnt);
   if (g->rinfo) (void)hipFree(g->rinfo);
@@ -1070,51 +1082,156 @@ static adds_status ensure_transpose(adds_graph* g) {
   return ADDS_OK;
 }
 
-extern "C" adds_status adds_sssp_backward_f32(adds_graph* g, uint32_t k, const float* dist, const float* grad_dist,
-                                              float* grad_w, double* ms, void* stream) {
-  if (!g || !k || !dist || !grad_dist || !grad_w) return ADDS_EINVAL;
+// Backward workspace for k sources: P and C/Acc (8 bytes per vertex-instance each), cnt (4), the list
+// of nonzero g entries (4 per 32 vertex-instances), GW (8 per edge), counters and sources.
+struct BwdLayout {
+  uint64_t p, c, cnt, list, segup, attach, part, gw, ctr, srcs, bytes;
+  uint32_t list_cap;
+};
+static BwdLayout bwd_layout(uint32_t n, uint64_t m, uint32_t k) {
+  auto up = [](uint64_t x) { return (x + 255) & ~255ull; };
+  const uint64_t kn = (uint64_t)k * n;
+  BwdLayout L;
+  L.list_cap = (uint32_t)(kn / 32 + 1);
+  L.p = 0;
+  L.c = up(L.p + 8 * kn);
+  L.cnt = up(L.c + 8 * kn);
+  L.list = up(L.cnt + 4 * kn);
+  L.segup = up(L.list + 4 * (uint64_t)L.list_cap);
+  L.attach = up(L.segup + 4 * (uint64_t)L.list_cap);
+  L.part = up(L.attach + 4 * (uint64_t)L.list_cap);
+  L.gw = up(L.part + 32 * (uint64_t)BWD_MAX_PARENT_BLOCKS);
+  L.ctr = up(L.gw + 8 * std::max<uint64_t>(m, 1));
+  L.srcs = up(L.ctr + 64);
+  L.bytes = up(L.srcs + 4 * (uint64_t)k);
+  return L;
+}
+
+// Fixed-point scale of a batch: M = largest |g| = f 2^eM (0.5 <= f < 1), 2^eN >= nonzero count:
+// S = 62 - eM - eN keeps every sum below 2^62.
+static int grad_scale(uint32_t maxbits, uint64_t nnz) {
+  float mx;
+  memcpy(&mx, &maxbits, 4);
+  int em = 0, en = 0;
+  (void)frexpf(mx, &em);
+  while ((1ull << en) < nnz) en++;
+  return 62 - em - en;
+}
+
+extern "C" adds_status adds_sssp_backward_f32(adds_graph* g, const uint32_t* srcs, uint32_t k, const float* dist,
+                                              const float* grad_dist, float* grad_w, double* ms, void* stream) {
+  if (!g || !srcs || !k || !dist || !grad_dist || !grad_w) return ADDS_EINVAL;
   if (g->wt != ADDS_W_F32) {
     g_detail = "backward needs a float graph";
     return ADDS_EINVAL;
   }
   if ((uint64_t)k * g->n >= 0xFFFFFFFFull) return ADDS_EINVAL;
+  for (uint32_t i = 0; i < k; i++)
+    if (srcs[i] >= g->n) {
+      g_detail = "source out of range";
+      return ADDS_EINVAL;
+    }
   HIPCHK(hipSetDevice(g->device));
+  hipStream_t s = (hipStream_t)stream;
   adds_status rc = ensure_transpose(g);
   if (rc != ADDS_OK) return rc;
+  if (!g->rw) HIPCHK(hipMalloc(&g->rw, std::max<uint64_t>(g->m, 1) * 4));
+  if (g->rw_gen != g->gen) {
+    HIPCHK(launch_bwd_tweights(g->reid, (const float*)g->w, g->rw, g->m, s));
+    g->rw_gen = g->gen;
+  }
+  const BwdLayout L = bwd_layout(g->n, g->m, k);
+  if (g->bwd_bytes < L.bytes) {
+    HIPCHK(hipStreamSynchronize(s));
+    if (g->bwd) (void)hipFree(g->bwd);
+    g->bwd = nullptr;
+    g->bwd_bytes = 0;
+    HIPCHK(hipMalloc(&g->bwd, L.bytes));
+    g->bwd_bytes = L.bytes;
+  }
+  if (!g->bwd_host) HIPCHK(hipHostMalloc((void**)&g->bwd_host, 64, hipHostMallocDefault));
   const uint64_t kn = (uint64_t)k * g->n;
-  rc = ensure_outbuf(g, 5 * kn);
-  if (rc != ADDS_OK) return rc;
-  hipStream_t s = (hipStream_t)stream;
   BwdArgs b;
-  b.row_ptr = g->row;
-  b.col = g->col;
-  b.w = (const float*)g->w;
+  memset(&b, 0, sizeof(b));
   b.rrow = g->rrow;
   b.reid = g->reid;
   b.rsrc = g->rsrc;
+  b.rw = g->rw;
   b.dist = dist;
   b.gdist = grad_dist;
-  b.pe = g->outbuf;
-  b.cnt = g->outbuf + kn;
-  b.adj = (float*)(g->outbuf + 2 * kn);  // then the parent vertices at 3 * kn (launch_backward)
-  b.cmask = g->outbuf + 4 * kn;
+  b.srcs = (const uint32_t*)(g->bwd + L.srcs);
+  b.P = (uint2*)(g->bwd + L.p);
+  b.C = (uint2*)(g->bwd + L.c);
+  b.Acc = (long long*)(g->bwd + L.c);  // the dense form runs after the rounds: C is free by then
+  b.cnt = (uint32_t*)(g->bwd + L.cnt);
+  b.list = (uint32_t*)(g->bwd + L.list);
+  b.list_cap = L.list_cap;
+  b.owner = b.cnt;  // the sparse form's owner words use the dense form's counters
+  b.seg_up = (uint32_t*)(g->bwd + L.segup);
+  b.attach = (uint32_t*)(g->bwd + L.attach);
+  b.part = (unsigned long long*)(g->bwd + L.part);
+  b.GW = (unsigned long long*)(g->bwd + L.gw);
   b.gw = grad_w;
+  b.ctr = (unsigned long long*)(g->bwd + L.ctr);
   b.n = g->n;
   b.k = k;
   b.m = g->m;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  auto counters = [&]() -> adds_status {
+    HIPCHK(hipMemcpyAsync(g->bwd_host, b.ctr, 48, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ADDS_OK;
+  };
+  auto finish = [&](adds_status r) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return r;
+  };
+  HIPCHK(hipMemcpyAsync((void*)b.srcs, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
   HIPCHK(hipEventRecord(e0, s));
-  HIPCHK(launch_backward(b, s));
+  HIPCHK(launch_fill_u64((uint64_t*)b.ctr, 8, 0, s));
+  HIPCHK(launch_fill_u64((uint64_t*)b.GW, g->m, 0, s));
+  HIPCHK(launch_bwd_parent(b, s));
+  rc = counters();
+  if (rc != ADDS_OK) return finish(rc);
+  if (g->bwd_host[3]) {
+    g_detail = "grad_dist holds a non-finite value at a reached vertex";
+    return finish(ADDS_EINVAL);
+  }
+  // equal-distance rounds, while vertices are left without a parent
+  for (uint64_t unresolved = g->bwd_host[0]; unresolved;) {
+    HIPCHK(launch_fill_u64((uint64_t*)b.ctr + 5, 1, 0, s));
+    HIPCHK(launch_bwd_round(b, s));
+    rc = counters();
+    if (rc != ADDS_OK) return finish(rc);
+    if (g->bwd_host[5] == 0 || g->bwd_host[5] > unresolved) {
+      g_detail = "dist is not a shortest-path solution from srcs (a reached vertex has no tight path)";
+      return finish(ADDS_EINVAL);
+    }
+    unresolved -= g->bwd_host[5];
+  }
+  const uint64_t nnz = g->bwd_host[1], listed = g->bwd_host[4];
+  b.S = nnz ? grad_scale((uint32_t)g->bwd_host[2], nnz) : 0;
+  if (nnz) {
+    if (listed <= b.list_cap) {
+      HIPCHK(launch_fill_u32(b.owner, kn, 0xFFFFFFFFu, s));
+      HIPCHK(launch_fill_u64((uint64_t*)b.Acc, kn, 0, s));
+      HIPCHK(launch_bwd_sparse(b, (uint32_t)listed, s));
+    } else {
+      HIPCHK(launch_fill_u64((uint64_t*)b.Acc, kn, 0, s));
+      HIPCHK(launch_fill_u32(b.cnt, kn, 0, s));
+      HIPCHK(launch_bwd_climb(b, s));
+    }
+  }
+  HIPCHK(launch_bwd_convert(b, s));
   HIPCHK(hipEventRecord(e1, s));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0;
   (void)hipEventElapsedTime(&t, e0, e1);
   if (ms) *ms = t;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return ADDS_OK;
+  return finish(ADDS_OK);
 }
 
 // ---- autotuner ----

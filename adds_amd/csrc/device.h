@@ -140,23 +140,41 @@ hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_
 
 // Backward (backward_kernels.hip)
 struct BwdArgs {
-  const uint32_t* row_ptr;
-  const uint32_t* col;
-  const float* w;
   const uint32_t* rrow;  // transpose: in-edges of v at rrow[v]..rrow[v+1]
-  const uint32_t* reid;  // transpose: forward edge id of each in-edge (sorted ascending per row)
+  const uint32_t* reid;  // transpose: forward edge id of each in-edge (ascending per row)
   const uint32_t* rsrc;  // transpose: tail vertex of each in-edge
+  const float* rw;       // transpose: weight of each in-edge
   const float* dist;     // [k][n]
   const float* gdist;    // [k][n]
-  uint32_t* pe;          // [k][n] parent edge (or ~0)
-  uint32_t* cnt;         // [k][n] pending children
-  uint32_t* cmask;       // [k][n] which out-edges (by position in the row) are tree edges
-  float* adj;            // [k][n] adjoint
-  float* gw;             // [m]
+  const uint32_t* srcs;  // [k] (device)
+  uint2* P;              // [k][n] (parent edge, parent vertex); edge 0xFFFFFFFF = unreached/unresolved,
+                         // 0xFFFFFFFE = root
+  uint2* C;              // [k][n] choices of an equal-distance round
+  long long* Acc;        // [k][n] children's sums (dense form) or sums attached at a vertex (sparse
+                         // form); shares C's memory
+  uint32_t* cnt;         // [k][n] pending children (dense form)
+  uint32_t* list;        // nonzero g entries (sparse form), list_cap of them
+  uint32_t list_cap;
+  uint32_t* owner;       // [k][n] sparse form: the entry whose segment holds the vertex (shares cnt's memory)
+  uint32_t* seg_up;      // [list_cap] sparse form: the segment an entry's segment attaches to (or NONE)
+  uint32_t* attach;      // [list_cap] sparse form: the vertex where it attaches
+  unsigned long long* part;  // [4 * parent blocks] per-block counts of k_bwd_parent
+  unsigned long long* GW;   // [m] exact fixed-point gradient
+  float* gw;                // [m] output
+  unsigned long long* ctr;  // [0] unresolved, [1] nonzero g, [2] largest |g| bits, [3] non-finite g,
+                            // [4] listed entries, [5] resolved by a round
   uint32_t n;
   uint32_t k;
   uint64_t m;
+  int S;  // fixed-point scale: G = rint(g * 2^S)
 };
-hipError_t launch_backward(const BwdArgs& b, hipStream_t s);
+hipError_t launch_bwd_tweights(const uint32_t* reid, const float* w, float* rw, uint64_t m, hipStream_t s);
+hipError_t launch_bwd_parent(const BwdArgs& b, hipStream_t s);
+hipError_t launch_bwd_round(const BwdArgs& b, hipStream_t s);
+hipError_t launch_bwd_sparse(const BwdArgs& b, uint32_t count, hipStream_t s);
+constexpr uint32_t BWD_MAX_PARENT_BLOCKS = 16384;
+uint32_t bwd_parent_blocks(uint32_t n);
+hipError_t launch_bwd_climb(const BwdArgs& b, hipStream_t s);
+hipError_t launch_bwd_convert(const BwdArgs& b, hipStream_t s);
 
 }  // namespace adds

@@ -251,9 +251,13 @@ class Graph:
         L.check(L._lib().adds_graph_set_weights_device(self._h, ctypes.c_void_p(w_ptr), _stream_ptr(stream)),
                 "set_weights")
 
-    def backward_device(self, k, dist_ptr, gdist_ptr, gw_ptr, stream=None):
+    def backward_device(self, srcs, dist_ptr, gdist_ptr, gw_ptr, stream=None):
+        """dL/dw into gw_ptr ([m] float32) from dist [k][n] (solved from srcs) and dL/d(dist) [k][n], all
+        device pointers; returns the pass's time in ms."""
+        srcs = np.ascontiguousarray(np.atleast_1d(srcs), dtype=np.uint32)
         ms = ctypes.c_double()
-        L.check(L._lib().adds_sssp_backward_f32(self._h, k, ctypes.c_void_p(dist_ptr), ctypes.c_void_p(gdist_ptr),
+        L.check(L._lib().adds_sssp_backward_f32(self._h, srcs.ctypes.data_as(L.c_u32p), len(srcs),
+                                                ctypes.c_void_p(dist_ptr), ctypes.c_void_p(gdist_ptr),
                                                 ctypes.c_void_p(gw_ptr), ctypes.byref(ms), _stream_ptr(stream)),
                 "backward")
         return ms.value

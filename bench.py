@@ -273,12 +273,12 @@ def run_c5(steps, torch, adds_amd, oracle):
         dg.sssp_multi_device(srcs, dist.data_ptr(), **params)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        bms = dg.backward_device(k, dist.data_ptr(), gdist.data_ptr(), gw.data_ptr())
+        bms = dg.backward_device(srcs, dist.data_ptr(), gdist.data_ptr(), gw.data_ptr())
         torch.cuda.synchronize()
         if i:
             fwd.append((t1 - t0) * 1e3)
             bwd.append(bms)
-    ref_gw, _ = oracle.tree_grad(host.row_ptr, host.col, host.w, dist.cpu().numpy(), gdist.cpu().numpy())
+    ref_gw, _, _ = oracle.tree_grad(host.row_ptr, host.col, host.w, srcs, dist.cpu().numpy(), gdist.cpu().numpy())
     res = {"n": host.n, "m": host.m, "sources": k, "params": params, "forward_ms": float(np.median(fwd)),
            "backward_ms": float(np.median(bwd)),
            "grad_bit_exact_vs_cpu_tree": bool(np.array_equal(ref_gw.view(np.uint32),

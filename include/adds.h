@@ -228,12 +228,21 @@ adds_status adds_autotune_batched(adds_graph* const* graphs, const uint32_t* src
 adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint64_t* forbidden, uint64_t* lost);
 
 /* ---- backward (float graphs) ----
- * Given forward distances dist [k][n] from srcs and dL/d(dist) grad_dist [k][n] (device), writes
- * dL/dw into grad_w [m] (device).  The tree is the tight-predecessor forest of each source: parent edge
- * of v = the smallest edge id e=(u->v) with fl(d[u]+w_e) == d[v] and d[u] < d[v].  No reference
- * counterpart. */
-adds_status adds_sssp_backward_f32(adds_graph* g, uint32_t k, const float* dist, const float* grad_dist,
-                                   float* grad_w, double* ms, void* stream);
+ * No reference counterpart (the reference keeps no predecessor tree: ads_float/kernel.cu:82).  Given
+ * forward distances dist [k][n] (device) from the k sources srcs (host array) and dL/d(dist) grad_dist
+ * [k][n] (device), writes dL/dw into grad_w [m] (device).
+ * Tree of each source, over its reached vertices: the source is the root; a vertex's parent edge is the
+ * smallest edge id e = (u -> v) with d[u] < d[v] and fl(d[u] + w_e) == d[v]; a vertex with no such edge
+ * (reached only through zero weights or adds that round away) takes, in rounds, the smallest tight edge
+ * from an equal-distance vertex whose parent was chosen in an earlier round.
+ * Gradient: with S = 62 - eM - eN (largest reached |g| = f 2^eM, 0.5 <= f < 1; 2^eN >= number of nonzero
+ * reached g), dL/dw_e = ldexpf((float)GW_e, -S) where GW_e = sum of rint(g_s[t] 2^S) over the (s, t)
+ * whose tree path uses e, summed exactly in 64-bit integers: the exact sum of the (scaled) inputs rounded
+ * once, independent of any order -- bit-identical run to run and to the CPU reference.  ms: hipEvent
+ * time of the pass.  ADDS_EINVAL when a reached grad_dist value is not finite or dist is not the
+ * shortest-path solution from srcs. */
+adds_status adds_sssp_backward_f32(adds_graph* g, const uint32_t* srcs, uint32_t k, const float* dist,
+                                   const float* grad_dist, float* grad_w, double* ms, void* stream);
 
 /* ---- host CSR, .gr files and distance text (reference csr_graph.cu / support.cu) ---- */
 typedef struct {

@@ -25,8 +25,10 @@ def _lib():
             fn = getattr(L, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_int, u32, p, p, p, p, u32, p]
-        L.oracle_tree_grad.restype = None
-        L.oracle_tree_grad.argtypes = [u32, p, p, p, u32, p, p, p, p]
+        L.oracle_tree_grad.restype = ctypes.c_int
+        L.oracle_tree_grad.argtypes = [u32, p, p, p, u32, p, p, p, p, p, p]
+        L.oracle_grad_scale.restype = ctypes.c_int
+        L.oracle_grad_scale.argtypes = [u32, u32, p, p, p]
         L.oracle_delta_step.restype = ctypes.c_int
         L.oracle_delta_step.argtypes = [ctypes.c_int, u32, p, p, p, u32, ctypes.c_double, ctypes.c_uint, p, p]
         _L = L
@@ -90,15 +92,34 @@ def certify(row_ptr, col, w, srcs, dist, is_float):
                                      len(srcs), _p(d)))
 
 
-def tree_grad(row_ptr, col, w, dist, gdist):
-    """dL/dw (float32[m]) and parent edges ([k][n]) for forward distances dist [k][n]."""
+def tree_grad(row_ptr, col, w, srcs, dist, gdist):
+    """dL/dw (float32[m]), the parent edges ([k][n], 0xFFFFFFFF for roots and unreached vertices) and the
+    fixed-point scale S, for forward distances dist [k][n] from srcs and dL/d(dist) gdist [k][n]
+    (oracle.c: tree rule with equal-distance rounds, exact 64-bit sums of rint(g * 2^S), one rounding).
+    Raises ValueError when dist is not a shortest-path solution from srcs or a reached g is not finite."""
     row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
     col = np.ascontiguousarray(col, dtype=np.uint32)
     w = np.ascontiguousarray(w, dtype=np.float32)
     dist = np.ascontiguousarray(np.atleast_2d(dist), dtype=np.float32)
     gdist = np.ascontiguousarray(np.atleast_2d(gdist), dtype=np.float32)
     k, n = dist.shape
+    srcs = np.ascontiguousarray(np.atleast_1d(srcs), dtype=np.uint32)
+    assert len(srcs) == k
     gw = np.empty(len(col), dtype=np.float32)
     pe = np.empty((k, n), dtype=np.uint32)
-    _lib().oracle_tree_grad(n, _p(row_ptr), _p(col), _p(w), k, _p(dist), _p(gdist), _p(gw), _p(pe))
-    return gw, pe
+    scale = ctypes.c_int()
+    rc = _lib().oracle_tree_grad(n, _p(row_ptr), _p(col), _p(w), k, _p(srcs), _p(dist), _p(gdist), _p(gw), _p(pe),
+                                 ctypes.byref(scale))
+    if rc != 0:
+        raise ValueError("dist is not a shortest-path solution from srcs, or a reached gradient is not finite")
+    return gw, pe, scale.value
+
+
+def grad_scale(dist, gdist):
+    """The fixed-point scale S of a batch and its number of nonzero reached gradients."""
+    dist = np.ascontiguousarray(np.atleast_2d(dist), dtype=np.float32)
+    gdist = np.ascontiguousarray(np.atleast_2d(gdist), dtype=np.float32)
+    k, n = dist.shape
+    nnz = ctypes.c_uint64()
+    s = _lib().oracle_grad_scale(n, k, _p(dist), _p(gdist), ctypes.byref(nnz))
+    return s, nnz.value

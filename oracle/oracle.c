@@ -19,6 +19,7 @@
  * Non-negative floats order like their bit patterns, so one heap serves both.
  * Compile with -ffp-contract=off (see Makefile): every float add is one IEEE binary32 add.
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -162,24 +163,56 @@ int oracle_certify(int is_float, uint32_t n, const uint32_t* row, const uint32_t
 }
 
 /* ---- backward reference ----
- * For each source s (dist_s = dist + s*n, g_s = gdist + s*n):
- *   pe(v) = smallest edge id e = (u -> v) with fl(d[u] + w_e) == d[v] and d[u] < d[v] (none: root);
- *   adj[v] = g[v] + sum over out-edges e = (v -> c) in ascending e with pe(c) == e of adj[c],
- *            evaluated left to right in float, vertices in descending d (children before parents);
- *   gw[pe(v)] += adj[v], sources in ascending s, gw starting at +0.0.
- * This is the definition the device backward must reproduce bit for bit. */
-static const float* g_sort_d;
-static int cmp_desc(const void* a, const void* b) {
-  uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
-  float dx = g_sort_d[x], dy = g_sort_d[y];
-  if (dx > dy) return -1;
-  if (dx < dy) return 1;
-  return (x < y) ? -1 : (x > y);
+ * The reference has no backward pass (ads_float/kernel.cu:82 stores the vertex's own id as "parent"), so
+ * this defines the one the device must reproduce bit for bit.  For each source s (dist_s = dist + s*n,
+ * g_s = gdist + s*n), over the reached vertices (finite distance):
+ *   tree: the source is the root.  Round 0: pe(v) = the smallest edge id e = (u -> v) with d[u] < d[v] and
+ *     fl(d[u] + w_e) == d[v].  Round r >= 1: a vertex still without a parent takes the smallest e = (u -> v)
+ *     with d[u] == d[v] and fl(d[u] + w_e) == d[v] whose tail was resolved (root, or parent chosen) in an
+ *     earlier round -- zero-weight edges and adds that round away are tree edges too, and the rounds keep
+ *     the parent pointers acyclic.  Rounds end when one resolves nothing.
+ *   fixed point: M = max |g| and N = number of nonzero g over the reached vertex-instances of the batch;
+ *     with M = m 2^eM (0.5 <= m < 1, frexpf) and 2^eN the least power of two >= N, the scale is
+ *     S = 62 - eM - eN, so that N * M * 2^S < 2^62.  G = rint(g * 2^S) as a 64-bit integer (exact for g a
+ *     multiple of 2^-S).
+ *   gradient: GW[e] = sum over sources s and reached vertices t of G_s[t] * [e is on t's tree path to
+ *     the root], an exact 64-bit integer sum (no order dependence); gw[e] = ldexpf((float)GW[e], -S):
+ *     the exact sum of the scaled inputs, rounded once to float32 (round to nearest even).
+ * Here the sums are accumulated children before parents (descending distance, later rounds first within
+ * a distance), an order the device does not use. */
+typedef struct { float d; uint32_t round, v; } ordent;
+static int cmp_ord(const void* a, const void* b) {
+  const ordent* x = (const ordent*)a;
+  const ordent* y = (const ordent*)b;
+  if (x->d != y->d) return x->d > y->d ? -1 : 1;
+  if (x->round != y->round) return x->round > y->round ? -1 : 1;
+  return (x->v > y->v) - (x->v < y->v);
 }
 
-void oracle_tree_grad(uint32_t n, const uint32_t* row, const uint32_t* col, const float* w, uint32_t k,
-                      const float* dist, const float* gdist, float* gw, uint32_t* pe_out) {
+int oracle_grad_scale(uint32_t n, uint32_t k, const float* dist, const float* gdist, uint64_t* nnz_out) {
+  float mx = 0.0f;
+  uint64_t nnz = 0;
+  for (uint64_t i = 0; i < (uint64_t)k * n; i++) {
+    if (f_bits(dist[i]) == INF_F32) continue;
+    float a = gdist[i] < 0 ? -gdist[i] : gdist[i];
+    if (a != 0.0f) nnz++;
+    if (a > mx) mx = a;
+  }
+  if (nnz_out) *nnz_out = nnz;
+  if (nnz == 0) return 0;
+  int em = 0, en = 0;
+  (void)frexpf(mx, &em);
+  while (((uint64_t)1 << en) < nnz) en++;
+  return 62 - em - en;
+}
+
+/* Returns 0, or -1 when a reached vertex of some source is not resolved (dist is not a shortest-path
+ * solution from srcs) or a reached g is not finite. */
+int oracle_tree_grad(uint32_t n, const uint32_t* row, const uint32_t* col, const float* w, uint32_t k,
+                     const uint32_t* srcs, const float* dist, const float* gdist, float* gw, uint32_t* pe_out,
+                     int* scale_out) {
   const uint64_t m = row[n];
+  int rc = 0;
   uint32_t* rrow = (uint32_t*)calloc((size_t)n + 1, 4);
   uint32_t* reid = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
   uint32_t* rsrc = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
@@ -193,37 +226,74 @@ void oracle_tree_grad(uint32_t n, const uint32_t* row, const uint32_t* col, cons
       reid[p] = e;
       rsrc[p] = u;
     }
-  for (uint64_t e = 0; e < m; e++) gw[e] = 0.0f;
+  for (uint64_t i = 0; i < (uint64_t)k * n; i++)
+    if (f_bits(dist[i]) != INF_F32 && !(gdist[i] - gdist[i] == 0.0f)) rc = -1; /* inf or nan */
+  const int S = oracle_grad_scale(n, k, dist, gdist, NULL);
+  if (scale_out) *scale_out = S;
+  int64_t* GW = (int64_t*)calloc(m ? m : 1, sizeof(int64_t));
   uint32_t* pe = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n ? n : 1));
-  float* adj = (float*)malloc(sizeof(float) * ((size_t)n ? n : 1));
-  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n ? n : 1));
-  for (uint32_t s = 0; s < k; s++) {
+  uint32_t* rnd = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n ? n : 1));
+  int64_t* A = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n ? n : 1));
+  ordent* order = (ordent*)malloc(sizeof(ordent) * ((size_t)n ? n : 1));
+  const uint32_t UNRES = 0xFFFFFFFFu, ROOT = 0xFFFFFFFEu;
+  for (uint32_t s = 0; s < k && rc == 0; s++) {
     const float* d = dist + (uint64_t)s * n;
     const float* g = gdist + (uint64_t)s * n;
     for (uint32_t v = 0; v < n; v++) {
-      pe[v] = 0xFFFFFFFFu;
+      pe[v] = UNRES;
+      rnd[v] = UNRES;
       if (f_bits(d[v]) == INF_F32) continue;
+      if (v == srcs[s]) { pe[v] = ROOT; rnd[v] = 0; continue; }
       for (uint32_t p = rrow[v]; p < rrow[v + 1]; p++) {
         float du = d[rsrc[p]];
-        if (du < d[v] && du + w[reid[p]] == d[v]) { pe[v] = reid[p]; break; }
+        if (du < d[v] && du + w[reid[p]] == d[v]) { pe[v] = reid[p]; rnd[v] = 0; break; }
       }
     }
-    if (pe_out) memcpy(pe_out + (uint64_t)s * n, pe, sizeof(uint32_t) * n);
-    for (uint32_t v = 0; v < n; v++) order[v] = v;
-    g_sort_d = d;
-    qsort(order, n, sizeof(uint32_t), cmp_desc);
-    for (uint32_t i = 0; i < n; i++) {
-      uint32_t v = order[i];
-      if (f_bits(d[v]) == INF_F32) continue;
-      float a = g[v];
-      for (uint32_t e = row[v]; e < row[v + 1]; e++)
-        if (pe[col[e]] == e) a += adj[col[e]];
-      adj[v] = a;
+    for (uint32_t r = 1;; r++) {
+      uint32_t changed = 0;
+      for (uint32_t v = 0; v < n; v++) {
+        if (f_bits(d[v]) == INF_F32 || pe[v] != UNRES) continue;
+        for (uint32_t p = rrow[v]; p < rrow[v + 1]; p++) {
+          uint32_t u = rsrc[p];
+          if (d[u] == d[v] && d[u] + w[reid[p]] == d[v] && rnd[u] < r) { pe[v] = reid[p]; rnd[v] = r; changed++; break; }
+        }
+      }
+      if (!changed) break;
     }
-    for (uint32_t v = 0; v < n; v++)
-      if (pe[v] != 0xFFFFFFFFu) gw[pe[v]] += adj[v];
+    if (pe_out)
+      for (uint32_t v = 0; v < n; v++) pe_out[(uint64_t)s * n + v] = pe[v] >= ROOT ? 0xFFFFFFFFu : pe[v];
+    uint32_t cnt = 0;
+    for (uint32_t v = 0; v < n; v++) {
+      if (f_bits(d[v]) == INF_F32) continue;
+      if (pe[v] == UNRES) { rc = -1; break; }
+      A[v] = (int64_t)rint(ldexp((double)g[v], S));
+      order[cnt].d = d[v];
+      order[cnt].round = rnd[v];
+      order[cnt].v = v;
+      cnt++;
+    }
+    if (rc) break;
+    qsort(order, cnt, sizeof(ordent), cmp_ord);
+    /* children come before parents in `order`: when v is reached, A[v] holds its whole subtree */
+    for (uint32_t i = 0; i < cnt; i++) {
+      uint32_t v = order[i].v;
+      if (pe[v] == ROOT) continue;
+      GW[pe[v]] += A[v];
+      /* tail of the parent edge: the vertex whose row holds pe[v] (the last u with row[u] <= pe[v]
+         whose row is not empty) */
+      uint32_t lo = 0, hi = n;
+      while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (row[mid] <= pe[v]) lo = mid; else hi = mid;
+      }
+      A[lo] += A[v];
+    }
   }
-  free(rrow); free(reid); free(rsrc); free(cur); free(pe); free(adj); free(order);
+  for (uint64_t e = 0; e < m; e++) gw[e] = 0.0f;
+  if (rc == 0)
+    for (uint64_t e = 0; e < m; e++) gw[e] = ldexpf((float)GW[e], -S);
+  free(rrow); free(reid); free(rsrc); free(cur); free(pe); free(rnd); free(A); free(order); free(GW);
+  return rc;
 }
 
 /* ---- .gr v1 reader (csr_graph.cu:236-281) ----

@@ -177,45 +177,117 @@ def test_int_saturation_closed_form():
 
 
 # ---- backward reference ----
-def tree_paths_grad(row, col, w, dist, gdist):
-    """Independent float64 evaluation of dL/dw = sum_t c_t * [e on the tree path to t]."""
+def exact_tree_grad(row, col, w, srcs, dist, gdist):
+    """Independent evaluation for dyadic gradients (multiples of 2^-23): the tree by brute force over the
+    edge list (round 0: smallest tight in-edge from a strictly closer vertex; then rounds over
+    equal-distance tight in-edges from vertices resolved earlier), the gradient as exact Python-integer
+    sums of c * 2^23 over each edge's tree descendants, rounded once to float32."""
     k, n = dist.shape
     m = len(col)
-    src_of = np.repeat(np.arange(n), np.diff(row.astype(np.int64)))
-    gw = np.zeros(m)
+    tail = np.repeat(np.arange(n), np.diff(row.astype(np.int64)))
+    total = [0] * m
     for s in range(k):
         d = dist[s]
-        pe = np.full(n, -1)
+        pe = [None] * n
+        rnd = [None] * n
         for v in range(n):
             if not np.isfinite(d[v]):
                 continue
-            cands = [e for e in range(m) if col[e] == v and d[src_of[e]] < d[v]
-                     and np.float32(d[src_of[e]] + w[e]) == d[v]]
+            if v == srcs[s]:
+                rnd[v] = 0
+                continue
+            cands = [e for e in range(m) if col[e] == v and d[tail[e]] < d[v]
+                     and np.float32(d[tail[e]] + w[e]) == d[v]]
             if cands:
-                pe[v] = min(cands)
+                pe[v], rnd[v] = min(cands), 0
+        r = 1
+        while True:
+            new = {}
+            for v in range(n):
+                if not np.isfinite(d[v]) or rnd[v] is not None:
+                    continue
+                cands = [e for e in range(m) if col[e] == v and d[tail[e]] == d[v]
+                         and np.float32(d[tail[e]] + w[e]) == d[v] and rnd[tail[e]] is not None]
+                if cands:
+                    new[v] = min(cands)
+            if not new:
+                break
+            for v, e in new.items():
+                pe[v], rnd[v] = e, r
+            r += 1
         for t in range(n):
             if gdist[s, t] == 0 or not np.isfinite(d[t]):
                 continue
+            c = int(gdist[s, t] * (1 << 23))
+            assert c == gdist[s, t] * (1 << 23)
             x = t
-            while pe[x] >= 0:
-                gw[pe[x]] += gdist[s, t]
-                x = src_of[pe[x]]
-    return gw
+            while pe[x] is not None:
+                total[pe[x]] += c
+                x = tail[pe[x]]
+    return np.array([np.float32(t / float(1 << 23)) for t in total], dtype=np.float32)
 
 
-@pytest.mark.parametrize("seed", range(4))
-def test_tree_grad_matches_path_sums(seed):
+def dyadic(rng, shape, density=1.0):
+    g = (rng.integers(-(1 << 23), 1 << 23, shape) / float(1 << 23)).astype(np.float32)
+    return np.where(rng.random(shape) < density, g, 0).astype(np.float32)
+
+
+@pytest.mark.parametrize("seed,zero_ok", [(0, False), (1, False), (2, True), (3, True)])
+def test_tree_grad_matches_exact_path_sums(seed, zero_ok):
+    """Bit-exact against the independent evaluation, with zero weights allowed in half the cases (so
+    equal-distance tree edges occur); a wrong parent, a dropped edge or a wrong sign changes bits."""
     rng = np.random.default_rng(50 + seed)
-    row, col, w = rand_graph(rng, 40, 200, is_float=True)
+    row, col, w = rand_graph(rng, 40, 200, is_float=True, zero_ok=zero_ok)
+    if zero_ok:
+        w[rng.random(len(w)) < 0.3] = 0.0
     srcs = [0, 7, 13]
     dist = np.stack([oracle.dijkstra(row, col, w, s, True).view(np.float32) for s in srcs])
-    gd = (rng.integers(-(1 << 23), 1 << 23, dist.shape) / float(1 << 23)).astype(np.float32)
-    gw, pe = oracle.tree_grad(row, col, w, dist, gd)
-    ref = tree_paths_grad(row, col, w, dist, gd)
-    assert np.allclose(gw, ref, rtol=1e-5, atol=1e-6)
-    # a planted mistake in the oracle's output (one parent edge dropped) must be visible
-    used = np.nonzero(gw != 0)[0]
-    assert len(used) > 0
+    gd = dyadic(rng, dist.shape)
+    gw, pe, scale = oracle.tree_grad(row, col, w, srcs, dist, gd)
+    assert scale >= 23
+    ref = exact_tree_grad(row, col, w, srcs, dist, gd)
+    assert np.array_equal(gw.view(np.uint32), ref.view(np.uint32))
+    assert np.count_nonzero(gw) > 0
+    # planted mistakes must be visible: another sign or another coefficient
+    gw2, _, _ = oracle.tree_grad(row, col, w, srcs, dist, -gd)
+    assert np.array_equal(gw2, -gw)
+    gd3 = gd.copy()
+    t = int(np.nonzero(np.isfinite(dist[0]) & (np.arange(40) != srcs[0]))[0][0])
+    gd3[0, t] += np.float32(0.5)
+    assert not np.array_equal(oracle.tree_grad(row, col, w, srcs, dist, gd3)[0], gw)
+
+
+def test_tree_grad_zero_weight_chain_and_cycle():
+    """Zero-weight edges are tree edges: source 0 -0-> 1 -0-> 2 -1.5-> 3, plus a zero-weight back edge
+    2 -> 1 (a cycle at equal distance).  With dL/dd[3] = 1 and dL/dd[2] = 0.25, d[3] = w01 + w12 + w23
+    gives dL/dw = 1.25 on 0->1 and 1->2, 1 on 2->3, and 0 on the back edge."""
+    # edges (CSR order): 0->1, 1->2, 2->1, 2->3
+    row = np.array([0, 1, 2, 4, 4], dtype=np.uint32)
+    col = np.array([1, 2, 1, 3], dtype=np.uint32)
+    w = np.array([0.0, 0.0, 0.0, 1.5], dtype=np.float32)
+    d = oracle.dijkstra(row, col, w, 0, True).view(np.float32)
+    assert list(d) == [0.0, 0.0, 0.0, 1.5]
+    gd = np.array([[0.0, 0.0, 0.25, 1.0]], dtype=np.float32)
+    gw, pe, _ = oracle.tree_grad(row, col, w, [0], d[None, :], gd)
+    assert list(gw) == [1.25, 1.25, 0.0, 1.0]
+    assert list(pe[0]) == [0xFFFFFFFF, 0, 1, 3]
+    # an absorbed add: 2^25 + 1 rounds to 2^25 in float32, so the unit edge is a tree edge too
+    row = np.array([0, 1, 2, 2], dtype=np.uint32)
+    col = np.array([1, 2], dtype=np.uint32)
+    w = np.array([2.0 ** 25, 1.0], dtype=np.float32)
+    d = oracle.dijkstra(row, col, w, 0, True).view(np.float32)
+    assert d[2] == d[1]
+    gw, _, _ = oracle.tree_grad(row, col, w, [0], d[None, :], np.array([[0, 0, 1.0]], dtype=np.float32))
+    assert list(gw) == [1.0, 1.0]
+
+
+def test_tree_grad_scale_rule():
+    """S = 62 - eM - eN (M = max |g| = m 2^eM with 0.5 <= m < 1; 2^eN >= nonzero count); unreached
+    vertices do not count; inputs off the grid 2^-S are rounded to it."""
+    dist = np.array([[0.0, 1.0, np.inf, 2.0]], dtype=np.float32)
+    g = np.array([[0.5, -3.0, 1e30, 0.0]], dtype=np.float32)  # M = 3 = 0.75 * 2^2, N = 2 = 2^1
+    assert oracle.grad_scale(dist, g) == (62 - 2 - 1, 2)
+    assert oracle.grad_scale(dist, np.zeros_like(g))[1] == 0
 
 
 def fd_loss(row, col, w64, src, gd):
@@ -233,8 +305,8 @@ def test_tree_grad_central_differences():
     row, col, w = rand_graph(rng, 80, 480, is_float=True)
     src = 3
     d = oracle.dijkstra(row, col, w, src, True).view(np.float32)
-    gd = (rng.integers(-(1 << 23), 1 << 23, 80) / float(1 << 23)).astype(np.float32)
-    gw, pe = oracle.tree_grad(row, col, w, d[None, :], gd[None, :])
+    gd = dyadic(rng, 80)
+    gw, pe, _ = oracle.tree_grad(row, col, w, [src], d[None, :], gd[None, :])
     src_of = np.repeat(np.arange(80), np.diff(row.astype(np.int64)))
     h = 1e-3
     checked = 0
@@ -253,6 +325,27 @@ def test_tree_grad_central_differences():
         assert abs(fd - gw[e]) <= 1e-3 * max(1.0, abs(fd)), (v, e, fd, gw[e])
         checked += 1
     assert checked >= 10
+
+
+def test_tree_grad_c5_fixture_sources():
+    """The committed C5 finite differences (scipy, float64, full 1024^2 grid) against the oracle's
+    gradient of single sources (four of the 32; the GPU test checks all 1000 edges)."""
+    import os
+    import zlib
+
+    from adds_amd import workloads
+
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c5_fd.npz"))
+    g, srcs, gd = workloads.c5()
+    assert zlib.crc32(g.w_bits.tobytes()) == int(fx["w_crc32"][0])
+    for s in (0, 9, 22, 31):
+        sel = np.nonzero(fx["source"] == s)[0]
+        d = oracle.dijkstra(g.row_ptr, g.col, g.w, srcs[s], True).view(np.float32)
+        gw, _, _ = oracle.tree_grad(g.row_ptr, g.col, g.w, [srcs[s]], d[None, :], gd[s][None, :])
+        got = gw[fx["edge"][sel]].astype(np.float64)
+        want = fx["fd"][sel]
+        err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3)
+        assert err.max() <= 1e-3, (s, err.max())
 
 
 @pytest.mark.parametrize("is_float,delta", [(False, 8.0), (False, 1.0), (True, 0.25), (True, 64.0)])
