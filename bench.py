@@ -2,25 +2,30 @@
 """bench.py -- ADDS SSSP on one MI355X.
 
 Headline workload (the metric's single-GPU configuration): single-source SSSP on RMAT scale 24, edge
-factor 32 (a/b/c/d = .57/.19/.19/.05, symmetrised, deduplicated: 16.8M vertices, ~0.5B directed edges),
+factor 32 (a/b/c/d = .57/.19/.19/.05, symmetrised, deduplicated: 16.8M vertices, 1.02G directed edges),
 uint32 weights uniform in [1, 65535], source = highest-degree vertex.  One step = one solve (dist fill +
 persistent kernel + worklist cleanup) with the CSR resident in HBM.  value = GTEPS = m_r / t, where m_r
 is the sum of out-degrees of the reached vertices and t the wall time per step.
 
-Also reported (extras): the float variant of the same graph; RMAT-16 (C1) and the 2048^2 grid (C2);
-256 sources on RMAT-20 in one persistent launch replayed from a HIP graph vs a loop of single-source
-launches (C4a); 64 uneven grid/ER graphs in one launch vs a loop (C4b); forward and hand-written
-backward on a 1024^2 grid with 32 sources (C5).
+Also reported (extras), each with its own roofline entry and CPU leg: the float variant of the same graph
+(C3 float); RMAT-16 (C1) and the 2048^2 grid (C2); 256 sources on RMAT-20 in one persistent launch replayed
+from a HIP graph vs a loop of single-source launches (C4a); 64 uneven grid/ER graphs in one captured
+launch vs a loop of per-graph launches (C4b); forward and backward on a 1024^2 grid with 32 sources (C5).
+Workloads: adds_amd/workloads.py.  Solver parameters: the pinned entries of adds_amd/tuned_params.json
+(--retune searches again and --write-params stores the result).
 
-roofline: HBM-bound; algorithmic bytes per solve B = 4n + 8 n_r + 12 m_r; achieved = B / (mean
-persistent-kernel time from HIP events on the solve stream); peak 8 TB/s.
-cpu_baseline: the oracle's single-thread Dijkstra on the same graph and source (kind "port").
+roofline: HBM-bound; algorithmic bytes per solve B = 4n + 8 n_r + 12 m_r (backward: 4m + 20 k n_r);
+achieved = B / (mean persistent-kernel time from HIP events on the solve stream); peak 8 TB/s.
+cpu_baseline: the oracle's single-thread Dijkstra on the same graph and source (kind "port"); the
+parallel delta-stepping leg runs on the host's CPU share (OMP_NUM_THREADS, else the affinity mask).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -28,6 +33,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_SOURCES = ("adds_amd/csrc/sssp_kernels.hip", "adds_amd/csrc/device.h")
 
 
 def parse():
@@ -36,10 +42,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--only", default="all", help="all | c3int | c3float | c1 | c2 | c4a | c4b | c5 (comma list)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs")
+    ap.add_argument("--retune", action="store_true", help="search the solver parameters again (ignore pinned)")
+    ap.add_argument("--write-params", default=None, help="store the parameters used per workload key here")
     ap.add_argument("--scale", type=int, default=24, help="RMAT scale of the headline graph")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
-                    help="per-launch HBM bytes measured by rocprofv3 --pmc (optional)")
+                    help="per-launch HBM bytes measured by rocprofv3 --pmc (used when the kernel source matches)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB), 3 times: the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -52,6 +60,74 @@ def log(*a):
 
 def bytes_alg(n, n_r, m_r):
     return 4 * n + 8 * n_r + 12 * m_r
+
+
+def roofline_entry(bytes_, ms, note=None):
+    gbs = bytes_ / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    r = {"bytes_alg": int(bytes_), "ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    if note:
+        r["note"] = note
+    return r
+
+
+def kernel_source_sha():
+    h = hashlib.sha256()
+    for p in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, p), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS") or affinity)
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "threads": max(1, min(share, affinity))}
+
+
+class Params:
+    """Parameter lookup per workload: pinned / cached entries, or a new search (--retune)."""
+
+    def __init__(self, retune):
+        from adds_amd import autotune
+        self.at = autotune
+        self.retune = retune
+        self.used = {}
+
+    @staticmethod
+    def _origin(src):
+        return os.path.basename(src) if src and os.sep in src else src
+
+    def single(self, dg, host, srcs, name):
+        key = self.at.workload_key(host, len(np.atleast_1d(srcs)))
+        p, src = (None, None) if self.retune else self.at.lookup(key)
+        if p is None:
+            t0 = time.perf_counter()
+            p = self.at.tune(dg, host, srcs, refresh=True)
+            src = f"tuned in {time.perf_counter() - t0:.1f}s"
+        self.used[key] = p
+        log(f"{name}: params {p} ({self._origin(src)})")
+        return p
+
+    def batched(self, dgs, hosts, srcs, name):
+        key = self.at.batched_key(hosts)
+        p, src = (None, None) if self.retune else self.at.lookup(key)
+        if p is None:
+            t0 = time.perf_counter()
+            p = self.at.tune_batched(dgs, hosts, srcs, refresh=True)
+            src = f"tuned in {time.perf_counter() - t0:.1f}s"
+        self.used[key] = p
+        log(f"{name}: params {p} ({self._origin(src)})")
+        return p
 
 
 def time_single(dg, src, dist_ptr, steps, warmup, torch, **params):
@@ -78,16 +154,34 @@ def near_far(dg, src, adds_dist, adds_ms, torch, reps=3):
             "same_as_adds": bool(torch.equal(buf, adds_dist)), "adds_speedup": ms / adds_ms}
 
 
-def run_c3(host, name, steps, warmup, torch, adds_amd, oracle, do_cpu, certify=True, tune=True):
-    src = host.max_degree_vertex()
+def cpu_single(host, src, m_r, oracle, hinfo, delta):
+    """CPU legs of a single-source workload: the oracle's Dijkstra on 1 thread and the parallel
+    delta-stepping on the CPU share; both checked against each other."""
+    t0 = time.perf_counter()
+    dj = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)
+    tdj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ds = oracle.delta_step(host.row_ptr, host.col, host.w, src, host.is_float, delta, hinfo["threads"])
+    tds = time.perf_counter() - t0
+    out = {"dijkstra_1thread": {"s": tdj, "gteps": m_r / tdj / 1e9},
+           "delta_step": {"s": tds, "gteps": m_r / tds / 1e9, "threads": hinfo["threads"], "delta": delta,
+                          "same_as_dijkstra": bool(np.array_equal(ds, dj))}}
+    return out, dj
+
+
+def run_c3(host, src, name, steps, warmup, torch, adds_amd, oracle, prm, do_cpu, hinfo):
+    t0 = time.perf_counter()
     dg = adds_amd.Graph(host)
-    params = {}
-    if tune:
-        from adds_amd.autotune import tune as autotune
-        t0 = time.perf_counter()
-        params = autotune(dg, host, [src])
-        log(f"{name}: tuned {params} ({time.perf_counter() - t0:.1f}s)")
+    upload_s = time.perf_counter() - t0
+    params = prm.single(dg, host, [src], name)
     dist = torch.empty(host.n, dtype=torch.int32, device="cuda")
+    # cold: the first solve on a fresh handle builds the light-first rows for its light threshold and
+    # allocates the worklist (the one-time cost of a drop-in run); then the warm steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cold = dg.sssp_device(src, dist.data_ptr(), **params)
+    torch.cuda.synchronize()
+    cold_ms = (time.perf_counter() - t0) * 1e3
     wall, stats = time_single(dg, src, dist.data_ptr(), steps, warmup, torch, **params)
     d = dist.cpu().numpy().view(np.uint32)
     n_r, m_r = host.reached_counts(d)
@@ -96,16 +190,14 @@ def run_c3(host, name, steps, warmup, torch, adds_amd, oracle, do_cpu, certify=T
     res = {
         "n": host.n, "m": host.m, "src": src, "n_r": n_r, "m_r": m_r, "params": params,
         "ms_per_step": wall * 1e3, "solve_ms_event": sms, "kernel_ms_event": kms,
+        "kernel_ms_min": float(min(s["kernel_ms"] for s in stats)),
         "gteps": m_r / wall / 1e9, "gteps_kernel": m_r / (kms / 1e3) / 1e9,
         "bytes_alg": bytes_alg(host.n, n_r, m_r),
+        "cold_first_solve_ms": cold_ms, "cold_kernel_ms": cold["kernel_ms"], "graph_upload_s": upload_s,
         "work": {k: stats[-1][k] for k in ("vertices_expanded", "edges_scanned", "relaxations", "pushes",
                                            "stale_skips", "hub_tasks", "rebuckets")},
     }
-    if certify and not do_cpu:
-        t0 = time.perf_counter()
-        c = oracle.certify(host.row_ptr, host.col, host.w, src, d, host.is_float)
-        res["certificate"] = "ok" if c == 0 else f"FAILED({c})"
-        log(f"{name}: certificate {res['certificate']} ({time.perf_counter() - t0:.1f}s)")
+    res["roofline"] = roofline_entry(res["bytes_alg"], kms)
     # the same solve with the distances copied back to host memory (the CLI's path): PCIe-inclusive
     hw = []
     for _ in range(3):
@@ -116,59 +208,49 @@ def run_c3(host, name, steps, warmup, torch, adds_amd, oracle, do_cpu, certify=T
     res["host_out_gteps"] = m_r / np.median(hw) / 1e9
     res["near_far"] = near_far(dg, src, dist, sms, torch)
     log(f"{name}: near-far baseline {res['near_far']}")
-    if do_cpu:
-        t0 = time.perf_counter()
-        ref = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)
-        el = time.perf_counter() - t0
-        res["cpu_dijkstra_s"] = el
-        res["bit_exact_vs_dijkstra"] = bool(np.array_equal(ref, d))
-        log(f"{name}: cpu dijkstra {el:.1f}s, bit-exact {res['bit_exact_vs_dijkstra']}")
-        # the reference's other CPU baseline: parallel delta-stepping on the host's cores
-        threads = min(os.cpu_count() or 1, 16)
-        delta = params.get("delta") or 4096.0
-        t0 = time.perf_counter()
-        ds = oracle.delta_step(host.row_ptr, host.col, host.w, src, host.is_float, delta, threads)
-        el = time.perf_counter() - t0
-        res["cpu_delta_step"] = {"s": el, "threads": threads, "delta": delta, "gteps": m_r / el / 1e9,
-                                 "bit_exact": bool(np.array_equal(ds, ref))}
-        log(f"{name}: cpu delta-step {el:.1f}s on {threads} threads")
     dg.close()
     del dist
-    log(f"{name}: {res['ms_per_step']:.3f} ms/step, {res['gteps']:.2f} GTEPS, kernel {kms:.3f} ms")
+    if do_cpu:
+        cpu, ref = cpu_single(host, src, m_r, oracle, hinfo, params.get("delta") or 4096.0)
+        res["cpu"] = cpu
+        res["bit_exact_vs_dijkstra"] = bool(np.array_equal(ref, d))
+        log(f"{name}: cpu {cpu}, bit-exact {res['bit_exact_vs_dijkstra']}")
+    else:
+        c = oracle.certify(host.row_ptr, host.col, host.w, src, d, host.is_float)
+        res["certificate"] = "ok" if c == 0 else f"FAILED({c})"
+    log(f"{name}: {res['ms_per_step']:.3f} ms/step, {res['gteps']:.2f} GTEPS, kernel {kms:.3f} ms, "
+        f"cold {cold_ms:.1f} ms")
     return res
 
 
-def tuned(dg, host, srcs, name):
-    from adds_amd.autotune import tune as autotune
-    t0 = time.perf_counter()
-    params = autotune(dg, host, srcs)
-    log(f"{name}: tuned {params} ({time.perf_counter() - t0:.1f}s)")
-    return params
-
-
-def run_small(host, name, src, steps, warmup, torch, adds_amd, oracle):
+def run_small(host, name, src, steps, warmup, torch, adds_amd, oracle, prm, do_cpu, hinfo, note=None):
     dg = adds_amd.Graph(host)
-    params = tuned(dg, host, [src], name)
+    params = prm.single(dg, host, [src], name)
     dist = torch.empty(host.n, dtype=torch.int32, device="cuda")
     wall, stats = time_single(dg, src, dist.data_ptr(), steps, warmup, torch, **params)
     d = dist.cpu().numpy().view(np.uint32)
     n_r, m_r = host.reached_counts(d)
-    ref = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)
+    kms = float(np.mean([s["kernel_ms"] for s in stats]))
     res = {"n": host.n, "m": host.m, "n_r": n_r, "m_r": m_r, "params": params, "ms_per_step": wall * 1e3,
-           "kernel_ms_event": float(np.mean([s["kernel_ms"] for s in stats])),
-           "gteps": m_r / wall / 1e9, "bit_exact_vs_dijkstra": bool(np.array_equal(ref, d))}
+           "kernel_ms_event": kms, "gteps": m_r / wall / 1e9,
+           "roofline": roofline_entry(bytes_alg(host.n, n_r, m_r), kms, note)}
     res["near_far"] = near_far(dg, src, dist, float(np.mean([s["ms"] for s in stats])), torch)
     dg.close()
+    if do_cpu:
+        cpu, ref = cpu_single(host, src, m_r, oracle, hinfo, params.get("delta") or 1.0)
+        res["cpu"] = cpu
+    else:
+        ref = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)
+    res["bit_exact_vs_dijkstra"] = bool(np.array_equal(ref, d))
     log(f"{name}: {res}")
     return res
 
 
-def run_c4a(steps, torch, adds_amd, oracle):
-    host = adds_amd.HostGraph.rmat(20, 16, seed=4, wkind="f32", lo=1.0, hi=10.0)
-    srcs = host.sample_component(host.max_degree_vertex(), 256, seed=40)
+def run_c4a(steps, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo):
+    host, srcs = workloads.c4a()
     k = len(srcs)
     dg = adds_amd.Graph(host)
-    params = tuned(dg, host, srcs, "c4a")
+    params = prm.single(dg, host, srcs, "c4a")
     dist = torch.empty((k, host.n), dtype=torch.int32, device="cuda")
     srcs_dev = torch.from_numpy(srcs.astype(np.int32)).cuda()
     sg = adds_amd.SolveGraph(dg, srcs_dev.data_ptr(), k, dist.data_ptr(), **params)
@@ -176,15 +258,17 @@ def run_c4a(steps, torch, adds_amd, oracle):
     sg.status()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    replay_ms = []
     for _ in range(steps):
         sg.launch()
         st = sg.status()
+        replay_ms.append(st["ms"])
     torch.cuda.synchronize()
     batched_s = (time.perf_counter() - t0) / steps
     dall = dist.cpu().numpy().view(np.uint32)
-    # loop of single-source launches into the same buffer rows
+    # loop of single-source launches into one buffer, with the best parameters for one source
     one = torch.empty(host.n, dtype=torch.int32, device="cuda")
-    p1 = tuned(dg, host, [int(srcs[0])], "c4a single-source")  # the loop gets its own best parameters
+    p1 = prm.single(dg, host, [int(srcs[0])], "c4a single-source")
     dg.sssp_device(int(srcs[0]), one.data_ptr(), **p1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -192,98 +276,139 @@ def run_c4a(steps, torch, adds_amd, oracle):
         dg.sssp_device(int(s), one.data_ptr(), **p1)
     torch.cuda.synchronize()
     loop_s = time.perf_counter() - t0
-    check = [0, k // 2, k - 1]
-    exact = all(np.array_equal(oracle.dijkstra(host.row_ptr, host.col, host.w, srcs[i], True), dall[i])
-                for i in check)
-    m_r_total = sum(host.reached_counts(dall[i])[1] for i in range(k))
-    res = {"n": host.n, "m": host.m, "sources": k, "batched_ms": batched_s * 1e3, "loop_ms": loop_s * 1e3,
-           "batched_sources_per_s": k / batched_s, "loop_sources_per_s": k / loop_s,
-           "speedup": loop_s / batched_s, "batched_gteps": m_r_total / batched_s / 1e9,
-           "bit_exact_sampled": exact, "work": {"edges_scanned": st["edges_scanned"]}}
     sg.close()
     dg.close()
+    # every source checked: the certificate (pins the solution) on all, Dijkstra bit for bit on 16
+    with ThreadPoolExecutor(hinfo["threads"]) as ex:
+        certs = list(ex.map(lambda i: oracle.certify(host.row_ptr, host.col, host.w, srcs[i], dall[i], True),
+                            range(k)))
+        pick = list(range(0, k, max(1, k // 16)))
+        t0 = time.perf_counter()
+        refs = list(ex.map(lambda i: oracle.dijkstra(host.row_ptr, host.col, host.w, srcs[i], True), pick))
+        par_s = time.perf_counter() - t0
+    exact16 = all(np.array_equal(dall[i], r) for i, r in zip(pick, refs))
+    reach = [host.reached_counts(dall[i]) for i in range(k)]
+    m_r_total = sum(r[1] for r in reach)
+    b_total = sum(bytes_alg(host.n, r[0], r[1]) for r in reach)
+    mean_replay = float(np.mean(replay_ms))
+    res = {"n": host.n, "m": host.m, "sources": k, "params": params, "loop_params": p1,
+           "batched_ms": batched_s * 1e3, "replay_ms_event": mean_replay, "loop_ms": loop_s * 1e3,
+           "batched_sources_per_s": k / batched_s, "loop_sources_per_s": k / loop_s,
+           "speedup": loop_s / batched_s, "batched_gteps": m_r_total / batched_s / 1e9,
+           "certificate_all_256": all(c == 0 for c in certs), "bit_exact_16_vs_dijkstra": exact16,
+           "roofline": roofline_entry(b_total, mean_replay, "replay time (fill + kernel + cleanup)"),
+           "work": {"edges_scanned": st["edges_scanned"]}}
+    if do_cpu:
+        t0 = time.perf_counter()
+        for i in pick[:4]:
+            oracle.dijkstra(host.row_ptr, host.col, host.w, srcs[i], True)
+        one_s = (time.perf_counter() - t0) / 4
+        res["cpu"] = {"dijkstra_1thread_sources_per_s": 1.0 / one_s,
+                      "dijkstra_parallel_sources_per_s": len(pick) / par_s, "threads": hinfo["threads"],
+                      "sample": f"4 sources on 1 thread, {len(pick)} on {hinfo['threads']} threads"}
     log(f"c4a: {res}")
     return res
 
 
-def c4b_graphs(adds_amd):
-    rng = np.random.default_rng(4242)  # sizes only; graph content comes from the library's seeded PRNG
-    gs = []
-    for i in range(64):
-        n = int(round(np.exp(np.log(1e4) + rng.random() * (np.log(1e6) - np.log(1e4)))))
-        if i % 2 == 0:
-            side = max(2, int(round(np.sqrt(n))))
-            gs.append(adds_amd.HostGraph.grid(side, side, 0, seed=100 + i, wkind="u32", lo=1, hi=1000))
-        else:
-            gs.append(adds_amd.HostGraph.er(n, 8.0, seed=100 + i, wkind="u32", lo=1, hi=1000))
-    return gs
-
-
-def run_c4b(steps, torch, adds_amd, oracle):
-    hosts = c4b_graphs(adds_amd)
+def run_c4b(steps, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo):
+    hosts, srcs = workloads.c4b()
     dgs = [adds_amd.Graph(h) for h in hosts]
-    srcs = [0] * len(hosts)
-    from adds_amd.autotune import tune_batched
-    t0 = time.perf_counter()
-    params = tune_batched(dgs, hosts, srcs)  # builds and tunes the union graph
-    log(f"c4b: tuned {params} ({time.perf_counter() - t0:.1f}s)")
-    dists, _ = adds_amd.sssp_batched(dgs, srcs, **params)  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        dists, st = adds_amd.sssp_batched(dgs, srcs, **params)
-    batched_s = (time.perf_counter() - t0) / steps
+    params = prm.batched(dgs, hosts, srcs, "c4b")
     outs = [torch.empty(h.n, dtype=torch.int32, device="cuda") for h in hosts]
+    sg = adds_amd.SolveGraph.batched(dgs, srcs, [o.data_ptr() for o in outs], **params)
+    sg.launch()
+    sg.status()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for dg, o in zip(dgs, outs):
-        dg.sssp_device(0, o.data_ptr())
+    replay_ms = []
+    for _ in range(steps):
+        sg.launch()
+        st = sg.status()
+        replay_ms.append(st["ms"])
+    torch.cuda.synchronize()
+    batched_s = (time.perf_counter() - t0) / steps
+    dists = [o.cpu().numpy().view(np.uint32) for o in outs]
+    sg.close()
+    # loop: each graph with its own tuned parameters
+    lp = [prm.single(dg, h, [0], f"c4b graph {i}") for i, (dg, h) in enumerate(zip(dgs, hosts))]
+    for dg, o, p in zip(dgs, outs, lp):
+        dg.sssp_device(0, o.data_ptr(), **p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for dg, o, p in zip(dgs, outs, lp):
+        dg.sssp_device(0, o.data_ptr(), **p)
     torch.cuda.synchronize()
     loop_s = time.perf_counter() - t0
-    exact = all(np.array_equal(oracle.dijkstra(h.row_ptr, h.col, h.w, 0, False), d.view(np.uint32))
-                for h, d in zip(hosts, dists))
-    res = {"graphs": len(hosts), "total_n": int(sum(h.n for h in hosts)), "total_m": int(sum(h.m for h in hosts)),
-           "params": params, "loop_params": "defaults of each graph",
-           "batched_ms": batched_s * 1e3, "loop_ms": loop_s * 1e3, "speedup": loop_s / batched_s,
-           "instances_per_s": len(hosts) / batched_s, "bit_exact_all": exact}
     for d in dgs:
         d.close()
+    with ThreadPoolExecutor(hinfo["threads"]) as ex:
+        t0 = time.perf_counter()
+        refs = list(ex.map(lambda i: oracle.dijkstra(hosts[i].row_ptr, hosts[i].col, hosts[i].w, srcs[i], False),
+                           range(len(hosts))))
+        par_s = time.perf_counter() - t0
+    exact = all(np.array_equal(r, d) for r, d in zip(refs, dists))
+    reach = [h.reached_counts(d) for h, d in zip(hosts, dists)]
+    b_total = sum(bytes_alg(h.n, r[0], r[1]) for h, r in zip(hosts, reach))
+    mean_replay = float(np.mean(replay_ms))
+    res = {"graphs": len(hosts), "total_n": int(sum(h.n for h in hosts)), "total_m": int(sum(h.m for h in hosts)),
+           "params": params, "loop_params": "each graph's own tuned parameters",
+           "batched_ms": batched_s * 1e3, "replay_ms_event": mean_replay, "loop_ms": loop_s * 1e3,
+           "speedup": loop_s / batched_s, "instances_per_s": len(hosts) / batched_s, "bit_exact_all": exact,
+           "roofline": roofline_entry(b_total, mean_replay, "replay time (fill + kernel + cleanup + copies)")}
+    if do_cpu:
+        t0 = time.perf_counter()
+        for i in range(len(hosts)):
+            oracle.dijkstra(hosts[i].row_ptr, hosts[i].col, hosts[i].w, srcs[i], False)
+        res["cpu"] = {"dijkstra_1thread_instances_per_s": len(hosts) / (time.perf_counter() - t0),
+                      "dijkstra_parallel_instances_per_s": len(hosts) / par_s, "threads": hinfo["threads"]}
     log(f"c4b: {res}")
     return res
 
 
-def run_c5(steps, torch, adds_amd, oracle):
-    from adds_amd.autograd import SSSPFunction  # noqa: F401  (binding exercised by tests)
-    host = adds_amd.HostGraph.grid(1024, 1024, 0, seed=5, wkind="lognormal", lo=0.5, hi=0.0)
+def run_c5(steps, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo):
+    host, srcs, gd = workloads.c5()
     dg = adds_amd.Graph(host)
-    k = 32
-    srcs = host.sample_component(0, k, seed=55)
+    k = len(srcs)
     dist = torch.empty((k, host.n), dtype=torch.float32, device="cuda")
-    gdist = torch.zeros((k, host.n), dtype=torch.float32, device="cuda")
-    rng = np.random.default_rng(5)
-    for s in range(k):
-        tg = rng.choice(host.n, 4096, replace=False)
-        c = rng.integers(-(1 << 23), 1 << 23, size=4096).astype(np.float32) / np.float32(1 << 23)
-        gdist[s, torch.from_numpy(tg).cuda()] = torch.from_numpy(c).cuda()
+    gdist = torch.from_numpy(gd).cuda()
     gw = torch.empty(host.m, dtype=torch.float32, device="cuda")
-    params = tuned(dg, host, srcs, "c5")
-    fwd, bwd = [], []
+    params = prm.single(dg, host, srcs, "c5")
+    fwd, fwd_k, bwd = [], [], []
     for i in range(steps + 1):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        dg.sssp_multi_device(srcs, dist.data_ptr(), **params)
+        st = dg.sssp_multi_device(srcs, dist.data_ptr(), **params)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         bms = dg.backward_device(srcs, dist.data_ptr(), gdist.data_ptr(), gw.data_ptr())
         torch.cuda.synchronize()
         if i:
             fwd.append((t1 - t0) * 1e3)
+            fwd_k.append(st["kernel_ms"])
             bwd.append(bms)
-    ref_gw, _, _ = oracle.tree_grad(host.row_ptr, host.col, host.w, srcs, dist.cpu().numpy(), gdist.cpu().numpy())
-    res = {"n": host.n, "m": host.m, "sources": k, "params": params, "forward_ms": float(np.median(fwd)),
-           "backward_ms": float(np.median(bwd)),
-           "grad_bit_exact_vs_cpu_tree": bool(np.array_equal(ref_gw.view(np.uint32),
-                                                             gw.cpu().numpy().view(np.uint32)))}
+    d = dist.cpu().numpy()
+    got = gw.cpu().numpy()
     dg.close()
+    t0 = time.perf_counter()
+    ref_gw, _, scale = oracle.tree_grad(host.row_ptr, host.col, host.w, srcs, d, gd)
+    cpu_grad_s = time.perf_counter() - t0
+    reach = [host.reached_counts(d[i]) for i in range(k)]
+    n_r_total = sum(r[0] for r in reach)
+    res = {"n": host.n, "m": host.m, "sources": k, "params": params, "forward_ms": float(np.median(fwd)),
+           "forward_kernel_ms": float(np.median(fwd_k)), "backward_ms": float(np.median(bwd)),
+           "grad_scale": scale,
+           "grad_bit_exact_vs_cpu_reference": bool(np.array_equal(ref_gw.view(np.uint32), got.view(np.uint32))),
+           "roofline_forward": roofline_entry(sum(bytes_alg(host.n, r[0], r[1]) for r in reach),
+                                              float(np.median(fwd_k))),
+           "roofline_backward": roofline_entry(4 * host.m + 20 * n_r_total, float(np.median(bwd)),
+                                               "B = 4m + 20 k n_r (a latency-bound tree pass)")}
+    if do_cpu:
+        t0 = time.perf_counter()
+        for i in range(4):
+            oracle.dijkstra(host.row_ptr, host.col, host.w, srcs[i], True)
+        res["cpu"] = {"dijkstra_1thread_forward_ms_per_batch": (time.perf_counter() - t0) / 4 * k * 1e3,
+                      "tree_gradient_1thread_ms_per_batch": cpu_grad_s * 1e3,
+                      "sample": "forward: 4 sources scaled to 32; gradient: the whole batch (oracle.tree_grad)"}
     log(f"c5: {res}")
     return res
 
@@ -308,54 +433,80 @@ def main():
 
     import adds_amd
     import oracle
+    from adds_amd import workloads
 
     only = set(args.only.split(",")) if args.only != "all" else {"c3int", "c3float", "c1", "c2", "c4a", "c4b", "c5"}
     torch.cuda.init()
     ncu = adds_amd._lib._lib().adds_device_cu_count()
-    log(f"device CUs: {ncu}")
+    hinfo = host_info()
+    log(f"device CUs: {ncu}; host {hinfo}")
+    prm = Params(args.retune)
+    do_cpu = not args.no_cpu
     extras = {}
     head = None
-    host = None
     if "c3int" in only or "c3float" in only:
         t0 = time.perf_counter()
-        host = adds_amd.HostGraph.rmat(args.scale, 32, seed=1, wkind="u32", lo=1, hi=65535)
+        host, src = workloads.c3("u32", scale=args.scale)
         log(f"rmat{args.scale} generated n={host.n} m={host.m} in {time.perf_counter() - t0:.1f}s")
-    if "c3int" in only:
-        head = run_c3(host, "c3int", args.steps, args.warmup, torch, adds_amd, oracle, not args.no_cpu)
-    if "c3float" in only:
-        host.reweight(seed=1, wkind="f32", lo=0.0, hi=1.0)
-        extras["c3_float"] = run_c3(host, "c3float", args.steps, args.warmup, torch, adds_amd, oracle,
-                                    do_cpu=False)
-    host = None
+        if "c3int" in only:
+            head = run_c3(host, src, "c3int", args.steps, args.warmup, torch, adds_amd, oracle, prm, do_cpu, hinfo)
+        if "c3float" in only:
+            workloads.c3_reweight_float(host)
+            extras["c3_float"] = run_c3(host, src, "c3float", args.steps, args.warmup, torch, adds_amd, oracle, prm,
+                                        do_cpu, hinfo)
+        host = None
     if "c1" in only:
-        h = adds_amd.HostGraph.rmat(16, 16, seed=1, wkind="u32", lo=1, hi=100)
-        extras["c1_rmat16_int"] = run_small(h, "c1", h.max_degree_vertex(), 20, 3, torch, adds_amd, oracle)
+        h, s = workloads.c1()
+        extras["c1_rmat16_int"] = run_small(h, "c1", s, 20, 3, torch, adds_amd, oracle, prm, do_cpu, hinfo,
+                                            note="16.5 MB working set: L2/MALL-resident, not an HBM case")
     if "c2" in only:
-        h = adds_amd.HostGraph.grid(2048, 2048, 8385, seed=2, wkind="f32", lo=1.0, hi=2.0)
-        extras["c2_grid2048_float"] = run_small(h, "c2", 0, 5, 2, torch, adds_amd, oracle)
+        h, s = workloads.c2()
+        extras["c2_grid2048_float"] = run_small(h, "c2", s, 5, 2, torch, adds_amd, oracle, prm, do_cpu, hinfo,
+                                                note="high-diameter grid: latency-bound")
     if "c4a" in only:
-        extras["c4a_rmat20_256src"] = run_c4a(3, torch, adds_amd, oracle)
+        extras["c4a_rmat20_256src"] = run_c4a(3, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo)
     if "c4b" in only:
-        extras["c4b_64graphs"] = run_c4b(3, torch, adds_amd, oracle)
+        extras["c4b_64graphs"] = run_c4b(3, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo)
     if "c5" in only:
-        extras["c5_grid1024_backward"] = run_c5(3, torch, adds_amd, oracle)
+        extras["c5_grid1024_backward"] = run_c5(3, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo)
+
+    if args.write_params:
+        cache = {}
+        if os.path.exists(args.write_params):
+            with open(args.write_params) as f:
+                cache = json.load(f)
+        for key, p in prm.used.items():
+            cache[key] = {"params": p, "kernel_ms": None}
+        with open(args.write_params, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+        log(f"wrote {len(prm.used)} parameter sets to {args.write_params}")
 
     if head is None:
-        print(json.dumps({"extras": extras}))
+        print(json.dumps({"extras": extras, "host": hinfo}))
         return
-    traffic = None
+    traffic, traffic_note = None, "no rocprofv3 traffic measurement for this kernel source"
+    sha = kernel_source_sha()
     if os.path.exists(args.traffic_json):
         try:
             with open(args.traffic_json) as f:
-                traffic = json.load(f).get("c3int_bytes_per_launch")
+                tj = json.load(f)
+            if tj.get("kernel_source_sha") == sha and tj.get("params") == head["params"]:
+                traffic = tj.get("c3int_bytes_per_launch")
+                traffic_note = tj.get("source")
+            else:
+                traffic_note = "profiles/traffic.json was measured on another kernel source or parameter set"
         except (OSError, ValueError):
-            traffic = None
+            pass
     achieved = head["bytes_alg"] / (head["kernel_ms_event"] / 1e3) / 1e9
     cpu = None
-    if "cpu_dijkstra_s" in head:
-        cpu = {"value": head["m_r"] / head["cpu_dijkstra_s"] / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
-               "sample": f"one full single-source Dijkstra (oracle, binary heap) on the same RMAT-{args.scale} "
-                         f"graph and source: {head['cpu_dijkstra_s']:.1f}s"}
+    if "cpu" in head:
+        dj = head["cpu"]["dijkstra_1thread"]
+        cpu = {"value": dj["gteps"], "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": f"one full single-source Dijkstra (oracle, binary heap, 1 thread) on the same "
+                         f"RMAT-{args.scale} graph and source: {dj['s']:.1f}s",
+               "host_cpu_model": hinfo["cpu_model"], "host_nproc": hinfo["nproc"],
+               "host_cpu_share": hinfo["threads"],
+               "parallel_delta_step": head["cpu"]["delta_step"]}
     line = {
         "metric": "SSSP GTEPS, RMAT-24 ef32 uint32 weights, single source, 1 GPU",
         "value": head["gteps"], "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -366,14 +517,16 @@ def main():
                    "certificate": head.get("certificate"),
                    "bit_exact_vs_dijkstra": head.get("bit_exact_vs_dijkstra")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": f"k_sssp<false, false, {head['params'].get('edge_windows') or 4}>",
-                     "kernel_ms": head["kernel_ms_event"],
+                     "kernel_ms": head["kernel_ms_event"], "kernel_source_sha": sha,
                      "bytes_alg_per_launch": head["bytes_alg"]},
         "cpu_baseline": cpu,
-        "extras": dict(extras, c3_int={k: head[k] for k in ("params", "kernel_ms_event", "solve_ms_event",
-                                                             "gteps_kernel", "host_out_ms", "host_out_gteps",
-                                                             "work", "cpu_delta_step", "near_far") if k in head}),
+        "extras": dict(extras, c3_int={k: head[k] for k in ("params", "kernel_ms_event", "kernel_ms_min",
+                                                             "solve_ms_event", "gteps_kernel", "host_out_ms",
+                                                             "host_out_gteps", "cold_first_solve_ms",
+                                                             "cold_kernel_ms", "work", "cpu", "near_far")
+                                       if k in head}),
     }
     print(json.dumps(line))
 
