@@ -45,6 +45,115 @@ static int fail(const char* what, adds_status s) {
   return 1;
 }
 
+#if !ADDS_NF
+// ---- tuned parameters ----
+// The reference adapts delta inside the binary (wl.h:395-449, 968-1310); this solver is tuned offline
+// (adds_autotune) and the binary looks the result up by workload key in tuned_params.json (committed
+// entries) and then tune_cache.json (local tuning runs), both in the package directory above bin/, or
+// in the file ADDS_TUNE_FILE names ("none": defaults).  The key is the one adds_amd/autotune.py
+// workload_key() writes: n, m, weight type, the mean of every max(1, m / 65536)-th weight (%.4g) and the
+// number of sources.
+static std::string tuned_key(const adds_host_csr& h) {
+  const uint64_t step = h.m / 65536 > 1 ? h.m / 65536 : 1;
+  double s = 0;
+  uint64_t cnt = 0;
+  for (uint64_t i = 0; i < h.m; i += step, cnt++) {
+    if (h.wtype == ADDS_W_F32) {
+      float f;
+      memcpy(&f, &h.w[i], 4);
+      s += f;
+    } else {
+      s += h.w[i];
+    }
+  }
+  char mean[64];
+  snprintf(mean, sizeof(mean), "%.4g", cnt ? s / (double)cnt : 0.0);
+  return "n=" + std::to_string(h.n) + ";m=" + std::to_string(h.m) + ";w=" +
+         (h.wtype == ADDS_W_F32 ? "f32" : "u32") + ";meanw=" + mean + ";k=1";
+}
+
+static bool read_file(const std::string& path, std::string& out) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  char buf[1 << 16];
+  size_t r;
+  while ((r = fread(buf, 1, sizeof(buf), f)) > 0) out.append(buf, r);
+  fclose(f);
+  return true;
+}
+
+// Fills p from the "params" object of `key` in a JSON cache written by autotune.py (a flat object of
+// numbers); returns false when the key is absent.
+static bool params_from_json(const std::string& text, const std::string& key, adds_params* p) {
+  const size_t at = text.find("\"" + key + "\"");
+  if (at == std::string::npos) return false;
+  const size_t po = text.find("\"params\"", at);
+  const size_t open = po == std::string::npos ? po : text.find('{', po);
+  const size_t close = open == std::string::npos ? open : text.find('}', open);
+  if (close == std::string::npos) return false;
+  const std::string obj = text.substr(open, close - open);
+  auto num = [&](const char* name, double* v) {
+    const size_t f = obj.find("\"" + std::string(name) + "\"");
+    if (f == std::string::npos) return;
+    const size_t colon = obj.find(':', f);
+    if (colon != std::string::npos) *v = strtod(obj.c_str() + colon + 1, nullptr);
+  };
+  double delta = 0, light = 0, wg = 0, chunk = 0, nb = 0, parts = 0, conc = 0, te = 0, flags = 0, win = 0, tail = 0,
+         hub = 0;
+  num("delta", &delta);
+  num("light_delta", &light);
+  num("wg_per_cu", &wg);
+  num("chunk", &chunk);
+  num("nbuckets", &nb);
+  num("parts", &parts);
+  num("concurrency", &conc);
+  num("task_edges", &te);
+  num("flags", &flags);
+  num("edge_windows", &win);
+  num("tail_items", &tail);
+  num("hub_degree", &hub);
+  p->delta = (float)delta;
+  p->light_delta = (float)light;
+  p->wg_per_cu = (uint32_t)wg;
+  p->chunk = (uint32_t)chunk;
+  p->nbuckets = (uint32_t)nb;
+  p->parts = (uint32_t)parts;
+  p->concurrency = (uint32_t)conc;
+  p->task_edges = (uint32_t)te;
+  p->flags = (uint32_t)flags & (ADDS_F_NO_LOCAL | ADDS_F_NO_WAIT);  // tuning flags only
+  p->edge_windows = (uint32_t)win;
+  p->tail_items = (uint32_t)tail;
+  p->hub_degree = (uint32_t)hub;
+  return true;
+}
+
+static std::string lookup_tuned(const adds_host_csr& h, adds_params* p) {
+  std::vector<std::string> files;
+  const char* env = getenv("ADDS_TUNE_FILE");
+  if (env) {
+    if (strcmp(env, "none") == 0) return "";
+    files.push_back(env);
+  } else {
+    char exe[4096];
+    const ssize_t len = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+    if (len > 0) {
+      exe[len] = '\0';
+      std::string dir(exe);
+      dir = dir.substr(0, dir.find_last_of('/'));   // .../bin
+      dir = dir.substr(0, dir.find_last_of('/'));   // the package directory
+      files.push_back(dir + "/tuned_params.json");
+      files.push_back(dir + "/tune_cache.json");
+    }
+  }
+  const std::string key = tuned_key(h);
+  for (const std::string& f : files) {
+    std::string text;
+    if (read_file(f, text) && params_from_json(text, key, p)) return f;
+  }
+  return "";
+}
+#endif
+
 int main(int argc, char** argv) {
   if (argc == 1) {
     usage(argv[0]);
@@ -105,7 +214,11 @@ int main(int argc, char** argv) {
   std::vector<uint32_t> dist(h.n);
   adds_params p;
   memset(&p, 0, sizeof(p));
-  p.flags = ADDS_F_HOST_OUT;
+#if !ADDS_NF
+  const std::string tuned_from = lookup_tuned(h, &p);
+  if (!tuned_from.empty()) printf("tuned parameters from %s\n", tuned_from.c_str());
+#endif
+  p.flags |= ADDS_F_HOST_OUT;
   double agg_ms = 0;
   unsigned long long agg_work = 0;
   for (int loop = 0; loop < run_loop; loop++) {

@@ -45,6 +45,42 @@ def test_cli_on_golden_fixtures(tmp_path):
         float(parts[1]), int(parts[2])
 
 
+@pytest.mark.parametrize("kind", ["int", "float"])
+def test_cli_uses_tuned_parameters(tmp_path, kind):
+    """The binaries find the tuned parameters of their graph under the key autotune.workload_key()
+    writes (here in a file named by ADDS_TUNE_FILE, with distinctive values), and the output is unchanged."""
+    import numpy as np
+
+    import adds_amd
+    import oracle
+    from adds_amd import autotune
+
+    wk = ("u32", 1, 500) if kind == "int" else ("f32", 0.0, 1.0)
+    g = adds_amd.HostGraph.rmat(11, 8, seed=5, wkind=wk[0], lo=wk[1], hi=wk[2])
+    path = tmp_path / "g.gr"
+    g.write_gr(path)
+    tune = tmp_path / "tune.json"
+    params = {"delta": 37.0 if kind == "int" else 0.013, "light_delta": 0.0, "wg_per_cu": 1, "chunk": 32,
+              "nbuckets": 16, "parts": 2, "concurrency": 2, "task_edges": 0, "flags": 16, "edge_windows": 8,
+              "tail_items": 0, "hub_degree": 0}
+    tune.write_text(json.dumps({"other": {"params": {}}, autotune.workload_key(g, 1): {"params": params}}, indent=1))
+    out = tmp_path / "dist"
+    env = dict(os.environ, ADDS_TUNE_FILE=str(tune))
+    r = subprocess.run(f"{os.path.join(BIN, 'sssp_' + kind)} -s 3 -o {out} {path} 3>/dev/null", shell=True,
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stderr
+    assert f"tuned parameters from {tune}" in r.stdout
+    ref = oracle.dijkstra(g.row_ptr, g.col, g.w, 3, g.is_float)
+    ref_txt = tmp_path / "ref"
+    adds_amd.write_dist_text(ref_txt, ref.view(np.float32) if g.is_float else ref, "float" if g.is_float else "int")
+    assert out.read_text() == ref_txt.read_text()
+    # a key that is not in the file: defaults, no message
+    env["ADDS_TUNE_FILE"] = str(tmp_path / "missing.json")
+    r = subprocess.run(f"{os.path.join(BIN, 'sssp_' + kind)} -s 3 {path} 3>/dev/null", shell=True,
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0 and "tuned parameters" not in r.stdout
+
+
 def test_cli_rejects_out_of_range_source(tmp_path):
     r = subprocess.run([os.path.join(BIN, "sssp_int"), "-s", "999999", os.path.join(GOLDEN, "rmat8_int.gr")],
                        capture_output=True, text=True, timeout=120)
