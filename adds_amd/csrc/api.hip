@@ -1632,23 +1632,45 @@ extern "C" adds_status adds_sssp_noalloc(adds_graph* g, uint32_t src, void* dist
 
 // ---- cross-XCD ordering litmus (diagnostic) ----
 extern "C" adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint64_t* forbidden,
-                                   uint64_t* lost) {
+                                   uint64_t* lost, uint64_t* cross_xcd_pairs) {
   if (!pairs || !rounds || variant > 3 || !forbidden || !lost) return ADDS_EINVAL;
   int cnt = 0;
   if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return ADDS_ENODEV;
   uint32_t* mem = nullptr;
+  uint32_t* xcc = nullptr;
   unsigned long long* out = nullptr;
   HIPCHK(hipMalloc(&mem, (size_t)pairs * 128 * 4));
+  HIPCHK(hipMalloc(&xcc, (size_t)pairs * 2 * 4));
   HIPCHK(hipMalloc(&out, 16));
   HIPCHK(launch_fill_u32(mem, (uint64_t)pairs * 128, 0, nullptr));
   for (uint32_t p = 0; p < pairs; p++) HIPCHK(hipMemset(mem + (size_t)p * 128 + 96, 0xFF, 4));
   HIPCHK(hipMemset(out, 0, 16));
-  HIPCHK(launch_litmus(mem, pairs, rounds, variant, out, nullptr));
+  HIPCHK(launch_litmus(mem, pairs, rounds, variant, out, xcc, nullptr));
   unsigned long long h[2] = {0, 0};
+  std::vector<uint32_t> hx((size_t)pairs * 2);
   HIPCHK(hipMemcpy(h, out, 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hx.data(), xcc, hx.size() * 4, hipMemcpyDeviceToHost));
   (void)hipFree(mem);
+  (void)hipFree(xcc);
   (void)hipFree(out);
   *forbidden = h[0];
   *lost = h[1];
+  if (cross_xcd_pairs) {
+    uint64_t c = 0;
+    for (uint32_t p = 0; p < pairs; p++) c += hx[2 * p] != hx[2 * p + 1];
+    *cross_xcd_pairs = c;
+  }
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_xcd_map(uint32_t workgroups, uint32_t threads, uint32_t* xcc) {
+  if (!workgroups || !threads || threads > 1024 || !xcc) return ADDS_EINVAL;
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return ADDS_ENODEV;
+  uint32_t* d = nullptr;
+  HIPCHK(hipMalloc(&d, (size_t)workgroups * 4));
+  HIPCHK(launch_xcd_map(d, workgroups, threads, nullptr));
+  HIPCHK(hipMemcpy(xcc, d, (size_t)workgroups * 4, hipMemcpyDeviceToHost));
+  (void)hipFree(d);
   return ADDS_OK;
 }

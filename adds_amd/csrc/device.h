@@ -136,7 +136,8 @@ constexpr uint64_t split_aux_words(uint64_t m) { return m / 4096 + 2; }
 
 // Litmus (litmus.hip)
 hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_t variant, unsigned long long* out,
-                         hipStream_t s);
+                         uint32_t* xcc, hipStream_t s);
+hipError_t launch_xcd_map(uint32_t* xcc, uint32_t workgroups, uint32_t threads, hipStream_t s);
 
 // Backward (backward_kernels.hip)
 struct BwdArgs {

@@ -1,6 +1,8 @@
 // Cross-XCD message-passing litmus test for the memory-ordering recipe the solver relies on.
 //
-// Workgroup pairs (2p, 2p+1) run on different XCDs (workgroups are dealt round-robin over the 8 XCDs).
+// Workgroup pairs (2p, 2p+1) are meant to run on different XCDs (workgroups are dealt round-robin over the
+// 8 XCDs); every workgroup records the XCD it ran on (the XCC_ID hardware register), so the host counts the
+// pairs that really crossed XCDs instead of assuming it.
 // In every round the producer
 //   1. writes a payload word with a relaxed agent-scope store        (like a worklist entry / adjoint),
 //   2. waits for its completion (workgroup-scope fence = s_waitcnt vmcnt(0)),
@@ -28,9 +30,13 @@ __device__ __forceinline__ void lwait() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// The XCD (XCC) this wavefront runs on: hardware register HW_REG_XCC_ID, bits [3:0].
+__device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xFu; }
+
 // words per pair: [0] payload, [32] flag, [64] ack, [96] dist   (each on its own 128-byte line)
-__global__ void k_litmus(uint32_t* mem, uint32_t rounds, uint32_t variant, unsigned long long* out) {
+__global__ void k_litmus(uint32_t* mem, uint32_t rounds, uint32_t variant, unsigned long long* out, uint32_t* xcc) {
   if (threadIdx.x != 0) return;
+  xcc[blockIdx.x] = xcc_id();
   const uint32_t pair = blockIdx.x >> 1;
   const bool producer = (blockIdx.x & 1) == 0;
   uint32_t* w = mem + (size_t)pair * 128;
@@ -73,8 +79,17 @@ __global__ void k_litmus(uint32_t* mem, uint32_t rounds, uint32_t variant, unsig
 }
 
 hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_t variant, unsigned long long* out,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_litmus, dim3(2 * pairs), dim3(64), 0, s, mem, rounds, variant, out);
+                         uint32_t* xcc, hipStream_t s) {
+  hipLaunchKernelGGL(k_litmus, dim3(2 * pairs), dim3(64), 0, s, mem, rounds, variant, out, xcc);
+  return hipGetLastError();
+}
+
+// XCD of every workgroup of a launch of the given shape (the solver's: workgroups_per_cu x CUs, 256 threads).
+__global__ void k_xcd_map(uint32_t* xcc) {
+  if (threadIdx.x == 0) xcc[blockIdx.x] = xcc_id();
+}
+hipError_t launch_xcd_map(uint32_t* xcc, uint32_t workgroups, uint32_t threads, hipStream_t s) {
+  hipLaunchKernelGGL(k_xcd_map, dim3(workgroups), dim3(threads), 0, s, xcc);
   return hipGetLastError();
 }
 

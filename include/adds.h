@@ -221,11 +221,17 @@ adds_status adds_autotune_batched(adds_graph* const* graphs, const uint32_t* src
 
 /* Diagnostic with no reference counterpart: cross-XCD message-passing litmus test of the ordering recipe
  * the solver uses (relaxed agent-scope atomics ordered by waiting for completion).  `pairs` producer /
- * consumer workgroup pairs on different XCDs run `rounds` rounds; variant 0 publishes with a store,
- * 1 with a fetch_add, 2 checks atomicMin visibility, 3 is variant 0 without the wait (a control: it may
- * show reordering).  *forbidden counts stale observations (must be 0 for variants 0-2),
- * *lost counts rounds whose bounded wait expired. */
-adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint64_t* forbidden, uint64_t* lost);
+ * consumer workgroup pairs run `rounds` rounds; variant 0 publishes with a store, 1 with a fetch_add,
+ * 2 checks atomicMin visibility, 3 is variant 0 without the wait (a control: it may show reordering).
+ * *forbidden counts stale observations (must be 0 for variants 0-2), *lost counts rounds whose bounded
+ * wait expired, *cross_xcd_pairs (may be NULL) the pairs whose two workgroups ran on different XCDs, read
+ * from the XCC_ID hardware register of each workgroup. */
+adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint64_t* forbidden, uint64_t* lost,
+                        uint64_t* cross_xcd_pairs);
+/* Diagnostic: the XCD (XCC_ID register) each workgroup of a launch of `workgroups` x `threads` ran on,
+ * into xcc (host, `workgroups` words).  The solver groups its termination words by workgroup index mod 8,
+ * which is the XCD under round-robin dispatch; this shows the placement. */
+adds_status adds_xcd_map(uint32_t workgroups, uint32_t threads, uint32_t* xcc);
 
 /* ---- backward (float graphs) ----
  * No reference counterpart (the reference keeps no predecessor tree: ads_float/kernel.cu:82).  Given

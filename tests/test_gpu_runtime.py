@@ -14,21 +14,54 @@ from adds_amd import _lib as L
 pytestmark = pytest.mark.gpu
 
 
+def _xcd_count():
+    """XCDs seen by one wide launch (8 on a whole MI355X in SPX mode)."""
+    n = 4096
+    xcc = (ctypes.c_uint32 * n)()
+    L.check(L._lib().adds_xcd_map(n, 64, xcc), "xcd map")
+    return len(set(xcc))
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2])
 def test_cross_xcd_litmus(variant):
-    f, lost = ctypes.c_uint64(), ctypes.c_uint64()
-    L.check(L._lib().adds_litmus(256, 2000, variant, ctypes.byref(f), ctypes.byref(lost)), "litmus")
+    """Every producer/consumer pair really runs on two different XCDs (read from each workgroup's
+    XCC_ID register), and no consumer sees a stale payload after the flag."""
+    f, lost, cross = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    L.check(L._lib().adds_litmus(256, 2000, variant, ctypes.byref(f), ctypes.byref(lost), ctypes.byref(cross)),
+            "litmus")
     assert lost.value == 0
     assert f.value == 0, f"{f.value} forbidden outcomes"
+    if _xcd_count() > 1:
+        assert cross.value == 256, f"only {cross.value} of 256 pairs crossed XCDs"
 
 
 def test_litmus_control_without_wait():
     """Variant 3 drops the completion wait; it is a control that reports how often a stale payload is
     seen without it (no assertion: the hardware may or may not reorder in a given run)."""
-    f, lost = ctypes.c_uint64(), ctypes.c_uint64()
-    L.check(L._lib().adds_litmus(256, 2000, 3, ctypes.byref(f), ctypes.byref(lost)), "litmus")
+    f, lost, cross = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    L.check(L._lib().adds_litmus(256, 2000, 3, ctypes.byref(f), ctypes.byref(lost), ctypes.byref(cross)), "litmus")
     assert lost.value == 0
-    print(f"litmus without the wait: {f.value} stale payloads in {256 * 2000} rounds")
+    print(f"litmus without the wait: {f.value} stale payloads in {256 * 2000} rounds, {cross.value} pairs cross-XCD")
+
+
+@pytest.mark.parametrize("wg_per_cu", [1, 4])
+def test_solver_grid_xcd_placement(wg_per_cu):
+    """The solver's persistent grid (workgroups_per_cu x CUs workgroups of 256 threads) is dealt
+    round-robin over the XCDs: workgroup b runs on the same XCD as workgroup b % nx, and the first nx
+    workgroups cover nx different XCDs.  The termination words are grouped by b % 8 on that basis, so
+    each group's waves share one XCD's L2."""
+    cu = L._lib().adds_device_cu_count()
+    grid = cu * wg_per_cu
+    xcc = (ctypes.c_uint32 * grid)()
+    L.check(L._lib().adds_xcd_map(grid, 256, xcc), "xcd map")
+    ids = list(xcc)
+    nx = len(set(ids))
+    print(f"{grid} workgroups on {nx} XCDs; first 16: {ids[:16]}")
+    assert nx in (1, 2, 4, 8)
+    assert len(set(ids[:nx])) == nx
+    assert all(ids[b] == ids[b % nx] for b in range(grid)), "placement is not round-robin"
+    if nx > 1:
+        assert 8 % nx == 0  # b % 8 groups then never straddle two XCDs
 
 
 def test_graph_replays_match_eager():
