@@ -1,6 +1,6 @@
 /* delta_step.c -- parallel CPU delta-stepping SSSP: the reference's second CPU comparison,
  * Galois "-algo=deltaStep -t <threads>" (run_all.sh:77-91, NUM_THREADS = hardware threads), restated.
- * Test / baseline infrastructure only: bench.py's cpu_baseline leg and tests may use it; the product
+ * A CPU baseline: run_all.py's ds leg and the benchmark's parallel CPU leg time it; the GPU library
  * never links it.
  *
  * Galois' delta semantics (cpu_graph_list_* column 3): int bucket = dist >> shift
@@ -37,6 +37,11 @@ static void vpush(vec* x, uint32_t v, uint32_t d) {
 }
 
 static float bf(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+/* int sums saturate at 2^32-2, one below the unreached marker, as the GPU solver's do */
+static uint32_t sat_add(uint32_t d, uint32_t w) {
+  uint64_t s = (uint64_t)d + w;
+  return s >= INF_U32 ? INF_U32 - 1 : (uint32_t)s;
+}
 static uint32_t fb(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
 
 typedef struct shared_s {
@@ -123,7 +128,7 @@ static void* worker(void* p) {
       if (__atomic_load_n(&S->dist[u], __ATOMIC_RELAXED) < du) continue; /* stale */
       t->work++; /* Galois WORK_COUNT (cpu_sssp_int.patch:144): requests processed, stale ones excluded */
       for (uint32_t e = S->row[u]; e < S->row[u + 1]; e++) {
-        uint32_t nd = S->is_float ? fb(bf(du) + bf(S->w[e])) : du + S->w[e];
+        uint32_t nd = S->is_float ? fb(bf(du) + bf(S->w[e])) : sat_add(du, S->w[e]);
         relax_min(S, mine, S->col[e], nd);
       }
     }
@@ -133,7 +138,7 @@ static void* worker(void* p) {
 
 /* Returns 0 on success.  delta > 0 (distance units).  *work (may be null) receives the number of
  * non-stale requests processed, the count Galois reports as total_work. */
-int oracle_delta_step(int is_float, uint32_t n, const uint32_t* row, const uint32_t* col, const uint32_t* w,
+int cpu_delta_step(int is_float, uint32_t n, const uint32_t* row, const uint32_t* col, const uint32_t* w,
                       uint32_t src, double delta, unsigned nthreads, uint32_t* dist, uint64_t* work) {
   if (nthreads < 1) nthreads = 1;
   shared_t S;

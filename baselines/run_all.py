@@ -1,6 +1,6 @@
 """The evaluation driver of the reference (run_all.sh + verify_against_*), restated over this build.
 
-    python tests/run_all.py --inputs-int <dir> --inputs-float <dir> [--lists <dir>] [--threads N]
+    python baselines/run_all.py --inputs-int <dir> --inputs-float <dir> [--lists <dir>] [--threads N]
                             [--legs ads,nf,dj,ds] [--workdir <dir>]
 
 For each weight type it runs, like run_all.sh:
@@ -12,8 +12,8 @@ For each weight type it runs, like run_all.sh:
   * ds_{int,float}: parallel delta-step on --threads threads with the list's third column as Galois'
     -delta (run_ds, run_all.sh:78-92);
 each program appending "<graph> <ms> <work>" to fd 3, redirected to <impl>_result.  The CPU legs run
-oracle/oracle_sssp_cpu (the CPU oracle behind Galois sssp-cpu's command line), which is why this driver
-lives with the tests.  Then it checks ads against dj, ds and nf with adds_amd/bin/adds_verify (verify.py
+baselines/sssp_cpu (Galois sssp-cpu's command line over this directory's CPU baselines).  Then it checks
+ads against dj, ds and nf with adds_amd/bin/adds_verify (verify.py
 semantics; verify_against_dj, verify_against_ds, verify_against_nf) and aggregates the fd-3 files into one
 table per type (mean ms and work per graph and program, ads speed-up over each other leg), also written
 to run_all_summary.json.  Exit status 1 when any comparison mismatches.
@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SSSP_BIN = {t: os.path.join(ROOT, "adds_amd", "bin", f"sssp_{t}") for t in ("int", "float")}
 NF_BIN = {t: os.path.join(ROOT, "adds_amd", "bin", f"sssp_nf_{t}") for t in ("int", "float")}
 VERIFY_BIN = os.path.join(ROOT, "adds_amd", "bin", "adds_verify")
-CPU_BIN = os.path.join(ROOT, "oracle", "oracle_sssp_cpu")
+CPU_BIN = os.path.join(ROOT, "baselines", "sssp_cpu")
 
 
 def read_list(path):

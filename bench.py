@@ -17,7 +17,8 @@ Workloads: adds_amd/workloads.py.  Solver parameters: the pinned entries of adds
 roofline: HBM-bound; algorithmic bytes per solve B = 4n + 8 n_r + 12 m_r (backward: 4m + 20 k n_r);
 achieved = B / (mean persistent-kernel time from HIP events on the solve stream); peak 8 TB/s.
 cpu_baseline: the oracle's single-thread Dijkstra on the same graph and source (kind "port"); the
-parallel delta-stepping leg runs on the host's CPU share (OMP_NUM_THREADS, else the affinity mask).
+parallel delta-stepping leg (baselines/, Galois -algo=deltaStep restated) runs on the host's CPU share
+(OMP_NUM_THREADS, else the affinity mask).
 """
 import argparse
 import hashlib
@@ -161,7 +162,8 @@ def cpu_single(host, src, m_r, oracle, hinfo, delta):
     dj = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)
     tdj = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ds = oracle.delta_step(host.row_ptr, host.col, host.w, src, host.is_float, delta, hinfo["threads"])
+    import baselines
+    ds, _ = baselines.delta_step(host.row_ptr, host.col, host.w, src, host.is_float, delta, hinfo["threads"])
     tds = time.perf_counter() - t0
     out = {"dijkstra_1thread": {"s": tdj, "gteps": m_r / tdj / 1e9},
            "delta_step": {"s": tds, "gteps": m_r / tds / 1e9, "threads": hinfo["threads"], "delta": delta,

@@ -29,8 +29,6 @@ def _lib():
         L.oracle_tree_grad.argtypes = [u32, p, p, p, u32, p, p, p, p, p, p]
         L.oracle_grad_scale.restype = ctypes.c_int
         L.oracle_grad_scale.argtypes = [u32, u32, p, p, p]
-        L.oracle_delta_step.restype = ctypes.c_int
-        L.oracle_delta_step.argtypes = [ctypes.c_int, u32, p, p, p, u32, ctypes.c_double, ctypes.c_uint, p, p]
         _L = L
     return _L
 
@@ -69,17 +67,6 @@ def bellman_ford(row_ptr, col, w, srcs, is_float, saturate=False):
     if _lib().oracle_bellman_ford(int(is_float), n, _p(row_ptr), _p(col), _p(wb), _p(srcs), len(srcs), _p(d)) \
             and not saturate:
         raise OverflowError("integer distance overflow")
-    return d
-
-
-def delta_step(row_ptr, col, w, src, is_float, delta, threads=None):
-    """Parallel delta-stepping (all host threads by default); distance words like dijkstra()."""
-    row_ptr, col, wb = _prep(row_ptr, col, w)
-    n = len(row_ptr) - 1
-    d = np.empty(n, dtype=np.uint32)
-    threads = threads or os.cpu_count() or 1
-    _lib().oracle_delta_step(int(is_float), n, _p(row_ptr), _p(col), _p(wb), int(src), float(delta), threads, _p(d),
-                             None)
     return d
 
 

@@ -12,8 +12,10 @@
  *      binary-heap Dijkstra with the same arithmetic.  Galois itself is an un-vendored dependency
  *      (IntelligentSoftwareSystems/Galois @ 38cd91cfb59a30cf0b4f7cf7d19f29d4d7188548, cloned by
  *      build_all.sh:45-65) and is not available here.
- *  - Galois .gr v1 parsing (csr_graph.cu:236-281) and the distance text (support.cu:78-96,
- *      cpu_sssp_float.patch:241-248).
+ *  - the tree gradient of the differentiable backward (no reference counterpart: ads_float/kernel.cu:82
+ *      stores no predecessor), defined in oracle_tree_grad below.
+ * The CPU comparison legs (Galois Dijkstra -t 1 and delta-step -t NUM_THREADS) live in baselines/, not
+ * here: this file is the checker only.
  *
  * Distances are 32-bit words: uint32 (INF 0xFFFFFFFF) or IEEE binary32 bits (INF = +inf, 0x7F800000).
  * Non-negative floats order like their bit patterns, so one heap serves both.
@@ -294,44 +296,4 @@ int oracle_tree_grad(uint32_t n, const uint32_t* row, const uint32_t* col, const
     for (uint64_t e = 0; e < m; e++) gw[e] = ldexpf((float)GW[e], -S);
   free(rrow); free(reid); free(rsrc); free(cur); free(pe); free(rnd); free(A); free(order); free(GW);
   return rc;
-}
-
-/* ---- .gr v1 reader (csr_graph.cu:236-281) ----
- * Returns 0 on success; the arrays are malloc'd (free with oracle_free). */
-int oracle_read_gr(const char* path, uint32_t* n_out, uint64_t* m_out, uint32_t** row_out, uint32_t** col_out,
-                   uint32_t** w_out) {
-  FILE* f = fopen(path, "rb");
-  if (!f) return 1;
-  uint64_t hdr[4];
-  if (fread(hdr, 8, 4, f) != 4 || hdr[0] != 1) { fclose(f); return 2; }
-  uint64_t nn = hdr[2], ne = hdr[3], esz = hdr[1];
-  uint32_t* row = (uint32_t*)malloc(sizeof(uint32_t) * (nn + 1));
-  uint32_t* col = (uint32_t*)malloc(sizeof(uint32_t) * (ne ? ne : 1));
-  uint32_t* w = (uint32_t*)calloc(ne ? ne : 1, sizeof(uint32_t));
-  uint64_t* idx = (uint64_t*)malloc(sizeof(uint64_t) * (nn ? nn : 1));
-  int rc = 0;
-  if (fread(idx, 8, nn, f) != nn) rc = 3;
-  if (!rc && fread(col, 4, ne, f) != ne) rc = 3;
-  if (!rc && (ne & 1)) { uint32_t pad; if (fread(&pad, 4, 1, f) != 1) rc = 3; }
-  if (!rc && esz && fread(w, 4, ne, f) != ne) rc = 3;
-  fclose(f);
-  row[0] = 0;
-  for (uint64_t i = 0; i < nn && !rc; i++) row[i + 1] = (uint32_t)idx[i];
-  free(idx);
-  if (rc) { free(row); free(col); free(w); return rc; }
-  *n_out = (uint32_t)nn; *m_out = ne; *row_out = row; *col_out = col; *w_out = w;
-  return 0;
-}
-void oracle_free(void* p) { free(p); }
-
-/* ---- distance text: "%d %u" / "%d %f" / "%d INF" ---- */
-int oracle_write_dist(const char* path, int is_float, uint32_t n, const uint32_t* dist) {
-  FILE* f = fopen(path, "w");
-  if (!f) return 1;
-  for (uint32_t i = 0; i < n; i++) {
-    if (dist[i] == (is_float ? INF_F32 : INF_U32)) fprintf(f, "%d INF\n", (int)i);
-    else if (is_float) fprintf(f, "%d %f\n", (int)i, (double)bits_f(dist[i]));
-    else fprintf(f, "%d %u\n", (int)i, dist[i]);
-  }
-  return fclose(f) ? 1 : 0;
 }

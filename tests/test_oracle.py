@@ -346,14 +346,3 @@ def test_tree_grad_c5_fixture_sources():
         want = fx["fd"][sel]
         err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3)
         assert err.max() <= 1e-3, (s, err.max())
-
-
-@pytest.mark.parametrize("is_float,delta", [(False, 8.0), (False, 1.0), (True, 0.25), (True, 64.0)])
-def test_parallel_delta_step_equals_dijkstra(is_float, delta):
-    """The CPU delta-stepping baseline (Galois -algo=deltaStep restated) reaches the same distances."""
-    rng = np.random.default_rng(17)
-    row, col, w = rand_graph(rng, 3000, 24000, is_float=is_float, zero_ok=True)
-    for src in (0, 1234):
-        d = oracle.dijkstra(row, col, w, src, is_float)
-        for threads in (1, 4):
-            assert np.array_equal(oracle.delta_step(row, col, w, src, is_float, delta, threads), d)

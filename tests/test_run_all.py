@@ -1,4 +1,4 @@
-"""tests/run_all.py, the reference's evaluation driver (run_all.sh + verify_against_*) over this build: the
+"""baselines/run_all.py, the reference's evaluation driver (run_all.sh + verify_against_*) over this build: the
 CPU legs (Dijkstra on one thread, parallel delta-step with Galois' -delta) on the golden fixtures, their
 output files and fd-3 lines; on the GPU, the whole driver with the drop-in binaries and the verifier."""
 import json
@@ -10,8 +10,8 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
-sys.path.insert(0, HERE)
-import run_all  # noqa: E402
+sys.path.insert(0, os.path.dirname(HERE))
+from baselines import run_all  # noqa: E402
 
 DELTA = {"int": "3", "float": "0.25"}  # Galois -delta: shift 3 (bucket width 8) / divisor 0.25
 
@@ -99,7 +99,7 @@ def test_full_driver(tmp_path):
 
 
 def test_cpu_command_contract(tmp_path):
-    """oracle_sssp_cpu: usage errors exit 2; a missing output directory is reported (exit 1); the int -delta is
+    """sssp_cpu: usage errors exit 2; a missing output directory is reported (exit 1); the int -delta is
     a shift and the float -delta a divisor, and neither changes the distances."""
     import subprocess
     g = os.path.join(GOLD, "rmat8_int.gr")
