@@ -117,7 +117,7 @@ EXPORTS = [
     "adds_gen_rmat", "adds_gen_grid", "adds_gen_er", "adds_gen_weights", "adds_max_degree_vertex",
     "adds_sample_component", "adds_reached_counts", "adds_last_error_detail", "adds_autotune",
     "adds_autotune_batched",
-    "adds_litmus", "adds_xcd_map",
+    "adds_litmus", "adds_xcd_map", "adds_gather_ceiling",
 ]
 
 _LIB = None
@@ -171,6 +171,7 @@ def _lib():
         "adds_autotune_batched": (i32, [vp, c_u32p, u32, u32, pp, ctypes.POINTER(ctypes.c_double)]),
         "adds_litmus": (i32, [u32, u32, u32, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "adds_xcd_map": (i32, [u32, u32, c_u32p]),
+        "adds_gather_ceiling": (i32, [u32, u64, ctypes.POINTER(ctypes.c_double)]),
         "adds_host_csr_free": (None, [hp]),
         "adds_read_gr": (i32, [ctypes.c_char_p, i32, hp]),
         "adds_read_mtx": (i32, [ctypes.c_char_p, i32, hp]),

@@ -1674,3 +1674,46 @@ extern "C" adds_status adds_xcd_map(uint32_t workgroups, uint32_t threads, uint3
   (void)hipFree(d);
   return ADDS_OK;
 }
+
+extern "C" adds_status adds_gather_ceiling(uint32_t n, uint64_t m, double* gathers_per_s) {
+  if (!n || !m || m > (1ull << 31) || !gathers_per_s) return ADDS_EINVAL;
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return ADDS_ENODEV;
+  const uint32_t cu = std::max(1u, adds_device_cu_count());
+  uint2* cw = nullptr;
+  uint32_t* d = nullptr;
+  uint32_t* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  adds_status rc = ADDS_OK;
+  double best = 0.0;
+  do {
+    if (hipMalloc(&cw, m * sizeof(uint2)) != hipSuccess || hipMalloc(&d, (size_t)n * 4) != hipSuccess ||
+        hipMalloc(&out, (size_t)cu * 8 * 256 * 4) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || launch_gather_probe_init(cw, m, n, d, nullptr) != hipSuccess) {
+      rc = ADDS_EHIP;
+      break;
+    }
+    // one to eight workgroups of 256 threads per CU; the best rate over the shapes
+    for (uint32_t wpc = 1; wpc <= 8 && rc == ADDS_OK; wpc *= 2) {
+      const uint32_t grid = cu * wpc;
+      for (int r = 0; r < 2; r++) launch_gather_probe(cw, d, m, out, grid, nullptr);
+      const int R = 5;
+      (void)hipEventRecord(e0, nullptr);
+      for (int r = 0; r < R; r++) launch_gather_probe(cw, d, m, out, grid, nullptr);
+      (void)hipEventRecord(e1, nullptr);
+      float ms = 0.0f;
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.0f) {
+        rc = ADDS_EHIP;
+        break;
+      }
+      best = std::max(best, (double)m * R / (ms * 1e-3));
+    }
+  } while (false);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(cw);
+  (void)hipFree(d);
+  (void)hipFree(out);
+  if (rc == ADDS_OK) *gathers_per_s = best;
+  return rc;
+}

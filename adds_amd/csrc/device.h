@@ -138,6 +138,9 @@ constexpr uint64_t split_aux_words(uint64_t m) { return m / 4096 + 2; }
 hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_t variant, unsigned long long* out,
                          uint32_t* xcc, hipStream_t s);
 hipError_t launch_xcd_map(uint32_t* xcc, uint32_t workgroups, uint32_t threads, hipStream_t s);
+hipError_t launch_gather_probe_init(uint2* cw, uint64_t m, uint32_t n, uint32_t* d, hipStream_t s);
+hipError_t launch_gather_probe(const uint2* cw, const uint32_t* d, uint64_t m, uint32_t* out, uint32_t workgroups,
+                               hipStream_t s);
 
 // Backward (backward_kernels.hip)
 struct BwdArgs {

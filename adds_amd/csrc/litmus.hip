@@ -93,4 +93,52 @@ hipError_t launch_xcd_map(uint32_t* xcc, uint32_t workgroups, uint32_t threads, 
   return hipGetLastError();
 }
 
+// Random-gather ceiling: the memory pattern of the solver's edge windows without the solver.  Each
+// element i of a stream of m (col, w) pairs names a uniformly random vertex col < n (a 64-bit hash of
+// i); a thread loads GP_U pairs of its grid-stride sequence, then gathers d[col] for each and compares
+// it with w, so GP_U gathers per lane are in flight.  The rate this reaches for a given n is what a
+// kernel issuing one d[v] gather per edge can expect from the memory system when the targets have no
+// locality.
+__global__ void k_gather_probe_init(uint2* cw, uint64_t m, uint32_t n, uint32_t* d) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m; i += stride) {
+    uint64_t h = (i + 1) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 32;
+    cw[i] = make_uint2((uint32_t)(h % n), (uint32_t)(h >> 40));
+  }
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) d[i] = (uint32_t)i * 2654435761u;
+}
+constexpr int GP_U = 8;
+__global__ void __launch_bounds__(256) k_gather_probe(const uint2* __restrict__ cw, const uint32_t* __restrict__ d,
+                                                      uint64_t m, uint32_t* out) {
+  const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint32_t acc = 0;
+  for (uint64_t base = t; base < m; base += T * GP_U) {
+    uint64_t x[GP_U];
+    uint32_t y[GP_U];
+#pragma unroll
+    for (int j = 0; j < GP_U; j++) {
+      const uint64_t i = base + j * T;
+      x[j] = i < m ? __builtin_nontemporal_load((const uint64_t*)cw + i) : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < GP_U; j++) y[j] = d[(uint32_t)x[j]];
+#pragma unroll
+    for (int j = 0; j < GP_U; j++) acc += y[j] > (uint32_t)(x[j] >> 32);
+  }
+  out[t] = acc;
+}
+hipError_t launch_gather_probe_init(uint2* cw, uint64_t m, uint32_t n, uint32_t* d, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_probe_init, dim3(4096), dim3(256), 0, s, cw, m, n, d);
+  return hipGetLastError();
+}
+hipError_t launch_gather_probe(const uint2* cw, const uint32_t* d, uint64_t m, uint32_t* out, uint32_t workgroups,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_probe, dim3(workgroups), dim3(256), 0, s, cw, d, m, out);
+  return hipGetLastError();
+}
+
 }  // namespace adds

@@ -71,6 +71,18 @@ def roofline_entry(bytes_, ms, note=None):
     return r
 
 
+def gather_ceiling(adds_amd, n, edges_scanned, kernel_ms):
+    """The device's random-gather rate for an n-word distance array (adds_gather_ceiling: 2^28 uniformly
+    random targets, no locality) beside the solve's edge rate (edges scanned, one d[v] gather each, per
+    second of kernel time)."""
+    import ctypes
+    g = ctypes.c_double()
+    adds_amd._lib.check(adds_amd._lib._lib().adds_gather_ceiling(n, 1 << 28, ctypes.byref(g)), "gather ceiling")
+    rate = edges_scanned / (kernel_ms / 1e3)
+    return {"random_gathers_per_s": g.value, "edges_scanned": int(edges_scanned), "edge_gathers_per_s": rate,
+            "edge_rate_over_ceiling": rate / g.value if g.value else None}
+
+
 def kernel_source_sha():
     h = hashlib.sha256()
     for p in KERNEL_SOURCES:
@@ -200,6 +212,8 @@ def run_c3(host, src, name, steps, warmup, torch, adds_amd, oracle, prm, do_cpu,
                                            "stale_skips", "hub_tasks", "rebuckets")},
     }
     res["roofline"] = roofline_entry(res["bytes_alg"], kms)
+    res["gather_ceiling"] = gather_ceiling(adds_amd, host.n, res["work"]["edges_scanned"], kms)
+    log(f"{name}: random-gather ceiling {res['gather_ceiling']}")
     # the same solve with the distances copied back to host memory (the CLI's path): PCIe-inclusive
     hw = []
     for _ in range(3):
@@ -522,7 +536,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": f"k_sssp<false, false, {head['params'].get('edge_windows') or 4}>",
                      "kernel_ms": head["kernel_ms_event"], "kernel_source_sha": sha,
-                     "bytes_alg_per_launch": head["bytes_alg"]},
+                     "bytes_alg_per_launch": head["bytes_alg"],
+                     "random_gather_ceiling": head["gather_ceiling"]},
         "cpu_baseline": cpu,
         "extras": dict(extras, c3_int={k: head[k] for k in ("params", "kernel_ms_event", "kernel_ms_min",
                                                              "solve_ms_event", "gteps_kernel", "host_out_ms",

@@ -232,6 +232,12 @@ adds_status adds_litmus(uint32_t pairs, uint32_t rounds, uint32_t variant, uint6
  * into xcc (host, `workgroups` words).  The solver groups its termination words by workgroup index mod 8,
  * which is the XCD under round-robin dispatch; this shows the placement. */
 adds_status adds_xcd_map(uint32_t workgroups, uint32_t threads, uint32_t* xcc);
+/* Diagnostic: the random-gather ceiling of the device for a distance array of n words.  A stream of m
+ * (m <= 2^31) 8-byte (col, w) pairs with uniformly random col < n is read once and d[col] gathered for
+ * every pair, with 8 gathers per lane in flight, at one to eight 256-thread workgroups per CU;
+ * *gathers_per_s receives the best rate.  The solver's edge windows issue the same pattern (one d[v]
+ * gather per edge), so this bounds their edge rate on graphs whose targets have no locality. */
+adds_status adds_gather_ceiling(uint32_t n, uint64_t m, double* gathers_per_s);
 
 /* ---- backward (float graphs) ----
  * No reference counterpart (the reference keeps no predecessor tree: ads_float/kernel.cu:82).  Given

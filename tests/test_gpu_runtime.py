@@ -210,3 +210,14 @@ def test_autotune_batched_keeps_results():
     dists, _ = adds_amd.sssp_batched(dgs, srcs, **params)
     for h, d, s in zip(hosts, dists, srcs):
         assert np.array_equal(d.view(np.uint32), oracle.dijkstra(h.row_ptr, h.col, h.w, s, False))
+
+
+def test_gather_ceiling_reports_a_rate():
+    """The random-gather diagnostic returns a plausible rate, higher for a distance array that fits the
+    caches than for one that does not, and refuses empty shapes."""
+    small, large = ctypes.c_double(), ctypes.c_double()
+    L.check(L._lib().adds_gather_ceiling(1 << 16, 1 << 24, ctypes.byref(small)), "gather ceiling")
+    L.check(L._lib().adds_gather_ceiling(1 << 24, 1 << 24, ctypes.byref(large)), "gather ceiling")
+    print(f"random gathers/s: {small.value:.3e} (256 KiB array), {large.value:.3e} (64 MiB array)")
+    assert 1e9 < large.value < small.value < 1e13
+    assert L._lib().adds_gather_ceiling(0, 1 << 20, ctypes.byref(small)) == L.EINVAL
