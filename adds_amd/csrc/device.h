@@ -115,6 +115,24 @@ struct KArgs {
   uint32_t trace_slice;  // wall-clock ticks per histogram bin
 };
 
+// Inclusive prefix sum over the 64 lanes of a wavefront (every lane active), with DPP: row shifts by
+// 1, 2, 4 and 8 scan each row of 16 lanes (lanes shifted in from outside the row add 0), then
+// row_bcast:15 adds the last lane of rows 0 and 2 to rows 1 and 3, and row_bcast:31 adds lane 31 to
+// rows 2 and 3.  Each step compiles to one v_add_u32_dpp.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_or_zero(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xF, true);
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  x += dpp_or_zero<0x111, 0xF>(x);  // row_shr:1
+  x += dpp_or_zero<0x112, 0xF>(x);  // row_shr:2
+  x += dpp_or_zero<0x114, 0xF>(x);  // row_shr:4
+  x += dpp_or_zero<0x118, 0xF>(x);  // row_shr:8
+  x += dpp_or_zero<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_or_zero<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+  return x;
+}
+
 // Mean sampled weight of a graph (the statistic the reference's profiler_kernel computes), api.hip.
 }  // namespace adds
 struct adds_graph;

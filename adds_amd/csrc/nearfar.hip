@@ -209,11 +209,7 @@ __global__ void __launch_bounds__(64 * NF_WAVES) k_nf_items(NfArgs a) {
       for (uint32_t c = 0; c < nt; c++) a.tasks[t0 + c] = make_uint2(u, c);
     }
     const uint32_t d = big ? 0u : deg;
-    uint32_t incl = d;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(incl, o, 64);
-      if (lane >= (uint32_t)o) incl += y;
-    }
+    const uint32_t incl = wave_incl_scan(d);
     const uint32_t total = __shfl(incl, 63, 64);
     s_pref[wv][lane] = incl - d;
     s_rs[wv][lane] = rs;
