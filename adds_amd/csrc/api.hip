@@ -1696,11 +1696,16 @@ extern "C" adds_status adds_gather_ceiling(uint32_t n, uint64_t m, double* gathe
     // one to eight workgroups of 256 threads per CU; the best rate over the shapes
     for (uint32_t wpc = 1; wpc <= 8 && rc == ADDS_OK; wpc *= 2) {
       const uint32_t grid = cu * wpc;
-      for (int r = 0; r < 2; r++) launch_gather_probe(cw, d, m, out, grid, nullptr);
       const int R = 5;
+      bool ok = true;
+      for (int r = 0; r < 2; r++) ok = ok && launch_gather_probe(cw, d, m, out, grid, nullptr) == hipSuccess;
       (void)hipEventRecord(e0, nullptr);
-      for (int r = 0; r < R; r++) launch_gather_probe(cw, d, m, out, grid, nullptr);
+      for (int r = 0; r < R; r++) ok = ok && launch_gather_probe(cw, d, m, out, grid, nullptr) == hipSuccess;
       (void)hipEventRecord(e1, nullptr);
+      if (!ok) {
+        rc = ADDS_EHIP;
+        break;
+      }
       float ms = 0.0f;
       if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.0f) {
         rc = ADDS_EHIP;

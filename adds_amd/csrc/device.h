@@ -150,7 +150,12 @@ hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
 // (col, w) pairs, with per-row light counts.  aux: split_aux_words(m) words of scratch (long rows).
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
                         bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, uint4* rinfo, hipStream_t s);
-constexpr uint64_t split_aux_words(uint64_t m) { return m / 4096 + 2; }
+// Rows longer than SPLIT_BIG edges are split by one 1024-thread workgroup each (k_split_big); the list
+// of such rows needs at most m / SPLIT_BIG entries, after SPLIT_AUX_ROWS words: the list's length and
+// the work counters of the two split kernels.
+constexpr uint32_t SPLIT_BIG = 1024;
+constexpr uint32_t SPLIT_AUX_ROWS = 3;
+constexpr uint64_t split_aux_words(uint64_t m) { return m / SPLIT_BIG + SPLIT_AUX_ROWS + 1; }
 
 // Litmus (litmus.hip)
 hipError_t launch_litmus(uint32_t* mem, uint32_t pairs, uint32_t rounds, uint32_t variant, unsigned long long* out,

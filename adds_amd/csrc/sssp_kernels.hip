@@ -1483,10 +1483,11 @@ __global__ void k_cleanup(KArgs a) {
 }
 
 // Stable per-row partition of (col, w) into light (w <= delta) then heavy edges.  Rows of up to
-// SPLIT_BIG edges: one wavefront per row.  Longer rows are appended to a list (aux[0] = count,
-// aux[1..] = rows) and partitioned by k_split_big, one 1024-thread workgroup per row, so that the
-// few very long rows of a power-law graph do not serialise on one wavefront.
-constexpr uint32_t SPLIT_BIG = 4096;
+// SPLIT_BIG edges: k_split, 64 rows per wavefront.  Longer rows are appended to a list (aux[0] =
+// count, aux[SPLIT_AUX_ROWS..] = rows) and partitioned by k_split_big, one 1024-thread workgroup per
+// row, so that
+// the long rows of a power-law graph (on RMAT-24, rows above 1024 edges hold over half the edges) do
+// not serialise on one wavefront.
 
 __device__ __forceinline__ bool is_light(uint32_t x, float delta, uint32_t idelta, int is_float) {
   return is_float ? __uint_as_float(x) <= delta : x <= idelta;
@@ -1495,42 +1496,141 @@ __device__ __forceinline__ uint32_t int_delta(float delta) {
   return delta >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)delta;  // int weights: w <= floor(delta)
 }
 
-__global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        int is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux) {
+// Rows of up to SPLIT_BIG edges, 64 consecutive rows per wavefront at a time: their edges are
+// flattened (row k owns [off[k], off[k+1]) of the order) and read in 64-edge windows, four in flight.
+// Pass 1 keeps each window's light mask in LDS; a scan of the masks' popcounts gives the light edges
+// before any flattened position, so an edge's rank among its row's light (or heavy) edges is a
+// difference of two such counts.  Pass 2 writes every edge to its place.  The result is the same stable
+// partition as one wavefront per row, with a few round trips per 64 rows instead of several per row.
+// When the 64 rows hold more than SPLIT_CAP edges they are taken as consecutive sub-groups of rows that
+// each fit (a row of at most SPLIT_BIG <= SPLIT_CAP edges always does).
+constexpr uint32_t SPLIT_CAP = 16384;
+constexpr uint32_t SPLIT_WIN = SPLIT_CAP / 64;
+static_assert(SPLIT_BIG <= SPLIT_CAP, "a row must fit one sub-group");
+__global__ void __launch_bounds__(256) k_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w,
+                                               uint32_t n, float delta, int is_float, uint2* cw2, uint32_t* lcnt,
+                                               uint32_t* aux) {
+  __shared__ uint32_t s_off[WAVES_PER_WG][65];
+  __shared__ uint32_t s_rs[WAVES_PER_WG][64], s_ls[WAVES_PER_WG][64], s_lc[WAVES_PER_WG][64];
+  __shared__ uint64_t s_mask[WAVES_PER_WG][SPLIT_WIN + 1];
+  __shared__ uint32_t s_cb[WAVES_PER_WG][SPLIT_WIN + 1];
   const uint32_t lane = lane_id();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t wv = threadIdx.x >> 6;
   const uint32_t idelta = int_delta(delta);
-  for (uint32_t v = wave; v < n; v += nwaves) {
-    const uint32_t rs = row_ptr[v], re = row_ptr[v + 1];
-    if (re - rs > SPLIT_BIG) {
-      if (lane == 0) aux[1 + afadd(aux, 1u)] = v;
-      continue;
+  uint32_t* off = s_off[wv];
+  uint64_t* mask = s_mask[wv];
+  uint32_t* cb = s_cb[wv];
+  // light edges before flattened position i (i <= the sub-group's edges)
+  auto before = [&](uint32_t i) -> uint32_t {
+    const uint32_t j = i >> 6, b = i & 63;
+    return cb[j] + (uint32_t)__popcll(b ? (mask[j] & ((1ull << b) - 1)) : 0ull);
+  };
+  auto locate = [&](uint32_t i) -> uint32_t {  // last row k with off[k] <= i
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1)
+      if (off[lo + step] <= i) lo += step;
+    return lo;
+  };
+  // groups of 64 rows are handed out by a counter (aux[1]), so that the dense groups of a power-law
+  // graph's low ids do not pile up on the wavefronts that a static stride would give them to
+  for (;;) {
+    uint32_t grp = 0;
+    if (lane == 0) grp = afadd(aux + 1, 1u);
+    const uint64_t base = (uint64_t)rdlane(grp, 0) * 64;
+    if (base >= n) break;
+    const uint32_t v = (uint32_t)base + lane;
+    const bool valid = v < n;
+    const uint32_t rs = valid ? row_ptr[v] : 0u, re = valid ? row_ptr[v + 1] : 0u;
+    const bool big = valid && re - rs > SPLIT_BIG;
+    const uint64_t mb = ballot(big);
+    if (mb) {
+      uint32_t at = 0;
+      if (lane == 0) at = afadd(aux, (uint32_t)__popcll(mb));
+      at = rdlane(at, 0);
+      if (big) aux[SPLIT_AUX_ROWS + at + mbcnt(mb)] = v;
     }
-    uint32_t nl = 0;
-    for (uint32_t b = rs; b < re; b += 64) {
-      const uint32_t e = b + lane;
-      nl += (uint32_t)__popcll(ballot(e < re && is_light(w[e], delta, idelta, is_float)));
-    }
-    if (lane == 0) lcnt[v] = nl;
-    uint32_t lo = rs, hi = rs + nl;
-    for (uint32_t b = rs; b < re; b += 64) {
-      const uint32_t e = b + lane;
-      const bool in = e < re;
-      uint32_t c = 0, x = 0;
-      bool l = false;
-      if (in) {
-        c = col[e];
-        x = w[e];
-        l = is_light(x, delta, idelta, is_float);
+    const uint32_t d = (valid && !big) ? re - rs : 0u;
+    const uint32_t incl = wave_incl_scan(d), excl = incl - d;
+    s_rs[wv][lane] = rs;
+    for (uint32_t first = 0; first < 64;) {
+      const uint32_t e0 = rdlane(excl, first);
+      const uint64_t fit = ballot(lane >= first && incl - e0 <= SPLIT_CAP);  // holds bit `first`
+      const uint32_t last = 63u - (uint32_t)__clzll((long long)fit);
+      const uint32_t total = rdlane(incl, last) - e0;
+      const bool mine = lane >= first && lane <= last;
+      off[lane] = lane < first ? 0u : (mine ? excl - e0 : total);
+      if (lane == 0) off[64] = total;
+      const uint32_t nwin = (total + 63) >> 6;
+      for (uint32_t j = nwin + lane; j <= SPLIT_WIN; j += 64) mask[j] = 0ull;
+      wave_sync();
+      // pass 1: light masks
+      for (uint32_t w0 = 0; w0 < nwin; w0 += 4) {
+        uint32_t x[4];
+        bool in[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint32_t i = (w0 + q) * 64 + lane;
+          in[q] = i < total;
+          const uint32_t k = locate(in[q] ? i : 0u);
+          x[q] = in[q] ? w[s_rs[wv][k] + (i - off[k])] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint64_t m = ballot(in[q] && is_light(x[q], delta, idelta, is_float));
+          if (lane == 0 && w0 + q < nwin) mask[w0 + q] = m;
+        }
       }
-      const uint64_t ml = ballot(in && l), mh = ballot(in && !l);
-      if (in) {
-        const uint32_t dst = l ? lo + mbcnt(ml) : hi + mbcnt(mh);
-        cw2[dst] = make_uint2(c, x);
+      wave_sync();
+      // light edges before each window: lane t sums windows 4t .. 4t+3
+      {
+        uint32_t c[4], sum = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          c[q] = (uint32_t)__popcll(mask[4 * lane + q]);
+          sum += c[q];
+        }
+        const uint32_t inc = wave_incl_scan(sum);
+        uint32_t run = inc - sum;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          cb[4 * lane + q] = run;
+          run += c[q];
+        }
+        if (lane == 63) cb[SPLIT_WIN] = inc;
       }
-      lo += (uint32_t)__popcll(ml);
-      hi += (uint32_t)__popcll(mh);
+      wave_sync();
+      {
+        const uint32_t dk = mine ? d : 0u;
+        const uint32_t a = before(off[lane]), b = before(off[lane] + dk);
+        s_ls[wv][lane] = a;
+        s_lc[wv][lane] = b - a;
+        if (mine && valid && !big) lcnt[v] = b - a;
+      }
+      wave_sync();
+      // pass 2: every edge to its place
+      for (uint32_t w0 = 0; w0 < nwin; w0 += 4) {
+        uint32_t c[4], x[4], dst[4];
+        bool in[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint32_t i = (w0 + q) * 64 + lane;
+          in[q] = i < total;
+          const uint32_t ii = in[q] ? i : 0u;
+          const uint32_t k = locate(ii);
+          const uint32_t e = s_rs[wv][k] + (ii - off[k]);
+          c[q] = in[q] ? col[e] : 0u;
+          x[q] = in[q] ? w[e] : 0u;
+          const uint32_t lb = before(ii) - s_ls[wv][k];
+          const bool l = (mask[ii >> 6] >> (ii & 63)) & 1ull;
+          dst[q] = s_rs[wv][k] + (l ? lb : s_lc[wv][k] + (ii - off[k]) - lb);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (in[q]) cw2[dst[q]] = make_uint2(c[q], x[q]);
+      }
+      wave_sync();
+      first = last + 1;
     }
   }
 }
@@ -1540,14 +1640,21 @@ __global__ void k_split(const uint32_t* row_ptr, const uint32_t* col, const uint
 // order by one wavefront.
 __global__ void __launch_bounds__(1024) k_split_big(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w,
                                                     float delta, int is_float, uint2* cw2,
-                                                    uint32_t* lcnt, const uint32_t* aux) {
+                                                    uint32_t* lcnt, uint32_t* aux) {
   __shared__ uint32_t s_l[64], s_h[64], s_pl[64], s_ph[64];
   __shared__ uint32_t s_tl, s_th;
   const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const uint32_t idelta = int_delta(delta);
+  __shared__ uint32_t s_row;
   const uint32_t nbig = aux[0];
-  for (uint32_t i = blockIdx.x; i < nbig; i += gridDim.x) {
-    const uint32_t v = aux[1 + i];
+  for (;;) {
+    // rows are handed out by a counter (aux[2]): one very long row does not hold up the rows behind it
+    if (t == 0) s_row = afadd(aux + 2, 1u);
+    __syncthreads();
+    const uint32_t i = s_row;
+    __syncthreads();
+    if (i >= nbig) break;
+    const uint32_t v = aux[SPLIT_AUX_ROWS + i];
     const uint32_t rs = row_ptr[v], re = row_ptr[v + 1];
     uint32_t c = 0;
     for (uint32_t e = rs + t; e < re; e += 1024) c += is_light(w[e], delta, idelta, is_float) ? 1u : 0u;
@@ -1660,13 +1767,13 @@ __global__ void k_rinfo(const uint32_t* row_ptr, const uint32_t* lcnt, uint32_t 
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
                         bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, uint4* rinfo, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipError_t e = launch_fill_u32(aux, 1, 0, s);
+  hipError_t e = launch_fill_u32(aux, SPLIT_AUX_ROWS, 0, s);  // long-row count and the two work counters
   if (e != hipSuccess) return e;
-  const uint32_t blocks = std::min<uint32_t>((n + 3) / 4, 16384);
+  const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 4096);  // groups of 64 rows, taken dynamically
   hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, row_ptr, col, w, n, delta, is_float ? 1 : 0, cw2,
                      lcnt, aux);
   hipLaunchKernelGGL(k_split_big, dim3(1024), dim3(1024), 0, s, row_ptr, col, w, delta, is_float ? 1 : 0, cw2,
-                     lcnt, (const uint32_t*)aux);
+                     lcnt, aux);
   if (rinfo)
     hipLaunchKernelGGL(k_rinfo, dim3(std::min<uint32_t>((n + 255) / 256, 8192)), dim3(256), 0, s, row_ptr, lcnt, n,
                        rinfo);
