@@ -504,6 +504,9 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.nb = r.nb;
   a.parts = r.parts;
   a.inv_delta = 1.0f / r.delta;
+  // replicated solves may place each replica's items in one part (ADDS_F_REPLICA_PARTS)
+  a.inst_part = (total_n > g->n && r.parts > 1 && (r.flags & ADDS_F_REPLICA_PARTS)) ? 1u : 0u;
+  a.n_magic = g->n > 1 ? (uint32_t)((((uint64_t)1 << 32) + g->n - 1) / g->n) : 0u;
   a.chunk = r.chunk;
   // guided claims: a wavefront takes its share of what is published among the wavefronts of one
   // part, at least claim_min items.  Measured on RMAT graphs, full claims (claim_min = chunk) are
@@ -1408,6 +1411,11 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       q.flags |= ADDS_F_NO_WAIT;
       try_point(q);
     }
+    if (repl) {
+      adds_params q = p;
+      q.flags |= ADDS_F_REPLICA_PARTS;
+      try_point(q);
+    }
     // 8) a second, shorter pass over the choices that interact most (the best bucket width depends on
     //    how long ranges are split and on the tail, and so on), from the point reached so far
     for (float f : {0.5f, 2.0f}) {
@@ -1444,7 +1452,8 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
     }
     // the two flags again: the first pass can settle on a point from which the no-wait flush or local
     // continuation looks worse than it is
-    for (uint32_t f : {ADDS_F_NO_LOCAL, ADDS_F_NO_WAIT}) {
+    for (uint32_t f : {ADDS_F_NO_LOCAL, ADDS_F_NO_WAIT, ADDS_F_REPLICA_PARTS}) {
+      if (f == ADDS_F_REPLICA_PARTS && !repl) continue;
       adds_params q = p;
       q.flags ^= f;
       try_point(q);

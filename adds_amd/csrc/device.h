@@ -80,6 +80,12 @@ struct KArgs {
   const uint32_t* host_abort;  // host-visible abort word (may be null)
   uint32_t n;         // vertices of the graph (per replica)
   uint32_t total_n;   // replicas * n
+  // Replicated solves with ADDS_F_REPLICA_PARTS: the items of replica r go to sub-queue part r % parts
+  // (inst_part != 0), so that with parts = 8 each replica's items are mostly taken by one XCD;
+  // r = mulhi(base, n_magic) for a replica's base offset r * n (n_magic = ceil(2^32 / n), exact for
+  // multiples of n below 2^32).
+  uint32_t inst_part;
+  uint32_t n_magic;
   uint32_t pool_chunks;
   uint32_t maxv;      // virtual chunks per queue
   uint32_t nb;        // bucket ring length (power of two)

@@ -289,6 +289,13 @@ struct PushBuf {
 // Target queue of a light-queue push: the bucket of nd, clamped to the window of the bucket being
 // processed (cb): never before it (such items are simply early work) and never so far ahead that the
 // ring slot would alias the behind bucket.
+// The sub-queue part that the items of the replica at base offset gb go to (the pusher's own part unless
+// the solve places replicas by part).
+__device__ __forceinline__ uint32_t push_part(const KArgs& a, uint32_t gb, uint32_t part) {
+  if (!a.inst_part) return part;
+  const uint32_t r = a.n_magic ? __umulhi(gb, a.n_magic) : gb;
+  return r & (a.parts - 1);
+}
 template <bool FLT>
 __device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uint32_t part, uint32_t cb, bool tail) {
   uint32_t b = tail ? cb : bucket_key<FLT>(nd, a.inv_delta);  // the tail is processed as one bucket
@@ -498,7 +505,7 @@ __device__ __forceinline__ void load_cw(const KArgs& a, EdgeGroup<UN>& g) {
 template <int UN>
 struct Gathered {
   bool act[UN];
-  uint32_t tgt[UN], nd[UN], dv[UN];
+  uint32_t tgt[UN], nd[UN], dv[UN], pp[UN];  // pp: the part its improvements go to
 };
 
 // Turns a group whose (col, w) loads have been issued into a gathered group: candidate distances and
@@ -510,6 +517,7 @@ __device__ __forceinline__ void gather(const KArgs& a, const EdgeGroup<UN>& g, G
     r.act[j] = g.act[j];
     r.tgt[j] = g.act[j] ? g.gb[j] + g.cl[j] : 0u;
     r.nd[j] = dadd<FLT>(g.du[j], g.wl[j]);
+    r.pp[j] = g.gb[j];
   }
 #pragma unroll
   for (int j = 0; j < UN; j++) r.dv[j] = a.dist[r.tgt[j]];
@@ -532,7 +540,7 @@ struct PairGroup {
 template <int UN>
 struct PairGathered {
   bool a0[UN], a1[UN];
-  uint32_t t0[UN], t1[UN], n0[UN], n1[UN], d0[UN], d1[UN];
+  uint32_t t0[UN], t1[UN], n0[UN], n1[UN], d0[UN], d1[UN], pp[UN];  // pp: the pair's replica base
 };
 template <int UN>
 __device__ __forceinline__ void load_pairs(const KArgs& a, PairGroup<UN>& g) {
@@ -556,6 +564,7 @@ __device__ __forceinline__ void gather_pairs(const KArgs& a, const PairGroup<UN>
     r.t1[j] = r.a1[j] ? g.gb[j] + g.c1[j] : 0u;
     r.n0[j] = dadd<FLT>(g.du[j], g.w0[j]);
     r.n1[j] = dadd<FLT>(g.du[j], g.w1[j]);
+    r.pp[j] = g.gb[j];
   }
 #pragma unroll
   for (int j = 0; j < UN; j++) {
@@ -563,7 +572,7 @@ __device__ __forceinline__ void gather_pairs(const KArgs& a, const PairGroup<UN>
     r.d1[j] = a.dist[r.t1[j]];
   }
 }
-template <bool FLT, int UN>
+template <bool FLT, bool REPL, int UN>
 __device__ __forceinline__ void finish_pairs(const KArgs& a, const PairGathered<UN>& r, uint32_t part, uint32_t cb,
                                              bool heavy, bool tail, PushBuf& pb, WaveStats& ws) {
   bool f0[UN], f1[UN];
@@ -580,15 +589,16 @@ __device__ __forceinline__ void finish_pairs(const KArgs& a, const PairGathered<
   }
 #pragma unroll
   for (int j = 0; j < UN; j++) {
-    pb_add(a, pb, f0[j], ((uint64_t)r.n0[j] << 32) | r.t0[j], light_queue<FLT>(a, r.n0[j], part, cb, tail) | PB_RELAX, ws);
-    pb_add(a, pb, f1[j], ((uint64_t)r.n1[j] << 32) | r.t1[j], light_queue<FLT>(a, r.n1[j], part, cb, tail) | PB_RELAX, ws);
+    const uint32_t pp = REPL ? push_part(a, r.pp[j], part) : part;
+    pb_add(a, pb, f0[j], ((uint64_t)r.n0[j] << 32) | r.t0[j], light_queue<FLT>(a, r.n0[j], pp, cb, tail) | PB_RELAX, ws);
+    pb_add(a, pb, f1[j], ((uint64_t)r.n1[j] << 32) | r.t1[j], light_queue<FLT>(a, r.n1[j], pp, cb, tail) | PB_RELAX, ws);
   }
 }
 
 // Consumes a gathered group: improvements over the pre-filter value go to the push buffer, whose
 // flush applies the atomicMin.  No store or atomic is issued in the edge loop, so the compiler's
 // waits there cover loads only and the pipeline keeps two groups' loads in flight.
-template <bool FLT, int UN>
+template <bool FLT, bool REPL, int UN>
 __device__ __forceinline__ void finish(const KArgs& a, const Gathered<UN>& r, uint32_t part, uint32_t cb, bool heavy,
                                        bool tail, PushBuf& pb, WaveStats& ws) {
   bool f[UN];
@@ -604,7 +614,8 @@ __device__ __forceinline__ void finish(const KArgs& a, const Gathered<UN>& r, ui
   }
 #pragma unroll
   for (int j = 0; j < UN; j++)
-    pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j], light_queue<FLT>(a, r.nd[j], part, cb, tail) | PB_RELAX, ws);
+    pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j],
+           light_queue<FLT>(a, r.nd[j], REPL ? push_part(a, r.pp[j], part) : part, cb, tail) | PB_RELAX, ws);
 }
 
 // Appends a bucket-timeline record when tracing is on (called by one lane).
@@ -1055,8 +1066,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
         nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], a.inv_delta) + 1 < cb));
-        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], part, cb, tail_mode), ws);
-        pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + part, ws);
+        const uint32_t pp = REPL ? push_part(a, gbase[j], part) : part;
+        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], pp, cb, tail_mode), ws);
+        pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + pp, ws);
       }
       if (lane == 0) {
         ws.v[ST_STALE] += nstale;
@@ -1202,7 +1214,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             locate_p(base + 128 * UP, ga);
             load_pairs(a, ga);
             gather_pairs<FLT, UP>(a, gb, rb);
-            finish_pairs<FLT, UP>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
+            finish_pairs<FLT, REPL, UP>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
             ra = rb;
             gb = ga;
           }
@@ -1221,7 +1233,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           locate(base + 128 * UN, ga);
           load_cw(a, ga);
           gather<FLT, UN>(a, gb, rb);
-          finish<FLT, UN>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
+          finish<FLT, REPL, UN>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
           ra = rb;
           gb = ga;
         }

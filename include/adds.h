@@ -84,6 +84,11 @@ typedef struct {
 #define ADDS_F_NO_WAIT 0x10u  /* flushes issue their atomicMins without waiting for the results and queue
                                  every improvement (one that lost becomes a stale entry, dropped when
                                  claimed): a round trip less per flush; pays on throughput-bound solves */
+#define ADDS_F_REPLICA_PARTS 0x20u /* many-source solves (adds_sssp_multi and its captures): the items of
+                                 source i go to sub-queue part i % parts, so that with parts = 8 each
+                                 source's distance row is worked on mostly by one XCD and its L2; pays on
+                                 many sources of a power-law graph (256 sources on RMAT-20: -6%), loses on
+                                 grids (hops leave the wavefront's part, so local continuation stops) */
 
 typedef struct {
   double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */

@@ -1,14 +1,14 @@
 """GPU parity: device distances equal CPU Dijkstra bit for bit (int and float), through the C ABI.
 
-Small cases compare against the oracle's Dijkstra directly; the full-size configurations are checked
-with the oracle's certificate (feasible + tight-reachable + INF exactly on the unreachable set), which
-pins the shortest-path solution uniquely, plus Dijkstra where it finishes in seconds.
+Every case here compares against the oracle's Dijkstra directly; the full-size configurations are in
+test_gpu_fullsize.py.
 """
 import numpy as np
 import pytest
 
 import adds_amd
 import oracle
+from adds_amd import _lib as L
 
 pytestmark = pytest.mark.gpu
 
@@ -84,6 +84,24 @@ def test_multi_source_replicated():
     dist, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT)
     for i, s in enumerate(srcs):
         _check(g, dist[i], s, True)
+
+
+@pytest.mark.parametrize("params", [dict(parts=8), dict(parts=4, edge_windows=8, flags=0x10),
+                                    dict(parts=2, flags=0x8), dict(parts=8, light_delta=1e30, delta=2.0)])
+@pytest.mark.parametrize("wkind", ["f32", "u32"])
+def test_multi_source_replica_parts(params, wkind):
+    """ADDS_F_REPLICA_PARTS: each source's items go to sub-queue part (source index mod parts), whatever
+    part the pushing wavefront works on; distances are unchanged, on a power-law graph and on a grid
+    (where hops leave the wavefront's own part)."""
+    lo, hi = (1.0, 10.0) if wkind == "f32" else (1, 1000)
+    p = dict(params)
+    p["flags"] = p.get("flags", 0) | L.F_REPLICA_PARTS
+    for g in (adds_amd.HostGraph.rmat(13, 8, seed=14, wkind=wkind, lo=lo, hi=hi),
+              adds_amd.HostGraph.grid(90, 70, 0, seed=15, wkind=wkind, lo=lo, hi=hi)):
+        srcs = g.sample_component(0, 19, seed=3)
+        dist, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT, **p)
+        for i, s in enumerate(srcs):
+            _check(g, dist[i], s, g.is_float)
 
 
 def test_batched_union_graphs():
