@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--only", default="all", help="all | c3int | c3float | c1 | c2 | c4a | c4b | c5 (comma list)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs")
+    ap.add_argument("--no-cli", action="store_true", help="skip the drop-in binary on the headline graph")
     ap.add_argument("--retune", action="store_true", help="search the solver parameters again (ignore pinned)")
     ap.add_argument("--write-params", default=None, help="store the parameters used per workload key here")
     ap.add_argument("--scale", type=int, default=24, help="RMAT scale of the headline graph")
@@ -69,6 +70,45 @@ def roofline_entry(bytes_, ms, note=None):
     if note:
         r["note"] = note
     return r
+
+
+def run_cli(host, src, wtype):
+    """The drop-in binary on the same graph and source, as the reference's run_all.sh calls it
+    (`sssp -s <src> <graph>` with fd 3 redirected): the graph goes to a .gr file in /dev/shm (memory), the
+    binary reads it, copies it to the device, looks its tuned parameters up and solves RUN_LOOP times
+    with the distances copied back to host memory.  Reports the per-run times it prints (the first
+    includes building the light-first rows) and its fd-3 line (mean ms, mean work)."""
+    import re
+    import subprocess
+    import tempfile
+    binp = os.path.join(ROOT, "adds_amd", "bin", f"sssp_{wtype}")
+    if not os.path.exists(binp):
+        return {"skipped": f"{binp} not built"}
+    d = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    with tempfile.TemporaryDirectory(dir=d) as tmp:
+        gr = os.path.join(tmp, "c3.gr")
+        res = os.path.join(tmp, "result")
+        t0 = time.perf_counter()
+        host.write_gr(gr)
+        write_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = subprocess.run(["bash", "-c", '"$@" 3>"$RES"', "cli", binp, "-s", str(src), gr], capture_output=True,
+                           text=True, env=dict(os.environ, RES=res), timeout=600)
+        wall_s = time.perf_counter() - t0
+        if r.returncode != 0:
+            return {"failed": r.returncode, "stderr": r.stderr[-500:]}
+        runs = [float(x) * 1e3 for x in re.findall(r"Measured time for sample = ([0-9.]+)s", r.stdout)]
+        tuned = re.findall(r"tuned parameters from (\S+)", r.stdout)
+        with open(res) as f:
+            fd3 = f.read().strip()
+    tok = fd3.split()
+    return {"binary": os.path.relpath(binp, ROOT), "run_loop": len(runs), "first_ms": runs[0] if runs else None,
+            "runs_ms": runs, "fd3_line": fd3, "fd3_mean_ms": float(tok[1]) if len(tok) == 3 else None,
+            "fd3_mean_gteps": host.m / (float(tok[1]) * 1e6) if len(tok) == 3 and float(tok[1]) > 0 else None,
+            "tuned_parameters_from": tuned[0] if tuned else None, "gr_write_s": write_s, "process_wall_s": wall_s,
+            "note": "per-run times as the binary prints them (\"Measured time for sample = %.3fs\", 1 ms "
+                    "resolution) and the fd-3 mean: the library's solve time (fill, persistent kernel, "
+                    "cleanup), the light-first rows built before the first run; GTEPS on m"}
 
 
 def gather_ceiling(adds_amd, n, edges_scanned, kernel_ms):
@@ -466,6 +506,9 @@ def main():
         log(f"rmat{args.scale} generated n={host.n} m={host.m} in {time.perf_counter() - t0:.1f}s")
         if "c3int" in only:
             head = run_c3(host, src, "c3int", args.steps, args.warmup, torch, adds_amd, oracle, prm, do_cpu, hinfo)
+            if not args.no_cli:
+                head["cli"] = run_cli(host, src, "int")
+                log(f"c3int: drop-in binary {head['cli']}")
         if "c3float" in only:
             workloads.c3_reweight_float(host)
             extras["c3_float"] = run_c3(host, src, "c3float", args.steps, args.warmup, torch, adds_amd, oracle, prm,
@@ -542,7 +585,7 @@ def main():
         "extras": dict(extras, c3_int={k: head[k] for k in ("params", "kernel_ms_event", "kernel_ms_min",
                                                              "solve_ms_event", "gteps_kernel", "host_out_ms",
                                                              "host_out_gteps", "cold_first_solve_ms",
-                                                             "cold_kernel_ms", "work", "cpu", "near_far")
+                                                             "cold_kernel_ms", "work", "cpu", "near_far", "cli")
                                        if k in head}),
     }
     print(json.dumps(line))
