@@ -104,6 +104,43 @@ def test_multi_source_replica_parts(params, wkind):
             _check(g, dist[i], s, g.is_float)
 
 
+@pytest.mark.parametrize("wkind", ["u32", "f32"])
+def test_split_row_shapes(wkind):
+    """The light-first rows at the boundaries of their two build paths: rows of 1023, 1024 (the last
+    taken 64 to a wavefront) and 1025 edges (the first taken one to a workgroup), runs of empty rows,
+    a row count that is not a multiple of 64, and 64-row groups of more than 16384 edges (built as
+    consecutive sub-groups).  Every row mixes light and heavy edges; distances from several sources must
+    equal Dijkstra's."""
+    rng = np.random.default_rng(77)
+    n = 64 * 40 + 37
+    deg = rng.integers(0, 40, n)
+    deg[rng.random(n) < 0.2] = 0
+    deg[64:128] = 300                     # a group of 19200 edges
+    deg[200:264] = rng.integers(200, 1100, 64)
+    deg[400], deg[401], deg[402], deg[403] = 1023, 1024, 1025, 5000
+    deg[500:505] = 0
+    deg[n - 3:] = [1025, 0, 7]
+    row = np.zeros(n + 1, dtype=np.uint32)
+    row[1:] = np.cumsum(deg)
+    m = int(row[-1])
+    src = np.repeat(np.arange(n), deg)
+    col = rng.integers(0, n, m).astype(np.uint32)
+    col = np.where(col == src, (col + 1) % n, col).astype(np.uint32)
+    if wkind == "f32":
+        w = rng.uniform(0.1, 10.0, m).astype(np.float32)
+        delta = 2.0
+    else:
+        w = rng.integers(1, 1000, m).astype(np.uint32)
+        delta = 150.0
+    g = adds_amd.HostGraph.from_arrays(row, col, w, wkind)
+    dg = adds_amd.Graph(g)
+    srcs = [0, 65, 401, n - 3]
+    for params in (dict(delta=delta), dict(delta=delta, edge_windows=8, hub_degree=512)):
+        dist, _ = dg.sssp_multi(srcs, timeout_ms=TIMEOUT, **params)
+        for i, s in enumerate(srcs):
+            _check(g, dist[i], s, wkind == "f32")
+
+
 def test_batched_union_graphs():
     gs = [adds_amd.HostGraph.grid(40 + 7 * i, 33, 0, seed=i, wkind="u32", lo=1, hi=1000) if i % 2 == 0 else
           adds_amd.HostGraph.er(3000 + 500 * i, 8, seed=i, wkind="u32", lo=1, hi=1000) for i in range(6)]
