@@ -246,12 +246,15 @@ def run_c3(host, src, name, steps, warmup, torch, adds_amd, oracle, prm, do_cpu,
         "ms_per_step": wall * 1e3, "solve_ms_event": sms, "kernel_ms_event": kms,
         "kernel_ms_min": float(min(s["kernel_ms"] for s in stats)),
         "gteps": m_r / wall / 1e9, "gteps_kernel": m_r / (kms / 1e3) / 1e9,
+        "gteps_undirected": m_r / 2 / wall / 1e9,
         "bytes_alg": bytes_alg(host.n, n_r, m_r),
         "cold_first_solve_ms": cold_ms, "cold_kernel_ms": cold["kernel_ms"], "graph_upload_s": upload_s,
         "work": {k: stats[-1][k] for k in ("vertices_expanded", "edges_scanned", "relaxations", "pushes",
                                            "stale_skips", "hub_tasks", "rebuckets")},
     }
     res["roofline"] = roofline_entry(res["bytes_alg"], kms)
+    res["work_efficiency"] = {"relaxations_per_m_r": res["work"]["relaxations"] / m_r,
+                              "edges_scanned_per_m_r": res["work"]["edges_scanned"] / m_r}
     res["gather_ceiling"] = gather_ceiling(adds_amd, host.n, res["work"]["edges_scanned"], kms)
     log(f"{name}: random-gather ceiling {res['gather_ceiling']}")
     # the same solve with the distances copied back to host memory (the CLI's path): PCIe-inclusive
@@ -573,6 +576,7 @@ def main():
         "dtype": "u32", "data": "synthetic (seeded RMAT generator, no dataset)",
         "config": {"workload": f"rmat{args.scale}-ef32-u32[1,65535]-single-source", "n": head["n"], "m": head["m"],
                    "source": head["src"], "n_reached": head["n_r"], "m_reached": head["m_r"],
+                   "gteps_undirected": head["gteps_undirected"],
                    "certificate": head.get("certificate"),
                    "bit_exact_vs_dijkstra": head.get("bit_exact_vs_dijkstra")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -585,7 +589,8 @@ def main():
         "extras": dict(extras, c3_int={k: head[k] for k in ("params", "kernel_ms_event", "kernel_ms_min",
                                                              "solve_ms_event", "gteps_kernel", "host_out_ms",
                                                              "host_out_gteps", "cold_first_solve_ms",
-                                                             "cold_kernel_ms", "work", "cpu", "near_far", "cli")
+                                                             "cold_kernel_ms", "work", "work_efficiency", "cpu",
+                                                             "near_far", "cli")
                                        if k in head}),
     }
     print(json.dumps(line))
