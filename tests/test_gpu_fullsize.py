@@ -61,13 +61,22 @@ def test_c3_full_size_bit_exact(c3_graph, kind):
     assert n_r > g.n // 2
 
 
-def test_c4a_all_256_sources():
+@pytest.mark.parametrize("point", ["default", "pinned"])
+def test_c4a_all_256_sources(point):
+    """All 256 sources at the default parameters and at the pinned benchmark point (which places each
+    source's items in one sub-queue part, ADDS_F_REPLICA_PARTS)."""
+    from adds_amd import autotune
+
     g, srcs = workloads.c4a()
     assert len(srcs) == 256
+    params = {}
+    if point == "pinned":
+        params, _ = autotune.lookup(autotune.workload_key(g, len(srcs)))
+        assert params is not None
     dg = adds_amd.Graph(g)
     dist = torch.empty((len(srcs), g.n), dtype=torch.int32, device="cuda")
-    st = dg.sssp_multi_device(srcs, dist.data_ptr(), timeout_ms=TIMEOUT)
-    print(f"c4a: kernel {st['kernel_ms']:.2f} ms", flush=True)
+    st = dg.sssp_multi_device(srcs, dist.data_ptr(), timeout_ms=TIMEOUT, **params)
+    print(f"c4a ({point} {params}): kernel {st['kernel_ms']:.2f} ms", flush=True)
     d = dist.cpu().numpy().view(np.uint32)
     del dist
     dg.close()
