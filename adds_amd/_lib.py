@@ -18,6 +18,7 @@ W_U32, W_F32 = 0, 1
 WK_U32_UNIFORM, WK_F32_UNIFORM, WK_F32_LOGNORMAL = 0, 1, 2
 # flags
 F_HOST_OUT, F_NO_SYNC, F_ATOMIC_PROBE, F_NO_LOCAL, F_NO_WAIT, F_REPLICA_PARTS = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20
+F_NO_PULL, F_PULL_EAGER = 0x40, 0x80
 
 INF_U32 = 0xFFFFFFFF
 INF_F32_BITS = 0x7F800000
@@ -97,6 +98,9 @@ class Stats(ctypes.Structure):
         ("local_items", ctypes.c_uint64),
         ("heavy_edges", ctypes.c_uint64),
         ("ticks_flush", ctypes.c_uint64),
+        ("pulls", ctypes.c_uint64),
+        ("pull_edges", ctypes.c_uint64),
+        ("pull_improved", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -107,7 +111,7 @@ class Stats(ctypes.Structure):
 EXPORTS = [
     "adds_strerror", "adds_set_device", "adds_device_cu_count",
     "adds_graph_create", "adds_graph_create_device", "adds_graph_set_weights_device",
-    "adds_graph_destroy", "adds_graph_info", "adds_graph_device_arrays",
+    "adds_graph_destroy", "adds_graph_info", "adds_graph_device_arrays", "adds_graph_set_symmetric",
     "adds_sssp", "adds_sssp_noalloc", "adds_sssp_nearfar", "adds_sssp_multi", "adds_sssp_batched", "adds_last_status",
     "adds_solve_graph_create", "adds_solve_graph_create_batched", "adds_solve_graph_launch",
     "adds_solve_graph_status", "adds_solve_graph_destroy",
@@ -152,6 +156,7 @@ def _lib():
         "adds_graph_create_device": (i32, [vp, vp, vp, u32, u64, i32, ctypes.POINTER(vp)]),
         "adds_graph_set_weights_device": (i32, [vp, vp, vp]),
         "adds_graph_destroy": (None, [vp]),
+        "adds_graph_set_symmetric": (i32, [vp, i32]),
         "adds_graph_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u64), ctypes.POINTER(i32)]),
         "adds_graph_device_arrays": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
         "adds_sssp": (i32, [vp, u32, vp, pp, sp, vp]),

@@ -44,7 +44,7 @@ struct Workspace {
   uint32_t hrec_cap = 0;
   uint32_t* srcs = nullptr;
   uint32_t nq = 0, nb = 0, maxv = 0, pool_chunks = 0, srcs_cap = 0;
-  uint32_t* host_state = nullptr;       // pinned copy of the first 8 state lines
+  uint32_t* host_state = nullptr;       // pinned copy of the state words before the queues
   uint32_t* host_abort = nullptr;       // pinned, device-visible abort word
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // solve start, solve end, state copied
   hipEvent_t evk0 = nullptr, evk1 = nullptr;                // around the persistent kernel
@@ -56,6 +56,8 @@ struct Workspace {
   uint64_t* trace = nullptr;            // bucket timeline (ADDS_TRACE=<file>), TRACE_CAP records
   uint64_t trace_t0 = 0;
   double timeout_ms = 0;
+  uint4* ptask = nullptr;               // pull phases: long-row tasks
+  uint32_t ptask_cap = 0;
 };
 
 // Every graph handle gets a number no other handle of this process has had, renewed whenever its
@@ -69,6 +71,10 @@ struct adds_graph {
   uint32_t n = 0;
   uint64_t m = 0;
   adds_wtype wt = ADDS_W_U32;
+  // Declared symmetric (every edge (u, v, w) has its reverse (v, u, w)): pull phases may then read a
+  // vertex's in-edges from its own row.  sym_dev mirrors it on the device, where captured solves read it.
+  bool symmetric = false;
+  uint32_t* sym_dev = nullptr;
   uint32_t* row = nullptr;
   uint32_t* col = nullptr;
   uint32_t* w = nullptr;
@@ -154,6 +160,7 @@ static adds_status free_workspace(Workspace& ws) {
   if (ws.hrec) (void)hipFree(ws.hrec);
   if (ws.srcs) (void)hipFree(ws.srcs);
   if (ws.trace) (void)hipFree(ws.trace);
+  if (ws.ptask) (void)hipFree(ws.ptask);
   if (ws.host_state) (void)hipHostFree(ws.host_state);
   if (ws.host_abort) (void)hipHostFree(ws.host_abort);
   if (ws.ev0) (void)hipEventDestroy(ws.ev0);
@@ -171,6 +178,8 @@ static void release_graph_memory(adds_graph* g) {
   if (g->col) (void)hipFree(g->col);
   if (g->w) (void)hipFree(g->w);
   if (g->outbuf) (void)hipFree(g->outbuf);
+  if (g->sym_dev) (void)hipFree(g->sym_dev);
+  g->sym_dev = nullptr;
   if (g->rrow) (void)hipFree(g->rrow);
   if (g->reid) (void)hipFree(g->reid);
   if (g->rsrc) (void)hipFree(g->rsrc);
@@ -238,6 +247,8 @@ static adds_status alloc_graph(uint32_t n, uint64_t m, adds_wtype wt, adds_graph
   if (e == hipSuccess) e = hipMalloc(&g->lcnt, std::max<uint64_t>(n, 1) * 4);
   if (e == hipSuccess) e = hipMalloc(&g->rinfo, std::max<uint64_t>(n, 1) * 16);
   if (e == hipSuccess) e = hipMalloc(&g->split_aux, split_aux_words(m) * 4);
+  if (e == hipSuccess) e = hipMalloc(&g->sym_dev, 256);
+  if (e == hipSuccess) e = launch_fill_u32(g->sym_dev, 1, 0u, nullptr);
   if (e != hipSuccess) {
     g_detail = std::string("graph allocation: ") + hipGetErrorString(e);
     release_graph_memory(g);
@@ -313,6 +324,12 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   if (g->split_delta != 0.f && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
     HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scw, g->lcnt,
                         g->split_aux, g->rinfo, (hipStream_t)stream));
+  // new weights need not be symmetric: pull phases stop (also in solves captured earlier) until the
+  // caller declares the graph symmetric again
+  if (g->symmetric) {
+    HIPCHK(launch_fill_u32(g->sym_dev, 1, 0u, (hipStream_t)stream));
+    g->symmetric = false;
+  }
   // a new generation: every cached union that holds this graph (on any handle) no longer matches
   g->gen = g_next_gen.fetch_add(1);
   if (g->union_g) {  // weights changed: the union copy is stale
@@ -321,6 +338,15 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
     g->union_g = nullptr;
     g->union_key.clear();
   }
+  return ADDS_OK;
+}
+
+extern "C" adds_status adds_graph_set_symmetric(adds_graph* g, int symmetric) {
+  if (!g) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  g->symmetric = symmetric != 0;
+  HIPCHK(launch_fill_u32(g->sym_dev, 1, g->symmetric ? 1u : 0u, nullptr));
+  HIPCHK(hipDeviceSynchronize());
   return ADDS_OK;
 }
 
@@ -358,6 +384,8 @@ struct Resolved {
   uint64_t pool_entries;
   bool pool_fixed;
   double timeout_ms;
+  uint32_t pull_slots, ptask_cap;  // pull phases per solve (0: none) and the long-row task list size
+  uint64_t pull_min;               // heavy edges a bucket needs before a pull phase is considered
 };
 
 // Default bucket width.  The reference derives it from the mean weight and degree
@@ -416,6 +444,12 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
     return ADDS_EINVAL;
   }
   if (r.pool_entries < CH) r.pool_entries = CH;
+  // Pull phases (symmetric graphs only): one per solve, for the first bucket whose heavy queues hold
+  // at least max(total_n / 2, 2^20) heavy edges -- in practice bucket 0 of a power-law graph with a
+  // high-degree source.  Every long row of every instance may become tasks of at least PULL_LONG edges.
+  r.pull_slots = (g->symmetric && !(r.flags & ADDS_F_NO_PULL)) ? 1u : 0u;
+  r.pull_min = std::max<uint64_t>(total_n / 2, 1u << 20);
+  r.ptask_cap = (uint32_t)std::min<uint64_t>(total_m / PULL_LONG + 4096, 1u << 24);
   return ADDS_OK;
 }
 
@@ -425,7 +459,7 @@ static adds_status ensure_workspace(Workspace& ws, const Resolved& r, uint32_t n
   if (chunks64 >= (1u << 24)) return ADDS_EINVAL;  // chunk ids are 24-bit in the kernel's chunk-map cache
   const uint32_t chunks = (uint32_t)chunks64;
   if (!ws.host_state) {
-    HIPCHK(hipHostMalloc((void**)&ws.host_state, 8 * LINE * 4, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&ws.host_state, HOST_STATE_WORDS * 4, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&ws.host_abort, 256, hipHostMallocCoherent | hipHostMallocMapped));
     *ws.host_abort = 0;
     HIPCHK(hipEventCreate(&ws.ev0));
@@ -460,6 +494,14 @@ static adds_status ensure_workspace(Workspace& ws, const Resolved& r, uint32_t n
     ws.nb = r.nb;
     ws.pool_chunks = pc;
     ws.maxv = pc;
+  }
+  if (r.pull_slots && ws.ptask_cap < r.ptask_cap) {
+    HIPCHK(hipDeviceSynchronize());
+    if (ws.ptask) (void)hipFree(ws.ptask);
+    ws.ptask = nullptr;
+    ws.ptask_cap = 0;
+    HIPCHK(hipMalloc(&ws.ptask, (size_t)r.ptask_cap * 16));
+    ws.ptask_cap = r.ptask_cap;
   }
   if (ws.srcs_cap < nsrc) {
     if (ws.srcs) (void)hipFree(ws.srcs);
@@ -531,6 +573,14 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
   a.trace_slice = std::max(1, rate_khz / 20);  // 50 us bins
+  // pull phases need the task list (the allocation-free solve has none) and the graph's symmetric word
+  a.pull_slots = (ws.ptask && g->sym_dev) ? r.pull_slots : 0u;
+  a.pull_min = r.pull_min;
+  a.pull_eager = (r.flags & ADDS_F_PULL_EAGER) ? 1u : 0u;
+  a.pull_ok = g->sym_dev;
+  a.ptask = ws.ptask;
+  a.ptask_cap = ws.ptask_cap;
+  a.launch_idx = 0;
   return a;
 }
 
@@ -560,6 +610,14 @@ static adds_status enqueue_solve(const adds_graph* g, const Workspace& ws, const
   HIPCHK(launch_init(a, srcs_dev, k, replicated ? 1 : 0, inf_bits(g->wt), s));
   if (kernel_events) HIPCHK(hipEventRecord(ws.evk0, s));
   HIPCHK(launch_solve(a, g->wt == ADDS_W_F32, replicated, grid, s));
+  // Pull phase l and the launch that continues after it; both return at once unless the previous
+  // launch asked for the pull (the sequence is fixed, so it is captured in HIP graphs as it is).
+  for (uint32_t l = 1; l <= a.pull_slots; l++) {
+    HIPCHK(launch_pull(a, g->wt == ADDS_W_F32, replicated, l, grid_for(r) / r.wg_per_cu * 4, s));
+    KArgs b = a;
+    b.launch_idx = l;
+    HIPCHK(launch_solve(b, g->wt == ADDS_W_F32, replicated, grid, s));
+  }
   if (kernel_events) HIPCHK(hipEventRecord(ws.evk1, s));
   HIPCHK(launch_cleanup(a, s));
   return ADDS_OK;
@@ -630,10 +688,12 @@ static void dump_trace(const adds_graph* g, const Workspace& ws) {
   fclose(f);
 }
 
-// The solve's counters and outcome from a host copy of the first 8 state lines.  ms / kernel_ms are
+// The solve's counters and outcome from the host copy of the state words before the queues.  ms / kernel_ms are
 // left to the caller.
 static void read_stats(const uint32_t* hs, adds_stats* stats) {
-  const uint64_t* sv = (const uint64_t*)(hs + S_STATS);
+  uint64_t sv[NSTATS] = {};
+  for (uint32_t c = 0; c < STAT_COPIES; c++)
+    for (uint32_t i = 0; i < NSTATS; i++) sv[i] += ((const uint64_t*)(hs + stat_copy(c)))[i];
   stats->vertices_expanded = sv[ST_EXPANDED];
   stats->edges_scanned = sv[ST_EDGES];
   stats->relaxations = sv[ST_RELAX];
@@ -653,6 +713,9 @@ static void read_stats(const uint32_t* hs, adds_stats* stats) {
   stats->local_items = sv[ST_LOCAL];
   stats->heavy_edges = sv[ST_HEDGES];
   stats->ticks_flush = sv[ST_T_FLUSH];
+  stats->pulls = hs[PULL_DONE];
+  stats->pull_edges = sv[ST_PULL_EDGES];
+  stats->pull_improved = sv[ST_PULL_IMPROVED];
 }
 
 static adds_status solve_outcome(const uint32_t* hs, double timeout_ms) {
@@ -729,7 +792,7 @@ static adds_status solve_core(adds_graph* g, const uint32_t* srcs_host, uint32_t
     ws.kernel_events = true;
     if (rc != ADDS_OK) return rc;
     HIPCHK(hipEventRecord(ws.ev1, s));
-    HIPCHK(hipMemcpyAsync(ws.host_state, ws.st, 8 * LINE * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ws.host_state, ws.st, HOST_STATE_WORDS * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(ws.ev2, s));
     ws.last_stream = s;
     ws.pending = true;
@@ -823,6 +886,14 @@ static adds_status build_union(adds_graph* const* graphs, uint32_t k, adds_graph
   HIPCHK(hipMemcpy(u->row + N, &last, 4, hipMemcpyHostToDevice));
   HIPCHK(hipDeviceSynchronize());
   u->ave_wt = M ? (float)(wsum / (double)M) : 1.f;
+  // block-diagonal: symmetric when every member is
+  bool sym = true;
+  for (uint32_t i = 0; i < k; i++) sym &= graphs[i]->symmetric;
+  if (sym) {
+    u->symmetric = true;
+    HIPCHK(launch_fill_u32(u->sym_dev, 1, 1u, nullptr));
+    HIPCHK(hipDeviceSynchronize());
+  }
   *out = u;
   return ADDS_OK;
 }
@@ -1043,7 +1114,7 @@ extern "C" adds_status adds_solve_graph_launch(adds_solve_graph* sg, void* strea
   ws.kernel_events = false;
   HIPCHK(hipGraphLaunch(sg->exec, s));
   HIPCHK(hipEventRecord(ws.ev1, s));
-  HIPCHK(hipMemcpyAsync(ws.host_state, ws.st, 8 * LINE * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(ws.host_state, ws.st, HOST_STATE_WORDS * 4, hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(ws.ev2, s));
   ws.pending = true;
   return ADDS_OK;
@@ -1628,7 +1699,7 @@ extern "C" adds_status adds_sssp_noalloc(adds_graph* g, uint32_t src, void* dist
   if (rc == ADDS_OK) rc = enqueue_solve(g, ws, r, (uint32_t*)dist, g->n, srcw, 1, false, s, false);
   if (rc != ADDS_OK) return rc;
   g_na_clean[dev] = true;
-  uint32_t hs[8 * LINE];
+  uint32_t hs[HOST_STATE_WORDS];
   HIPCHK(hipMemcpyAsync(hs, ws.st, sizeof(hs), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (stats) {

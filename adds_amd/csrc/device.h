@@ -29,26 +29,49 @@ constexpr uint32_t HREC_WORDS = 8;
 constexpr uint32_t LINE = 32;
 constexpr uint32_t S_CUR = 0 * LINE;        // absolute index of the head bucket
 constexpr uint32_t S_TAIL = 1 * LINE;       // [0] 1 once the solve is in its tail (see tail_items),
-                                            // [1] most items pending seen at a head move
+                                            // [1] most items pending seen at a head move,
+                                            // [2] pull requests made (PULL_GEN), [3] bucket of the last
+                                            // request, [4..5] u64 heavy edges queued in it, [6] pulls
+                                            // performed, [7] the last request's outcome (1 pull, 2 push)
 constexpr uint32_t S_DONE = 2 * LINE;       // [0] done flag, [1] error code
 constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
-constexpr uint32_t S_STATS = 4 * LINE;      // u64 stats[NSTATS] (two lines)
+// (lines 4 and 5 are spare)
 constexpr uint32_t S_ADV = 6 * LINE;        // try-lock serialising head-advance scans
 constexpr uint32_t S_HREC = 7 * LINE;       // next free edge-range record
 // Termination words, one line each for VER_GROUPS groups of workgroups (workgroup b is in group
 // b % VER_GROUPS, which on MI355X is its XCD): u64 (busy transitions << 32) | idle wavefronts.
 constexpr uint32_t VER_GROUPS = 8;
 constexpr uint32_t S_VER = 8 * LINE;
+// Pull-phase work counters (k_pull_*), on S_PULL2 (S_PULL is a spare line): the long-row task count
+// [0], the task claim counter [1], u64 heavy in-edges of unsettled vertices [2..3] and u64 unsettled
+// vertices [4..5].
+constexpr uint32_t S_PULL = (8 + VER_GROUPS) * LINE;
+constexpr uint32_t S_PULL2 = S_PULL + LINE;
+constexpr uint32_t PULL_GEN = S_TAIL + 2, PULL_BUCKET = S_TAIL + 3, PULL_H = S_TAIL + 4, PULL_DONE = S_TAIL + 6,
+                   PULL_OUTCOME = S_TAIL + 7;
+// Statistics: STAT_COPIES copies of u64 stats[NSTATS] (two lines each).  Workgroup b adds its counters,
+// summed over its wavefronts in LDS, to copy b % STAT_COPIES, and the host adds the copies up: atomics on
+// one address serialise (about 12 ns each), and one per wavefront and counter on a single copy cost a
+// persistent launch hundreds of microseconds at its end.
+constexpr uint32_t STAT_COPIES = 8;
+constexpr uint32_t S_STATC = (10 + VER_GROUPS) * LINE;
+__host__ __device__ constexpr uint32_t stat_copy(uint32_t c) { return S_STATC + c * 2 * LINE; }
 // Queues: nq = nb * parts light queues (index slot * parts + part) followed by nq heavy queues (index
 // nq + slot * parts + part).  Queue q: head (claimed) at S_QBASE + 3q*LINE, tail (reserved) at +LINE,
 // published-unclaimed count at +2*LINE.  Then the bucket slots (SLOT_WORDS below).
-constexpr uint32_t S_QBASE = (8 + VER_GROUPS) * LINE;
+constexpr uint32_t S_QBASE = S_STATC + STAT_COPIES * 2 * LINE;
+constexpr uint32_t HOST_STATE_WORDS = S_QBASE;  // what the host copies back after a solve: every word before the queues
 constexpr uint32_t QLINES = 3;
 constexpr uint32_t s_slotlines(uint32_t nq) { return S_QBASE + QLINES * 2 * nq * LINE; }
 // Per bucket slot: the in-flight claim counters (light claims of part p at index p, heavy at 16 + p),
 // each on a cache line of its own -- every claim updates one twice, and packed into one line they were
 // a point of contention for the whole GPU -- then the phase word (1 = the heavy queues of the head
-// bucket are open) on a line of its own.
+// bucket are open) on a line of its own.  Words 2..3 of light line p: u64 heavy edges of the vertices
+// that claims of part p filed in the bucket's heavy queues (the cost of its push-direction heavy phase).
+constexpr uint32_t INFL_HEAVY = 2;
+// Word 4 of every in-flight line: claims of that part and phase between their first step and the landing
+// of their in-flight count (see try_claim).
+constexpr uint32_t INFL_CLAIMING = 4;
 constexpr uint32_t INFL_STRIDE = LINE;
 constexpr uint32_t SLOT_WORDS = (2 * 16 + 1) * LINE;
 constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_slotlines(nq) + nb * SLOT_WORDS; }
@@ -62,7 +85,10 @@ enum StatIdx {
   ST_T_FLUSH,  // ticks in push-buffer flushes (part of the item and edge time below)
   // wavefront time by activity (wall-clock ticks): probing/claiming, item handling, edge windows,
   // idle (termination test, head advance, back-off)
-  ST_T_PROBE, ST_T_ITEMS, ST_T_EDGES, ST_T_IDLE, NSTATS
+  ST_T_PROBE, ST_T_ITEMS, ST_T_EDGES, ST_T_IDLE,
+  ST_PULL_EDGES,     // in-edges scanned by pull phases (also counted in ST_EDGES)
+  ST_PULL_IMPROVED,  // vertices whose distance a pull phase lowered
+  NSTATS
 };
 
 static_assert(NSTATS <= 32, "the stats block is two state lines of u64");
@@ -119,7 +145,23 @@ struct KArgs {
   uint64_t* trace;
   uint32_t trace_cap;    // records
   uint32_t trace_slice;  // wall-clock ticks per histogram bin
+  // Pull phases (symmetric graphs): launch l of the persistent kernel (l = 0 .. pull_slots) may request
+  // pull l + 1 when the head bucket's light phase is done and its heavy queues hold at least pull_min
+  // heavy edges; its wavefronts then leave, k_pull_* relax the bucket's heavy edges from the receiving
+  // side (or open the push-direction phase when that is cheaper), and launch l + 1 continues the solve.
+  uint32_t launch_idx;
+  uint32_t pull_slots;
+  uint64_t pull_min;
+  uint32_t pull_eager;   // request at every opportunity and always pull (testing and tuning)
+  const uint32_t* pull_ok;  // the graph's device word: nonzero while it is declared symmetric
+  uint32_t ptask_cap;
+  uint4* ptask;          // long-row tasks of a pull phase: (vertex-instance, first edge, end edge, 0)
 };
+
+// Pull phases: rows with more heavy edges than PULL_LONG are scanned by k_pull_b in tasks of at most
+// PULL_TASK edges (one wavefront each), the rest by k_pull_a, 256 vertex-instances per wavefront.
+constexpr uint32_t PULL_LONG = 128;
+constexpr uint32_t PULL_TASK = 1024;
 
 // Inclusive prefix sum over the 64 lanes of a wavefront (every lane active), with DPP: row shifts by
 // 1, 2, 4 and 8 scan each row of 16 lanes (lanes shifted in from outside the row add 0), then
@@ -152,6 +194,9 @@ hipError_t launch_init(const KArgs& a, const uint32_t* srcs, uint32_t k, int rep
                        hipStream_t s);
 hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s);
 hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
+// The kernels of pull phase `slot` (1 .. pull_slots): count, short rows, long-row tasks.  Each returns at
+// once unless launch slot - 1 of the persistent kernel requested this pull.
+hipError_t launch_pull(const KArgs& a, bool is_float, bool replicated, uint32_t slot, uint32_t grid, hipStream_t s);
 // Stable per-row partition of (col, w) into light (w <= delta) then heavy edges, written interleaved as
 // (col, w) pairs, with per-row light counts.  aux: split_aux_words(m) words of scratch (long rows).
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,

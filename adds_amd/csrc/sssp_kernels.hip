@@ -146,6 +146,10 @@ __device__ __forceinline__ uint32_t* inflight(const KArgs& a, uint32_t slot, uin
 __device__ __forceinline__ uint32_t* phase(const KArgs& a, uint32_t slot) {
   return a.st + s_slotlines(nq_of(a)) + slot * SLOT_WORDS + SLOT_WORDS - LINE;
 }
+// heavy edges filed in bucket slot `slot` by claims of part p (u64, on the part's light in-flight line)
+__device__ __forceinline__ uint64_t* heavy_ctr(const KArgs& a, uint32_t slot, uint32_t p) {
+  return (uint64_t*)(inflight(a, slot, p) + INFL_HEAVY);
+}
 // termination word of group g, and the wavefronts in group g (workgroups b < grid with b % VER_GROUPS == g)
 __device__ __forceinline__ uint64_t* verp(const KArgs& a, uint32_t g) { return (uint64_t*)(a.st + S_VER + g * LINE); }
 __device__ __forceinline__ uint32_t group_waves(uint32_t grid, uint32_t g) {
@@ -185,6 +189,25 @@ __device__ __forceinline__ uint32_t bucket_key(uint32_t d, float inv_delta) {
 struct WaveStats {
   uint64_t v[NSTATS];
 };
+
+// Adds the workgroup's counters to its stats copy: every wavefront leaves its sums (lane 0's values plus
+// `extra`, which the caller adds for counters it keeps outside ws) in s_stat, then one atomic per nonzero
+// counter.  Every wavefront of the workgroup calls it once (it holds a workgroup barrier).
+__device__ __forceinline__ void publish_stats(uint32_t* st, uint64_t (*s_stat)[NSTATS], const WaveStats& ws,
+                                              const uint64_t* extra) {
+  const uint32_t wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NSTATS; i++) {
+    const uint64_t x = wave_sum64(ws.v[i]) + (extra ? extra[i] : 0ull);
+    if (lane_id() == 0) s_stat[wv][i] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NSTATS) {
+    uint64_t x = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; w++) x += s_stat[w][threadIdx.x];
+    if (x) afadd64((uint64_t*)(st + stat_copy(blockIdx.x % STAT_COPIES)) + threadIdx.x, x);
+  }
+}
 
 // Spin until a pool-chunk mapping or entry is written.  Every wait here is on a wavefront that has
 // already reserved the slot and is running, so the loops end; the cap and the stop flag turn a broken
@@ -743,20 +766,27 @@ __device__ __forceinline__ bool try_claim(const KArgs& a, uint32_t qq, uint32_t 
   if (lane_id() == 0) {
     // guided size: a share of what is published (at least claim_min), at most the queue's chunk
     const int32_t chunk = (int32_t)(qq >= nq ? a.chunk_heavy : a.chunk);
+    // The claim is registered in the bucket's claiming counter from before it can move the head until its
+    // in-flight count has landed, so that the head advance never sees the head past the claimed items
+    // while the claim is counted nowhere (the head and the in-flight count are two words, updated at once).
+    uint32_t* fl = inflight(a, bb & (a.nb - 1), (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u));
+    afadd(fl + INFL_CLAIMING, 1u);
     const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - (uint32_t)chunk);
     const int32_t avail = max(old, 0);
     const int32_t share = max((int32_t)a.claim_min, avail / (int32_t)a.claim_div);
     const int32_t take = min(min(avail, chunk), share);
     if (take < chunk) afadd(qavail(a, qq), (uint32_t)(chunk - take));
+    mem_wait();  // registered before the head moves
     if (take > 0) {
+      afadd(fl, (uint32_t)take);
       hh = afadd(qhead(a, qq), (uint32_t)take);
-      const uint32_t ic = (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u);
-      afadd(inflight(a, bb & (a.nb - 1), ic), (uint32_t)take);
       cc = (uint32_t)take;
       ok = 1;
     } else {
       ws.v[ST_CASFAIL]++;
     }
+    mem_wait();  // the in-flight count has landed
+    afadd(fl + INFL_CLAIMING, 0u - 1u);
   }
   if (!rdlane(ok, 0)) return false;
   h = rdlane(hh, 0);
@@ -788,9 +818,16 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
   __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
   __shared__ uint64_t s_hl[WAVES_PER_WG][8];
+  __shared__ uint64_t s_stat[WAVES_PER_WG][NSTATS];
 
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
+  // Launch l > 0 continues the solve after pull phase l, and only if launch l - 1 requested it (and
+  // the solve has not ended or failed); it clears the pull phases' work counters for the next one.
+  if (a.launch_idx > 0) {
+    if (ald(a.st + PULL_GEN) != a.launch_idx || stopped(a)) return;
+    if (blockIdx.x == 0 && threadIdx.x < 6) ast(a.st + S_PULL2 + threadIdx.x, 0u);
+  }
   PushBuf pbuf{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
   for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;  // no queue has id 0xFFFF
   wave_sync();
@@ -827,6 +864,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     // wavefront starts at its own part.
     bool got = false, from_local = false;
     uint32_t q = 0, h = 0, cnt = 0, cb = 0;
+    // A pull phase has been requested: this launch ends once every wavefront has finished what it holds
+    // (claims in progress complete and push their improvements; local items are processed first).
+    if (a.pull_slots && !nlocal &&
+        hld(a.st + PULL_GEN, a.cached_probe && (idle_iter & 7) != 7) > a.launch_idx)
+      break;
     if (nlocal) {
       got = true;
       from_local = true;
@@ -1061,14 +1103,20 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         nstale += (uint32_t)__popcll(ballot(stale));
         nreb += (uint32_t)__popcll(ballot(early[j]));
       }
+      uint64_t filed_heavy = 0;  // heavy edges of the vertices filed below (the pull-phase cost model)
 #pragma unroll
       for (int j = 0; j < IPL; j++) {
         const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
+        filed_heavy += file_heavy ? r1v[j] - r0v[j] - lcv[j] : 0u;
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
         nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], a.inv_delta) + 1 < cb));
         const uint32_t pp = REPL ? push_part(a, gbase[j], part) : part;
         pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], pp, cb, tail_mode), ws);
         pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + pp, ws);
+      }
+      if (a.pull_slots) {
+        filed_heavy = wave_sum64(filed_heavy);
+        if (lane == 0 && filed_heavy) afadd64(heavy_ctr(a, cb & (a.nb - 1), part), filed_heavy);
       }
       if (lane == 0) {
         ws.v[ST_STALE] += nstale;
@@ -1320,30 +1368,77 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       uint32_t z = 0;
       locked = acas(a.st + S_ADV, z, 1u) ? 1u : 0u;
     }
-    if (rdlane(locked, 0)) {
+    const uint32_t pgen = (rdlane(locked, 0) && a.pull_slots) ? rdlane(lane == 0 ? ald(a.st + PULL_GEN) : 0u, 0) : 0u;
+    if (pgen > a.launch_idx) {
+      // a pull phase is pending: the head stays where it is until the next launch
+      if (lane == 0) ast(a.st + S_ADV, 0u);
+    } else if (rdlane(locked, 0)) {
       const uint32_t c2 = ald(a.st + S_CUR);
       const uint32_t hs = c2 & (a.nb - 1), bs = (c2 - 1) & (a.nb - 1);
-      // head/behind state, one lane per part
+      // Head/behind state, one lane per part, in three rounds: (1) each queue's head, then its tail;
+      // (2) the in-flight and claiming counters; (3) the tails again.  A claim is counted (claiming, then
+      // in flight) before the head passes its items, and stops counting only after its pushes have
+      // reserved their tail positions.
+      // So a queue that reads empty in (1), idle in (2) and unchanged in (3) had no claim of its bucket
+      // running at any time in between (a single pass can miss a claim that starts or ends between its
+      // reads, and open a heavy phase or move the head too early).
       bool head_light = false, head_heavy = false, behind_any = false;
       if (lane < a.parts) {
-        head_light = ald(qhead(a, hs * a.parts + lane)) < ald(qtail(a, hs * a.parts + lane)) ||
-                     ald(inflight(a, hs, lane)) != 0;
-        head_heavy = ald(qhead(a, nq + hs * a.parts + lane)) < ald(qtail(a, nq + hs * a.parts + lane)) ||
-                     ald(inflight(a, hs, MAX_PARTS + lane)) != 0;
-        if (c2 > 0)
-          behind_any = ald(qhead(a, bs * a.parts + lane)) < ald(qtail(a, bs * a.parts + lane)) ||
-                       ald(qhead(a, nq + bs * a.parts + lane)) < ald(qtail(a, nq + bs * a.parts + lane)) ||
-                       ald(inflight(a, bs, lane)) != 0 || ald(inflight(a, bs, MAX_PARTS + lane)) != 0;
+        const bool beh = c2 > 0;
+        const uint32_t ql[4] = {hs * a.parts + lane, nq + hs * a.parts + lane, bs * a.parts + lane,
+                                nq + bs * a.parts + lane};
+        const uint32_t fi[4] = {lane, MAX_PARTS + lane, lane, MAX_PARTS + lane};
+        const uint32_t fs[4] = {hs, hs, bs, bs};
+        uint32_t h0[4], t0[4], f[4], t1[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const bool use = i < 2 || beh;
+          h0[i] = use ? ald(qhead(a, ql[i])) : 0u;
+          t0[i] = use ? ald(qtail(a, ql[i])) : 0u;
+        }
+        mem_wait();
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          f[i] = (i < 2 || beh) ? ald(inflight(a, fs[i], fi[i])) | ald(inflight(a, fs[i], fi[i]) + INFL_CLAIMING) : 0u;
+        mem_wait();
+#pragma unroll
+        for (int i = 0; i < 4; i++) t1[i] = (i < 2 || beh) ? ald(qtail(a, ql[i])) : 0u;
+        bool busy_q[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) busy_q[i] = h0[i] < t0[i] || f[i] != 0 || t1[i] != t0[i];
+        head_light = busy_q[0];
+        head_heavy = busy_q[1];
+        behind_any = beh && (busy_q[2] || busy_q[3]);
       }
       const bool hl = ballot(head_light) != 0, hh = ballot(head_heavy) != 0, ba = ballot(behind_any) != 0;
-      if (!hl) {
-        uint32_t op = 0;
-        if (lane == 0 && ald(phase(a, hs)) == 0) {
-          ast(phase(a, hs), 1u);
-          trace_event(a, 1, c2, c2);
-          op = 1;
+      if (!hl && rdlane(lane == 0 ? ald(phase(a, hs)) : 1u, 0) == 0) {
+        // The head bucket's light phase is done.  Its heavy edges are relaxed from the sending side
+        // (open the heavy queues), or -- when the queues hold many heavy edges and a pull slot is left
+        // -- from the receiving side by a pull phase: request it and let this launch end.
+        bool req = false;
+        uint64_t hv = 0;
+        if (a.launch_idx < a.pull_slots && pgen == a.launch_idx &&
+            rdlane(lane == 0 ? (ald(a.st + S_TAIL) == 0 && ald(a.pull_ok) != 0 ? 1u : 0u) : 0u, 0) != 0) {
+          hv = wave_sum64(lane < a.parts ? ald64(heavy_ctr(a, hs, lane)) : 0ull);
+          req = a.pull_eager ? hv > 0 : hv >= a.pull_min;
         }
-        opened = rdlane(op, 0) != 0;
+        if (req && ba) {
+          // a claim of the behind bucket can still push into this one: decide on a later pass
+        } else if (req) {
+          if (lane == 0) {
+            ast(a.st + PULL_BUCKET, c2);
+            ast64((uint64_t*)(a.st + PULL_H), hv);
+            mem_wait();
+            ast(a.st + PULL_GEN, pgen + 1);
+            trace_event(a, 5, c2, (uint32_t)min(hv, (uint64_t)0xFFFFFFFFu));
+          }
+        } else {
+          if (lane == 0) {
+            ast(phase(a, hs), 1u);
+            trace_event(a, 1, c2, c2);
+          }
+          opened = true;
+        }
       }
       if (!hl && !hh && !ba) {
         uint32_t firstj = 0xFFFFFFFFu;
@@ -1370,7 +1465,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           const uint32_t c3 = c2 - 1 + firstj;
           // buckets c2-1 .. c3-2 leave the window; their slots come back later as fresh buckets
           for (uint32_t b = c2 - 1 + lane; b + 1 < c3 && b - (c2 - 1) < a.nb; b += 64)
-            if (c2 > 0 || b != c2 - 1) ast(phase(a, b & (a.nb - 1)), 0u);
+            if (c2 > 0 || b != c2 - 1) {
+              ast(phase(a, b & (a.nb - 1)), 0u);
+              if (a.pull_slots)
+                for (uint32_t pi = 0; pi < a.parts; pi++) ast64(heavy_ctr(a, b & (a.nb - 1), pi), 0ull);
+            }
           // Few items left, and well past the peak of the solve: from now on the solve runs as one
           // bucket, whose heavy queues are open at once (a heavy edge relaxed before its vertex has
           // settled is relaxed again later, at negligible cost this late).  The peak test keeps the
@@ -1437,13 +1536,316 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     mem_wait();
     if (lane == 0) afadd64(verp(a, vgrp), 1ull);
   }
+  {
+    uint64_t extra[NSTATS];
 #pragma unroll
-  for (int i = 0; i < NSTATS; i++) {
-    const uint64_t s = wave_sum64(ws.v[i]);
-    if (lane == 0 && s) afadd64((uint64_t*)(a.st + S_STATS) + i, s);
+    for (int i = 0; i < NSTATS; i++) extra[i] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) extra[ST_T_PROBE + i] = tk[i];  // wave-uniform
+    publish_stats(a.st, s_stat, ws, extra);
   }
-  if (lane == 0)
-    for (int i = 0; i < 4; i++) afadd64((uint64_t*)(a.st + S_STATS) + ST_T_PROBE + i, tk[i]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// pull phases: a bucket's heavy edges relaxed from the receiving side
+// ------------------------------------------------------------------------------------------------
+// Delta-stepping relaxes the heavy edges of a settled bucket c once, from their sources (the push
+// direction above).  A heavy edge can only lower the distance of a vertex that is not yet settled
+// (d[u] >= c * delta and w > delta give d[u] + w > (c + 1) * delta > d[v] for every v of buckets <= c),
+// so the same relaxations are covered by letting every unsettled vertex v take min(d[v], d[u] + w) over
+// its heavy in-edges (u, w).  On a symmetric graph those are the heavy part of v's own light-first row.
+// When bucket c holds most of a power-law graph's edges and the unsettled vertices few (on RMAT-24 with
+// a hub source: 931M heavy edges leave bucket 0, 29M heavy edges enter the vertices still unsettled),
+// this scans a few percent of the edges, and every random gather is one the result needs.  The vertices
+// it lowers are queued in their buckets like any improvement; distances and their uniqueness are those
+// of the push direction (each is a real path length, and every relaxation the push direction would make
+// is made or cannot lower anything).
+//
+// k_pull_count sums the heavy in-degrees of the unsettled vertices; k_pull_a takes the cheaper direction
+// (opening the heavy queues for the next launch when pushing costs less) and relaxes rows of up to
+// PULL_LONG heavy edges, 256 vertex-instances per wavefront claim, handing longer rows to k_pull_b as
+// tasks of at most PULL_TASK edges, each relaxed by one wavefront and applied with an atomicMin.
+
+template <class A>
+__device__ __forceinline__ bool pull_requested(const A& a, uint32_t slot) {
+  return ald(a.st + PULL_GEN) == slot && !stopped(a);
+}
+template <bool FLT>
+__host__ __device__ constexpr uint32_t inf_word() {
+  return FLT ? 0x7F800000u : 0xFFFFFFFFu;
+}
+// not settled at bucket c: unreached, or its distance lies beyond bucket c
+template <bool FLT>
+__device__ __forceinline__ bool unsettled(uint32_t d, uint32_t c, float inv_delta) {
+  return d == inf_word<FLT>() || bucket_key<FLT>(d, inv_delta) > c;
+}
+// A path through an unreached vertex is no path (the saturating integer add would turn it into 2^32-2).
+template <bool FLT>
+__device__ __forceinline__ uint32_t pull_cand(uint32_t du, uint32_t w) {
+  return du == inf_word<FLT>() ? du : dadd<FLT>(du, w);
+}
+// Pushing costs one gathered edge per queued heavy edge (h); pulling costs its in-edges (a gathered edge
+// each, on short rows: counted twice), a row record per unsettled vertex and a distance per vertex.
+__device__ __forceinline__ bool pull_pays(uint64_t h, uint64_t in_edges, uint64_t unsettled_v, uint64_t total_n) {
+  return 2 * in_edges + unsettled_v / 2 + total_n / 16 <= h;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, d, 64));
+  return x;
+}
+
+constexpr uint32_t PULL_COUNT_THREADS = 1024;  // k_pull_count: few large workgroups, two atomics each
+template <bool FLT, bool REPL>
+__global__ void __launch_bounds__(PULL_COUNT_THREADS) k_pull_count(KArgs a, uint32_t slot) {
+  __shared__ uint64_t s_red[PULL_COUNT_THREADS / 64][2];
+  if (!pull_requested(a, slot)) return;
+  const uint32_t c = ald(a.st + PULL_BUCKET);
+  uint64_t in_edges = 0, uns = 0;
+  for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < a.total_n; x += (uint64_t)gridDim.x * blockDim.x) {
+    if (unsettled<FLT>(a.dist[x], c, a.inv_delta)) {
+      const uint4 ri = a.rinfo[REPL ? (uint32_t)x % a.n : (uint32_t)x];
+      in_edges += ri.y - ri.x - ri.z;
+      uns++;
+    }
+  }
+  in_edges = wave_sum64(in_edges);
+  uns = wave_sum64(uns);
+  if (lane_id() == 0) {
+    s_red[threadIdx.x >> 6][0] = in_edges;
+    s_red[threadIdx.x >> 6][1] = uns;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint64_t x = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; w++) x += s_red[w][threadIdx.x];
+    if (x) afadd64((uint64_t*)(a.st + S_PULL2 + 2 + 2 * threadIdx.x), x);
+  }
+}
+
+constexpr int PULL_UN = 8;  // 64-edge windows per wavefront in flight in the pull kernels
+
+template <bool FLT, bool REPL>
+__global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
+  __shared__ uint32_t s_pref[WAVES_PER_WG][64 * IPL + 1];
+  __shared__ uint32_t s_rs[WAVES_PER_WG][64 * IPL];
+  __shared__ uint32_t s_best[WAVES_PER_WG][64 * IPL];
+  __shared__ uint32_t s_gb[WAVES_PER_WG][REPL ? 64 * IPL : 1];
+  __shared__ uint64_t s_pbe[WAVES_PER_WG][PB_CAP];
+  __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
+  __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
+  __shared__ uint64_t s_stat[WAVES_PER_WG][NSTATS];
+  if (!pull_requested(a, slot)) return;
+  const uint32_t c = ald(a.st + PULL_BUCKET);
+  const bool pull = a.pull_eager || pull_pays(ald64((const uint64_t*)(a.st + PULL_H)),
+                                              ald64((const uint64_t*)(a.st + S_PULL2 + 2)),
+                                              ald64((const uint64_t*)(a.st + S_PULL2 + 4)), a.total_n);
+  const uint32_t cs = c & (a.nb - 1);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the bucket's heavy queues are done with: pulled (their items dropped) or opened for the next launch
+    ast(a.st + PULL_OUTCOME, pull ? 1u : 2u);
+    if (pull) {
+      ast(a.st + PULL_DONE, ald(a.st + PULL_DONE) + 1);
+      const uint32_t nq = nq_of(a);
+      for (uint32_t p = 0; p < a.parts; p++) {
+        const uint32_t q = nq + cs * a.parts + p;
+        ast(qhead(a, q), ald(qtail(a, q)));
+        ast(qavail(a, q), 0u);
+      }
+    }
+    ast(phase(a, cs), 1u);
+    trace_event(a, 6, c, pull ? 1u : 2u);
+  }
+  if (!pull) return;
+  const uint32_t lane = lane_id();
+  const uint32_t wv = threadIdx.x >> 6;
+  PushBuf pb{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
+  for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;
+  wave_sync();
+  const uint32_t part = blockIdx.x % a.parts;
+  WaveStats ws;
+#pragma unroll
+  for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
+  constexpr uint32_t INF = inf_word<FLT>();
+  // chunks of 256 vertex-instances dealt round-robin over the wavefronts (no shared counter: one address
+  // taking an atomic per chunk from every wavefront costs more than the scan itself)
+  const uint32_t nwaves = gridDim.x * WAVES_PER_WG;
+  for (uint64_t ch = (uint64_t)blockIdx.x * WAVES_PER_WG + wv; ch * (64 * IPL) < a.total_n; ch += nwaves) {
+    const uint32_t base = (uint32_t)(ch * (64 * IPL));
+    // items: vertex-instance x = base + j * 64 + lane (coalesced); range index k = lane * IPL + j
+    uint32_t x[IPL], dv[IPL], rs[IPL], deg[IPL], gb[IPL];
+    bool uns[IPL];
+#pragma unroll
+    for (int j = 0; j < IPL; j++) {
+      x[j] = base + j * 64 + lane;
+      dv[j] = x[j] < a.total_n ? a.dist[x[j]] : 0u;
+      uns[j] = x[j] < a.total_n && unsettled<FLT>(dv[j], c, a.inv_delta);
+      gb[j] = REPL ? x[j] - x[j] % a.n : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < IPL; j++) {
+      const uint4 ri = a.rinfo[uns[j] ? x[j] - gb[j] : 0u];
+      rs[j] = ri.x + ri.z;
+      deg[j] = uns[j] ? ri.y - rs[j] : 0u;
+    }
+    // rows longer than PULL_LONG become tasks for k_pull_b (all of this claim's, or, when the task list
+    // is full, none: then they are relaxed here and the reserved positions hold empty tasks)
+    {
+      uint32_t nt[IPL], mine = 0;
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        nt[j] = deg[j] > PULL_LONG ? (deg[j] + PULL_TASK - 1) / PULL_TASK : 0u;
+        mine += nt[j];
+      }
+      const uint32_t incl = wave_incl_scan(mine);
+      const uint32_t total = rdlane(incl, 63);
+      if (total) {
+        uint32_t tb = 0;
+        if (lane == 0) tb = afadd(a.st + S_PULL2, total);
+        tb = rdlane(tb, 0);
+        const bool fits = (uint64_t)tb + total <= a.ptask_cap;
+        uint32_t pos = tb + incl - mine;
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          for (uint32_t t = 0; t < nt[j]; t++, pos++) {
+            if (pos >= a.ptask_cap) break;
+            const uint32_t e0 = rs[j] + t * PULL_TASK;
+            a.ptask[pos] = fits ? make_uint4(x[j], e0, min(e0 + PULL_TASK, rs[j] + deg[j]), 0u)
+                                : make_uint4(0u, 0u, 0u, 0u);
+          }
+          if (fits && nt[j]) deg[j] = 0;
+        }
+      }
+    }
+    // flatten the remaining rows (range k = lane * IPL + j owns [s_pref[k], s_pref[k+1]))
+    uint32_t mysum = 0;
+#pragma unroll
+    for (int j = 0; j < IPL; j++) mysum += deg[j];
+    const uint32_t incl = wave_incl_scan(mysum);
+    const uint32_t total = rdlane(incl, 63);
+    uint32_t run = incl - mysum;
+#pragma unroll
+    for (int j = 0; j < IPL; j++) {
+      const uint32_t k = lane * IPL + j;
+      s_pref[wv][k] = run;
+      s_rs[wv][k] = rs[j];
+      s_best[wv][k] = dv[j];
+      if (REPL) s_gb[wv][k] = gb[j];
+      run += deg[j];
+    }
+    if (lane == 0) s_pref[wv][64 * IPL] = total;
+    wave_sync();
+    for (uint32_t eb = 0; eb < total; eb += 64 * PULL_UN) {
+      uint32_t idx[PULL_UN], lo[PULL_UN], u[PULL_UN], w[PULL_UN], du[PULL_UN];
+#pragma unroll
+      for (int j = 0; j < PULL_UN; j++) {
+        idx[j] = eb + 64 * j + lane;
+        lo[j] = 0;
+      }
+#pragma unroll
+      for (uint32_t step = 32 * IPL; step >= 1; step >>= 1) {
+#pragma unroll
+        for (int j = 0; j < PULL_UN; j++)
+          if (s_pref[wv][lo[j] + step] <= idx[j]) lo[j] += step;
+      }
+#pragma unroll
+      for (int j = 0; j < PULL_UN; j++) {
+        const uint32_t e = idx[j] < total ? s_rs[wv][lo[j]] + (idx[j] - s_pref[wv][lo[j]]) : 0u;
+        const uint64_t cw = __builtin_nontemporal_load((const uint64_t*)a.cw + e);
+        u[j] = (uint32_t)cw;
+        w[j] = (uint32_t)(cw >> 32);
+      }
+#pragma unroll
+      for (int j = 0; j < PULL_UN; j++) du[j] = a.dist[(REPL ? s_gb[wv][lo[j]] : 0u) + (idx[j] < total ? u[j] : 0u)];
+#pragma unroll
+      for (int j = 0; j < PULL_UN; j++) {
+        const uint32_t cand = idx[j] < total ? pull_cand<FLT>(du[j], w[j]) : INF;
+        if (cand < INF)
+          __hip_atomic_fetch_min(&s_best[wv][lo[j]], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    wave_sync();
+    uint32_t nimp = 0;
+#pragma unroll
+    for (int j = 0; j < IPL; j++) {
+      const uint32_t best = s_best[wv][lane * IPL + j];
+      const bool imp = uns[j] && best < dv[j];
+      if (imp) ast(a.dist + x[j], best);
+      nimp += (uint32_t)__popcll(ballot(imp));
+      const uint32_t pp = REPL ? push_part(a, gb[j], part) : part;
+      pb_add(a, pb, imp, ((uint64_t)best << 32) | x[j], light_queue<FLT>(a, best, pp, c, false), ws);
+    }
+    wave_sync();
+    if (lane == 0) {
+      ws.v[ST_EDGES] += total;
+      ws.v[ST_PULL_EDGES] += total;
+      ws.v[ST_PULL_IMPROVED] += nimp;
+    }
+  }
+  pb_flush(flush_args(a), pb, ws);
+  publish_stats(a.st, s_stat, ws, nullptr);
+}
+
+template <bool FLT, bool REPL>
+__global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
+  __shared__ uint64_t s_pbe[WAVES_PER_WG][PB_CAP];
+  __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
+  __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
+  __shared__ uint64_t s_stat[WAVES_PER_WG][NSTATS];
+  if (!pull_requested(a, slot) || ald(a.st + PULL_OUTCOME) != 1u) return;
+  const uint32_t c = ald(a.st + PULL_BUCKET);
+  const uint32_t ntask = min(ald(a.st + S_PULL2), a.ptask_cap);
+  const uint32_t lane = lane_id();
+  const uint32_t wv = threadIdx.x >> 6;
+  PushBuf pb{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
+  for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;
+  wave_sync();
+  const uint32_t part = blockIdx.x % a.parts;
+  WaveStats ws;
+#pragma unroll
+  for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
+  constexpr uint32_t INF = inf_word<FLT>();
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = afadd(a.st + S_PULL2 + 1, 1u);
+    t = rdlane(t, 0);
+    if (t >= ntask) break;
+    const uint4 tk = a.ptask[t];
+    if (tk.y >= tk.z) continue;  // an empty task (the list was full)
+    const uint32_t gb = REPL ? tk.x - tk.x % a.n : 0u;
+    uint32_t best = INF;
+    for (uint32_t e0 = tk.y; e0 < tk.z; e0 += 64 * PULL_UN) {
+      uint32_t u[PULL_UN], w[PULL_UN], du[PULL_UN];
+      bool act[PULL_UN];
+#pragma unroll
+      for (int j = 0; j < PULL_UN; j++) {
+        const uint32_t e = e0 + 64 * j + lane;
+        act[j] = e < tk.z;
+        const uint64_t cw = __builtin_nontemporal_load((const uint64_t*)a.cw + (act[j] ? e : tk.y));
+        u[j] = (uint32_t)cw;
+        w[j] = (uint32_t)(cw >> 32);
+      }
+#pragma unroll
+      for (int j = 0; j < PULL_UN; j++) du[j] = a.dist[gb + u[j]];
+#pragma unroll
+      for (int j = 0; j < PULL_UN; j++)
+        if (act[j]) best = min(best, pull_cand<FLT>(du[j], w[j]));
+    }
+    best = wave_min_u32(best);
+    bool imp = false;
+    if (lane == 0 && best < INF) {
+      const uint32_t old = afmin(a.dist + tk.x, best);
+      imp = best < old;
+      ws.v[ST_RELAX]++;
+      ws.v[ST_EDGES] += tk.z - tk.y;
+      ws.v[ST_PULL_EDGES] += tk.z - tk.y;
+      if (imp) ws.v[ST_PULL_IMPROVED]++;
+    }
+    const uint32_t pp = REPL ? push_part(a, gb, part) : part;
+    pb_add(a, pb, imp, ((uint64_t)best << 32) | tk.x, light_queue<FLT>(a, best, pp, c, false), ws);
+  }
+  pb_flush(flush_args(a), pb, ws);
+  publish_stats(a.st, s_stat, ws, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1764,6 +2166,24 @@ static void launch_solve_un(const KArgs& a, bool is_float, bool replicated, uint
 hipError_t launch_solve(const KArgs& a, bool is_float, bool replicated, uint32_t grid, hipStream_t s) {
   if (a.windows == 8) launch_solve_un<8>(a, is_float, replicated, grid, s);
   else launch_solve_un<4>(a, is_float, replicated, grid, s);
+  return hipGetLastError();
+}
+template <bool FLT, bool REPL>
+static void launch_pull_t(const KArgs& a, uint32_t slot, uint32_t grid, hipStream_t s) {
+  const uint32_t cblocks = std::max<uint32_t>(1, std::min<uint32_t>((a.total_n + PULL_COUNT_THREADS - 1) /
+                                                                   PULL_COUNT_THREADS, 256));
+  hipLaunchKernelGGL((k_pull_count<FLT, REPL>), dim3(cblocks), dim3(PULL_COUNT_THREADS), 0, s, a, slot);
+  hipLaunchKernelGGL((k_pull_a<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
+  hipLaunchKernelGGL((k_pull_b<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
+}
+hipError_t launch_pull(const KArgs& a, bool is_float, bool replicated, uint32_t slot, uint32_t grid, hipStream_t s) {
+  if (is_float) {
+    if (replicated) launch_pull_t<true, true>(a, slot, grid, s);
+    else launch_pull_t<true, false>(a, slot, grid, s);
+  } else {
+    if (replicated) launch_pull_t<false, true>(a, slot, grid, s);
+    else launch_pull_t<false, false>(a, slot, grid, s);
+  }
   return hipGetLastError();
 }
 hipError_t launch_cleanup(const KArgs& a, hipStream_t s) {

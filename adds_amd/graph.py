@@ -24,8 +24,11 @@ def _wtype(w):
 class HostGraph:
     """A CSR graph in host memory (row_ptr[n+1], col[m], 4-byte weights w[m])."""
 
-    def __init__(self, csr: L.HostCSR):
+    def __init__(self, csr: L.HostCSR, symmetric=False):
         self._csr = csr
+        # every edge (u, v, w) has its reverse (v, u, w): the generators build such graphs (one weight per
+        # undirected pair); files and arrays are not assumed to be (see check_symmetric)
+        self.symmetric = bool(symmetric)
         n, m = csr.n, csr.m
         self.n, self.m = int(n), int(m)
         self.wtype = int(csr.wtype)
@@ -47,25 +50,25 @@ class HostGraph:
 
     # ---- construction ----
     @classmethod
-    def _from_call(cls, fn, *args, what="generator"):
+    def _from_call(cls, fn, *args, what="generator", symmetric=False):
         csr = L.HostCSR()
         L.check(fn(*args, ctypes.byref(csr)), what)
-        return cls(csr)
+        return cls(csr, symmetric=symmetric)
 
     @classmethod
     def rmat(cls, scale, edge_factor, seed, wkind="u32", lo=1, hi=100, a=0.57, b=0.19, c=0.19):
         return cls._from_call(L._lib().adds_gen_rmat, scale, edge_factor, a, b, c, seed, WKINDS[wkind],
-                              float(lo), float(hi), what="rmat")
+                              float(lo), float(hi), what="rmat", symmetric=True)
 
     @classmethod
     def grid(cls, rows, cols, shortcuts, seed, wkind="f32", lo=1.0, hi=2.0):
         return cls._from_call(L._lib().adds_gen_grid, rows, cols, shortcuts, seed, WKINDS[wkind], float(lo),
-                              float(hi), what="grid")
+                              float(hi), what="grid", symmetric=True)
 
     @classmethod
     def er(cls, n, avg_degree, seed, wkind="u32", lo=1, hi=1000):
         return cls._from_call(L._lib().adds_gen_er, n, float(avg_degree), seed, WKINDS[wkind], float(lo),
-                              float(hi), what="er")
+                              float(hi), what="er", symmetric=True)
 
     @classmethod
     def read_gr(cls, path, wtype="int"):
@@ -82,9 +85,9 @@ class HostGraph:
         return cls._from_call(L._lib().adds_read_dimacs, str(path).encode(), _wtype(wtype), what=str(path))
 
     @classmethod
-    def from_arrays(cls, row_ptr, col, w, wtype):
+    def from_arrays(cls, row_ptr, col, w, wtype, symmetric=False):
         """A host graph holding its own copies of the given arrays (they may be views into another graph,
-        which can then go away)."""
+        which can then go away); `symmetric` declares that every edge has its reverse with the same weight."""
         wt = _wtype(wtype)
         row_ptr = np.array(row_ptr, dtype=np.uint32, copy=True, order="C")
         col = np.array(col, dtype=np.uint32, copy=True, order="C")
@@ -94,6 +97,7 @@ class HostGraph:
         obj.n, obj.m, obj.wtype = len(row_ptr) - 1, len(col), wt
         obj.row_ptr, obj.col, obj.w_bits = row_ptr, col, wb
         obj.w = wb.view(np.float32) if wt == L.W_F32 else wb
+        obj.symmetric = bool(symmetric)
         return obj
 
     def _csr_view(self):
@@ -116,6 +120,16 @@ class HostGraph:
         return self
 
     # ---- utilities ----
+    def check_symmetric(self):
+        """True when every edge (u, v, w) has its reverse (v, u, w) with the same weight bits, counting
+        multiplicities (an exact test: the sorted (u, v, w) triples equal the sorted (v, u, w) triples)."""
+        u = np.repeat(np.arange(self.n, dtype=np.uint64), np.diff(self.row_ptr.astype(np.int64)))
+        v = self.col.astype(np.uint64)
+        wb = self.w_bits.astype(np.uint64)
+        a = np.lexsort((wb, v, u))
+        b = np.lexsort((wb, u, v))
+        return bool(np.array_equal(u[a], v[b]) and np.array_equal(v[a], u[b]) and np.array_equal(wb[a], wb[b]))
+
     @property
     def is_float(self):
         return self.wtype == L.W_F32
@@ -173,6 +187,8 @@ class Graph:
         self._h = h
         self.n, self.m, self.wtype = host.n, host.m, host.wtype
         self.host = host
+        if getattr(host, "symmetric", False):
+            L.check(L._lib().adds_graph_set_symmetric(h, 1), "graph")
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -89,6 +89,10 @@ typedef struct {
                                  source's distance row is worked on mostly by one XCD and its L2; pays on
                                  many sources of a power-law graph (256 sources on RMAT-20: -6%), loses on
                                  grids (hops leave the wavefront's part, so local continuation stops) */
+#define ADDS_F_NO_PULL 0x40u  /* no pull phases (see adds_graph_set_symmetric): every heavy phase relaxes
+                                 its edges from their sources */
+#define ADDS_F_PULL_EAGER 0x80u /* request a pull phase at the first heavy phase with any heavy edge and
+                                 always take it, whatever the cost estimate says (tests, tuning) */
 
 typedef struct {
   double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */
@@ -113,6 +117,9 @@ typedef struct {
   uint64_t local_items;       /* improvements processed by the wavefront that found them (no queue trip) */
   uint64_t heavy_edges;       /* of edges_scanned, those relaxed in heavy phases */
   uint64_t ticks_flush;       /* of ticks_items + ticks_edges, the time in push-buffer flushes */
+  uint64_t pulls;             /* pull phases performed (see adds_graph_set_symmetric) */
+  uint64_t pull_edges;        /* of edges_scanned, in-edges scanned by pull phases */
+  uint64_t pull_improved;     /* vertices whose distance a pull phase lowered */
 } adds_stats;
 
 /* ---- errors / device ---- */
@@ -136,6 +143,15 @@ adds_status adds_graph_create_device(const uint32_t* row_ptr_dev, const uint32_t
                                      adds_graph** out);
 /* Overwrites the edge weights from a device buffer of m 4-byte words (used by the autograd binding). */
 adds_status adds_graph_set_weights_device(adds_graph* g, const void* w_dev, void* stream);
+/* Declares (symmetric != 0) that every edge (u, v, w) of g has its reverse (v, u, w) with the same weight
+ * bits, or withdraws the declaration.  No reference counterpart: it enables pull phases, which relax a
+ * settled bucket's heavy edges from their targets, reading each unsettled vertex's in-edges from its own
+ * row; they are taken when the bucket's heavy edges (its push-direction work) outnumber the unsettled
+ * vertices' heavy in-edges (on RMAT-24 with a hub source, bucket 0: 931M against 29M).  Distances are
+ * the same either way.  A declaration that is false gives wrong distances; adds_graph_set_weights_device
+ * withdraws it (also for solves captured before).  A union of graphs (batched solves) is symmetric when
+ * every member is.  Graphs start undeclared. */
+adds_status adds_graph_set_symmetric(adds_graph* g, int symmetric);
 void adds_graph_destroy(adds_graph* g);
 adds_status adds_graph_info(const adds_graph* g, uint32_t* n, uint64_t* m, adds_wtype* wt);
 /* Device pointers of the resident CSR (row_ptr, col, w); valid until the handle is destroyed. */

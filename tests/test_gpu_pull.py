@@ -1,0 +1,217 @@
+"""GPU parity of pull phases: a settled bucket's heavy edges relaxed from their targets (symmetric graphs).
+
+Every case compares against the oracle's Dijkstra bit for bit and checks from the solve's counters that
+a pull phase really ran (or, where it must not, that none did).  ADDS_F_PULL_EAGER makes the solver take
+a pull at the first heavy phase whatever its cost estimate, so that small graphs exercise the path.
+"""
+import numpy as np
+import pytest
+import torch
+
+import adds_amd
+import oracle
+from adds_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+TIMEOUT = 60000
+EAGER = L.F_PULL_EAGER
+
+
+def _check(g, dist, srcs, is_float, saturate=False):
+    ref = oracle.dijkstra(g.row_ptr, g.col, g.w, srcs, is_float, saturate=saturate)
+    got = np.ascontiguousarray(dist).view(np.uint32)
+    bad = np.nonzero(got != ref)[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: got {got[bad[:5]]} want {ref[bad[:5]]}"
+
+
+def _symmetrize(row, col, w):
+    """Both directions of every edge, one weight per unordered pair (the smaller of the two)."""
+    n = len(row) - 1
+    u = np.repeat(np.arange(n, dtype=np.int64), np.diff(row.astype(np.int64)))
+    v = col.astype(np.int64)
+    a, b = np.minimum(u, v), np.maximum(u, v)
+    keep = a != b
+    a, b, ww = a[keep], b[keep], w[keep]
+    order = np.lexsort((ww, b, a))
+    a, b, ww = a[order], b[order], ww[order]
+    first = np.ones(len(a), dtype=bool)
+    first[1:] = (a[1:] != a[:-1]) | (b[1:] != b[:-1])
+    a, b, ww = a[first], b[first], ww[first]
+    src = np.concatenate([a, b])
+    dst = np.concatenate([b, a])
+    wts = np.concatenate([ww, ww])
+    o = np.lexsort((dst, src))
+    src, dst, wts = src[o], dst[o], wts[o]
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    np.add.at(rp, src + 1, 1)
+    rp = np.cumsum(rp).astype(np.uint32)
+    return rp, dst.astype(np.uint32), wts
+
+
+@pytest.mark.parametrize("is_float", [False, True])
+@pytest.mark.parametrize("params", [
+    {},
+    dict(flags=EAGER),
+    dict(flags=EAGER | L.F_NO_WAIT | L.F_NO_LOCAL, edge_windows=8, chunk=256, parts=4),
+    dict(flags=EAGER, parts=1, nbuckets=4, chunk=16),
+    dict(flags=EAGER, tail_items=4096, concurrency=4),
+    dict(flags=EAGER, hub_degree=64, task_edges=64),
+])
+def test_rmat_pull_identical(is_float, params):
+    """RMAT from its highest-degree vertex: bucket 0 holds most of the graph's edges.  With the default
+    cost model a graph this size may or may not pull; with EAGER it always does."""
+    wk = dict(wkind="f32", lo=0.0, hi=1.0) if is_float else dict(wkind="u32", lo=1, hi=65535)
+    g = adds_amd.HostGraph.rmat(15, 32, seed=21, **wk)
+    assert g.symmetric
+    src = g.max_degree_vertex()
+    delta = 0.0625 if is_float else 4096.0
+    dist, st = adds_amd.Graph(g).sssp(src, timeout_ms=TIMEOUT, delta=delta, **params)
+    _check(g, dist, src, is_float)
+    if params.get("flags", 0) & EAGER:
+        assert st["pulls"] == 1 and st["pull_edges"] > 0 and st["pull_improved"] > 0, st
+
+
+def test_rmat18_default_cost_model_pulls():
+    """RMAT-18 ef32 from the hub with the default rule: bucket 0's heavy edges far outnumber the unsettled
+    vertices' heavy in-edges, so the solver pulls without being told to, and scans fewer edges than the
+    push direction would."""
+    g = adds_amd.HostGraph.rmat(18, 32, seed=1, wkind="u32", lo=1, hi=65535)
+    src = g.max_degree_vertex()
+    dg = adds_amd.Graph(g)
+    dist, st = dg.sssp(src, timeout_ms=TIMEOUT, delta=4096.0)
+    _check(g, dist, src, False)
+    assert st["pulls"] == 1, st
+    dpush, sp = dg.sssp(src, timeout_ms=TIMEOUT, delta=4096.0, flags=L.F_NO_PULL)
+    _check(g, dpush, src, False)
+    assert sp["pulls"] == 0 and sp["pull_edges"] == 0
+    assert st["edges_scanned"] < sp["edges_scanned"] / 2, (st["edges_scanned"], sp["edges_scanned"])
+
+
+def test_low_degree_source_long_unsettled_rows():
+    """From a low-degree vertex the hubs are still unsettled when the first heavy phase opens: their
+    rows (thousands of heavy edges) go through the long-row tasks, each applied with an atomicMin."""
+    g = adds_amd.HostGraph.rmat(15, 32, seed=5, wkind="u32", lo=1, hi=65535)
+    deg = g.degrees()
+    cand = np.nonzero((deg >= 2) & (deg <= 4))[0]
+    src = int(cand[len(cand) // 2])
+    assert deg.max() > 4 * 1024  # the hub's row spans several tasks
+    for delta in (512.0, 4096.0):
+        dist, st = adds_amd.Graph(g).sssp(src, timeout_ms=TIMEOUT, delta=delta, flags=EAGER)
+        _check(g, dist, src, False)
+        assert st["pulls"] == 1
+
+
+@pytest.mark.parametrize("flags", [EAGER, EAGER | L.F_REPLICA_PARTS])
+def test_multi_source_pull(flags):
+    """Many sources in one solve (replicated distance rows): a pull covers every instance's row."""
+    g = adds_amd.HostGraph.rmat(13, 16, seed=9, wkind="f32", lo=1.0, hi=10.0)
+    srcs = g.sample_component(g.max_degree_vertex(), 12, seed=4)
+    dm, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT, delta=2.0, parts=4, flags=flags)
+    assert st["pulls"] == 1
+    for i, s in enumerate(srcs):
+        _check(g, dm[i], s, True)
+
+
+def test_batched_union_pull():
+    """Graphs of uneven size in one union solve: the union of symmetric graphs is symmetric."""
+    gs = [adds_amd.HostGraph.rmat(11, 16, seed=30 + i, wkind="u32", lo=1, hi=1000) for i in range(3)]
+    gs += [adds_amd.HostGraph.grid(40, 50, 10, seed=7, wkind="u32", lo=1, hi=1000),
+           adds_amd.HostGraph.er(3000, 8.0, seed=8, wkind="u32", lo=1, hi=1000)]
+    dgs = [adds_amd.Graph(h) for h in gs]
+    srcs = [h.max_degree_vertex() for h in gs]
+    dists, st = adds_amd.sssp_batched(dgs, srcs, timeout_ms=TIMEOUT, delta=64.0, flags=EAGER)
+    assert st["pulls"] == 1
+    for h, d, s in zip(gs, dists, srcs):
+        _check(h, d, s, False)
+
+
+def test_undeclared_and_reweighted_graphs_do_not_pull():
+    """A graph not declared symmetric never pulls; nor does one whose weights were replaced (new weights
+    need not be symmetric) until it is declared again."""
+    g = adds_amd.HostGraph.rmat(13, 16, seed=3, wkind="f32", lo=1.0, hi=10.0)
+    src = g.max_degree_vertex()
+    plain = adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, g.w, "f32")
+    assert not plain.symmetric
+    d0, st0 = adds_amd.Graph(plain).sssp(src, timeout_ms=TIMEOUT, delta=2.0, flags=EAGER)
+    assert st0["pulls"] == 0
+    _check(g, d0, src, True)
+    dg = adds_amd.Graph(g)
+    _, st1 = dg.sssp(src, timeout_ms=TIMEOUT, delta=2.0, flags=EAGER)
+    assert st1["pulls"] == 1
+    # asymmetric new weights: each direction of a pair gets its own value
+    rng = np.random.default_rng(1)
+    w2 = (rng.random(g.m).astype(np.float32) * 9 + 1).astype(np.float32)
+    wt = torch.from_numpy(w2).cuda()
+    dg.set_weights_device(wt.data_ptr())
+    torch.cuda.synchronize()
+    d2, st2 = dg.sssp(src, timeout_ms=TIMEOUT, delta=2.0, flags=EAGER)
+    assert st2["pulls"] == 0
+    h2 = adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, w2, "f32")
+    _check(h2, d2, src, True)
+
+
+def test_captured_pull_replays_and_weight_update():
+    """A captured solve with a pull phase replays bit-exact; after set_weights_device it stops pulling
+    (the graph's device word is cleared) and still gives the new weights' distances."""
+    g = adds_amd.HostGraph.rmat(13, 16, seed=6, wkind="u32", lo=1, hi=4000)
+    srcs = g.sample_component(g.max_degree_vertex(), 4, seed=3)
+    dg = adds_amd.Graph(g)
+    dist = torch.empty((len(srcs), g.n), dtype=torch.int32, device="cuda")
+    sd = torch.from_numpy(srcs.astype(np.int32)).cuda()
+    sg = adds_amd.SolveGraph(dg, sd.data_ptr(), len(srcs), dist.data_ptr(), delta=256.0, flags=EAGER)
+    ref = None
+    for i in range(3):
+        dist.fill_(5)
+        torch.cuda.synchronize()
+        sg.launch()
+        st = sg.status()
+        assert st["pulls"] == 1, st
+        if ref is None:
+            ref = dist.clone()
+            for j, s in enumerate(srcs):
+                _check(g, ref[j].cpu().numpy(), s, False)
+        assert torch.equal(dist, ref), f"replay {i} differs"
+    # symmetric new weights (the same rule per unordered pair): distances change, pulls stop
+    g2 = adds_amd.HostGraph.rmat(13, 16, seed=6, wkind="u32", lo=1, hi=4000).reweight(seed=77, wkind="u32", lo=1,
+                                                                                     hi=4000)
+    wt = torch.from_numpy(g2.w_bits.view(np.int32).copy()).cuda()
+    dg.set_weights_device(wt.data_ptr())
+    torch.cuda.synchronize()
+    sg.launch()
+    st = sg.status()
+    assert st["pulls"] == 0
+    for j, s in enumerate(srcs):
+        _check(g2, dist[j].cpu().numpy(), s, False)
+    sg.close()
+
+
+@pytest.mark.parametrize("params", [dict(flags=EAGER), dict(flags=EAGER | L.F_NO_WAIT, delta=1.0)])
+def test_saturating_sums_in_pull(params):
+    """The saturation case made symmetric: a pull reads unreached neighbours (no path) and saturated
+    ones (2^32-2 plus anything stays 2^32-2) exactly as the push direction and the oracle do."""
+    from test_oracle import saturation_case
+
+    row, col, w, _ = saturation_case()
+    rp, c2, w2 = _symmetrize(np.asarray(row), np.asarray(col), np.asarray(w, dtype=np.uint32))
+    g = adds_amd.HostGraph.from_arrays(rp, c2, w2, "u32", symmetric=True)
+    assert g.check_symmetric()
+    dg = adds_amd.Graph(g)
+    for s in range(g.n):
+        dist, _ = dg.sssp(s, timeout_ms=TIMEOUT, **params)
+        _check(g, dist, s, False, saturate=True)
+
+
+def test_disconnected_pull_keeps_unreached():
+    """Vertices of another component stay unreached: an in-edge from an unreached vertex is no path."""
+    a = adds_amd.HostGraph.rmat(12, 16, seed=2, wkind="u32", lo=1, hi=65535)
+    n = a.n
+    rp = np.concatenate([a.row_ptr, a.row_ptr[1:] + a.m]).astype(np.uint32)
+    col = np.concatenate([a.col, a.col + n]).astype(np.uint32)
+    w = np.concatenate([a.w_bits, a.w_bits]).astype(np.uint32)
+    g = adds_amd.HostGraph.from_arrays(rp, col, w, "u32", symmetric=True)
+    src = a.max_degree_vertex()
+    dist, st = adds_amd.Graph(g).sssp(src, timeout_ms=TIMEOUT, delta=1024.0, flags=EAGER)
+    assert st["pulls"] == 1
+    _check(g, dist, src, False)
+    assert np.all(dist.view(np.uint32)[n:] == L.INF_U32)

@@ -73,6 +73,31 @@ def test_generators_simple_symmetric_and_in_range():
     assert gl.w.min() > 0
 
 
+def test_symmetric_declaration_matches_exact_check():
+    """The generators declare their graphs symmetric (what enables pull phases); the exact check -- the
+    sorted (u, v, w) triples equal the sorted (v, u, w) triples -- agrees, and it rejects a graph with one
+    reversed pair's weight changed, one edge removed, or arrays not declared symmetric."""
+    gens = [adds_amd.HostGraph.rmat(10, 8, seed=3, wkind="u32", lo=1, hi=100),
+            adds_amd.HostGraph.rmat(9, 16, seed=3, wkind="f32", lo=0.0, hi=1.0).reweight(seed=9, wkind="f32"),
+            adds_amd.HostGraph.grid(30, 40, 25, seed=2, wkind="f32", lo=1.0, hi=2.0),
+            adds_amd.HostGraph.er(2000, 8, seed=5, wkind="u32", lo=1, hi=1000)]
+    for g in gens:
+        assert g.symmetric and g.check_symmetric()
+    g = gens[0]
+    w = g.w_bits.copy()
+    w[5] += 1  # one direction of a pair
+    bad = adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, w, "u32")
+    assert not bad.symmetric and not bad.check_symmetric()
+    # drop the last edge of vertex 0's row: its reverse is left alone
+    r = g.row_ptr.astype(np.int64)
+    keep = np.ones(g.m, dtype=bool)
+    keep[r[1] - 1] = False
+    rp = g.row_ptr.copy()
+    rp[1:] -= 1
+    assert not adds_amd.HostGraph.from_arrays(rp, g.col[keep], g.w_bits[keep], "u32").check_symmetric()
+    assert adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, g.w_bits, "u32").check_symmetric()
+
+
 def test_grid_shape():
     g = adds_amd.HostGraph.grid(2048, 2048, 0, seed=1)
     assert g.n == 2048 * 2048 and g.m == 16769024  # 4-neighbour directed edges
