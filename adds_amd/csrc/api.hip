@@ -350,6 +350,64 @@ extern "C" adds_status adds_graph_set_symmetric(adds_graph* g, int symmetric) {
   return ADDS_OK;
 }
 
+// Counts the edges that break "symmetric with strictly ascending rows": an edge (u, v, w) whose reverse
+// (v, u, w) is missing (binary search for u in v's row; equal weight bits), or whose destination is not
+// above its predecessor's in the row.  One edge per thread; its row is found by binary search over the
+// row offsets (the lanes of a wavefront mostly share one row, so those probes are broadcasts).
+__global__ static void k_check_sym(const uint32_t* row, const uint32_t* col, const uint32_t* w, uint32_t n,
+                                   uint64_t m, unsigned long long* bad) {
+  __shared__ unsigned long long s_bad[256 / 64];
+  unsigned long long nb = 0;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = n;  // row[lo] <= e < row[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (row[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t u = lo, v = col[e], wt = w[e];
+    if (e > row[u] && col[e - 1] >= v) nb++;
+    uint32_t a = row[v], b = row[v + 1];
+    const uint32_t end = b;
+    while (a < b) {
+      const uint32_t mid = a + (b - a) / 2;
+      if (col[mid] < u) a = mid + 1;
+      else b = mid;
+    }
+    if (!(a < end && col[a] == u && w[a] == wt)) nb++;
+  }
+  for (int d = 32; d >= 1; d >>= 1) nb += __shfl_xor(nb, d, 64);
+  if ((threadIdx.x & 63) == 0) s_bad[threadIdx.x >> 6] = nb;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t i = 0; i < blockDim.x / 64; i++) t += s_bad[i];
+    if (t) atomicAdd(bad, t);
+  }
+}
+
+extern "C" adds_status adds_graph_check_symmetric(adds_graph* g, int* symmetric) {
+  if (!g || !symmetric) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  unsigned long long* bad = nullptr;
+  HIPCHK(hipMalloc(&bad, 8));
+  unsigned long long hb = 0;
+  hipError_t e = hipMemcpy(bad, &hb, 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess && g->m) {
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((g->m + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_check_sym, dim3(blocks), dim3(256), 0, nullptr, g->row, g->col, g->w, g->n, g->m, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&hb, bad, 8, hipMemcpyDeviceToHost);
+  (void)hipFree(bad);
+  if (e != hipSuccess) {
+    g_detail = std::string("symmetry check: ") + hipGetErrorString(e);
+    return ADDS_EHIP;
+  }
+  *symmetric = hb == 0 ? 1 : 0;
+  return adds_graph_set_symmetric(g, *symmetric);
+}
+
 extern "C" void adds_graph_destroy(adds_graph* g) {
   if (!g) return;
   (void)hipSetDevice(g->device);

@@ -120,7 +120,7 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
   p->parts = (uint32_t)parts;
   p->concurrency = (uint32_t)conc;
   p->task_edges = (uint32_t)te;
-  p->flags = (uint32_t)flags & (ADDS_F_NO_LOCAL | ADDS_F_NO_WAIT);  // tuning flags only
+  p->flags = (uint32_t)flags & (ADDS_F_NO_LOCAL | ADDS_F_NO_WAIT | ADDS_F_NO_PULL);  // tuning flags only
   p->edge_windows = (uint32_t)win;
   p->tail_items = (uint32_t)tail;
   p->hub_degree = (uint32_t)hub;
@@ -217,6 +217,11 @@ int main(int argc, char** argv) {
 #if !ADDS_NF
   const std::string tuned_from = lookup_tuned(h, &p);
   if (!tuned_from.empty()) printf("tuned parameters from %s\n", tuned_from.c_str());
+  // a symmetric graph (every edge with its reverse) lets settled buckets' heavy edges be pulled
+  int symmetric = 0;
+  s = adds_graph_check_symmetric(g, &symmetric);
+  if (s != ADDS_OK) return fail("symmetry check", s);
+  printf("symmetric: %s\n", symmetric ? "yes" : "no");
 #endif
   p.flags |= ADDS_F_HOST_OUT;
   double agg_ms = 0;

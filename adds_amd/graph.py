@@ -205,6 +205,13 @@ class Graph:
     def handle(self):
         return self._h
 
+    def check_symmetric(self):
+        """Tests on the device whether the graph is symmetric with strictly ascending rows and declares it
+        accordingly (pull phases follow the declaration); returns the outcome."""
+        out = ctypes.c_int()
+        L.check(L._lib().adds_graph_check_symmetric(self._h, ctypes.byref(out)), "check_symmetric")
+        return bool(out.value)
+
     @property
     def dist_dtype(self):
         return np.float32 if self.wtype == L.W_F32 else np.uint32

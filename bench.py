@@ -237,7 +237,7 @@ def run_c3(host, src, name, steps, warmup, torch, adds_amd, oracle, prm, do_cpu,
     torch.cuda.synchronize()
     cold_ms = (time.perf_counter() - t0) * 1e3
     wall, stats = time_single(dg, src, dist.data_ptr(), steps, warmup, torch, **params)
-    d = dist.cpu().numpy().view(np.uint32)
+    d = dist.cpu().numpy().view(np.uint32).copy()
     n_r, m_r = host.reached_counts(d)
     kms = float(np.mean([s["kernel_ms"] for s in stats]))
     sms = float(np.mean([s["ms"] for s in stats]))
@@ -250,8 +250,18 @@ def run_c3(host, src, name, steps, warmup, torch, adds_amd, oracle, prm, do_cpu,
         "bytes_alg": bytes_alg(host.n, n_r, m_r),
         "cold_first_solve_ms": cold_ms, "cold_kernel_ms": cold["kernel_ms"], "graph_upload_s": upload_s,
         "work": {k: stats[-1][k] for k in ("vertices_expanded", "edges_scanned", "relaxations", "pushes",
-                                           "stale_skips", "hub_tasks", "rebuckets")},
+                                           "stale_skips", "hub_tasks", "rebuckets", "heavy_edges", "pulls",
+                                           "pull_edges", "pull_improved")},
     }
+    # the same solve with every heavy phase in the push direction (ADDS_F_NO_PULL), for comparison
+    from adds_amd import _lib as L
+    _, pst = time_single(dg, src, dist.data_ptr(), 3, 1, torch,
+                         **dict(params, flags=params.get("flags", 0) | L.F_NO_PULL))
+    pk = float(np.mean([s["kernel_ms"] for s in pst]))
+    res["push_direction_only"] = {
+        "kernel_ms_event": pk, "gteps_kernel": m_r / (pk / 1e3) / 1e9, "edges_scanned": pst[-1]["edges_scanned"],
+        "heavy_edges": pst[-1]["heavy_edges"],
+        "same_distances": bool(np.array_equal(dist.cpu().numpy().view(np.uint32), d))}
     res["roofline"] = roofline_entry(res["bytes_alg"], kms)
     res["work_efficiency"] = {"relaxations_per_m_r": res["work"]["relaxations"] / m_r,
                               "edges_scanned_per_m_r": res["work"]["edges_scanned"] / m_r}
@@ -356,7 +366,7 @@ def run_c4a(steps, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo):
            "speedup": loop_s / batched_s, "batched_gteps": m_r_total / batched_s / 1e9,
            "certificate_all_256": all(c == 0 for c in certs), "bit_exact_16_vs_dijkstra": exact16,
            "roofline": roofline_entry(b_total, mean_replay, "replay time (fill + kernel + cleanup)"),
-           "work": {"edges_scanned": st["edges_scanned"]}}
+           "work": {k: st[k] for k in ("edges_scanned", "heavy_edges", "pulls", "pull_edges")}}
     if do_cpu:
         t0 = time.perf_counter()
         for i in pick[:4]:
@@ -413,6 +423,7 @@ def run_c4b(steps, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo):
            "params": params, "loop_params": "each graph's own tuned parameters",
            "batched_ms": batched_s * 1e3, "replay_ms_event": mean_replay, "loop_ms": loop_s * 1e3,
            "speedup": loop_s / batched_s, "instances_per_s": len(hosts) / batched_s, "bit_exact_all": exact,
+           "work": {k: st[k] for k in ("edges_scanned", "heavy_edges", "pulls", "pull_edges")},
            "roofline": roofline_entry(b_total, mean_replay, "replay time (fill + kernel + cleanup + copies)")}
     if do_cpu:
         t0 = time.perf_counter()

@@ -54,7 +54,7 @@ typedef struct {
   uint32_t chunk;        /* worklist items a wavefront claims at once (1..256; default 64, 128 or 256 by size) */
   uint32_t flags;        /* ADDS_F_* */
   uint32_t nbuckets;     /* bucket ring length (power of two, 4..256) */
-  uint32_t parts;        /* sub-queues per bucket (1, 2, 4 or 8; 8 = one per XCD); (concurrency + 3) * parts <= 48 */
+  uint32_t parts;        /* sub-queues per bucket (1, 2, 4 or 8; 8 = one per XCD); (concurrency + 3) * parts <= 64 */
   uint32_t hub_degree;   /* edge ranges longer than this are split into chunks any wavefront can take */
   uint32_t task_edges;   /* edges per chunk of a split range (1..65536, default 64) */
   uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
@@ -152,6 +152,12 @@ adds_status adds_graph_set_weights_device(adds_graph* g, const void* w_dev, void
  * withdraws it (also for solves captured before).  A union of graphs (batched solves) is symmetric when
  * every member is.  Graphs start undeclared. */
 adds_status adds_graph_set_symmetric(adds_graph* g, int symmetric);
+/* Tests on the device whether g is symmetric in that sense with every row in strictly ascending order of
+ * destination (an edge whose reverse is missing, with other weight bits, or out of order in its row makes
+ * it read as not symmetric: a graph with unsorted rows or repeated edges is never declared), stores 1 or 0
+ * in *symmetric and declares the graph accordingly.  No reference counterpart; the drop-in binaries call it
+ * once per graph.  About 0.1 s per billion edges. */
+adds_status adds_graph_check_symmetric(adds_graph* g, int* symmetric);
 void adds_graph_destroy(adds_graph* g);
 adds_status adds_graph_info(const adds_graph* g, uint32_t* n, uint64_t* m, adds_wtype* wt);
 /* Device pointers of the resident CSR (row_ptr, col, w); valid until the handle is destroyed. */

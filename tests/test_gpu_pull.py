@@ -215,3 +215,37 @@ def test_disconnected_pull_keeps_unreached():
     assert st["pulls"] == 1
     _check(g, dist, src, False)
     assert np.all(dist.view(np.uint32)[n:] == L.INF_U32)
+
+
+def test_device_symmetry_check():
+    """adds_graph_check_symmetric agrees with the exact host check: a generated graph passes (and then
+    pulls); one changed weight, one row out of order, or a one-way edge fails (and never pulls)."""
+    g = adds_amd.HostGraph.rmat(12, 16, seed=4, wkind="u32", lo=1, hi=65535)
+    src = g.max_degree_vertex()
+    plain = adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, g.w_bits, "u32")
+    dg = adds_amd.Graph(plain)
+    assert dg.check_symmetric()
+    dist, st = dg.sssp(src, timeout_ms=TIMEOUT, delta=2048.0, flags=EAGER)
+    assert st["pulls"] == 1
+    _check(g, dist, src, False)
+    w = g.w_bits.copy()
+    w[7] += 1
+    assert not adds_amd.Graph(adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, w, "u32")).check_symmetric()
+    # swap the first two entries (destination and weight) of vertex src's row: same edges, unsorted row
+    c, w = g.col.copy(), g.w_bits.copy()
+    r0 = int(g.row_ptr[src])
+    c[r0], c[r0 + 1] = c[r0 + 1], c[r0]
+    w[r0], w[r0 + 1] = w[r0 + 1], w[r0]
+    h = adds_amd.HostGraph.from_arrays(g.row_ptr, c, w, "u32")
+    assert h.check_symmetric()  # as an edge multiset it is symmetric
+    dh = adds_amd.Graph(h)
+    assert not dh.check_symmetric()
+    _, st = dh.sssp(src, timeout_ms=TIMEOUT, delta=2048.0, flags=EAGER)
+    assert st["pulls"] == 0
+    # one-way edge: drop the last edge of the source's row
+    keep = np.ones(g.m, dtype=bool)
+    keep[int(g.row_ptr[src + 1]) - 1] = False
+    rp = g.row_ptr.astype(np.int64)
+    rp[src + 1:] -= 1
+    d1 = adds_amd.Graph(adds_amd.HostGraph.from_arrays(rp.astype(np.uint32), g.col[keep], g.w_bits[keep], "u32"))
+    assert not d1.check_symmetric()
