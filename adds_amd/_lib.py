@@ -61,6 +61,7 @@ class Params(ctypes.Structure):
         ("edge_windows", ctypes.c_uint32),
         ("light_delta", ctypes.c_float),
         ("pool_entries", ctypes.c_uint64),
+        ("pull_delta", ctypes.c_float),
     ]
 
     @classmethod

@@ -443,6 +443,7 @@ struct Resolved {
   bool pool_fixed;
   double timeout_ms;
   uint32_t pull_slots, ptask_cap;  // pull phases per solve (0: none) and the long-row task list size
+  float pull_delta;                // bucket width after the first pull (0: delta)
   uint64_t pull_min;               // heavy edges a bucket needs before a pull phase is considered
 };
 
@@ -506,6 +507,11 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   // at least max(total_n / 2, 2^20) heavy edges -- in practice bucket 0 of a power-law graph with a
   // high-degree source.  Every long row of every instance may become tasks of at least PULL_LONG edges.
   r.pull_slots = (g->symmetric && !(r.flags & ADDS_F_NO_PULL)) ? 1u : 0u;
+  r.pull_delta = p->pull_delta;
+  if (!(r.pull_delta >= 0.f) || !std::isfinite(r.pull_delta)) {
+    g_detail = "parameter out of range";
+    return ADDS_EINVAL;
+  }
   r.pull_min = std::max<uint64_t>(total_n / 2, 1u << 20);
   r.ptask_cap = (uint32_t)std::min<uint64_t>(total_m / PULL_LONG + 4096, 1u << 24);
   return ADDS_OK;
@@ -636,6 +642,7 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.pull_min = r.pull_min;
   a.pull_eager = (r.flags & ADDS_F_PULL_EAGER) ? 1u : 0u;
   a.pull_ok = g->sym_dev;
+  a.inv_delta_resume = r.pull_delta > 0.f ? 1.0f / r.pull_delta : 0.f;
   a.ptask = ws.ptask;
   a.ptask_cap = ws.ptask_cap;
   a.launch_idx = 0;
@@ -1450,9 +1457,9 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
         return;
       }
       if (getenv("ADDS_TUNE_VERBOSE"))
-        fprintf(stderr, "[tune] delta=%g light=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u hub=%u windows=%u tail=%u flags=%u -> %.3f ms\n",
-                q.delta, q.light_delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk, q.task_edges,
-                q.hub_degree, q.edge_windows, q.tail_items, q.flags, ms);
+        fprintf(stderr, "[tune] delta=%g light=%g pull_delta=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u hub=%u windows=%u tail=%u flags=%u -> %.3f ms\n",
+                q.delta, q.light_delta, q.pull_delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk,
+                q.task_edges, q.hub_degree, q.edge_windows, q.tail_items, q.flags, ms);
       tried.push_back({ms, q});
       if (ms < bms) {
         bms = ms;
@@ -1471,6 +1478,13 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       q.light_delta = 1e30f;
       try_point(q);
     }
+    // pull phases (symmetric graphs): a wider bucket for the sparse rest of the solve after the pull
+    if (g->symmetric)
+      for (float f : {16.0f, 64.0f}) {
+        adds_params q = p;
+        q.pull_delta = p.delta * f;
+        try_point(q);
+      }
     // light threshold above the bucket width: all edges light (no heavy phases) pays where vertices are
     // rarely expanded twice (many sources on one graph)
     for (float ld : {4.0f * p.delta, 1e30f}) {
@@ -1554,6 +1568,14 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       if (q.light_delta != 0.0f && q.light_delta < q.delta) q.light_delta = 0.0f;
       try_point(q);
     }
+    // with a pull the best bucket width before it is narrower (its rest is handled at pull_delta)
+    if (g->symmetric)
+      for (float f : {0.0f, 32.0f, 64.0f}) {
+        adds_params q = p;
+        q.pull_delta = p.delta * f;
+        if (q.pull_delta == p.pull_delta) continue;
+        try_point(q);
+      }
     for (uint32_t hd : {0u, 4096u, 16384u}) {
       adds_params q = p;
       if (q.hub_degree == hd) continue;

@@ -32,7 +32,9 @@ constexpr uint32_t S_TAIL = 1 * LINE;       // [0] 1 once the solve is in its ta
                                             // [1] most items pending seen at a head move,
                                             // [2] pull requests made (PULL_GEN), [3] bucket of the last
                                             // request, [4..5] u64 heavy edges queued in it, [6] pulls
-                                            // performed, [7] the last request's outcome (1 pull, 2 push)
+                                            // performed, [7] the last request's outcome (1 pull, 2 push),
+                                            // [8] the requester's 1/delta (float bits), [9] 1 once a pull
+                                            // has re-bucketed the solve (pull_delta), [10] its new head
 constexpr uint32_t S_DONE = 2 * LINE;       // [0] done flag, [1] error code
 constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
 // (lines 4 and 5 are spare)
@@ -48,7 +50,8 @@ constexpr uint32_t S_VER = 8 * LINE;
 constexpr uint32_t S_PULL = (8 + VER_GROUPS) * LINE;
 constexpr uint32_t S_PULL2 = S_PULL + LINE;
 constexpr uint32_t PULL_GEN = S_TAIL + 2, PULL_BUCKET = S_TAIL + 3, PULL_H = S_TAIL + 4, PULL_DONE = S_TAIL + 6,
-                   PULL_OUTCOME = S_TAIL + 7;
+                   PULL_OUTCOME = S_TAIL + 7, PULL_IDL = S_TAIL + 8, PULL_REBUCKET = S_TAIL + 9,
+                   PULL_NEWHEAD = S_TAIL + 10;
 // Statistics: STAT_COPIES copies of u64 stats[NSTATS] (two lines each).  Workgroup b adds its counters,
 // summed over its wavefronts in LDS, to copy b % STAT_COPIES, and the host adds the copies up: atomics on
 // one address serialise (about 12 ns each), and one per wavefront and counter on a single copy cost a
@@ -154,6 +157,7 @@ struct KArgs {
   uint64_t pull_min;
   uint32_t pull_eager;   // request at every opportunity and always pull (testing and tuning)
   const uint32_t* pull_ok;  // the graph's device word: nonzero while it is declared symmetric
+  float inv_delta_resume;   // 1 / pull_delta: the bucket width after the first pull (0: unchanged)
   uint32_t ptask_cap;
   uint4* ptask;          // long-row tasks of a pull phase: (vertex-instance, first edge, end edge, 0)
 };

@@ -320,8 +320,9 @@ __device__ __forceinline__ uint32_t push_part(const KArgs& a, uint32_t gb, uint3
   return r & (a.parts - 1);
 }
 template <bool FLT>
-__device__ __forceinline__ uint32_t light_queue(const KArgs& a, uint32_t nd, uint32_t part, uint32_t cb, bool tail) {
-  uint32_t b = tail ? cb : bucket_key<FLT>(nd, a.inv_delta);  // the tail is processed as one bucket
+__device__ __forceinline__ uint32_t light_queue(const KArgs& a, float idl, uint32_t nd, uint32_t part, uint32_t cb,
+                                                bool tail) {
+  uint32_t b = tail ? cb : bucket_key<FLT>(nd, idl);  // the tail is processed as one bucket
   b = max(b, cb);
   b = min(b, cb + a.nb - 3);
   return (b & (a.nb - 1)) * a.parts + part;
@@ -596,8 +597,8 @@ __device__ __forceinline__ void gather_pairs(const KArgs& a, const PairGroup<UN>
   }
 }
 template <bool FLT, bool REPL, int UN>
-__device__ __forceinline__ void finish_pairs(const KArgs& a, const PairGathered<UN>& r, uint32_t part, uint32_t cb,
-                                             bool heavy, bool tail, PushBuf& pb, WaveStats& ws) {
+__device__ __forceinline__ void finish_pairs(const KArgs& a, float idl, const PairGathered<UN>& r, uint32_t part,
+                                             uint32_t cb, bool heavy, bool tail, PushBuf& pb, WaveStats& ws) {
   bool f0[UN], f1[UN];
   uint32_t nact = 0;
 #pragma unroll
@@ -613,8 +614,8 @@ __device__ __forceinline__ void finish_pairs(const KArgs& a, const PairGathered<
 #pragma unroll
   for (int j = 0; j < UN; j++) {
     const uint32_t pp = REPL ? push_part(a, r.pp[j], part) : part;
-    pb_add(a, pb, f0[j], ((uint64_t)r.n0[j] << 32) | r.t0[j], light_queue<FLT>(a, r.n0[j], pp, cb, tail) | PB_RELAX, ws);
-    pb_add(a, pb, f1[j], ((uint64_t)r.n1[j] << 32) | r.t1[j], light_queue<FLT>(a, r.n1[j], pp, cb, tail) | PB_RELAX, ws);
+    pb_add(a, pb, f0[j], ((uint64_t)r.n0[j] << 32) | r.t0[j], light_queue<FLT>(a, idl, r.n0[j], pp, cb, tail) | PB_RELAX, ws);
+    pb_add(a, pb, f1[j], ((uint64_t)r.n1[j] << 32) | r.t1[j], light_queue<FLT>(a, idl, r.n1[j], pp, cb, tail) | PB_RELAX, ws);
   }
 }
 
@@ -622,8 +623,8 @@ __device__ __forceinline__ void finish_pairs(const KArgs& a, const PairGathered<
 // flush applies the atomicMin.  No store or atomic is issued in the edge loop, so the compiler's
 // waits there cover loads only and the pipeline keeps two groups' loads in flight.
 template <bool FLT, bool REPL, int UN>
-__device__ __forceinline__ void finish(const KArgs& a, const Gathered<UN>& r, uint32_t part, uint32_t cb, bool heavy,
-                                       bool tail, PushBuf& pb, WaveStats& ws) {
+__device__ __forceinline__ void finish(const KArgs& a, float idl, const Gathered<UN>& r, uint32_t part, uint32_t cb,
+                                       bool heavy, bool tail, PushBuf& pb, WaveStats& ws) {
   bool f[UN];
   uint32_t nact = 0;
 #pragma unroll
@@ -638,7 +639,7 @@ __device__ __forceinline__ void finish(const KArgs& a, const Gathered<UN>& r, ui
 #pragma unroll
   for (int j = 0; j < UN; j++)
     pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j],
-           light_queue<FLT>(a, r.nd[j], REPL ? push_part(a, r.pp[j], part) : part, cb, tail) | PB_RELAX, ws);
+           light_queue<FLT>(a, idl, r.nd[j], REPL ? push_part(a, r.pp[j], part) : part, cb, tail) | PB_RELAX, ws);
 }
 
 // Appends a bucket-timeline record when tracing is on (called by one lane).
@@ -828,6 +829,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     if (ald(a.st + PULL_GEN) != a.launch_idx || stopped(a)) return;
     if (blockIdx.x == 0 && threadIdx.x < 6) ast(a.st + S_PULL2 + threadIdx.x, 0u);
   }
+  // bucket width: after a pull phase that re-bucketed the pending work (pull_delta), the coarser one
+  const float idl = (a.launch_idx > 0 && a.inv_delta_resume > 0.f && ald(a.st + PULL_REBUCKET) != 0)
+                        ? a.inv_delta_resume : a.inv_delta;
   PushBuf pbuf{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
   for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;  // no queue has id 0xFFFF
   wave_sync();
@@ -1088,7 +1092,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         // clamped to the window's end when pushed and would be clamped there again at every head move (a
         // distance 2^31 past the head with delta 1 would take tens of millions of moves), so it is
         // relaxed now: the order of relaxations only affects the work, never the result.
-        const uint32_t key = bucket_key<FLT>(du[j], a.inv_delta);
+        const uint32_t key = bucket_key<FLT>(du[j], idl);
         early[j] = have[j] && !stale && !is_ref[j] && !heavy_claim && !tail_mode && key > cb &&
                    key <= cb + a.nb - 3;
         ok[j] = have[j] && !stale && !early[j];
@@ -1109,9 +1113,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
         filed_heavy += file_heavy ? r1v[j] - r0v[j] - lcv[j] : 0u;
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
-        nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], a.inv_delta) + 1 < cb));
+        nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], idl) + 1 < cb));
         const uint32_t pp = REPL ? push_part(a, gbase[j], part) : part;
-        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, du[j], pp, cb, tail_mode), ws);
+        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, idl, du[j], pp, cb, tail_mode), ws);
         pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + pp, ws);
       }
       if (a.pull_slots) {
@@ -1262,7 +1266,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             locate_p(base + 128 * UP, ga);
             load_pairs(a, ga);
             gather_pairs<FLT, UP>(a, gb, rb);
-            finish_pairs<FLT, REPL, UP>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
+            finish_pairs<FLT, REPL, UP>(a, idl, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
             ra = rb;
             gb = ga;
           }
@@ -1281,7 +1285,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           locate(base + 128 * UN, ga);
           load_cw(a, ga);
           gather<FLT, UN>(a, gb, rb);
-          finish<FLT, REPL, UN>(a, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
+          finish<FLT, REPL, UN>(a, idl, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
           ra = rb;
           gb = ga;
         }
@@ -1427,6 +1431,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         } else if (req) {
           if (lane == 0) {
             ast(a.st + PULL_BUCKET, c2);
+            ast(a.st + PULL_IDL, __float_as_uint(idl));
             ast64((uint64_t*)(a.st + PULL_H), hv);
             mem_wait();
             ast(a.st + PULL_GEN, pgen + 1);
@@ -1561,15 +1566,26 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 // of the push direction (each is a real path length, and every relaxation the push direction would make
 // is made or cannot lower anything).
 //
-// k_pull_count sums the heavy in-degrees of the unsettled vertices; k_pull_a takes the cheaper direction
-// (opening the heavy queues for the next launch when pushing costs less) and relaxes rows of up to
-// PULL_LONG heavy edges, 256 vertex-instances per wavefront claim, handing longer rows to k_pull_b as
-// tasks of at most PULL_TASK edges, each relaxed by one wavefront and applied with an atomicMin.
+// k_pull_count sums the heavy in-degrees of the unsettled vertices; k_pull_decide (one workgroup) takes
+// the cheaper direction (opening the heavy queues for the next launch when pushing costs less); k_pull_a
+// relaxes rows of up to PULL_LONG heavy edges, 256 vertex-instances per wavefront, handing longer rows to
+// k_pull_b as tasks of at most PULL_TASK edges, each relaxed by one wavefront and applied with an
+// atomicMin.
+//
+// Re-bucketing (pull_delta): what is left after such a pull is the sparse rest of the graph, whose many
+// narrow buckets each cost a few claim round trips for little work.  The first pull may therefore move
+// the solve to a wider bucket: k_pull_decide empties every queue, clears the bucket ring's phase words
+// and heavy-edge counters and sets the head to the first wide bucket; k_pull_a then queues every
+// unsettled vertex that has a distance (the dropped items were all of such vertices: every vertex of a
+// bucket up to c is settled, and c's own heavy items are covered by the pull) at the wide width, and the
+// launches that follow use it.
 
 template <class A>
 __device__ __forceinline__ bool pull_requested(const A& a, uint32_t slot) {
   return ald(a.st + PULL_GEN) == slot && !stopped(a);
 }
+// the bucket width of the launch that requested the pull (its bucket index c is in these units)
+__device__ __forceinline__ float pull_req_idl(const KArgs& a) { return __uint_as_float(ald(a.st + PULL_IDL)); }
 template <bool FLT>
 __host__ __device__ constexpr uint32_t inf_word() {
   return FLT ? 0x7F800000u : 0xFFFFFFFFu;
@@ -1601,9 +1617,10 @@ __global__ void __launch_bounds__(PULL_COUNT_THREADS) k_pull_count(KArgs a, uint
   __shared__ uint64_t s_red[PULL_COUNT_THREADS / 64][2];
   if (!pull_requested(a, slot)) return;
   const uint32_t c = ald(a.st + PULL_BUCKET);
+  const float idl = pull_req_idl(a);
   uint64_t in_edges = 0, uns = 0;
   for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < a.total_n; x += (uint64_t)gridDim.x * blockDim.x) {
-    if (unsettled<FLT>(a.dist[x], c, a.inv_delta)) {
+    if (unsettled<FLT>(a.dist[x], c, idl)) {
       const uint4 ri = a.rinfo[REPL ? (uint32_t)x % a.n : (uint32_t)x];
       in_edges += ri.y - ri.x - ri.z;
       uns++;
@@ -1625,6 +1642,52 @@ __global__ void __launch_bounds__(PULL_COUNT_THREADS) k_pull_count(KArgs a, uint
 
 constexpr int PULL_UN = 8;  // 64-edge windows per wavefront in flight in the pull kernels
 
+// One workgroup: pull or push, and the queue and ring state for the next launch.
+__global__ void __launch_bounds__(256) k_pull_decide(KArgs a, uint32_t slot) {
+  if (!pull_requested(a, slot)) return;
+  const uint32_t c = ald(a.st + PULL_BUCKET);
+  const bool pull = a.pull_eager || pull_pays(ald64((const uint64_t*)(a.st + PULL_H)),
+                                              ald64((const uint64_t*)(a.st + S_PULL2 + 2)),
+                                              ald64((const uint64_t*)(a.st + S_PULL2 + 4)), a.total_n);
+  const bool rebucket = pull && a.inv_delta_resume > 0.f && ald(a.st + PULL_DONE) == 0;
+  const uint32_t nq = nq_of(a), cs = c & (a.nb - 1);
+  if (rebucket) {
+    // every queue emptied (positions stay reserved, so the chunk maps and the cleanup are unaffected),
+    // every slot's phase word and heavy-edge counters cleared; in-flight counts are zero (no launch runs)
+    for (uint32_t q = threadIdx.x; q < 2 * nq; q += blockDim.x) {
+      ast(qhead(a, q), ald(qtail(a, q)));
+      ast(qavail(a, q), 0u);
+    }
+    for (uint32_t i = threadIdx.x; i < a.nb * a.parts; i += blockDim.x) {
+      const uint32_t sl = i / a.parts, p = i % a.parts;
+      ast64(heavy_ctr(a, sl, p), 0ull);
+      if (p == 0) ast(phase(a, sl), 0u);
+    }
+  } else if (threadIdx.x == 0 && pull) {
+    // the bucket's heavy queues are done with: their edges are pulled
+    for (uint32_t p = 0; p < a.parts; p++) {
+      const uint32_t q = nq + cs * a.parts + p;
+      ast(qhead(a, q), ald(qtail(a, q)));
+      ast(qavail(a, q), 0u);
+    }
+  }
+  if (threadIdx.x == 0) {
+    ast(a.st + PULL_OUTCOME, pull ? 1u : 2u);
+    if (pull) ast(a.st + PULL_DONE, ald(a.st + PULL_DONE) + 1);
+    if (rebucket) {
+      // the first wide bucket at or below every unsettled distance ((c + 1) * delta and up)
+      const float ratio = a.inv_delta_resume / pull_req_idl(a);
+      const uint32_t nh = (uint32_t)fmaxf(floorf((float)(c + 1) * ratio) - 1.f, 0.f);
+      ast(a.st + PULL_NEWHEAD, nh);
+      ast(a.st + S_CUR, nh);
+      ast(a.st + PULL_REBUCKET, 1u);
+    } else {
+      ast(phase(a, cs), 1u);  // pushed: the next launch relaxes the heavy queues; pulled: they are empty
+    }
+    trace_event(a, 6, c, pull ? (rebucket ? 3u : 1u) : 2u);
+  }
+}
+
 template <bool FLT, bool REPL>
 __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
   __shared__ uint32_t s_pref[WAVES_PER_WG][64 * IPL + 1];
@@ -1635,28 +1698,14 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
   __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
   __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
   __shared__ uint64_t s_stat[WAVES_PER_WG][NSTATS];
-  if (!pull_requested(a, slot)) return;
+  if (!pull_requested(a, slot) || ald(a.st + PULL_OUTCOME) != 1u) return;
   const uint32_t c = ald(a.st + PULL_BUCKET);
-  const bool pull = a.pull_eager || pull_pays(ald64((const uint64_t*)(a.st + PULL_H)),
-                                              ald64((const uint64_t*)(a.st + S_PULL2 + 2)),
-                                              ald64((const uint64_t*)(a.st + S_PULL2 + 4)), a.total_n);
-  const uint32_t cs = c & (a.nb - 1);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // the bucket's heavy queues are done with: pulled (their items dropped) or opened for the next launch
-    ast(a.st + PULL_OUTCOME, pull ? 1u : 2u);
-    if (pull) {
-      ast(a.st + PULL_DONE, ald(a.st + PULL_DONE) + 1);
-      const uint32_t nq = nq_of(a);
-      for (uint32_t p = 0; p < a.parts; p++) {
-        const uint32_t q = nq + cs * a.parts + p;
-        ast(qhead(a, q), ald(qtail(a, q)));
-        ast(qavail(a, q), 0u);
-      }
-    }
-    ast(phase(a, cs), 1u);
-    trace_event(a, 6, c, pull ? 1u : 2u);
-  }
-  if (!pull) return;
+  const float idl = pull_req_idl(a);
+  // pushes: at the requester's width behind head c, or (re-bucketed) at the wide width behind its new head,
+  // and then every unsettled vertex with a distance is queued, improved or not
+  const bool rebucket = ald(a.st + PULL_REBUCKET) != 0;
+  const float pidl = rebucket ? a.inv_delta_resume : idl;
+  const uint32_t pc = rebucket ? ald(a.st + PULL_NEWHEAD) : c;
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
   PushBuf pb{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
@@ -1679,7 +1728,7 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
     for (int j = 0; j < IPL; j++) {
       x[j] = base + j * 64 + lane;
       dv[j] = x[j] < a.total_n ? a.dist[x[j]] : 0u;
-      uns[j] = x[j] < a.total_n && unsettled<FLT>(dv[j], c, a.inv_delta);
+      uns[j] = x[j] < a.total_n && unsettled<FLT>(dv[j], c, idl);
       gb[j] = REPL ? x[j] - x[j] % a.n : 0u;
     }
 #pragma unroll
@@ -1773,7 +1822,8 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
       if (imp) ast(a.dist + x[j], best);
       nimp += (uint32_t)__popcll(ballot(imp));
       const uint32_t pp = REPL ? push_part(a, gb[j], part) : part;
-      pb_add(a, pb, imp, ((uint64_t)best << 32) | x[j], light_queue<FLT>(a, best, pp, c, false), ws);
+      const bool q = imp || (rebucket && uns[j] && best < INF);
+      pb_add(a, pb, q, ((uint64_t)best << 32) | x[j], light_queue<FLT>(a, pidl, best, pp, pc, false), ws);
     }
     wave_sync();
     if (lane == 0) {
@@ -1793,7 +1843,9 @@ __global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
   __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
   __shared__ uint64_t s_stat[WAVES_PER_WG][NSTATS];
   if (!pull_requested(a, slot) || ald(a.st + PULL_OUTCOME) != 1u) return;
-  const uint32_t c = ald(a.st + PULL_BUCKET);
+  const bool rebucket = ald(a.st + PULL_REBUCKET) != 0;
+  const float pidl = rebucket ? a.inv_delta_resume : pull_req_idl(a);
+  const uint32_t pc = rebucket ? ald(a.st + PULL_NEWHEAD) : ald(a.st + PULL_BUCKET);
   const uint32_t ntask = min(ald(a.st + S_PULL2), a.ptask_cap);
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
@@ -1842,7 +1894,7 @@ __global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
       if (imp) ws.v[ST_PULL_IMPROVED]++;
     }
     const uint32_t pp = REPL ? push_part(a, gb, part) : part;
-    pb_add(a, pb, imp, ((uint64_t)best << 32) | tk.x, light_queue<FLT>(a, best, pp, c, false), ws);
+    pb_add(a, pb, imp, ((uint64_t)best << 32) | tk.x, light_queue<FLT>(a, pidl, best, pp, pc, false), ws);
   }
   pb_flush(flush_args(a), pb, ws);
   publish_stats(a.st, s_stat, ws, nullptr);
@@ -2173,6 +2225,7 @@ static void launch_pull_t(const KArgs& a, uint32_t slot, uint32_t grid, hipStrea
   const uint32_t cblocks = std::max<uint32_t>(1, std::min<uint32_t>((a.total_n + PULL_COUNT_THREADS - 1) /
                                                                    PULL_COUNT_THREADS, 256));
   hipLaunchKernelGGL((k_pull_count<FLT, REPL>), dim3(cblocks), dim3(PULL_COUNT_THREADS), 0, s, a, slot);
+  hipLaunchKernelGGL(k_pull_decide, dim3(1), dim3(256), 0, s, a, slot);
   hipLaunchKernelGGL((k_pull_a<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
   hipLaunchKernelGGL((k_pull_b<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
 }

@@ -73,6 +73,10 @@ typedef struct {
   uint64_t pool_entries; /* worklist pool capacity in entries (default 4 per vertex-instance + 1 per 16
                             edge-instances + 8M, doubled
                             and remembered per graph after an overflow; a solve that overflows is rerun) */
+  float pull_delta;      /* bucket width after the solve's first pull phase (0, the default: delta).  What
+                            is left after such a pull is the sparse rest of the graph; a wider bucket there
+                            saves phase changes (each a few claim round trips) at the price of a few
+                            re-expansions of low-degree vertices */
 } adds_params;
 
 /* flags */

@@ -57,6 +57,9 @@ def _symmetrize(row, col, w):
     dict(flags=EAGER, parts=1, nbuckets=4, chunk=16),
     dict(flags=EAGER, tail_items=4096, concurrency=4),
     dict(flags=EAGER, hub_degree=64, task_edges=64),
+    dict(flags=EAGER, pull_delta=4.0),  # re-bucketed after the pull: 4 (float) or 16 (int) times wider
+    dict(flags=EAGER | L.F_NO_WAIT | L.F_NO_LOCAL, edge_windows=8, pull_delta=4.0, tail_items=4096),
+    dict(flags=EAGER, pull_delta=0.5),  # narrower after the pull
 ])
 def test_rmat_pull_identical(is_float, params):
     """RMAT from its highest-degree vertex: bucket 0 holds most of the graph's edges.  With the default
@@ -66,6 +69,9 @@ def test_rmat_pull_identical(is_float, params):
     assert g.symmetric
     src = g.max_degree_vertex()
     delta = 0.0625 if is_float else 4096.0
+    params = dict(params)
+    if "pull_delta" in params:
+        params["pull_delta"] *= delta * (1 if is_float else 4)
     dist, st = adds_amd.Graph(g).sssp(src, timeout_ms=TIMEOUT, delta=delta, **params)
     _check(g, dist, src, is_float)
     if params.get("flags", 0) & EAGER:
@@ -102,12 +108,13 @@ def test_low_degree_source_long_unsettled_rows():
         assert st["pulls"] == 1
 
 
-@pytest.mark.parametrize("flags", [EAGER, EAGER | L.F_REPLICA_PARTS])
-def test_multi_source_pull(flags):
+@pytest.mark.parametrize("flags,pull_delta", [(EAGER, 0.0), (EAGER | L.F_REPLICA_PARTS, 0.0), (EAGER, 16.0)])
+def test_multi_source_pull(flags, pull_delta):
     """Many sources in one solve (replicated distance rows): a pull covers every instance's row."""
     g = adds_amd.HostGraph.rmat(13, 16, seed=9, wkind="f32", lo=1.0, hi=10.0)
     srcs = g.sample_component(g.max_degree_vertex(), 12, seed=4)
-    dm, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT, delta=2.0, parts=4, flags=flags)
+    dm, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT, delta=2.0, parts=4, flags=flags,
+                                          pull_delta=pull_delta)
     assert st["pulls"] == 1
     for i, s in enumerate(srcs):
         _check(g, dm[i], s, True)
@@ -120,10 +127,11 @@ def test_batched_union_pull():
            adds_amd.HostGraph.er(3000, 8.0, seed=8, wkind="u32", lo=1, hi=1000)]
     dgs = [adds_amd.Graph(h) for h in gs]
     srcs = [h.max_degree_vertex() for h in gs]
-    dists, st = adds_amd.sssp_batched(dgs, srcs, timeout_ms=TIMEOUT, delta=64.0, flags=EAGER)
-    assert st["pulls"] == 1
-    for h, d, s in zip(gs, dists, srcs):
-        _check(h, d, s, False)
+    for pd in (0.0, 512.0):
+        dists, st = adds_amd.sssp_batched(dgs, srcs, timeout_ms=TIMEOUT, delta=64.0, flags=EAGER, pull_delta=pd)
+        assert st["pulls"] == 1
+        for h, d, s in zip(gs, dists, srcs):
+            _check(h, d, s, False)
 
 
 def test_undeclared_and_reweighted_graphs_do_not_pull():
@@ -159,7 +167,8 @@ def test_captured_pull_replays_and_weight_update():
     dg = adds_amd.Graph(g)
     dist = torch.empty((len(srcs), g.n), dtype=torch.int32, device="cuda")
     sd = torch.from_numpy(srcs.astype(np.int32)).cuda()
-    sg = adds_amd.SolveGraph(dg, sd.data_ptr(), len(srcs), dist.data_ptr(), delta=256.0, flags=EAGER)
+    sg = adds_amd.SolveGraph(dg, sd.data_ptr(), len(srcs), dist.data_ptr(), delta=256.0, flags=EAGER,
+                             pull_delta=2048.0)
     ref = None
     for i in range(3):
         dist.fill_(5)
