@@ -79,6 +79,10 @@ struct adds_graph {
   uint32_t* col = nullptr;
   uint32_t* w = nullptr;
   float ave_wt = 1.f;  // mean sampled weight (the reference's profiler statistic)
+  // largest weight (bits; float weights are non-negative, so bits order them), when known: with a light
+  // threshold at or above it no edge is heavy and no pull phase can be requested
+  bool max_w_known = false;
+  uint32_t max_w = 0;
   // light-first copy of the rows for the solver (col/w stay in input order for the backward pass and
   // the device-array accessor); valid for split_delta
   uint2* scw = nullptr;
@@ -272,11 +276,13 @@ extern "C" adds_status adds_graph_create(const uint32_t* row_ptr, const uint32_t
       return ADDS_EINVAL;
     }
   const uint32_t* wb = (const uint32_t*)w;
+  uint32_t maxw = 0;
   for (uint64_t i = 0; i < m; i++) {
     if (col[i] >= n) {
       g_detail = "edge destination out of range";
       return ADDS_EINVAL;
     }
+    maxw = std::max(maxw, wt == ADDS_W_F32 ? (wb[i] & 0x7FFFFFFFu) : wb[i]);
     if (wt == ADDS_W_F32) {
       uint32_t x = wb[i] & 0x7FFFFFFFu;  // -0.0 is accepted (it adds like +0.0)
       if (x >= 0x7F800000u || (wb[i] >> 31 && x != 0)) {
@@ -294,6 +300,8 @@ extern "C" adds_status adds_graph_create(const uint32_t* row_ptr, const uint32_t
     HIPCHK(hipMemcpy(g->w, w, m * 4, hipMemcpyHostToDevice));
   }
   g->ave_wt = sample_mean_weight(wb, m, wt);
+  g->max_w = maxw;
+  g->max_w_known = true;
   return ADDS_OK;
 }
 
@@ -324,6 +332,7 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   if (g->split_delta != 0.f && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
     HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scw, g->lcnt,
                         g->split_aux, g->rinfo, (hipStream_t)stream));
+  g->max_w_known = false;
   // new weights need not be symmetric: pull phases stop (also in solves captured earlier) until the
   // caller declares the graph symmetric again
   if (g->symmetric) {
@@ -507,6 +516,13 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
   // at least max(total_n / 2, 2^20) heavy edges -- in practice bucket 0 of a power-law graph with a
   // high-degree source.  Every long row of every instance may become tasks of at least PULL_LONG edges.
   r.pull_slots = (g->symmetric && !(r.flags & ADDS_F_NO_PULL)) ? 1u : 0u;
+  // no heavy edge (every weight at or below the light threshold): no pull can be asked for, so its
+  // launches are left out
+  if (g->max_w_known) {
+    const bool all_light = g->wt == ADDS_W_F32 ? __builtin_bit_cast(float, g->max_w) <= r.light_delta
+                                               : (double)g->max_w <= std::floor((double)r.light_delta);
+    if (all_light) r.pull_slots = 0;
+  }
   r.pull_delta = p->pull_delta;
   if (!(r.pull_delta >= 0.f) || !std::isfinite(r.pull_delta)) {
     g_detail = "parameter out of range";
@@ -951,9 +967,14 @@ static adds_status build_union(adds_graph* const* graphs, uint32_t k, adds_graph
   HIPCHK(hipMemcpy(u->row + N, &last, 4, hipMemcpyHostToDevice));
   HIPCHK(hipDeviceSynchronize());
   u->ave_wt = M ? (float)(wsum / (double)M) : 1.f;
-  // block-diagonal: symmetric when every member is
-  bool sym = true;
-  for (uint32_t i = 0; i < k; i++) sym &= graphs[i]->symmetric;
+  // block-diagonal: symmetric when every member is; its largest weight is known when every member's is
+  bool sym = true, mk = true;
+  for (uint32_t i = 0; i < k; i++) {
+    sym &= graphs[i]->symmetric;
+    mk &= graphs[i]->max_w_known;
+    u->max_w = std::max(u->max_w, graphs[i]->max_w);
+  }
+  u->max_w_known = mk;
   if (sym) {
     u->symmetric = true;
     HIPCHK(launch_fill_u32(u->sym_dev, 1, 1u, nullptr));

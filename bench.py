@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--only", default="all", help="all | c3int | c3float | c1 | c2 | c4a | c4b | c5 (comma list)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs")
     ap.add_argument("--no-cli", action="store_true", help="skip the drop-in binary on the headline graph")
+    ap.add_argument("--profile", action="store_true",
+                    help="run only the warmup and timed solves of the selected configurations (pinned parameters, "
+                         "no checks, comparisons or CPU legs): the command the rocprofv3 summaries come from")
     ap.add_argument("--retune", action="store_true", help="search the solver parameters again (ignore pinned)")
     ap.add_argument("--write-params", default=None, help="store the parameters used per workload key here")
     ap.add_argument("--scale", type=int, default=24, help="RMAT scale of the headline graph")
@@ -483,6 +486,77 @@ def run_c5(steps, torch, adds_amd, oracle, workloads, prm, do_cpu, hinfo):
     return res
 
 
+def profile_runs(only, args, torch, adds_amd, workloads, prm):
+    """The solves of each selected configuration and nothing else, so that every dispatch a profiler sees
+    belongs to a timed solve: `warmup` + `steps` solves (single source), captured replays (c4a, c4b) or
+    forward + backward passes (c5).  Prints one JSON line of mean kernel times."""
+    out = {}
+
+    def single(name, host, src):
+        dg = adds_amd.Graph(host)
+        p = prm.single(dg, host, [src], name)
+        dist = torch.empty(host.n, dtype=torch.int32, device="cuda")
+        _, st = time_single(dg, src, dist.data_ptr(), args.steps, args.warmup, torch, **p)
+        out[name] = {"kernel_ms": float(np.mean([s["kernel_ms"] for s in st])), "solves": args.warmup + args.steps,
+                     "pulls_per_solve": st[-1]["pulls"], "params": p}
+        dg.close()
+
+    if "c3int" in only or "c3float" in only:
+        host, src = workloads.c3("u32", scale=args.scale)
+        if "c3int" in only:
+            single("c3int", host, src)
+        if "c3float" in only:
+            workloads.c3_reweight_float(host)
+            single("c3float", host, src)
+    if "c1" in only:
+        single("c1", *workloads.c1())
+    if "c2" in only:
+        single("c2", *workloads.c2())
+    if "c4a" in only or "c4b" in only:
+        for name in ("c4a", "c4b"):
+            if name not in only:
+                continue
+            if name == "c4a":
+                host, srcs = workloads.c4a()
+                dg = adds_amd.Graph(host)
+                p = prm.single(dg, host, srcs, name)
+                dist = torch.empty((len(srcs), host.n), dtype=torch.int32, device="cuda")
+                sd = torch.from_numpy(srcs.astype(np.int32)).cuda()
+                sg = adds_amd.SolveGraph(dg, sd.data_ptr(), len(srcs), dist.data_ptr(), **p)
+                keep = [dg]
+            else:
+                hosts, srcs = workloads.c4b()
+                keep = [adds_amd.Graph(h) for h in hosts]
+                p = prm.batched(keep, hosts, srcs, name)
+                outs = [torch.empty(h.n, dtype=torch.int32, device="cuda") for h in hosts]
+                sg = adds_amd.SolveGraph.batched(keep, srcs, [o.data_ptr() for o in outs], **p)
+            ms = []
+            for i in range(args.warmup + args.steps):
+                sg.launch()
+                ms.append(sg.status()["ms"])
+            out[name] = {"replay_ms": float(np.mean(ms[args.warmup:])), "replays": len(ms), "params": p}
+            sg.close()
+            for g in keep:
+                g.close()
+    if "c5" in only:
+        host, srcs, gd = workloads.c5()
+        dg = adds_amd.Graph(host)
+        p = prm.single(dg, host, srcs, "c5")
+        dist = torch.empty((len(srcs), host.n), dtype=torch.float32, device="cuda")
+        gdist = torch.from_numpy(gd).cuda()
+        gw = torch.empty(host.m, dtype=torch.float32, device="cuda")
+        fk, bk = [], []
+        for i in range(args.warmup + args.steps):
+            st = dg.sssp_multi_device(srcs, dist.data_ptr(), **p)
+            bk.append(dg.backward_device(srcs, dist.data_ptr(), gdist.data_ptr(), gw.data_ptr()))
+            fk.append(st["kernel_ms"])
+        out["c5"] = {"forward_kernel_ms": float(np.mean(fk[args.warmup:])), "backward_ms": float(np.mean(bk[args.warmup:])),
+                     "passes": len(fk), "params": p}
+        dg.close()
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": out}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -511,6 +585,9 @@ def main():
     hinfo = host_info()
     log(f"device CUs: {ncu}; host {hinfo}")
     prm = Params(args.retune)
+    if args.profile:
+        profile_runs(only, args, torch, adds_amd, workloads, prm)
+        return
     do_cpu = not args.no_cpu
     extras = {}
     head = None
@@ -592,7 +669,9 @@ def main():
                    "bit_exact_vs_dijkstra": head.get("bit_exact_vs_dijkstra")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
-                     "kernel": f"k_sssp<false, false, {head['params'].get('edge_windows') or 4}>",
+                     "kernel": f"one solve: k_sssp<false, false, {head['params'].get('edge_windows') or 4}> "
+                               "(persistent launch), k_pull_count, k_pull_decide, k_pull_a, k_pull_b, k_sssp "
+                               "(continuation); HIP events around the sequence",
                      "kernel_ms": head["kernel_ms_event"], "kernel_source_sha": sha,
                      "bytes_alg_per_launch": head["bytes_alg"],
                      "random_gather_ceiling": head["gather_ceiling"]},
