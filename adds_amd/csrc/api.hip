@@ -659,6 +659,7 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.pull_eager = (r.flags & ADDS_F_PULL_EAGER) ? 1u : 0u;
   a.pull_ok = g->sym_dev;
   a.inv_delta_resume = r.pull_delta > 0.f ? 1.0f / r.pull_delta : 0.f;
+  a.resume_all_light = (r.flags & ADDS_F_PULL_ALL_LIGHT) ? 1u : 0u;
   a.ptask = ws.ptask;
   a.ptask_cap = ws.ptask_cap;
   a.launch_idx = 0;
@@ -1589,14 +1590,21 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       if (q.light_delta != 0.0f && q.light_delta < q.delta) q.light_delta = 0.0f;
       try_point(q);
     }
-    // with a pull the best bucket width before it is narrower (its rest is handled at pull_delta)
-    if (g->symmetric)
-      for (float f : {0.0f, 32.0f, 64.0f}) {
+    // with a pull the best bucket width before it is narrower (its rest is handled at pull_delta), and the
+    // rest may do best with every edge light
+    if (g->symmetric) {
+      for (float f : {0.0f, 8.0f, 32.0f, 64.0f}) {
         adds_params q = p;
         q.pull_delta = p.delta * f;
         if (q.pull_delta == p.pull_delta) continue;
         try_point(q);
       }
+      if (p.pull_delta > 0.f) {
+        adds_params q = p;
+        q.flags ^= ADDS_F_PULL_ALL_LIGHT;
+        try_point(q);
+      }
+    }
     for (uint32_t hd : {0u, 4096u, 16384u}) {
       adds_params q = p;
       if (q.hub_degree == hd) continue;

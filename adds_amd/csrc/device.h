@@ -158,6 +158,7 @@ struct KArgs {
   uint32_t pull_eager;   // request at every opportunity and always pull (testing and tuning)
   const uint32_t* pull_ok;  // the graph's device word: nonzero while it is declared symmetric
   float inv_delta_resume;   // 1 / pull_delta: the bucket width after the first pull (0: unchanged)
+  uint32_t resume_all_light;  // after a re-bucketing pull every edge is light (ADDS_F_PULL_ALL_LIGHT)
   uint32_t ptask_cap;
   uint4* ptask;          // long-row tasks of a pull phase: (vertex-instance, first edge, end edge, 0)
 };

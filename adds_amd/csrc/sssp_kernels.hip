@@ -829,9 +829,12 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     if (ald(a.st + PULL_GEN) != a.launch_idx || stopped(a)) return;
     if (blockIdx.x == 0 && threadIdx.x < 6) ast(a.st + S_PULL2 + threadIdx.x, 0u);
   }
-  // bucket width: after a pull phase that re-bucketed the pending work (pull_delta), the coarser one
-  const float idl = (a.launch_idx > 0 && a.inv_delta_resume > 0.f && ald(a.st + PULL_REBUCKET) != 0)
-                        ? a.inv_delta_resume : a.inv_delta;
+  // bucket width: after a pull phase that re-bucketed the pending work (pull_delta), the coarser one;
+  // and with resume_all_light every edge is light from then on (a light claim relaxes the whole row and
+  // files nothing heavy)
+  const bool rebucketed = a.launch_idx > 0 && a.inv_delta_resume > 0.f && ald(a.st + PULL_REBUCKET) != 0;
+  const float idl = rebucketed ? a.inv_delta_resume : a.inv_delta;
+  const bool all_light = rebucketed && a.resume_all_light;
   PushBuf pbuf{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
   for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;  // no queue has id 0xFFFF
   wave_sync();
@@ -1098,7 +1101,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         ok[j] = have[j] && !stale && !early[j];
         has_heavy[j] = false;
         if (!is_ref[j]) {
-          const uint32_t lc = lcv[j];
+          const uint32_t lc = all_light ? r1v[j] - r0v[j] : lcv[j];
           rs[j] = heavy_claim ? r0v[j] + lc : r0v[j];
           deg[j] = ok[j] ? (heavy_claim ? r1v[j] - rs[j] : lc) : 0u;
           has_heavy[j] = r1v[j] - r0v[j] > lc;
@@ -1111,7 +1114,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 #pragma unroll
       for (int j = 0; j < IPL; j++) {
         const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
-        filed_heavy += file_heavy ? r1v[j] - r0v[j] - lcv[j] : 0u;
+        filed_heavy += file_heavy ? r1v[j] - r0v[j] - lcv[j] : 0u;  // (never filed when all_light)
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
         nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], idl) + 1 < cb));
         const uint32_t pp = REPL ? push_part(a, gbase[j], part) : part;

@@ -121,7 +121,7 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
   p->parts = (uint32_t)parts;
   p->concurrency = (uint32_t)conc;
   p->task_edges = (uint32_t)te;
-  p->flags = (uint32_t)flags & (ADDS_F_NO_LOCAL | ADDS_F_NO_WAIT | ADDS_F_NO_PULL);  // tuning flags only
+  p->flags = (uint32_t)flags & (ADDS_F_NO_LOCAL | ADDS_F_NO_WAIT | ADDS_F_NO_PULL | ADDS_F_PULL_ALL_LIGHT);  // tuning flags only
   p->edge_windows = (uint32_t)win;
   p->tail_items = (uint32_t)tail;
   p->hub_degree = (uint32_t)hub;

@@ -97,6 +97,9 @@ typedef struct {
                                  its edges from their sources */
 #define ADDS_F_PULL_EAGER 0x80u /* request a pull phase at the first heavy phase with any heavy edge and
                                  always take it, whatever the cost estimate says (tests, tuning) */
+#define ADDS_F_PULL_ALL_LIGHT 0x100u /* with pull_delta: after the re-bucketing pull every edge is light
+                                 (expanding a vertex relaxes its whole row; no heavy phases), which suits
+                                 the sparse rest of a power-law solve */
 
 typedef struct {
   double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */

@@ -1,7 +1,8 @@
 """Full-size parity on the benchmark workloads (adds_amd.workloads), through the C ABI.
 
 * C3: RMAT-24 ef32 (16.8M vertices, 1.02G directed edges), uint32 and float32 weights, single source:
-  bit-exact against the oracle's Dijkstra.
+  bit-exact against the oracle's Dijkstra at the default point, the pinned point (pull phase and
+  re-bucketing) and the pinned point without pulls.
 * C4a: 256 sources on RMAT-20 in one replicated launch: every source checked with the oracle's
   certificate (feasible + tight-reachable + INF exactly on the unreachable set, which pins the solution
   uniquely) and 16 of them against Dijkstra bit for bit.
@@ -43,20 +44,34 @@ def c3_graph():
 
 @pytest.mark.parametrize("kind", ["u32", "f32"])
 def test_c3_full_size_bit_exact(c3_graph, kind):
+    """The default parameters, the pinned benchmark point (a pull phase for bucket 0, then the rest of the
+    solve re-bucketed at pull_delta) and the pinned point with every heavy phase pushed: all three equal
+    Dijkstra bit for bit."""
+    from adds_amd import _lib as L, autotune
+
     g, src = c3_graph
     if kind == "f32":
         workloads.c3_reweight_float(g)
     assert g.is_float == (kind == "f32")
+    pinned, _ = autotune.lookup(autotune.workload_key(g, 1))
+    assert pinned is not None and pinned.get("pull_delta")
     dg = adds_amd.Graph(g)
     dist = torch.empty(g.n, dtype=torch.int32, device="cuda")
-    st = dg.sssp_device(src, dist.data_ptr(), timeout_ms=TIMEOUT)
-    print(f"c3 {kind}: kernel {st['kernel_ms']:.2f} ms, edges scanned {st['edges_scanned']}", flush=True)
-    got = _dist_words(dist)
+    got = {}
+    for name, params in (("default", {}), ("pinned", pinned),
+                         ("pinned, push only", dict(pinned, flags=pinned.get("flags", 0) | L.F_NO_PULL))):
+        st = dg.sssp_device(src, dist.data_ptr(), timeout_ms=TIMEOUT, **params)
+        print(f"c3 {kind} {name}: kernel {st['kernel_ms']:.2f} ms, edges scanned {st['edges_scanned']}, "
+              f"pulls {st['pulls']}", flush=True)
+        if name == "pinned":
+            assert st["pulls"] == 1
+        got[name] = _dist_words(dist).copy()
     dg.close()
     del dist
     ref = oracle.dijkstra(g.row_ptr, g.col, g.w, src, g.is_float)
-    bad = np.nonzero(got != ref)[0]
-    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}"
+    for name, d in got.items():
+        bad = np.nonzero(d != ref)[0]
+        assert bad.size == 0, f"{name}: {bad.size} mismatches, first at {bad[:5]}"
     n_r = int(np.count_nonzero(ref != (0x7F800000 if g.is_float else 0xFFFFFFFF)))
     assert n_r > g.n // 2
 

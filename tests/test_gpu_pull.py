@@ -60,6 +60,9 @@ def _symmetrize(row, col, w):
     dict(flags=EAGER, pull_delta=4.0),  # re-bucketed after the pull: 4 (float) or 16 (int) times wider
     dict(flags=EAGER | L.F_NO_WAIT | L.F_NO_LOCAL, edge_windows=8, pull_delta=4.0, tail_items=4096),
     dict(flags=EAGER, pull_delta=0.5),  # narrower after the pull
+    dict(flags=EAGER | L.F_PULL_ALL_LIGHT, pull_delta=4.0),  # ... and every edge light after the pull
+    dict(flags=EAGER | L.F_PULL_ALL_LIGHT | L.F_NO_WAIT, edge_windows=8, pull_delta=64.0, hub_degree=64,
+         task_edges=32),
 ])
 def test_rmat_pull_identical(is_float, params):
     """RMAT from its highest-degree vertex: bucket 0 holds most of the graph's edges.  With the default
@@ -108,7 +111,8 @@ def test_low_degree_source_long_unsettled_rows():
         assert st["pulls"] == 1
 
 
-@pytest.mark.parametrize("flags,pull_delta", [(EAGER, 0.0), (EAGER | L.F_REPLICA_PARTS, 0.0), (EAGER, 16.0)])
+@pytest.mark.parametrize("flags,pull_delta", [(EAGER, 0.0), (EAGER | L.F_REPLICA_PARTS, 0.0), (EAGER, 16.0),
+                                              (EAGER | L.F_PULL_ALL_LIGHT, 16.0)])
 def test_multi_source_pull(flags, pull_delta):
     """Many sources in one solve (replicated distance rows): a pull covers every instance's row."""
     g = adds_amd.HostGraph.rmat(13, 16, seed=9, wkind="f32", lo=1.0, hi=10.0)
@@ -127,8 +131,8 @@ def test_batched_union_pull():
            adds_amd.HostGraph.er(3000, 8.0, seed=8, wkind="u32", lo=1, hi=1000)]
     dgs = [adds_amd.Graph(h) for h in gs]
     srcs = [h.max_degree_vertex() for h in gs]
-    for pd in (0.0, 512.0):
-        dists, st = adds_amd.sssp_batched(dgs, srcs, timeout_ms=TIMEOUT, delta=64.0, flags=EAGER, pull_delta=pd)
+    for pd, fl in ((0.0, EAGER), (512.0, EAGER), (512.0, EAGER | L.F_PULL_ALL_LIGHT)):
+        dists, st = adds_amd.sssp_batched(dgs, srcs, timeout_ms=TIMEOUT, delta=64.0, flags=fl, pull_delta=pd)
         assert st["pulls"] == 1
         for h, d, s in zip(gs, dists, srcs):
             _check(h, d, s, False)
