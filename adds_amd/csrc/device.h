@@ -72,9 +72,6 @@ constexpr uint32_t s_slotlines(uint32_t nq) { return S_QBASE + QLINES * 2 * nq *
 // bucket are open) on a line of its own.  Words 2..3 of light line p: u64 heavy edges of the vertices
 // that claims of part p filed in the bucket's heavy queues (the cost of its push-direction heavy phase).
 constexpr uint32_t INFL_HEAVY = 2;
-// Word 4 of every in-flight line: claims of that part and phase between their first step and the landing
-// of their in-flight count (see try_claim).
-constexpr uint32_t INFL_CLAIMING = 4;
 constexpr uint32_t INFL_STRIDE = LINE;
 constexpr uint32_t SLOT_WORDS = (2 * 16 + 1) * LINE;
 constexpr uint32_t state_words(uint32_t nq, uint32_t nb) { return s_slotlines(nq) + nb * SLOT_WORDS; }

@@ -767,27 +767,24 @@ __device__ __forceinline__ bool try_claim(const KArgs& a, uint32_t qq, uint32_t 
   if (lane_id() == 0) {
     // guided size: a share of what is published (at least claim_min), at most the queue's chunk
     const int32_t chunk = (int32_t)(qq >= nq ? a.chunk_heavy : a.chunk);
-    // The claim is registered in the bucket's claiming counter from before it can move the head until its
-    // in-flight count has landed, so that the head advance never sees the head past the claimed items
-    // while the claim is counted nowhere (the head and the in-flight count are two words, updated at once).
+    // The in-flight count lands before the head moves past the claimed items (a round trip; a separate
+    // counter of claims in progress, updated without waiting, cost more: C4a +2%), so the head advance
+    // never sees the items gone while the claim is counted nowhere.
     uint32_t* fl = inflight(a, bb & (a.nb - 1), (qq % a.parts) + (qq >= nq ? MAX_PARTS : 0u));
-    afadd(fl + INFL_CLAIMING, 1u);
     const int32_t old = (int32_t)afadd(qavail(a, qq), 0u - (uint32_t)chunk);
     const int32_t avail = max(old, 0);
     const int32_t share = max((int32_t)a.claim_min, avail / (int32_t)a.claim_div);
     const int32_t take = min(min(avail, chunk), share);
     if (take < chunk) afadd(qavail(a, qq), (uint32_t)(chunk - take));
-    mem_wait();  // registered before the head moves
     if (take > 0) {
       afadd(fl, (uint32_t)take);
+      mem_wait();  // counted in flight before the head moves past the items
       hh = afadd(qhead(a, qq), (uint32_t)take);
       cc = (uint32_t)take;
       ok = 1;
     } else {
       ws.v[ST_CASFAIL]++;
     }
-    mem_wait();  // the in-flight count has landed
-    afadd(fl + INFL_CLAIMING, 0u - 1u);
   }
   if (!rdlane(ok, 0)) return false;
   h = rdlane(hh, 0);
@@ -848,6 +845,11 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
   bool busy = false;  // wave-uniform
   const uint64_t t_start = wall_clock64();
+  // histogram time origin (tracing only): the solve's start -- the first launch's, or the time of the
+  // timeline's first record for the launches that continue after a pull phase
+  const uint64_t t_hist = (a.trace && a.launch_idx > 0) ? __hip_atomic_load(a.trace + 4, __ATOMIC_RELAXED,
+                                                                           __HIP_MEMORY_SCOPE_AGENT)
+                                                        : t_start;
   uint32_t idle_iter = 0, work_iter = 0;
   bool missed = false;  // the last claim attempt lost: probe with atomic loads next time
   uint64_t tk[4] = {0, 0, 0, 0};  // activity ticks: probe, items, edges, idle (wave-uniform)
@@ -864,7 +866,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 
   for (;;) {
     const uint64_t t0 = wall_clock64();
-    if (a.trace) trace_hist(a, s_hl[wv], t0 - t_start, tk, ws.v[ST_EDGES], ws.v[ST_EXPANDED], ws.v[ST_T_FLUSH], false);
+    if (a.trace) trace_hist(a, s_hl[wv], t0 - t_hist, tk, ws.v[ST_EDGES], ws.v[ST_EXPANDED], ws.v[ST_T_FLUSH], false);
     // ---------------- probe the open queues in parallel ----------------
     // `parts` lanes each: light behind, light head, heavy behind, heavy head (only once the head's
     // phase word is set), light of the concur-1 buckets ahead.  Bit order is the claim priority; each
@@ -1383,9 +1385,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       const uint32_t c2 = ald(a.st + S_CUR);
       const uint32_t hs = c2 & (a.nb - 1), bs = (c2 - 1) & (a.nb - 1);
       // Head/behind state, one lane per part, in three rounds: (1) each queue's head, then its tail;
-      // (2) the in-flight and claiming counters; (3) the tails again.  A claim is counted (claiming, then
-      // in flight) before the head passes its items, and stops counting only after its pushes have
-      // reserved their tail positions.
+      // (2) the in-flight counters; (3) the tails again.  A claim is counted in flight before the head
+      // passes its items, and stops counting only after its pushes have reserved their tail positions.
       // So a queue that reads empty in (1), idle in (2) and unchanged in (3) had no claim of its bucket
       // running at any time in between (a single pass can miss a claim that starts or ends between its
       // reads, and open a heavy phase or move the head too early).
@@ -1406,7 +1407,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         mem_wait();
 #pragma unroll
         for (int i = 0; i < 4; i++)
-          f[i] = (i < 2 || beh) ? ald(inflight(a, fs[i], fi[i])) | ald(inflight(a, fs[i], fi[i]) + INFL_CLAIMING) : 0u;
+          f[i] = (i < 2 || beh) ? ald(inflight(a, fs[i], fi[i])) : 0u;
         mem_wait();
 #pragma unroll
         for (int i = 0; i < 4; i++) t1[i] = (i < 2 || beh) ? ald(qtail(a, ql[i])) : 0u;
@@ -1539,7 +1540,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
 
   // ---------------- exit: publish this wavefront's counters ----------------
   if (a.trace)
-    trace_hist(a, s_hl[wv], wall_clock64() - t_start, tk, ws.v[ST_EDGES], ws.v[ST_EXPANDED], ws.v[ST_T_FLUSH], true);
+    trace_hist(a, s_hl[wv], wall_clock64() - t_hist, tk, ws.v[ST_EDGES], ws.v[ST_EXPANDED], ws.v[ST_T_FLUSH], true);
   if (busy) {
     mem_wait();
     if (lane == 0) afadd64(verp(a, vgrp), 1ull);
