@@ -729,8 +729,9 @@ static adds_status wait_solve(Workspace& ws) {
 }
 
 // Appends the last solve's bucket timeline to the file named by ADDS_TRACE: one line per record,
-// "t_us kind bucket prev_head pool_chunks" (kind 0 = start, 1 = heavy phase opened, 2 = head moved,
-// 3 = done), after a
+// "t_us kind bucket prev_head pool_chunks" (kind 0 = start of a launch, 1 = heavy phase opened, 2 = head
+// moved, 3 = done, 5 = pull requested (prev_head: heavy edges queued), 6 = pull phase decided (prev_head:
+// 1 pulled, 2 pushed, 3 pulled and re-bucketed)), after a
 // header line per solve.  A diagnostic aid; solves are unaffected apart from the records' writes.
 static void dump_trace(const adds_graph* g, const Workspace& ws) {
   const char* path = getenv("ADDS_TRACE");
