@@ -26,7 +26,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_CACHE = os.path.join(_HERE, "tune_cache.json")
 PINNED = os.path.join(_HERE, "tuned_params.json")
 TUNED_FIELDS = ("delta", "wg_per_cu", "chunk", "nbuckets", "parts", "concurrency", "task_edges", "flags",
-                "light_delta", "edge_windows", "tail_items", "hub_degree", "pull_delta")
+                "light_delta", "edge_windows", "tail_items", "hub_degree", "pull_delta",
+                "order_degree")
 
 
 def workload_key(host, k):

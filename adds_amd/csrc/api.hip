@@ -454,6 +454,8 @@ struct Resolved {
   uint32_t pull_slots, ptask_cap;  // pull phases per solve (0: none) and the long-row task list size
   float pull_delta;                // bucket width after the first pull (0: delta)
   uint64_t pull_min;               // heavy edges a bucket needs before a pull phase is considered
+  uint32_t pull_at;                // ADDS_F_PULL_SPLIT: the bucket of the forced first pull (PULL_AT_OFF: off)
+  uint32_t order_degree;           // ... and the light degree from which vertices wait for their bucket there
 };
 
 // Default bucket width.  The reference derives it from the mean weight and degree
@@ -529,6 +531,24 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
     return ADDS_EINVAL;
   }
   r.pull_min = std::max<uint64_t>(total_n / 2, 1u << 20);
+  // Split first bucket: the last bucket of width delta that no heavy edge can reach, i.e. one below the
+  // bucket of the smallest heavy weight (the kernel's bucket key is monotone, and a heavy edge's target
+  // distance is at least its weight).  Heavy: w > floor(light_delta) (int), w > light_delta (float).
+  r.pull_at = PULL_AT_OFF;
+  if ((r.flags & ADDS_F_PULL_SPLIT) && r.pull_slots) {
+    float hmin;
+    if (g->wt == ADDS_W_F32) {
+      hmin = std::nextafter(r.light_delta, INFINITY);
+    } else {
+      const double lh = std::floor((double)r.light_delta) + 1.0;
+      hmin = lh >= 4294967295.0 ? INFINITY : (float)(uint32_t)lh;
+    }
+    const float inv = 1.0f / r.delta;  // the kernel's inv_delta
+    const float x = std::fmin(hmin * inv, 4.0e9f);
+    const uint32_t key = std::isfinite(hmin) ? (uint32_t)x : 0u;
+    if (key >= 1) r.pull_at = key - 1;
+  }
+  r.order_degree = r.pull_at != PULL_AT_OFF ? p->order_degree : 0u;
   r.ptask_cap = (uint32_t)std::min<uint64_t>(total_m / PULL_LONG + 4096, 1u << 24);
   return ADDS_OK;
 }
@@ -648,7 +668,7 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.timeout_ticks = (uint64_t)(r.timeout_ms * (double)rate_khz);
   a.cached_probe = (r.flags & ADDS_F_ATOMIC_PROBE) ? 0u : 1u;
   a.sticky = 1;
-  a.local_cont = (r.flags & ADDS_F_NO_LOCAL) ? 0u : 1u;
+  a.local_cont = (r.flags & ADDS_F_NO_LOCAL) ? 0u : (r.flags & ADDS_F_LOCAL_SPARSE) ? 2u : 1u;
   a.nowait = (r.flags & ADDS_F_NO_WAIT) ? 1u : 0u;
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
@@ -660,6 +680,8 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.pull_ok = g->sym_dev;
   a.inv_delta_resume = r.pull_delta > 0.f ? 1.0f / r.pull_delta : 0.f;
   a.resume_all_light = (r.flags & ADDS_F_PULL_ALL_LIGHT) ? 1u : 0u;
+  a.pull_at = r.pull_at;
+  a.order_degree = r.order_degree;
   a.ptask = ws.ptask;
   a.ptask_cap = ws.ptask_cap;
   a.launch_idx = 0;
@@ -1435,7 +1457,22 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       unsigned long long nd = 0;
       HIPCHK(hipMemcpy(&nd, diff, 8, hipMemcpyDeviceToHost));
       if (nd) {
-        g_detail = "autotuner: distances differ between tuning points (" + std::to_string(nd) + " words)";
+        // name the point and the first differing words (index: first point's word / this point's)
+        std::vector<uint32_t> hr(words), hc(words);
+        std::string where;
+        if (hipMemcpy(hr.data(), ref, words * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(hc.data(), cur, words * 4, hipMemcpyDeviceToHost) == hipSuccess)
+          for (uint64_t j = 0, shown = 0; j < words && shown < 4; j++)
+            if (hr[j] != hc[j]) {
+              where += " " + std::to_string(j) + ":" + std::to_string(hr[j]) + "/" + std::to_string(hc[j]);
+              shown++;
+            }
+        char pt[256];
+        snprintf(pt, sizeof(pt), "delta=%g light=%g pull_delta=%g order=%u nb=%u parts=%u concur=%u chunk=%u tail=%u flags=%u",
+                 p.delta, p.light_delta, p.pull_delta, p.order_degree, p.nbuckets, p.parts, p.concurrency, p.chunk,
+                 p.tail_items, p.flags);
+        g_detail = "autotuner: distances differ between tuning points (" + std::to_string(nd) + " words, run " +
+                   std::to_string(i) + " of " + pt + ";" + where + ")";
         return ADDS_EHIP;
       }
     }
@@ -1480,9 +1517,9 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
         return;
       }
       if (getenv("ADDS_TUNE_VERBOSE"))
-        fprintf(stderr, "[tune] delta=%g light=%g pull_delta=%g nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u hub=%u windows=%u tail=%u flags=%u -> %.3f ms\n",
-                q.delta, q.light_delta, q.pull_delta, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu, q.chunk,
-                q.task_edges, q.hub_degree, q.edge_windows, q.tail_items, q.flags, ms);
+        fprintf(stderr, "[tune] delta=%g light=%g pull_delta=%g order=%u nb=%u parts=%u concur=%u wg/cu=%u chunk=%u task_edges=%u hub=%u windows=%u tail=%u flags=%u -> %.3f ms\n",
+                q.delta, q.light_delta, q.pull_delta, q.order_degree, q.nbuckets, q.parts, q.concurrency, q.wg_per_cu,
+                q.chunk, q.task_edges, q.hub_degree, q.edge_windows, q.tail_items, q.flags, ms);
       tried.push_back({ms, q});
       if (ms < bms) {
         bms = ms;
@@ -1604,6 +1641,37 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
         adds_params q = p;
         q.flags ^= ADDS_F_PULL_ALL_LIGHT;
         try_point(q);
+      }
+    }
+    // split first bucket (symmetric graphs): the light threshold kept, narrower buckets below it, all
+    // their heavy edges pulled at once, optionally only the vertices of many light edges in bucket order;
+    // then more open buckets and local continuation for the sparse stretches
+    if (g->symmetric) {
+      const adds_params p4 = p;
+      const float lt = p4.light_delta > 0.f ? p4.light_delta : p4.delta;
+      if (lt < 1e29f && !(p4.flags & ADDS_F_NO_PULL)) {
+        for (float div : {1.0f, 8.0f, 16.0f})
+          for (uint32_t od : {0u, 16u, 32u}) {
+            if (div == 1.0f && od) continue;
+            adds_params q = p4;
+            q.flags |= ADDS_F_PULL_SPLIT;
+            q.delta = lt / div;
+            q.light_delta = lt;
+            q.order_degree = od;
+            try_point(q);
+          }
+        if (p.flags & ADDS_F_PULL_SPLIT) {
+          if (p.concurrency < 4 && (4 + 3) * p.parts <= MAX_PROBE_QUEUES) {
+            adds_params q = p;
+            q.concurrency = 4;
+            try_point(q);
+          }
+          {
+            adds_params q = p;
+            q.flags = (q.flags & ~ADDS_F_NO_LOCAL) | ADDS_F_LOCAL_SPARSE;
+            try_point(q);
+          }
+        }
       }
     }
     for (uint32_t hd : {0u, 4096u, 16384u}) {

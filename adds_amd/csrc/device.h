@@ -132,11 +132,14 @@ struct KArgs {
   uint64_t timeout_ticks;  // wall_clock64 ticks
   uint32_t cached_probe;   // probe queue counters with L2-cached loads (claims stay atomic)
   uint32_t sticky;         // after a full claim from the behind/head bucket, claim the same queue again
-  uint32_t local_cont;     // a light claim keeps up to 64 same-bucket improvements for itself
+  uint32_t local_cont;     // a light claim keeps up to 64 same-bucket improvements for itself (2: only a
+                           // claim of at most chunk / 4 items)
   uint32_t nowait;         // flushes do not wait for their atomicMins (every improvement is queued)
   // Optional bucket timeline (null = off): trace[0] = record count, then records of 4 words from
   // trace + 4: wall clock, (kind << 56 | bucket), previous head, pool chunks used.  Kinds: 1 = the
-  // head bucket's heavy phase opened, 2 = the head moved to `bucket`, 0 = start, 3 = done.
+  // head bucket's heavy phase opened, 2 = the head moved to `bucket`, 0 = start, 3 = done, 5 = pull phase
+  // requested (previous head: heavy edges queued), 6 = pull phase decided (previous head: 1 pulled,
+  // 2 pushed, 3 pulled and re-bucketed).
   // After the records: an activity histogram of TRACE_HBINS bins of trace_slice ticks, 8 words per bin
   // (wavefront ticks probing, on items, on edges, idle; edges relaxed; items expanded; ticks in
   // push-buffer flushes; 1 spare), in
@@ -156,6 +159,13 @@ struct KArgs {
   const uint32_t* pull_ok;  // the graph's device word: nonzero while it is declared symmetric
   float inv_delta_resume;   // 1 / pull_delta: the bucket width after the first pull (0: unchanged)
   uint32_t resume_all_light;  // after a re-bucketing pull every edge is light (ADDS_F_PULL_ALL_LIGHT)
+  // ADDS_F_PULL_SPLIT: launch 0 runs buckets 0 .. pull_at (narrower than the light threshold) through
+  // their light phases only and then asks for one pull phase at bucket pull_at, which relaxes the heavy
+  // edges of all of them (PULL_AT_OFF: off)
+  uint32_t pull_at;
+  // ... and below pull_at only the vertices of at least order_degree light edges wait for their bucket:
+  // the others are queued in the bucket of the claim that finds them (0: every vertex waits)
+  uint32_t order_degree;
   uint32_t ptask_cap;
   uint4* ptask;          // long-row tasks of a pull phase: (vertex-instance, first edge, end edge, 0)
 };
@@ -163,6 +173,7 @@ struct KArgs {
 // Pull phases: rows with more heavy edges than PULL_LONG are scanned by k_pull_b in tasks of at most
 // PULL_TASK edges (one wavefront each), the rest by k_pull_a, 256 vertex-instances per wavefront.
 constexpr uint32_t PULL_LONG = 128;
+constexpr uint32_t PULL_AT_OFF = 0xFFFFFFFFu;
 constexpr uint32_t PULL_TASK = 1024;
 
 // Inclusive prefix sum over the 64 lanes of a wavefront (every lane active), with DPP: row shifts by

@@ -309,9 +309,6 @@ struct PushBuf {
   uint32_t n;    // wave-uniform fill
 };
 
-// Target queue of a light-queue push: the bucket of nd, clamped to the window of the bucket being
-// processed (cb): never before it (such items are simply early work) and never so far ahead that the
-// ring slot would alias the behind bucket.
 // The sub-queue part that the items of the replica at base offset gb go to (the pusher's own part unless
 // the solve places replicas by part).
 __device__ __forceinline__ uint32_t push_part(const KArgs& a, uint32_t gb, uint32_t part) {
@@ -319,10 +316,17 @@ __device__ __forceinline__ uint32_t push_part(const KArgs& a, uint32_t gb, uint3
   const uint32_t r = a.n_magic ? __umulhi(gb, a.n_magic) : gb;
   return r & (a.parts - 1);
 }
+// Target queue of a light-queue push: QM_OWN, the bucket of nd; QM_TAIL, the claim's own bucket (the tail
+// of a solve is processed as one bucket); QM_ORDER (ordered split first bucket), the claim's own bucket
+// while nd lies below the split point, where a claim sends the vertices of many light edges on to their
+// own buckets.  Clamped to the window of the bucket being processed (cb): never before it (such items are
+// simply early work) and never so far ahead that the ring slot would alias the behind bucket.
+constexpr uint32_t QM_OWN = 0, QM_TAIL = 1, QM_ORDER = 2;
 template <bool FLT>
 __device__ __forceinline__ uint32_t light_queue(const KArgs& a, float idl, uint32_t nd, uint32_t part, uint32_t cb,
-                                                bool tail) {
-  uint32_t b = tail ? cb : bucket_key<FLT>(nd, idl);  // the tail is processed as one bucket
+                                                uint32_t mode) {
+  uint32_t b = bucket_key<FLT>(nd, idl);
+  if (mode == QM_TAIL || (mode == QM_ORDER && b <= a.pull_at)) b = cb;
   b = max(b, cb);
   b = min(b, cb + a.nb - 3);
   return (b & (a.nb - 1)) * a.parts + part;
@@ -598,7 +602,7 @@ __device__ __forceinline__ void gather_pairs(const KArgs& a, const PairGroup<UN>
 }
 template <bool FLT, bool REPL, int UN>
 __device__ __forceinline__ void finish_pairs(const KArgs& a, float idl, const PairGathered<UN>& r, uint32_t part,
-                                             uint32_t cb, bool heavy, bool tail, PushBuf& pb, WaveStats& ws) {
+                                             uint32_t cb, bool heavy, uint32_t qmode, PushBuf& pb, WaveStats& ws) {
   bool f0[UN], f1[UN];
   uint32_t nact = 0;
 #pragma unroll
@@ -614,8 +618,8 @@ __device__ __forceinline__ void finish_pairs(const KArgs& a, float idl, const Pa
 #pragma unroll
   for (int j = 0; j < UN; j++) {
     const uint32_t pp = REPL ? push_part(a, r.pp[j], part) : part;
-    pb_add(a, pb, f0[j], ((uint64_t)r.n0[j] << 32) | r.t0[j], light_queue<FLT>(a, idl, r.n0[j], pp, cb, tail) | PB_RELAX, ws);
-    pb_add(a, pb, f1[j], ((uint64_t)r.n1[j] << 32) | r.t1[j], light_queue<FLT>(a, idl, r.n1[j], pp, cb, tail) | PB_RELAX, ws);
+    pb_add(a, pb, f0[j], ((uint64_t)r.n0[j] << 32) | r.t0[j], light_queue<FLT>(a, idl, r.n0[j], pp, cb, qmode) | PB_RELAX, ws);
+    pb_add(a, pb, f1[j], ((uint64_t)r.n1[j] << 32) | r.t1[j], light_queue<FLT>(a, idl, r.n1[j], pp, cb, qmode) | PB_RELAX, ws);
   }
 }
 
@@ -624,7 +628,7 @@ __device__ __forceinline__ void finish_pairs(const KArgs& a, float idl, const Pa
 // waits there cover loads only and the pipeline keeps two groups' loads in flight.
 template <bool FLT, bool REPL, int UN>
 __device__ __forceinline__ void finish(const KArgs& a, float idl, const Gathered<UN>& r, uint32_t part, uint32_t cb,
-                                       bool heavy, bool tail, PushBuf& pb, WaveStats& ws) {
+                                       bool heavy, uint32_t qmode, PushBuf& pb, WaveStats& ws) {
   bool f[UN];
   uint32_t nact = 0;
 #pragma unroll
@@ -639,7 +643,7 @@ __device__ __forceinline__ void finish(const KArgs& a, float idl, const Gathered
 #pragma unroll
   for (int j = 0; j < UN; j++)
     pb_add(a, pb, f[j], ((uint64_t)r.nd[j] << 32) | r.tgt[j],
-           light_queue<FLT>(a, idl, r.nd[j], REPL ? push_part(a, r.pp[j], part) : part, cb, tail) | PB_RELAX, ws);
+           light_queue<FLT>(a, idl, r.nd[j], REPL ? push_part(a, r.pp[j], part) : part, cb, qmode) | PB_RELAX, ws);
 }
 
 // Appends a bucket-timeline record when tracing is on (called by one lane).
@@ -832,6 +836,17 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
   const bool rebucketed = a.launch_idx > 0 && a.inv_delta_resume > 0.f && ald(a.st + PULL_REBUCKET) != 0;
   const float idl = rebucketed ? a.inv_delta_resume : a.inv_delta;
   const bool all_light = rebucketed && a.resume_all_light;
+  // Split first bucket (ADDS_F_PULL_SPLIT): buckets 0 .. pull_at are narrower than the light threshold,
+  // so no heavy edge can lower a distance inside them; they run their light phases only (nothing is
+  // filed heavy), and at the end of bucket pull_at one pull phase relaxes the heavy edges of all of them.
+  // Until then the head stops at pull_at and the solve cannot end.
+  const bool defer = a.launch_idx == 0 && a.pull_at != PULL_AT_OFF && a.pull_slots != 0 && ald(a.pull_ok) != 0;
+  // Ordered split (order_degree): below the split point only vertices of at least order_degree light
+  // edges are expanded in bucket order.  Every push below it goes to the pusher's own bucket; a claim
+  // expands the few-edge vertices it finds there at once and sends the many-edge ones on to their own
+  // buckets (as early items).  An item's bucket never exceeds its distance's, so bucket pull_at's light
+  // phase still ends with every vertex below the split point settled.
+  const bool order = defer && a.order_degree != 0;
   PushBuf pbuf{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
   for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;  // no queue has id 0xFFFF
   wave_sync();
@@ -983,6 +998,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       // changes -- per distance step.  Read as a hint: a wavefront that sees it late only pushes some
       // items to their own buckets, which is also correct.
       const bool tail_mode = a.tail_items != 0 && hld(a.st + S_TAIL, true) != 0;
+      const uint32_t qmode = tail_mode ? QM_TAIL : order ? QM_ORDER : QM_OWN;
       // ---------------- load the claimed entries (IPL per lane: item k = j * 64 + lane) ----------------
       // Positions h .. h+cnt-1 span at most two pool chunks (cnt <= 256 < CH).
       uint32_t p0 = TAB_BAD, p1 = TAB_BAD;
@@ -1099,7 +1115,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         // relaxed now: the order of relaxations only affects the work, never the result.
         const uint32_t key = bucket_key<FLT>(du[j], idl);
         early[j] = have[j] && !stale && !is_ref[j] && !heavy_claim && !tail_mode && key > cb &&
-                   key <= cb + a.nb - 3;
+                   key <= cb + a.nb - 3 && (!order || key > a.pull_at || lcv[j] >= a.order_degree);
         ok[j] = have[j] && !stale && !early[j];
         has_heavy[j] = false;
         if (!is_ref[j]) {
@@ -1115,12 +1131,12 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       uint64_t filed_heavy = 0;  // heavy edges of the vertices filed below (the pull-phase cost model)
 #pragma unroll
       for (int j = 0; j < IPL; j++) {
-        const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j];
+        const bool file_heavy = ok[j] && !heavy_claim && has_heavy[j] && !(defer && cb <= a.pull_at);
         filed_heavy += file_heavy ? r1v[j] - r0v[j] - lcv[j] : 0u;  // (never filed when all_light)
         nexp += (uint32_t)__popcll(ballot(ok[j] && !is_ref[j] && !heavy_claim));
         nlate += (uint32_t)__popcll(ballot(ok[j] && bucket_key<FLT>(du[j], idl) + 1 < cb));
         const uint32_t pp = REPL ? push_part(a, gbase[j], part) : part;
-        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, idl, du[j], pp, cb, tail_mode), ws);
+        pb_add(a, pbuf, early[j], ((uint64_t)du[j] << 32) | gid[j], light_queue<FLT>(a, idl, du[j], pp, cb, QM_OWN), ws);
         pb_add(a, pbuf, file_heavy, ((uint64_t)du[j] << 32) | vgid[j], nq + (cb & (a.nb - 1)) * a.parts + pp, ws);
       }
       if (a.pull_slots) {
@@ -1271,7 +1287,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
             locate_p(base + 128 * UP, ga);
             load_pairs(a, ga);
             gather_pairs<FLT, UP>(a, gb, rb);
-            finish_pairs<FLT, REPL, UP>(a, idl, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
+            finish_pairs<FLT, REPL, UP>(a, idl, ra, part, cb, heavy_claim, qmode, pbuf, ws);
             ra = rb;
             gb = ga;
           }
@@ -1290,7 +1306,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           locate(base + 128 * UN, ga);
           load_cw(a, ga);
           gather<FLT, UN>(a, gb, rb);
-          finish<FLT, REPL, UN>(a, idl, ra, part, cb, heavy_claim, tail_mode, pbuf, ws);
+          finish<FLT, REPL, UN>(a, idl, ra, part, cb, heavy_claim, qmode, pbuf, ws);
           ra = rb;
           gb = ga;
         }
@@ -1301,7 +1317,10 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       // reservations have (atomics with a result), so the queues already read as non-empty to the head
       // advance and to the termination test.  Improvements kept for local continuation stay counted,
       // under this wavefront's own part of the bucket.
-      const uint32_t own_q = (a.local_cont && !heavy_claim) ? (cb & (a.nb - 1)) * a.parts + part : 0xFFFFFFFFu;
+      // (local_cont 2: only after claims of at most a quarter chunk -- the sparse stretches, where a hop's
+      // queue round trips are the cost -- while full claims leave their improvements to every wavefront)
+      const bool keep_local = a.local_cont == 1 || (a.local_cont == 2 && cnt * 4 <= a.chunk);
+      const uint32_t own_q = (keep_local && !heavy_claim) ? (cb & (a.nb - 1)) * a.parts + part : 0xFFFFFFFFu;
       const uint32_t nl = pb_flush(flush_args(a), pbuf, ws, own_q);
       if (nl) {
         local_e = lane < nl ? pbuf.e[lane] : 0ull;
@@ -1344,7 +1363,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     // Termination: every group's word reads all idle, every queue is empty, and no group's word has
     // changed since (a wavefront goes busy before it claims, and idle only after its pushes have
     // completed).  The own group's word is checked first, so idle wavefronts mostly read one line.
-    if ((uint32_t)hld64(verp(a, vgrp), a.cached_probe && (idle_iter & 7) != 7) == group_waves(gridDim.x, vgrp)) {
+    if (!defer &&
+        (uint32_t)hld64(verp(a, vgrp), a.cached_probe && (idle_iter & 7) != 7) == group_waves(gridDim.x, vgrp)) {
       const uint64_t w1 = lane < VER_GROUPS ? ald64(verp(a, lane)) : 0ull;
       const uint64_t idle = wave_sum64(w1 & 0xFFFFFFFFull);
       if (idle == a.total_waves) {
@@ -1425,7 +1445,9 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         // -- from the receiving side by a pull phase: request it and let this launch end.
         bool req = false;
         uint64_t hv = 0;
-        if (a.launch_idx < a.pull_slots && pgen == a.launch_idx &&
+        if (defer) {
+          req = c2 >= a.pull_at;  // the deferred heavy edges: pulled at bucket pull_at
+        } else if (a.launch_idx < a.pull_slots && pgen == a.launch_idx &&
             rdlane(lane == 0 ? (ald(a.st + S_TAIL) == 0 && ald(a.pull_ok) != 0 ? 1u : 0u) : 0u, 0) != 0) {
           hv = wave_sum64(lane < a.parts ? ald64(heavy_ctr(a, hs, lane)) : 0ull);
           req = a.pull_eager ? hv > 0 : hv >= a.pull_min;
@@ -1449,7 +1471,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           opened = true;
         }
       }
-      if (!hl && !hh && !ba) {
+      if (!hl && !hh && !ba && !(defer && c2 >= a.pull_at)) {
         uint32_t firstj = 0xFFFFFFFFu;
         uint64_t pend = 0;  // items queued or in flight in the window (this lane's buckets)
         for (uint32_t j0 = 0; j0 < a.nb; j0 += 64) {
@@ -1470,6 +1492,8 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           if (m && firstj == 0xFFFFFFFFu) firstj = j0 + ffs64(m);
         }
         pend = wave_sum64(pend);
+        // deferred heavy edges: the head stops at pull_at, empty or not
+        if (defer) firstj = min(firstj, a.pull_at + 1 - c2);
         if (firstj != 0xFFFFFFFFu) {
           const uint32_t c3 = c2 - 1 + firstj;
           // buckets c2-1 .. c3-2 leave the window; their slots come back later as fresh buckets
@@ -1485,7 +1509,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           // start of a solve, whose frontier is small too, ordered: run as one bucket from there, a
           // grid solve does many times the work.  Head moves are serialised (S_ADV), so the peak
           // word is read and written by one wavefront at a time.
-          if (lane == 0 && a.tail_items != 0) {
+          if (lane == 0 && a.tail_items != 0 && !defer) {
             const uint32_t p32 = (uint32_t)min(pend, (uint64_t)0xFFFFFFFFu);
             const uint32_t peak = max(ald(a.st + S_TAIL + 1), p32);
             ast(a.st + S_TAIL + 1, peak);
@@ -1619,7 +1643,7 @@ constexpr uint32_t PULL_COUNT_THREADS = 1024;  // k_pull_count: few large workgr
 template <bool FLT, bool REPL>
 __global__ void __launch_bounds__(PULL_COUNT_THREADS) k_pull_count(KArgs a, uint32_t slot) {
   __shared__ uint64_t s_red[PULL_COUNT_THREADS / 64][2];
-  if (!pull_requested(a, slot)) return;
+  if (!pull_requested(a, slot) || (slot == 1 && a.pull_at != PULL_AT_OFF)) return;  // forced: no estimate
   const uint32_t c = ald(a.st + PULL_BUCKET);
   const float idl = pull_req_idl(a);
   uint64_t in_edges = 0, uns = 0;
@@ -1650,7 +1674,8 @@ constexpr int PULL_UN = 8;  // 64-edge windows per wavefront in flight in the pu
 __global__ void __launch_bounds__(256) k_pull_decide(KArgs a, uint32_t slot) {
   if (!pull_requested(a, slot)) return;
   const uint32_t c = ald(a.st + PULL_BUCKET);
-  const bool pull = a.pull_eager || pull_pays(ald64((const uint64_t*)(a.st + PULL_H)),
+  // the first pull of a split first bucket is forced: its heavy edges were never queued for pushing
+  const bool pull = a.pull_eager || (slot == 1 && a.pull_at != PULL_AT_OFF) || pull_pays(ald64((const uint64_t*)(a.st + PULL_H)),
                                               ald64((const uint64_t*)(a.st + S_PULL2 + 2)),
                                               ald64((const uint64_t*)(a.st + S_PULL2 + 4)), a.total_n);
   const bool rebucket = pull && a.inv_delta_resume > 0.f && ald(a.st + PULL_DONE) == 0;
@@ -1827,7 +1852,7 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
       nimp += (uint32_t)__popcll(ballot(imp));
       const uint32_t pp = REPL ? push_part(a, gb[j], part) : part;
       const bool q = imp || (rebucket && uns[j] && best < INF);
-      pb_add(a, pb, q, ((uint64_t)best << 32) | x[j], light_queue<FLT>(a, pidl, best, pp, pc, false), ws);
+      pb_add(a, pb, q, ((uint64_t)best << 32) | x[j], light_queue<FLT>(a, pidl, best, pp, pc, QM_OWN), ws);
     }
     wave_sync();
     if (lane == 0) {
@@ -1898,7 +1923,7 @@ __global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
       if (imp) ws.v[ST_PULL_IMPROVED]++;
     }
     const uint32_t pp = REPL ? push_part(a, gb, part) : part;
-    pb_add(a, pb, imp, ((uint64_t)best << 32) | tk.x, light_queue<FLT>(a, pidl, best, pp, pc, false), ws);
+    pb_add(a, pb, imp, ((uint64_t)best << 32) | tk.x, light_queue<FLT>(a, pidl, best, pp, pc, QM_OWN), ws);
   }
   pb_flush(flush_args(a), pb, ws);
   publish_stats(a.st, s_stat, ws, nullptr);

@@ -99,7 +99,7 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
     if (colon != std::string::npos) *v = strtod(obj.c_str() + colon + 1, nullptr);
   };
   double delta = 0, light = 0, wg = 0, chunk = 0, nb = 0, parts = 0, conc = 0, te = 0, flags = 0, win = 0, tail = 0,
-         hub = 0, pdelta = 0;
+         hub = 0, pdelta = 0, order = 0;
   num("delta", &delta);
   num("light_delta", &light);
   num("wg_per_cu", &wg);
@@ -113,6 +113,7 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
   num("tail_items", &tail);
   num("hub_degree", &hub);
   num("pull_delta", &pdelta);
+  num("order_degree", &order);
   p->delta = (float)delta;
   p->light_delta = (float)light;
   p->wg_per_cu = (uint32_t)wg;
@@ -121,11 +122,13 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
   p->parts = (uint32_t)parts;
   p->concurrency = (uint32_t)conc;
   p->task_edges = (uint32_t)te;
-  p->flags = (uint32_t)flags & (ADDS_F_NO_LOCAL | ADDS_F_NO_WAIT | ADDS_F_NO_PULL | ADDS_F_PULL_ALL_LIGHT);  // tuning flags only
+  p->flags = (uint32_t)flags & (ADDS_F_NO_LOCAL | ADDS_F_NO_WAIT | ADDS_F_NO_PULL | ADDS_F_PULL_ALL_LIGHT |
+                                ADDS_F_PULL_SPLIT | ADDS_F_LOCAL_SPARSE);  // tuning flags only
   p->edge_windows = (uint32_t)win;
   p->tail_items = (uint32_t)tail;
   p->hub_degree = (uint32_t)hub;
   p->pull_delta = (float)pdelta;
+  p->order_degree = (uint32_t)order;
   return true;
 }
 

@@ -210,6 +210,15 @@ def near_far(dg, src, adds_dist, adds_ms, torch, reps=3):
             "same_as_adds": bool(torch.equal(buf, adds_dist)), "adds_speedup": ms / adds_ms}
 
 
+def cpu_delta(params, default):
+    """Bucket width for the CPU delta-stepping leg: the GPU point's delta, or with a split first bucket
+    (narrow buckets below the light threshold, no counterpart in plain delta-stepping) its light threshold."""
+    import adds_amd._lib as L
+    if params.get("flags", 0) & L.F_PULL_SPLIT and params.get("light_delta"):
+        return params["light_delta"]
+    return params.get("delta") or default
+
+
 def cpu_single(host, src, m_r, oracle, hinfo, delta):
     """CPU legs of a single-source workload: the oracle's Dijkstra on 1 thread and the parallel
     delta-stepping on the CPU share; both checked against each other."""
@@ -283,7 +292,7 @@ def run_c3(host, src, name, steps, warmup, torch, adds_amd, oracle, prm, do_cpu,
     dg.close()
     del dist
     if do_cpu:
-        cpu, ref = cpu_single(host, src, m_r, oracle, hinfo, params.get("delta") or 4096.0)
+        cpu, ref = cpu_single(host, src, m_r, oracle, hinfo, cpu_delta(params, 4096.0))
         res["cpu"] = cpu
         res["bit_exact_vs_dijkstra"] = bool(np.array_equal(ref, d))
         log(f"{name}: cpu {cpu}, bit-exact {res['bit_exact_vs_dijkstra']}")
@@ -309,7 +318,7 @@ def run_small(host, name, src, steps, warmup, torch, adds_amd, oracle, prm, do_c
     res["near_far"] = near_far(dg, src, dist, float(np.mean([s["ms"] for s in stats])), torch)
     dg.close()
     if do_cpu:
-        cpu, ref = cpu_single(host, src, m_r, oracle, hinfo, params.get("delta") or 1.0)
+        cpu, ref = cpu_single(host, src, m_r, oracle, hinfo, cpu_delta(params, 1.0))
         res["cpu"] = cpu
     else:
         ref = oracle.dijkstra(host.row_ptr, host.col, host.w, src, host.is_float)

@@ -77,6 +77,12 @@ typedef struct {
                             is left after such a pull is the sparse rest of the graph; a wider bucket there
                             saves phase changes (each a few claim round trips) at the price of a few
                             re-expansions of low-degree vertices */
+  uint32_t order_degree; /* with ADDS_F_PULL_SPLIT: only vertices with at least this many light edges are
+                            expanded in bucket order below the split point; the others are expanded in the
+                            bucket that finds them, whatever their distance (0, the default: every vertex in
+                            bucket order).  A vertex of many light edges expanded before its distance is
+                            final rescans all of them; one of few costs little, and running those out of
+                            order needs fewer, fuller buckets */
 } adds_params;
 
 /* flags */
@@ -100,6 +106,15 @@ typedef struct {
 #define ADDS_F_PULL_ALL_LIGHT 0x100u /* with pull_delta: after the re-bucketing pull every edge is light
                                  (expanding a vertex relaxes its whole row; no heavy phases), which suits
                                  the sparse rest of a power-law solve */
+#define ADDS_F_PULL_SPLIT 0x200u /* symmetric graphs with light_delta > delta: the buckets below the
+                                 lightest heavy edge's bucket run their light phases only (no heavy edge
+                                 can reach them), then one pull phase relaxes all their heavy edges at
+                                 once; narrow buckets order the first light phases (fewer re-expansions)
+                                 without pushing the heavy edges of the dense start of the solve */
+#define ADDS_F_LOCAL_SPARSE 0x400u /* local continuation only after claims of at most chunk / 4 items: the
+                                 sparse stretches (chains of single hops at the end of a bucket) keep it,
+                                 full claims hand their improvements to every wavefront (ignored with
+                                 ADDS_F_NO_LOCAL) */
 
 typedef struct {
   double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */

@@ -262,3 +262,118 @@ def test_device_symmetry_check():
     rp[src + 1:] -= 1
     d1 = adds_amd.Graph(adds_amd.HostGraph.from_arrays(rp.astype(np.uint32), g.col[keep], g.w_bits[keep], "u32"))
     assert not d1.check_symmetric()
+
+
+SPLIT = L.F_PULL_SPLIT
+
+
+@pytest.mark.parametrize("is_float", [False, True])
+@pytest.mark.parametrize("params", [
+    dict(flags=SPLIT),
+    dict(flags=SPLIT, pull_delta=16.0),
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT, pull_delta=16.0),
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT | L.F_NO_WAIT | L.F_NO_LOCAL, edge_windows=8, pull_delta=128.0),
+    dict(flags=SPLIT, tail_items=4096, concurrency=4, pull_delta=16.0),
+    dict(flags=SPLIT, parts=1, nbuckets=4, chunk=16),  # the ring narrower than the split bucket
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT, hub_degree=64, task_edges=64, pull_delta=8.0),
+    dict(flags=SPLIT, light_scale=3.3),  # a light threshold that is no multiple of the bucket width
+    # ordered split: only vertices of at least order_degree light edges wait for their bucket
+    dict(flags=SPLIT, order_degree=4),
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT, order_degree=16, pull_delta=16.0, concurrency=1),
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT | L.F_NO_WAIT, order_degree=2, edge_windows=8, pull_delta=64.0,
+         tail_items=4096, hub_degree=64, task_edges=32),
+    dict(flags=SPLIT, order_degree=1 << 30, parts=2, nbuckets=8),  # no vertex qualifies: none waits
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT | L.F_LOCAL_SPARSE, order_degree=8, pull_delta=16.0, chunk=64),
+])
+def test_rmat_split_first_bucket(is_float, params):
+    """ADDS_F_PULL_SPLIT: the buckets below the light threshold run their light phases only, then one
+    forced pull relaxes all their heavy edges (counted as one pull, whatever the cost estimate says)."""
+    wk = dict(wkind="f32", lo=0.0, hi=1.0) if is_float else dict(wkind="u32", lo=1, hi=65535)
+    g = adds_amd.HostGraph.rmat(15, 32, seed=21, **wk)
+    src = g.max_degree_vertex()
+    delta = 0.0078125 if is_float else 512.0
+    params = dict(params)
+    light = delta * params.pop("light_scale", 8.0)
+    if "pull_delta" in params:
+        params["pull_delta"] *= delta
+    dist, st = adds_amd.Graph(g).sssp(src, timeout_ms=TIMEOUT, delta=delta, light_delta=light, **params)
+    _check(g, dist, src, is_float)
+    assert st["pulls"] == 1 and st["pull_edges"] > 0, st
+
+
+def test_split_first_bucket_runs_dry_before_the_pull():
+    """Nothing queued below the split point: the head still stops at it (the solve cannot end while heavy
+    edges wait for the pull), the pull reaches the vertex only a heavy edge leads to, and an isolated
+    source ends with everything else unreached."""
+    # path 0 - 1 - 2 (weights 1) and a heavy edge 2 - 3 (weight 100000); vertex 4 isolated
+    rp = np.array([0, 1, 3, 5, 6, 6], dtype=np.uint32)
+    col = np.array([1, 0, 2, 1, 3, 2], dtype=np.uint32)
+    w = np.array([1, 1, 1, 1, 100000, 100000], dtype=np.uint32)
+    g = adds_amd.HostGraph.from_arrays(rp, col, w, "u32", symmetric=True)
+    assert g.check_symmetric()
+    dg = adds_amd.Graph(g)
+    for params in (dict(), dict(pull_delta=4096.0), dict(pull_delta=4096.0, flags=L.F_PULL_ALL_LIGHT),
+                   dict(tail_items=4096), dict(order_degree=2)):
+        params = dict(params)
+        params["flags"] = params.get("flags", 0) | SPLIT
+        for src in range(g.n):
+            dist, st = dg.sssp(src, timeout_ms=TIMEOUT, delta=64.0, light_delta=1024.0, **params)
+            _check(g, dist, src, False)
+            assert st["pulls"] == 1, (src, st)  # an isolated source too: the pull finds nothing
+    # the pull's vertex: reached through the heavy edge only
+    dist, _ = dg.sssp(0, timeout_ms=TIMEOUT, delta=64.0, light_delta=1024.0, flags=SPLIT)
+    assert list(dist.view(np.uint32)[:4]) == [0, 1, 2, 100002]
+
+
+@pytest.mark.parametrize("flags,order", [(SPLIT, 0), (SPLIT | L.F_REPLICA_PARTS | L.F_PULL_ALL_LIGHT, 0),
+                                         (SPLIT | L.F_REPLICA_PARTS | L.F_PULL_ALL_LIGHT, 3)])
+def test_multi_source_split(flags, order):
+    g = adds_amd.HostGraph.rmat(13, 16, seed=9, wkind="f32", lo=1.0, hi=10.0)
+    srcs = g.sample_component(g.max_degree_vertex(), 12, seed=4)
+    dm, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT, delta=0.5, light_delta=4.0, parts=4,
+                                          flags=flags, pull_delta=16.0 if flags & L.F_PULL_ALL_LIGHT else 0.0,
+                                          order_degree=order)
+    assert st["pulls"] == 1, st
+    for i, s in enumerate(srcs):
+        _check(g, dm[i], s, True)
+
+
+def test_captured_split_replays_and_weight_update():
+    """A captured solve with a split first bucket replays bit-exact; once the weights change (and the
+    graph is no longer declared symmetric) the same capture runs every heavy phase by pushing."""
+    g = adds_amd.HostGraph.rmat(13, 16, seed=6, wkind="u32", lo=1, hi=4000)
+    src = np.array([g.max_degree_vertex()], dtype=np.int32)
+    dg = adds_amd.Graph(g)
+    dist = torch.empty((1, g.n), dtype=torch.int32, device="cuda")
+    sd = torch.from_numpy(src).cuda()
+    sg = adds_amd.SolveGraph(dg, sd.data_ptr(), 1, dist.data_ptr(), delta=32.0, light_delta=256.0,
+                             flags=SPLIT | L.F_PULL_ALL_LIGHT, pull_delta=2048.0)
+    ref = None
+    for i in range(3):
+        dist.fill_(5)
+        torch.cuda.synchronize()
+        sg.launch()
+        st = sg.status()
+        assert st["pulls"] == 1, st
+        if ref is None:
+            ref = dist.clone()
+            _check(g, ref[0].cpu().numpy(), int(src[0]), False)
+        assert torch.equal(dist, ref), f"replay {i} differs"
+    g2 = adds_amd.HostGraph.rmat(13, 16, seed=6, wkind="u32", lo=1, hi=4000).reweight(seed=78, wkind="u32", lo=1,
+                                                                                     hi=4000)
+    wt = torch.from_numpy(g2.w_bits.view(np.int32).copy()).cuda()
+    dg.set_weights_device(wt.data_ptr())
+    torch.cuda.synchronize()
+    sg.launch()
+    assert sg.status()["pulls"] == 0
+    _check(g2, dist[0].cpu().numpy(), int(src[0]), False)
+    sg.close()
+
+
+def test_split_flag_without_symmetry_is_ignored():
+    g = adds_amd.HostGraph.rmat(12, 16, seed=4, wkind="u32", lo=1, hi=65535)
+    plain = adds_amd.HostGraph.from_arrays(g.row_ptr, g.col, g.w_bits, "u32")
+    src = g.max_degree_vertex()
+    dist, st = adds_amd.Graph(plain).sssp(src, timeout_ms=TIMEOUT, delta=256.0, light_delta=2048.0, flags=SPLIT)
+    assert st["pulls"] == 0
+    _check(g, dist, src, False)
