@@ -508,7 +508,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
                                    : (4 * total_n + total_m / 16 + (8u << 20)) << std::min(pool_shift, 6u);
   if (!(r.delta > 0) || !std::isfinite(r.delta) || !(r.light_delta >= r.delta) || r.wg_per_cu > 64 || r.chunk > 256 ||
       (r.nb & (r.nb - 1)) || r.nb < 4 || r.nb > 256 || (r.parts != 1 && r.parts != 2 && r.parts != 4 &&
-      r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 4 ||
+      r.parts != 8) || (r.concur + 3) * r.parts > MAX_PROBE_QUEUES || r.concur < 1 || r.concur > 16 ||
       r.task_edges == 0 || r.task_edges > 65536 || r.hub_degree < 64 || (r.windows != 4 && r.windows != 8)) {
     g_detail = "parameter out of range";
     return ADDS_EINVAL;
@@ -1661,9 +1661,16 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
             try_point(q);
           }
         if (p.flags & ADDS_F_PULL_SPLIT) {
-          if (p.concurrency < 4 && (4 + 3) * p.parts <= MAX_PROBE_QUEUES) {
-            adds_params q = p;
-            q.concurrency = 4;
+          // more open buckets: the narrow buckets' hubs are expanded ahead of the head, and the head's
+          // walk through the narrow buckets blocks less (4 parts leave probe lanes for 8 buckets)
+          const adds_params p5 = p;
+          const uint32_t pc_open[][2] = {{8, 4}, {4, 4}, {4, 8}, {4, 12}};
+          for (const auto& pc : pc_open) {
+            adds_params q = p5;
+            q.parts = pc[0];
+            q.concurrency = pc[1];
+            if (q.parts == p5.parts && q.concurrency == p5.concurrency) continue;
+            if ((q.concurrency + 3) * q.parts > MAX_PROBE_QUEUES) continue;
             try_point(q);
           }
           {

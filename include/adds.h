@@ -57,7 +57,8 @@ typedef struct {
   uint32_t parts;        /* sub-queues per bucket (1, 2, 4 or 8; 8 = one per XCD); (concurrency + 3) * parts <= 64 */
   uint32_t hub_degree;   /* edge ranges longer than this are split into chunks any wavefront can take */
   uint32_t task_edges;   /* edges per chunk of a split range (1..65536, default 64) */
-  uint32_t concurrency;  /* buckets open for claiming at once (1..4) */
+  uint32_t concurrency;  /* buckets open for claiming at once (1..16, at most nbuckets - 2, and
+                            (concurrency + 3) * parts <= 64) */
   uint32_t timeout_ms;   /* device watchdog (default 120000) */
   uint32_t tail_items;   /* when the head bucket moves on and at most this many items are left in the bucket
                             window, no more than a quarter of the most seen at a head move, the rest of the

@@ -284,6 +284,8 @@ SPLIT = L.F_PULL_SPLIT
          tail_items=4096, hub_degree=64, task_edges=32),
     dict(flags=SPLIT, order_degree=1 << 30, parts=2, nbuckets=8),  # no vertex qualifies: none waits
     dict(flags=SPLIT | L.F_PULL_ALL_LIGHT | L.F_LOCAL_SPARSE, order_degree=8, pull_delta=16.0, chunk=64),
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT, order_degree=8, pull_delta=16.0, parts=4, concurrency=13),
+    dict(flags=SPLIT, order_degree=8, parts=2, concurrency=8, nbuckets=8),  # concurrency capped at nb - 2
 ])
 def test_rmat_split_first_bucket(is_float, params):
     """ADDS_F_PULL_SPLIT: the buckets below the light threshold run their light phases only, then one
