@@ -113,7 +113,8 @@ class Stats(ctypes.Structure):
 EXPORTS = [
     "adds_strerror", "adds_set_device", "adds_device_cu_count",
     "adds_graph_create", "adds_graph_create_device", "adds_graph_set_weights_device",
-    "adds_graph_destroy", "adds_graph_info", "adds_graph_device_arrays", "adds_graph_set_symmetric",
+    "adds_graph_destroy", "adds_graph_info", "adds_graph_device_arrays", "adds_graph_split_rows",
+    "adds_graph_set_symmetric",
     "adds_graph_check_symmetric",
     "adds_sssp", "adds_sssp_noalloc", "adds_sssp_nearfar", "adds_sssp_multi", "adds_sssp_batched", "adds_last_status",
     "adds_solve_graph_create", "adds_solve_graph_create_batched", "adds_solve_graph_launch",
@@ -163,6 +164,7 @@ def _lib():
         "adds_graph_check_symmetric": (i32, [vp, ctypes.POINTER(i32)]),
         "adds_graph_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u64), ctypes.POINTER(i32)]),
         "adds_graph_device_arrays": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "adds_graph_split_rows": (i32, [vp, ctypes.c_float, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
         "adds_sssp": (i32, [vp, u32, vp, pp, sp, vp]),
         "adds_sssp_noalloc": (i32, [vp, u32, vp, pp, sp, vp]),
         "adds_sssp_nearfar": (i32, [vp, u32, vp, ctypes.c_float, u32, sp, vp]),

@@ -330,8 +330,8 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   HIPCHK(hipSetDevice(g->device));
   if (g->m) HIPCHK(hipMemcpyAsync(g->w, w_dev, g->m * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   if (g->split_delta != 0.f && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
-    HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->wt == ADDS_W_F32, g->scw, g->lcnt,
-                        g->split_aux, g->rinfo, (hipStream_t)stream));
+    HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->split_delta, 1.0f / g->split_delta,
+                        g->wt == ADDS_W_F32, g->scw, g->lcnt, g->split_aux, g->rinfo, (hipStream_t)stream));
   g->max_w_known = false;
   // new weights need not be symmetric: pull phases stop (also in solves captured earlier) until the
   // caller declares the graph symmetric again
@@ -617,14 +617,29 @@ static adds_status ensure_workspace(Workspace& ws, const Resolved& r, uint32_t n
 // The buffers are allocated with the graph; this launches kernels only.
 static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
   if (g->split_delta != delta) {
-    HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, g->wt == ADDS_W_F32, g->scw, g->lcnt, g->split_aux,
-                        g->rinfo, s));
+    // heavy weight bands of width delta above delta (the pull phases read the same two values)
+    HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, delta, 1.0f / delta, g->wt == ADDS_W_F32, g->scw,
+                        g->lcnt, g->split_aux, g->rinfo, s));
     g->split_delta = delta;
   }
   return ADDS_OK;
 }
 
 static uint32_t inf_bits(adds_wtype wt) { return wt == ADDS_W_F32 ? ADDS_INF_F32_BITS : ADDS_INF_U32; }
+
+extern "C" adds_status adds_graph_split_rows(adds_graph* g, float light_delta, const void** cw,
+                                             const uint32_t** light_counts) {
+  if (!g || !(light_delta > 0.f)) return ADDS_EINVAL;
+  HIPCHK(hipSetDevice(g->device));
+  if (g->n) {
+    const adds_status rc = ensure_split(g, light_delta, nullptr);
+    if (rc != ADDS_OK) return rc;
+    HIPCHK(hipStreamSynchronize(nullptr));
+  }
+  if (cw) *cw = g->scw;
+  if (light_counts) *light_counts = g->lcnt;
+  return ADDS_OK;
+}
 
 static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved& r, uint32_t* dist, uint64_t total_n,
                        uint32_t grid) {
@@ -682,6 +697,8 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.resume_all_light = (r.flags & ADDS_F_PULL_ALL_LIGHT) ? 1u : 0u;
   a.pull_at = r.pull_at;
   a.order_degree = r.order_degree;
+  a.band_lo = g->split_delta;  // the split's bands (ensure_split has run for r.light_delta)
+  a.band_inv = 1.0f / g->split_delta;
   a.ptask = ws.ptask;
   a.ptask_cap = ws.ptask_cap;
   a.launch_idx = 0;

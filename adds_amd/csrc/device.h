@@ -166,6 +166,8 @@ struct KArgs {
   // ... and below pull_at only the vertices of at least order_degree light edges wait for their bucket:
   // the others are queued in the bucket of the claim that finds them (0: every vertex waits)
   uint32_t order_degree;
+  // the heavy part of every row is in band order (heavy_band below, with these two values)
+  float band_lo, band_inv;
   uint32_t ptask_cap;
   uint4* ptask;          // long-row tasks of a pull phase: (vertex-instance, first edge, end edge, 0)
 };
@@ -210,14 +212,23 @@ hipError_t launch_cleanup(const KArgs& a, hipStream_t s);
 // The kernels of pull phase `slot` (1 .. pull_slots): count, short rows, long-row tasks.  Each returns at
 // once unless launch slot - 1 of the persistent kernel requested this pull.
 hipError_t launch_pull(const KArgs& a, bool is_float, bool replicated, uint32_t slot, uint32_t grid, hipStream_t s);
-// Stable per-row partition of (col, w) into light (w <= delta) then heavy edges, written interleaved as
-// (col, w) pairs, with per-row light counts.  aux: split_aux_words(m) words of scratch (long rows).
+// Per-row partition of (col, w) into light (w <= delta) then heavy edges, the heavy ones in increasing
+// weight band (heavy_band with band_lo, band_inv), written interleaved as (col, w) pairs, with per-row
+// light counts.  aux: split_aux_words(m) words of scratch (long rows).
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, uint4* rinfo, hipStream_t s);
+                        float band_lo, float band_inv, bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux,
+                        uint4* rinfo, hipStream_t s);
 // Rows longer than SPLIT_BIG edges are split by one 1024-thread workgroup each (k_split_big); the list
 // of such rows needs at most m / SPLIT_BIG entries, after SPLIT_AUX_ROWS words: the list's length and
 // the work counters of the two split kernels.
 constexpr uint32_t SPLIT_BIG = 1024;
+// Weight bands of a row's heavy part: band(x) = floor((x - band_lo) * band_inv) clamped to 0 .. SPLIT_BANDS-1,
+// x the weight as a float (int weights converted).  Monotone in x, so a heavy edge of a higher band than
+// a value's band weighs more than that value: a scan in band order can stop there (pull phases).
+constexpr uint32_t SPLIT_BANDS = 32;
+__host__ __device__ inline uint32_t heavy_band(float x, float band_lo, float band_inv) {
+  return (uint32_t)fminf(fmaxf((x - band_lo) * band_inv, 0.0f), (float)(SPLIT_BANDS - 1));
+}
 constexpr uint32_t SPLIT_AUX_ROWS = 3;
 constexpr uint64_t split_aux_words(uint64_t m) { return m / SPLIT_BIG + SPLIT_AUX_ROWS + 1; }
 

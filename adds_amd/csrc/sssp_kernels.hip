@@ -1669,6 +1669,12 @@ __global__ void __launch_bounds__(PULL_COUNT_THREADS) k_pull_count(KArgs a, uint
 }
 
 constexpr int PULL_UN = 8;  // 64-edge windows per wavefront in flight in the pull kernels
+constexpr uint32_t PULL_HEAD = 4;  // heavy edges per vertex scanned one at a time, in band order
+// a weight or distance word as the float that heavy_band takes (int words converted like the split does)
+template <bool FLT>
+__device__ __forceinline__ float wfloat(uint32_t x) {
+  return FLT ? __uint_as_float(x) : (float)x;
+}
 
 // One workgroup: pull or push, and the queue and ring state for the next launch.
 __global__ void __launch_bounds__(256) k_pull_decide(KArgs a, uint32_t slot) {
@@ -1751,7 +1757,7 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
   for (uint64_t ch = (uint64_t)blockIdx.x * WAVES_PER_WG + wv; ch * (64 * IPL) < a.total_n; ch += nwaves) {
     const uint32_t base = (uint32_t)(ch * (64 * IPL));
     // items: vertex-instance x = base + j * 64 + lane (coalesced); range index k = lane * IPL + j
-    uint32_t x[IPL], dv[IPL], rs[IPL], deg[IPL], gb[IPL];
+    uint32_t x[IPL], dv[IPL], rs[IPL], deg[IPL], gb[IPL], best[IPL], he[IPL];
     bool uns[IPL];
 #pragma unroll
     for (int j = 0; j < IPL; j++) {
@@ -1759,13 +1765,61 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
       dv[j] = x[j] < a.total_n ? a.dist[x[j]] : 0u;
       uns[j] = x[j] < a.total_n && unsettled<FLT>(dv[j], c, idl);
       gb[j] = REPL ? x[j] - x[j] % a.n : 0u;
+      best[j] = dv[j];
     }
 #pragma unroll
     for (int j = 0; j < IPL; j++) {
       const uint4 ri = a.rinfo[uns[j] ? x[j] - gb[j] : 0u];
       rs[j] = ri.x + ri.z;
-      deg[j] = uns[j] ? ri.y - rs[j] : 0u;
+      he[j] = uns[j] ? ri.y : rs[j];
     }
+    // The head of each heavy part, in band order, one edge per vertex per step: an edge of a higher band
+    // than the best value so far weighs more than it (and so does every edge after it), which ends the
+    // vertex's scan.  Most vertices end within PULL_HEAD edges; what is left of the others is relaxed in
+    // full below.
+    {
+      bool act[IPL];
+#pragma unroll
+      for (int j = 0; j < IPL; j++) act[j] = rs[j] < he[j];
+      uint32_t nscan = 0;
+      for (uint32_t step = 0; step < PULL_HEAD; step++) {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < IPL; j++) any |= act[j];
+        if (!ballot(any)) break;
+        uint32_t u[IPL], ww[IPL], du[IPL];
+#pragma unroll
+        for (int j = 0; j < IPL; j++) {
+          const uint64_t cw = act[j] ? __builtin_nontemporal_load((const uint64_t*)a.cw + rs[j]) : 0ull;
+          u[j] = (uint32_t)cw;
+          ww[j] = (uint32_t)(cw >> 32);
+        }
+#pragma unroll
+        for (int j = 0; j < IPL; j++)
+          if (act[j] && heavy_band(wfloat<FLT>(ww[j]), a.band_lo, a.band_inv) >
+                            heavy_band(wfloat<FLT>(best[j]), a.band_lo, a.band_inv)) {
+            act[j] = false;
+            he[j] = rs[j];  // nothing left to relax
+          }
+#pragma unroll
+        for (int j = 0; j < IPL; j++) du[j] = act[j] ? a.dist[gb[j] + u[j]] : INF;
+#pragma unroll
+        for (int j = 0; j < IPL; j++)
+          if (act[j]) {
+            best[j] = min(best[j], pull_cand<FLT>(du[j], ww[j]));
+            rs[j]++;
+            nscan++;
+            act[j] = rs[j] < he[j];
+          }
+      }
+      nscan = (uint32_t)wave_sum64(nscan);
+      if (lane == 0) {
+        ws.v[ST_EDGES] += nscan;
+        ws.v[ST_PULL_EDGES] += nscan;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < IPL; j++) deg[j] = he[j] - rs[j];
     // rows longer than PULL_LONG become tasks for k_pull_b (all of this claim's, or, when the task list
     // is full, none: then they are relaxed here and the reserved positions hold empty tasks)
     {
@@ -1807,7 +1861,7 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
       const uint32_t k = lane * IPL + j;
       s_pref[wv][k] = run;
       s_rs[wv][k] = rs[j];
-      s_best[wv][k] = dv[j];
+      s_best[wv][k] = best[j];
       if (REPL) s_gb[wv][k] = gb[j];
       run += deg[j];
     }
@@ -1846,13 +1900,13 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
     uint32_t nimp = 0;
 #pragma unroll
     for (int j = 0; j < IPL; j++) {
-      const uint32_t best = s_best[wv][lane * IPL + j];
-      const bool imp = uns[j] && best < dv[j];
-      if (imp) ast(a.dist + x[j], best);
+      const uint32_t bj = s_best[wv][lane * IPL + j];
+      const bool imp = uns[j] && bj < dv[j];
+      if (imp) ast(a.dist + x[j], bj);
       nimp += (uint32_t)__popcll(ballot(imp));
       const uint32_t pp = REPL ? push_part(a, gb[j], part) : part;
-      const bool q = imp || (rebucket && uns[j] && best < INF);
-      pb_add(a, pb, q, ((uint64_t)best << 32) | x[j], light_queue<FLT>(a, pidl, best, pp, pc, QM_OWN), ws);
+      const bool q = imp || (rebucket && uns[j] && bj < INF);
+      pb_add(a, pb, q, ((uint64_t)bj << 32) | x[j], light_queue<FLT>(a, pidl, bj, pp, pc, QM_OWN), ws);
     }
     wave_sync();
     if (lane == 0) {
@@ -1977,12 +2031,13 @@ __global__ void k_cleanup(KArgs a) {
   }
 }
 
-// Stable per-row partition of (col, w) into light (w <= delta) then heavy edges.  Rows of up to
-// SPLIT_BIG edges: k_split, 64 rows per wavefront.  Longer rows are appended to a list (aux[0] =
-// count, aux[SPLIT_AUX_ROWS..] = rows) and partitioned by k_split_big, one 1024-thread workgroup per
-// row, so that
-// the long rows of a power-law graph (on RMAT-24, rows above 1024 edges hold over half the edges) do
-// not serialise on one wavefront.
+// Per-row partition of (col, w) into classes: the light edges (w <= delta) first, then the heavy edges
+// band by band in increasing weight (heavy_band), so that a pull phase reading a row's heavy part can
+// stop at the first band above its best candidate.  The order inside a class is not kept (no reader
+// depends on it).  Rows of up to SPLIT_BIG edges: k_split, 64 rows per wavefront.  Longer rows are
+// appended to a list (aux[0] = count, aux[SPLIT_AUX_ROWS..] = rows) and partitioned by k_split_big, one
+// 1024-thread workgroup per row, so that the long rows of a power-law graph (on RMAT-24, rows above 1024
+// edges hold over half the edges) do not serialise on one wavefront.
 
 __device__ __forceinline__ bool is_light(uint32_t x, float delta, uint32_t idelta, int is_float) {
   return is_float ? __uint_as_float(x) <= delta : x <= idelta;
@@ -1990,36 +2045,32 @@ __device__ __forceinline__ bool is_light(uint32_t x, float delta, uint32_t idelt
 __device__ __forceinline__ uint32_t int_delta(float delta) {
   return delta >= 4294967295.0f ? 0xFFFFFFFFu : (uint32_t)delta;  // int weights: w <= floor(delta)
 }
+constexpr uint32_t SPLIT_NCLS = 1 + SPLIT_BANDS;  // light, then the heavy bands
+__device__ __forceinline__ uint32_t edge_class(uint32_t x, float delta, uint32_t idelta, int is_float, float blo,
+                                               float binv) {
+  if (is_light(x, delta, idelta, is_float)) return 0u;
+  return 1u + heavy_band(is_float ? __uint_as_float(x) : (float)x, blo, binv);
+}
 
 // Rows of up to SPLIT_BIG edges, 64 consecutive rows per wavefront at a time: their edges are
 // flattened (row k owns [off[k], off[k+1]) of the order) and read in 64-edge windows, four in flight.
-// Pass 1 keeps each window's light mask in LDS; a scan of the masks' popcounts gives the light edges
-// before any flattened position, so an edge's rank among its row's light (or heavy) edges is a
-// difference of two such counts.  Pass 2 writes every edge to its place.  The result is the same stable
-// partition as one wavefront per row, with a few round trips per 64 rows instead of several per row.
-// When the 64 rows hold more than SPLIT_CAP edges they are taken as consecutive sub-groups of rows that
-// each fit (a row of at most SPLIT_BIG <= SPLIT_CAP edges always does).
+// Pass 1 counts each row's edges per class in LDS; one lane per row turns its counts into the classes'
+// start positions; pass 2 sends every edge to its class's next position (LDS atomics).  When the 64 rows
+// hold more than SPLIT_CAP edges they are taken as consecutive sub-groups of rows that each fit (a row of
+// at most SPLIT_BIG <= SPLIT_CAP edges always does).
 constexpr uint32_t SPLIT_CAP = 16384;
-constexpr uint32_t SPLIT_WIN = SPLIT_CAP / 64;
 static_assert(SPLIT_BIG <= SPLIT_CAP, "a row must fit one sub-group");
 __global__ void __launch_bounds__(256) k_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w,
-                                               uint32_t n, float delta, int is_float, uint2* cw2, uint32_t* lcnt,
-                                               uint32_t* aux) {
+                                               uint32_t n, float delta, float blo, float binv, int is_float,
+                                               uint2* cw2, uint32_t* lcnt, uint32_t* aux) {
   __shared__ uint32_t s_off[WAVES_PER_WG][65];
-  __shared__ uint32_t s_rs[WAVES_PER_WG][64], s_ls[WAVES_PER_WG][64], s_lc[WAVES_PER_WG][64];
-  __shared__ uint64_t s_mask[WAVES_PER_WG][SPLIT_WIN + 1];
-  __shared__ uint32_t s_cb[WAVES_PER_WG][SPLIT_WIN + 1];
+  __shared__ uint32_t s_rs[WAVES_PER_WG][64];
+  __shared__ uint32_t s_cnt[WAVES_PER_WG][64 * SPLIT_NCLS];  // per row and class: count, then next position
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
   const uint32_t idelta = int_delta(delta);
   uint32_t* off = s_off[wv];
-  uint64_t* mask = s_mask[wv];
-  uint32_t* cb = s_cb[wv];
-  // light edges before flattened position i (i <= the sub-group's edges)
-  auto before = [&](uint32_t i) -> uint32_t {
-    const uint32_t j = i >> 6, b = i & 63;
-    return cb[j] + (uint32_t)__popcll(b ? (mask[j] & ((1ull << b) - 1)) : 0ull);
-  };
+  uint32_t* cnt = s_cnt[wv];
   auto locate = [&](uint32_t i) -> uint32_t {  // last row k with off[k] <= i
     uint32_t lo = 0;
 #pragma unroll
@@ -2056,73 +2107,56 @@ __global__ void __launch_bounds__(256) k_split(const uint32_t* row_ptr, const ui
       const bool mine = lane >= first && lane <= last;
       off[lane] = lane < first ? 0u : (mine ? excl - e0 : total);
       if (lane == 0) off[64] = total;
-      const uint32_t nwin = (total + 63) >> 6;
-      for (uint32_t j = nwin + lane; j <= SPLIT_WIN; j += 64) mask[j] = 0ull;
+      for (uint32_t i = lane; i < 64 * SPLIT_NCLS; i += 64) cnt[i] = 0u;
       wave_sync();
-      // pass 1: light masks
-      for (uint32_t w0 = 0; w0 < nwin; w0 += 4) {
-        uint32_t x[4];
+      // pass 1: class counts
+      for (uint32_t w0 = 0; w0 < total; w0 += 256) {
+        uint32_t x[4], k[4];
         bool in[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          const uint32_t i = (w0 + q) * 64 + lane;
+          const uint32_t i = w0 + q * 64 + lane;
           in[q] = i < total;
-          const uint32_t k = locate(in[q] ? i : 0u);
-          x[q] = in[q] ? w[s_rs[wv][k] + (i - off[k])] : 0u;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const uint64_t m = ballot(in[q] && is_light(x[q], delta, idelta, is_float));
-          if (lane == 0 && w0 + q < nwin) mask[w0 + q] = m;
-        }
-      }
-      wave_sync();
-      // light edges before each window: lane t sums windows 4t .. 4t+3
-      {
-        uint32_t c[4], sum = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          c[q] = (uint32_t)__popcll(mask[4 * lane + q]);
-          sum += c[q];
-        }
-        const uint32_t inc = wave_incl_scan(sum);
-        uint32_t run = inc - sum;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          cb[4 * lane + q] = run;
-          run += c[q];
-        }
-        if (lane == 63) cb[SPLIT_WIN] = inc;
-      }
-      wave_sync();
-      {
-        const uint32_t dk = mine ? d : 0u;
-        const uint32_t a = before(off[lane]), b = before(off[lane] + dk);
-        s_ls[wv][lane] = a;
-        s_lc[wv][lane] = b - a;
-        if (mine && valid && !big) lcnt[v] = b - a;
-      }
-      wave_sync();
-      // pass 2: every edge to its place
-      for (uint32_t w0 = 0; w0 < nwin; w0 += 4) {
-        uint32_t c[4], x[4], dst[4];
-        bool in[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const uint32_t i = (w0 + q) * 64 + lane;
-          in[q] = i < total;
-          const uint32_t ii = in[q] ? i : 0u;
-          const uint32_t k = locate(ii);
-          const uint32_t e = s_rs[wv][k] + (ii - off[k]);
-          c[q] = in[q] ? col[e] : 0u;
-          x[q] = in[q] ? w[e] : 0u;
-          const uint32_t lb = before(ii) - s_ls[wv][k];
-          const bool l = (mask[ii >> 6] >> (ii & 63)) & 1ull;
-          dst[q] = s_rs[wv][k] + (l ? lb : s_lc[wv][k] + (ii - off[k]) - lb);
+          k[q] = locate(in[q] ? i : 0u);
+          x[q] = in[q] ? w[s_rs[wv][k[q]] + (i - off[k[q]])] : 0u;
         }
 #pragma unroll
         for (int q = 0; q < 4; q++)
-          if (in[q]) cw2[dst[q]] = make_uint2(c[q], x[q]);
+          if (in[q]) atomicAdd(&cnt[k[q] * SPLIT_NCLS + edge_class(x[q], delta, idelta, is_float, blo, binv)], 1u);
+      }
+      wave_sync();
+      // each class's first position in its row: the light edges at the row start, then band after band
+      if (mine) {
+        uint32_t pos = rs, lc = 0;
+        for (uint32_t c = 0; c < SPLIT_NCLS; c++) {
+          const uint32_t t = cnt[lane * SPLIT_NCLS + c];
+          cnt[lane * SPLIT_NCLS + c] = pos;
+          pos += t;
+          if (c == 0) lc = t;
+        }
+        if (valid && !big) lcnt[v] = lc;
+      }
+      wave_sync();
+      // pass 2: every edge to its place
+      for (uint32_t w0 = 0; w0 < total; w0 += 256) {
+        uint32_t c[4], x[4], k[4];
+        bool in[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint32_t i = w0 + q * 64 + lane;
+          in[q] = i < total;
+          k[q] = locate(in[q] ? i : 0u);
+          const uint32_t e = s_rs[wv][k[q]] + ((in[q] ? i : 0u) - off[k[q]]);
+          c[q] = in[q] ? col[e] : 0u;
+          x[q] = in[q] ? w[e] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (in[q]) {
+            const uint32_t dst =
+                atomicAdd(&cnt[k[q] * SPLIT_NCLS + edge_class(x[q], delta, idelta, is_float, blo, binv)], 1u);
+            cw2[dst] = make_uint2(c[q], x[q]);
+          }
       }
       wave_sync();
       first = last + 1;
@@ -2130,88 +2164,61 @@ __global__ void __launch_bounds__(256) k_split(const uint32_t* row_ptr, const ui
   }
 }
 
-// One workgroup of 1024 threads per long row.  Windows of 4096 edges: edge base + j*1024 + t goes to
-// thread t, sub-window j; ranks come from per-(sub-window, wavefront) ballot counts scanned in edge
-// order by one wavefront.
+// One workgroup of 1024 threads per long row: class counts (LDS atomics), their start positions, then
+// every edge to its class's next position.
 __global__ void __launch_bounds__(1024) k_split_big(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w,
-                                                    float delta, int is_float, uint2* cw2,
+                                                    float delta, float blo, float binv, int is_float, uint2* cw2,
                                                     uint32_t* lcnt, uint32_t* aux) {
-  __shared__ uint32_t s_l[64], s_h[64], s_pl[64], s_ph[64];
-  __shared__ uint32_t s_tl, s_th;
-  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const uint32_t idelta = int_delta(delta);
+  __shared__ uint32_t s_cnt[SPLIT_NCLS];
   __shared__ uint32_t s_row;
+  const uint32_t t = threadIdx.x;
+  const uint32_t idelta = int_delta(delta);
   const uint32_t nbig = aux[0];
   for (;;) {
     // rows are handed out by a counter (aux[2]): one very long row does not hold up the rows behind it
     if (t == 0) s_row = afadd(aux + 2, 1u);
+    if (t < SPLIT_NCLS) s_cnt[t] = 0u;
     __syncthreads();
     const uint32_t i = s_row;
     __syncthreads();
     if (i >= nbig) break;
     const uint32_t v = aux[SPLIT_AUX_ROWS + i];
     const uint32_t rs = row_ptr[v], re = row_ptr[v + 1];
-    uint32_t c = 0;
-    for (uint32_t e = rs + t; e < re; e += 1024) c += is_light(w[e], delta, idelta, is_float) ? 1u : 0u;
+    for (uint32_t e = rs + t; e < re; e += 4096) {
+      uint32_t x[4];
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-    if (lane == 0) s_l[wv] = c;
+      for (int q = 0; q < 4; q++) x[q] = e + q * 1024 < re ? w[e + q * 1024] : 0u;
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (e + q * 1024 < re) atomicAdd(&s_cnt[edge_class(x[q], delta, idelta, is_float, blo, binv)], 1u);
+    }
     __syncthreads();
     if (t == 0) {
-      uint32_t tot = 0;
-      for (uint32_t k = 0; k < 16; k++) tot += s_l[k];
-      s_tl = tot;
-      lcnt[v] = tot;
+      uint32_t pos = rs;
+      for (uint32_t c = 0; c < SPLIT_NCLS; c++) {
+        const uint32_t k = s_cnt[c];
+        s_cnt[c] = pos;
+        pos += k;
+        if (c == 0) lcnt[v] = k;
+      }
     }
     __syncthreads();
-    uint32_t lo = rs, hi = rs + s_tl;
-    __syncthreads();
-    for (uint32_t base = rs; base < re; base += 4096) {
-      uint32_t cc[4], xx[4];
-      bool in[4], l[4];
-      uint64_t ml[4], mh[4];
+    for (uint32_t e = rs + t; e < re; e += 4096) {
+      uint32_t x[4], c[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const uint32_t e = base + j * 1024 + t;
-        in[j] = e < re;
-        cc[j] = 0;
-        xx[j] = 0;
-        l[j] = false;
-        if (in[j]) {
-          cc[j] = col[e];
-          xx[j] = w[e];
-          l[j] = is_light(xx[j], delta, idelta, is_float);
-        }
-        ml[j] = ballot(in[j] && l[j]);
-        mh[j] = ballot(in[j] && !l[j]);
-        if (lane == 0) {
-          s_l[j * 16 + wv] = (uint32_t)__popcll(ml[j]);
-          s_h[j * 16 + wv] = (uint32_t)__popcll(mh[j]);
-        }
+      for (int q = 0; q < 4; q++) {
+        const bool in = e + q * 1024 < re;
+        x[q] = in ? w[e + q * 1024] : 0u;
+        c[q] = in ? col[e + q * 1024] : 0u;
       }
-      __syncthreads();
-      if (wv == 0) {
-        const uint32_t xl = s_l[lane], xh = s_h[lane];
-        const uint32_t il = wave_incl_scan(xl), ih = wave_incl_scan(xh);
-        s_pl[lane] = il - xl;
-        s_ph[lane] = ih - xh;
-        if (lane == 63) {
-          s_tl = il;
-          s_th = ih;
-        }
-      }
-      __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if (in[j]) {
-          const uint32_t dst = l[j] ? lo + s_pl[j * 16 + wv] + mbcnt(ml[j]) : hi + s_ph[j * 16 + wv] + mbcnt(mh[j]);
-          cw2[dst] = make_uint2(cc[j], xx[j]);
+      for (int q = 0; q < 4; q++)
+        if (e + q * 1024 < re) {
+          const uint32_t dst = atomicAdd(&s_cnt[edge_class(x[q], delta, idelta, is_float, blo, binv)], 1u);
+          cw2[dst] = make_uint2(c[q], x[q]);
         }
-      }
-      lo += s_tl;
-      hi += s_th;
-      __syncthreads();  // the counts are rewritten by the next window
     }
+    __syncthreads();  // the positions are reset for the next row
   }
 }
 
@@ -2279,15 +2286,16 @@ __global__ void k_rinfo(const uint32_t* row_ptr, const uint32_t* lcnt, uint32_t 
 }
 
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
-                        bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux, uint4* rinfo, hipStream_t s) {
+                        float band_lo, float band_inv, bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux,
+                        uint4* rinfo, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipError_t e = launch_fill_u32(aux, SPLIT_AUX_ROWS, 0, s);  // long-row count and the two work counters
   if (e != hipSuccess) return e;
   const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 4096);  // groups of 64 rows, taken dynamically
-  hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, row_ptr, col, w, n, delta, is_float ? 1 : 0, cw2,
-                     lcnt, aux);
-  hipLaunchKernelGGL(k_split_big, dim3(1024), dim3(1024), 0, s, row_ptr, col, w, delta, is_float ? 1 : 0, cw2,
-                     lcnt, aux);
+  hipLaunchKernelGGL(k_split, dim3(blocks), dim3(256), 0, s, row_ptr, col, w, n, delta, band_lo, band_inv,
+                     is_float ? 1 : 0, cw2, lcnt, aux);
+  hipLaunchKernelGGL(k_split_big, dim3(1024), dim3(1024), 0, s, row_ptr, col, w, delta, band_lo, band_inv,
+                     is_float ? 1 : 0, cw2, lcnt, aux);
   if (rinfo)
     hipLaunchKernelGGL(k_rinfo, dim3(std::min<uint32_t>((n + 255) / 256, 8192)), dim3(256), 0, s, row_ptr, lcnt, n,
                        rinfo);

@@ -187,6 +187,14 @@ adds_status adds_graph_info(const adds_graph* g, uint32_t* n, uint64_t* m, adds_
 adds_status adds_graph_device_arrays(const adds_graph* g, const uint32_t** row_ptr, const uint32_t** col,
                                      const void** w);
 
+/* The solver's copy of the rows for light threshold light_delta (built if needed, then synchronised):
+ * m (col, w) pairs, each row light edges first (w <= light_delta; int weights: w <= floor(light_delta)),
+ * then its heavy edges in non-decreasing weight band, band(w) = min(31, floor((w - light_delta) /
+ * light_delta)) in float arithmetic (int weights converted); inside a class the order is unspecified.
+ * light_counts: n light-edge counts.  Device pointers owned by the graph, valid until the next split
+ * with another threshold or adds_graph_destroy.  No reference counterpart: inspection (tests). */
+adds_status adds_graph_split_rows(adds_graph* g, float light_delta, const void** cw, const uint32_t** light_counts);
+
 /* ---- solves ----
  * Replaces gg_main's timed solve (ads_int/kernel.cu:223-262): dist[v] = shortest distance from src.
  * dist is a device buffer of n words, or host memory with ADDS_F_HOST_OUT. */

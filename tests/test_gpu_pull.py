@@ -379,3 +379,47 @@ def test_split_flag_without_symmetry_is_ignored():
     dist, st = adds_amd.Graph(plain).sssp(src, timeout_ms=TIMEOUT, delta=256.0, light_delta=2048.0, flags=SPLIT)
     assert st["pulls"] == 0
     _check(g, dist, src, False)
+
+
+def _hip_to_host(ptr, nbytes):
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    out = np.empty(nbytes, dtype=np.uint8)
+    assert hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes), 2) == 0
+    return out
+
+
+@pytest.mark.parametrize("is_float,light", [(False, 2048.0), (False, 1.0), (True, 0.0625), (True, 1e-4)])
+def test_split_rows_light_first_then_bands(is_float, light):
+    """The solver's copy of the rows: every row holds the same (col, w) pairs as the input, light ones
+    first, then the heavy ones in non-decreasing weight band (what lets a pull phase stop early); rows of
+    more than 1024 edges (the workgroup-per-row split) included."""
+    import ctypes
+    wk = dict(wkind="f32", lo=0.0, hi=1.0) if is_float else dict(wkind="u32", lo=1, hi=65535)
+    g = adds_amd.HostGraph.rmat(14, 32, seed=8, **wk)
+    deg = g.degrees()
+    assert deg.max() > 1024
+    dg = adds_amd.Graph(g)
+    cw, lc = ctypes.c_void_p(), ctypes.c_void_p()
+    L.check(L._lib().adds_graph_split_rows(dg.handle, ctypes.c_float(light), ctypes.byref(cw), ctypes.byref(lc)),
+            "split rows")
+    pairs = _hip_to_host(cw.value, g.m * 8).view(np.uint32).reshape(-1, 2)
+    lcnt = _hip_to_host(lc.value, g.n * 4).view(np.uint32)
+    col, wb = pairs[:, 0], pairs[:, 1]
+    lf = np.float32(light)
+    wf = wb.view(np.float32) if is_float else wb.astype(np.float32)
+    is_l = (wb.view(np.float32) <= lf) if is_float else (wb <= np.uint32(int(light)))
+    band = np.minimum(np.maximum((wf - lf) * (np.float32(1.0) / lf), np.float32(0)), np.float32(31)).astype(np.int64)
+    rows = np.repeat(np.arange(g.n), deg)
+    pos = np.arange(g.m) - g.row_ptr[:-1].astype(np.int64)[rows]
+    # light exactly in the first lcnt positions of each row
+    assert np.array_equal(is_l, pos < lcnt[rows].astype(np.int64))
+    assert np.array_equal(lcnt, np.bincount(rows, weights=is_l, minlength=g.n).astype(np.uint32))
+    # heavy bands non-decreasing inside each row
+    same_row = rows[1:] == rows[:-1]
+    both_heavy = ~is_l[1:] & ~is_l[:-1]
+    assert np.all(band[1:][same_row & both_heavy] >= band[:-1][same_row & both_heavy])
+    # the same multiset of pairs per row
+    key_got = np.lexsort((wb, col, rows))
+    key_in = np.lexsort((g.w_bits, g.col, rows))
+    assert np.array_equal(col[key_got], g.col[key_in]) and np.array_equal(wb[key_got], g.w_bits[key_in])
