@@ -45,8 +45,7 @@ constexpr uint32_t S_HREC = 7 * LINE;       // next free edge-range record
 constexpr uint32_t VER_GROUPS = 8;
 constexpr uint32_t S_VER = 8 * LINE;
 // Pull-phase work counters (k_pull_*), on S_PULL2 (S_PULL is a spare line): the long-row task count
-// [0], the task claim counter [1], u64 heavy in-edges of unsettled vertices [2..3] and u64 unsettled
-// vertices [4..5].
+// [0], [1] spare, u64 heavy in-edges of unsettled vertices [2..3] and u64 unsettled vertices [4..5].
 constexpr uint32_t S_PULL = (8 + VER_GROUPS) * LINE;
 constexpr uint32_t S_PULL2 = S_PULL + LINE;
 constexpr uint32_t PULL_GEN = S_TAIL + 2, PULL_BUCKET = S_TAIL + 3, PULL_H = S_TAIL + 4, PULL_DONE = S_TAIL + 6,

@@ -1759,28 +1759,35 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
     // items: vertex-instance x = base + j * 64 + lane (coalesced); range index k = lane * IPL + j
     uint32_t x[IPL], dv[IPL], rs[IPL], deg[IPL], gb[IPL], best[IPL], he[IPL];
     bool uns[IPL];
+    // distances and row records together (a record read for a settled vertex is not used)
+    uint4 ri[IPL];
 #pragma unroll
     for (int j = 0; j < IPL; j++) {
       x[j] = base + j * 64 + lane;
-      dv[j] = x[j] < a.total_n ? a.dist[x[j]] : 0u;
-      uns[j] = x[j] < a.total_n && unsettled<FLT>(dv[j], c, idl);
+      const bool in = x[j] < a.total_n;
       gb[j] = REPL ? x[j] - x[j] % a.n : 0u;
-      best[j] = dv[j];
+      dv[j] = in ? a.dist[x[j]] : 0u;
+      ri[j] = a.rinfo[in ? x[j] - gb[j] : 0u];
     }
 #pragma unroll
     for (int j = 0; j < IPL; j++) {
-      const uint4 ri = a.rinfo[uns[j] ? x[j] - gb[j] : 0u];
-      rs[j] = ri.x + ri.z;
-      he[j] = uns[j] ? ri.y : rs[j];
+      uns[j] = x[j] < a.total_n && unsettled<FLT>(dv[j], c, idl);
+      best[j] = dv[j];
+      rs[j] = ri[j].x + ri[j].z;
+      he[j] = uns[j] ? ri[j].y : rs[j];
     }
     // The head of each heavy part, in band order, one edge per vertex per step: an edge of a higher band
     // than the best value so far weighs more than it (and so does every edge after it), which ends the
     // vertex's scan.  Most vertices end within PULL_HEAD edges; what is left of the others is relaxed in
-    // full below.
+    // full below.  The next edge's (col, w) is loaded alongside each step's distance gather.
     {
       bool act[IPL];
+      uint64_t cwn[IPL];
 #pragma unroll
-      for (int j = 0; j < IPL; j++) act[j] = rs[j] < he[j];
+      for (int j = 0; j < IPL; j++) {
+        act[j] = rs[j] < he[j];
+        cwn[j] = act[j] ? __builtin_nontemporal_load((const uint64_t*)a.cw + rs[j]) : 0ull;
+      }
       uint32_t nscan = 0;
       for (uint32_t step = 0; step < PULL_HEAD; step++) {
         bool any = false;
@@ -1788,29 +1795,33 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
         for (int j = 0; j < IPL; j++) any |= act[j];
         if (!ballot(any)) break;
         uint32_t u[IPL], ww[IPL], du[IPL];
+        uint64_t cw2[IPL];
 #pragma unroll
         for (int j = 0; j < IPL; j++) {
-          const uint64_t cw = act[j] ? __builtin_nontemporal_load((const uint64_t*)a.cw + rs[j]) : 0ull;
-          u[j] = (uint32_t)cw;
-          ww[j] = (uint32_t)(cw >> 32);
-        }
-#pragma unroll
-        for (int j = 0; j < IPL; j++)
+          u[j] = (uint32_t)cwn[j];
+          ww[j] = (uint32_t)(cwn[j] >> 32);
           if (act[j] && heavy_band(wfloat<FLT>(ww[j]), a.band_lo, a.band_inv) >
                             heavy_band(wfloat<FLT>(best[j]), a.band_lo, a.band_inv)) {
             act[j] = false;
             he[j] = rs[j];  // nothing left to relax
           }
+        }
 #pragma unroll
-        for (int j = 0; j < IPL; j++) du[j] = act[j] ? a.dist[gb[j] + u[j]] : INF;
+        for (int j = 0; j < IPL; j++) {
+          du[j] = act[j] ? a.dist[gb[j] + u[j]] : INF;
+          const bool more = act[j] && rs[j] + 1 < he[j] && step + 1 < PULL_HEAD;
+          cw2[j] = more ? __builtin_nontemporal_load((const uint64_t*)a.cw + rs[j] + 1) : 0ull;
+        }
 #pragma unroll
-        for (int j = 0; j < IPL; j++)
+        for (int j = 0; j < IPL; j++) {
           if (act[j]) {
             best[j] = min(best[j], pull_cand<FLT>(du[j], ww[j]));
             rs[j]++;
             nscan++;
             act[j] = rs[j] < he[j];
           }
+          cwn[j] = cw2[j];
+        }
       }
       nscan = (uint32_t)wave_sum64(nscan);
       if (lane == 0) {
@@ -1940,18 +1951,22 @@ __global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
 #pragma unroll
   for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
   constexpr uint32_t INF = inf_word<FLT>();
-  for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = afadd(a.st + S_PULL2 + 1, 1u);
-    t = rdlane(t, 0);
-    if (t >= ntask) break;
+  // tasks dealt round-robin over the wavefronts (a shared claim counter would take one same-address
+  // atomic per wavefront even when there are few tasks)
+  for (uint32_t t = blockIdx.x * WAVES_PER_WG + wv; t < ntask; t += gridDim.x * WAVES_PER_WG) {
     const uint4 tk = a.ptask[t];
     if (tk.y >= tk.z) continue;  // an empty task (the list was full)
     const uint32_t gb = REPL ? tk.x - tk.x % a.n : 0u;
-    uint32_t best = INF;
+    // The range is in band order: an edge of a higher band than this lane's bound (the vertex's distance
+    // when the task starts, or a smaller candidate found since) cannot lower it, nor can any edge after it,
+    // so the lane skips its gather, and the task ends once no lane of a window group has an edge left.
+    const uint32_t d0 = a.dist[tk.x];
+    uint32_t best = INF, scanned = 0;
     for (uint32_t e0 = tk.y; e0 < tk.z; e0 += 64 * PULL_UN) {
       uint32_t u[PULL_UN], w[PULL_UN], du[PULL_UN];
       bool act[PULL_UN];
+      const uint32_t lim = heavy_band(wfloat<FLT>(min(best, d0)), a.band_lo, a.band_inv);
+      bool any = false;
 #pragma unroll
       for (int j = 0; j < PULL_UN; j++) {
         const uint32_t e = e0 + 64 * j + lane;
@@ -1959,22 +1974,31 @@ __global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
         const uint64_t cw = __builtin_nontemporal_load((const uint64_t*)a.cw + (act[j] ? e : tk.y));
         u[j] = (uint32_t)cw;
         w[j] = (uint32_t)(cw >> 32);
+        act[j] = act[j] && heavy_band(wfloat<FLT>(w[j]), a.band_lo, a.band_inv) <= lim;
+        any |= act[j];
       }
+      if (!ballot(any)) break;
 #pragma unroll
-      for (int j = 0; j < PULL_UN; j++) du[j] = a.dist[gb + u[j]];
+      for (int j = 0; j < PULL_UN; j++) du[j] = act[j] ? a.dist[gb + u[j]] : INF;
 #pragma unroll
       for (int j = 0; j < PULL_UN; j++)
-        if (act[j]) best = min(best, pull_cand<FLT>(du[j], w[j]));
+        if (act[j]) {
+          best = min(best, pull_cand<FLT>(du[j], w[j]));
+          scanned++;
+        }
     }
     best = wave_min_u32(best);
+    scanned = (uint32_t)wave_sum64(scanned);
     bool imp = false;
     if (lane == 0 && best < INF) {
       const uint32_t old = afmin(a.dist + tk.x, best);
       imp = best < old;
       ws.v[ST_RELAX]++;
-      ws.v[ST_EDGES] += tk.z - tk.y;
-      ws.v[ST_PULL_EDGES] += tk.z - tk.y;
       if (imp) ws.v[ST_PULL_IMPROVED]++;
+    }
+    if (lane == 0) {
+      ws.v[ST_EDGES] += scanned;
+      ws.v[ST_PULL_EDGES] += scanned;
     }
     const uint32_t pp = REPL ? push_part(a, gb, part) : part;
     pb_add(a, pb, imp, ((uint64_t)best << 32) | tk.x, light_queue<FLT>(a, pidl, best, pp, pc, QM_OWN), ws);
