@@ -88,6 +88,7 @@ struct adds_graph {
   uint2* scw = nullptr;
   uint32_t* lcnt = nullptr;
   uint4* rinfo = nullptr;  // per-vertex row record (row start, row end, light edges)
+  uint4* phead = nullptr;  // per-vertex first two heavy edges of the split rows (pull phases)
   uint32_t* split_aux = nullptr;
   float split_delta = 0.f;
   Workspace ws;
@@ -197,7 +198,9 @@ static void release_graph_memory(adds_graph* g) {
   if (g->scw) (void)hipFree(g->scw);
   if (g->lcnt) (void)hipFree(g->lcnt);
   if (g->rinfo) (void)hipFree(g->rinfo);
+  if (g->phead) (void)hipFree(g->phead);
   g->rinfo = nullptr;
+  g->phead = nullptr;
   if (g->split_aux) (void)hipFree(g->split_aux);
   g->scw = nullptr;
   g->lcnt = g->split_aux = nullptr;
@@ -250,6 +253,7 @@ static adds_status alloc_graph(uint32_t n, uint64_t m, adds_wtype wt, adds_graph
   if (e == hipSuccess) e = hipMalloc(&g->scw, (m + 1) * 8);  // one pad entry: the last 16-byte pair is in bounds
   if (e == hipSuccess) e = hipMalloc(&g->lcnt, std::max<uint64_t>(n, 1) * 4);
   if (e == hipSuccess) e = hipMalloc(&g->rinfo, std::max<uint64_t>(n, 1) * 16);
+  if (e == hipSuccess) e = hipMalloc(&g->phead, std::max<uint64_t>(n, 1) * 16);
   if (e == hipSuccess) e = hipMalloc(&g->split_aux, split_aux_words(m) * 4);
   if (e == hipSuccess) e = hipMalloc(&g->sym_dev, 256);
   if (e == hipSuccess) e = launch_fill_u32(g->sym_dev, 1, 0u, nullptr);
@@ -331,7 +335,8 @@ extern "C" adds_status adds_graph_set_weights_device(adds_graph* g, const void* 
   if (g->m) HIPCHK(hipMemcpyAsync(g->w, w_dev, g->m * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   if (g->split_delta != 0.f && g->n)  // re-split in place (same buffers, so captured solves see the new weights)
     HIPCHK(launch_split(g->row, g->col, g->w, g->n, g->split_delta, g->split_delta, 1.0f / g->split_delta,
-                        g->wt == ADDS_W_F32, g->scw, g->lcnt, g->split_aux, g->rinfo, (hipStream_t)stream));
+                        g->wt == ADDS_W_F32, g->scw, g->lcnt, g->split_aux, g->rinfo, g->phead,
+                        (hipStream_t)stream));
   g->max_w_known = false;
   // new weights need not be symmetric: pull phases stop (also in solves captured earlier) until the
   // caller declares the graph symmetric again
@@ -619,7 +624,7 @@ static adds_status ensure_split(adds_graph* g, float delta, hipStream_t s) {
   if (g->split_delta != delta) {
     // heavy weight bands of width delta above delta (the pull phases read the same two values)
     HIPCHK(launch_split(g->row, g->col, g->w, g->n, delta, delta, 1.0f / delta, g->wt == ADDS_W_F32, g->scw,
-                        g->lcnt, g->split_aux, g->rinfo, s));
+                        g->lcnt, g->split_aux, g->rinfo, g->phead, s));
     g->split_delta = delta;
   }
   return ADDS_OK;
@@ -647,6 +652,7 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   memset(&a, 0, sizeof(a));
   a.cw = g->scw;
   a.rinfo = g->rinfo;
+  a.phead = g->phead;
   a.dist = dist;
   a.st = ws.st;
   a.pool = ws.pool;

@@ -165,8 +165,10 @@ struct KArgs {
   // ... and below pull_at only the vertices of at least order_degree light edges wait for their bucket:
   // the others are queued in the bucket of the claim that finds them (0: every vertex waits)
   uint32_t order_degree;
-  // the heavy part of every row is in band order (heavy_band below, with these two values)
+  // the heavy part of every row is in band order (heavy_band below, with these two values); phead[v]:
+  // its first two heavy edges (col, w, col, w), PHEAD_NONE as the col of an absent one
   float band_lo, band_inv;
+  const uint4* phead;
   uint32_t ptask_cap;
   uint4* ptask;          // long-row tasks of a pull phase: (vertex-instance, first edge, end edge, 0)
 };
@@ -175,6 +177,7 @@ struct KArgs {
 // PULL_TASK edges (one wavefront each), the rest by k_pull_a, 256 vertex-instances per wavefront.
 constexpr uint32_t PULL_LONG = 128;
 constexpr uint32_t PULL_AT_OFF = 0xFFFFFFFFu;
+constexpr uint32_t PHEAD_NONE = 0xFFFFFFFFu;  // no vertex id (vertex-instances stay below 2^32 - 1)
 constexpr uint32_t PULL_TASK = 1024;
 
 // Inclusive prefix sum over the 64 lanes of a wavefront (every lane active), with DPP: row shifts by
@@ -216,7 +219,7 @@ hipError_t launch_pull(const KArgs& a, bool is_float, bool replicated, uint32_t 
 // light counts.  aux: split_aux_words(m) words of scratch (long rows).
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
                         float band_lo, float band_inv, bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux,
-                        uint4* rinfo, hipStream_t s);
+                        uint4* rinfo, uint4* phead, hipStream_t s);
 // Rows longer than SPLIT_BIG edges are split by one 1024-thread workgroup each (k_split_big); the list
 // of such rows needs at most m / SPLIT_BIG entries, after SPLIT_AUX_ROWS words: the list's length and
 // the work counters of the two split kernels.

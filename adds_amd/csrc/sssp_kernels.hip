@@ -297,9 +297,10 @@ struct FlushArgs {
   uint32_t maxv;
   uint32_t pool_chunks;
   uint32_t nowait;
+  uint32_t publish;  // 0: positions are reserved and written but not published (a later step publishes)
 };
-__device__ __forceinline__ FlushArgs flush_args(const KArgs& a) {
-  return FlushArgs{a.dist, a.st, a.pool, a.tab, a.maxv, a.pool_chunks, a.nowait};
+__device__ __forceinline__ FlushArgs flush_args(const KArgs& a, bool publish = true) {
+  return FlushArgs{a.dist, a.st, a.pool, a.tab, a.maxv, a.pool_chunks, a.nowait, publish ? 1u : 0u};
 }
 
 struct PushBuf {
@@ -307,6 +308,7 @@ struct PushBuf {
   uint32_t* q;   // LDS: target queue
   uint64_t* tc;  // LDS: this wavefront's chunk-map cache
   uint32_t n;    // wave-uniform fill
+  bool publish = true;  // false in the pull kernels: k_pull_publish publishes every queue at once
 };
 
 // The sub-queue part that the items of the replica at base offset gb go to (the pusher's own part unless
@@ -470,7 +472,7 @@ __device__ FlushOut flush_buffer(const FlushArgs a, uint64_t* pbe, uint32_t* pbq
         if (ph != TAB_BAD) ast64(a.pool + ((size_t)ph << CH_SHIFT) + (pos & (CH - 1)), e[j]);
       }
     }
-    if (lane < ng) afadd(qavail(a, gq), gc);  // publish
+    if (lane < ng && a.publish) afadd(qavail(a, gq), gc);  // publish
   }
   wave_sync();
   return out;
@@ -496,7 +498,7 @@ __device__ __forceinline__ void pb_add(const KArgs& a, PushBuf& pb, bool f, uint
   const uint64_t m = ballot(f);
   if (m == 0) return;
   const uint32_t c = (uint32_t)__popcll(m);
-  if (pb.n + c > PB_CAP) pb_flush(flush_args(a), pb, ws);
+  if (pb.n + c > PB_CAP) pb_flush(flush_args(a, pb.publish), pb, ws);
   if (f) {
     const uint32_t i = pb.n + mbcnt(m);
     pb.e[i] = entry;
@@ -1743,7 +1745,7 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
   const uint32_t pc = rebucket ? ald(a.st + PULL_NEWHEAD) : c;
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
-  PushBuf pb{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
+  PushBuf pb{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u, false};
   for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;
   wave_sync();
   const uint32_t part = blockIdx.x % a.parts;
@@ -1759,27 +1761,57 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
     // items: vertex-instance x = base + j * 64 + lane (coalesced); range index k = lane * IPL + j
     uint32_t x[IPL], dv[IPL], rs[IPL], deg[IPL], gb[IPL], best[IPL], he[IPL];
     bool uns[IPL];
-    // distances and row records together (a record read for a settled vertex is not used)
-    uint4 ri[IPL];
+    // distances and the first two heavy edges together (coalesced); most vertices need nothing else
+    uint4 ph[IPL];
 #pragma unroll
     for (int j = 0; j < IPL; j++) {
       x[j] = base + j * 64 + lane;
       const bool in = x[j] < a.total_n;
       gb[j] = REPL ? x[j] - x[j] % a.n : 0u;
       dv[j] = in ? a.dist[x[j]] : 0u;
-      ri[j] = a.rinfo[in ? x[j] - gb[j] : 0u];
+      ph[j] = a.phead[in ? x[j] - gb[j] : 0u];
     }
+    {
+      bool g1[IPL], g2[IPL];
+      uint32_t d1[IPL], d2[IPL];
 #pragma unroll
-    for (int j = 0; j < IPL; j++) {
-      uns[j] = x[j] < a.total_n && unsettled<FLT>(dv[j], c, idl);
-      best[j] = dv[j];
-      rs[j] = ri[j].x + ri[j].z;
-      he[j] = uns[j] ? ri[j].y : rs[j];
+      for (int j = 0; j < IPL; j++) {
+        uns[j] = x[j] < a.total_n && unsettled<FLT>(dv[j], c, idl);
+        best[j] = dv[j];
+        const uint32_t bd = heavy_band(wfloat<FLT>(dv[j]), a.band_lo, a.band_inv);
+        g1[j] = uns[j] && ph[j].x != PHEAD_NONE && heavy_band(wfloat<FLT>(ph[j].y), a.band_lo, a.band_inv) <= bd;
+        g2[j] = uns[j] && ph[j].z != PHEAD_NONE && heavy_band(wfloat<FLT>(ph[j].w), a.band_lo, a.band_inv) <= bd;
+        d1[j] = g1[j] ? a.dist[gb[j] + ph[j].x] : INF;
+        d2[j] = g2[j] ? a.dist[gb[j] + ph[j].z] : INF;
+      }
+      uint32_t nscan = 0;
+      bool more[IPL];
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        if (g1[j]) best[j] = min(best[j], pull_cand<FLT>(d1[j], ph[j].y));
+        if (g2[j]) best[j] = min(best[j], pull_cand<FLT>(d2[j], ph[j].w));
+        nscan += (uint32_t)g1[j] + (uint32_t)g2[j];
+        // a third heavy edge (band at least the second's) can only matter if the second one's band is not
+        // above the best value's
+        more[j] = g2[j] && heavy_band(wfloat<FLT>(ph[j].w), a.band_lo, a.band_inv) <=
+                               heavy_band(wfloat<FLT>(best[j]), a.band_lo, a.band_inv);
+      }
+      nscan = (uint32_t)wave_sum64(nscan);
+      if (lane == 0) {
+        ws.v[ST_EDGES] += nscan;
+        ws.v[ST_PULL_EDGES] += nscan;
+      }
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        const uint4 ri = more[j] ? a.rinfo[x[j] - gb[j]] : make_uint4(0u, 0u, 0u, 0u);
+        rs[j] = ri.x + ri.z + 2;
+        he[j] = more[j] ? ri.y : rs[j];
+      }
     }
-    // The head of each heavy part, in band order, one edge per vertex per step: an edge of a higher band
-    // than the best value so far weighs more than it (and so does every edge after it), which ends the
-    // vertex's scan.  Most vertices end within PULL_HEAD edges; what is left of the others is relaxed in
-    // full below.  The next edge's (col, w) is loaded alongside each step's distance gather.
+    // The rest of the heavy part from its third edge, in band order, one edge per vertex per step: an edge
+    // of a higher band than the best value so far weighs more than it (and so does every edge after it),
+    // which ends the vertex's scan.  Most vertices end within PULL_HEAD more edges; what is left of the
+    // others is relaxed in full below.  The next edge's (col, w) is loaded alongside each step's gather.
     {
       bool act[IPL];
       uint64_t cwn[IPL];
@@ -1926,7 +1958,7 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
       ws.v[ST_PULL_IMPROVED] += nimp;
     }
   }
-  pb_flush(flush_args(a), pb, ws);
+  pb_flush(flush_args(a, false), pb, ws);
   publish_stats(a.st, s_stat, ws, nullptr);
 }
 
@@ -1943,7 +1975,7 @@ __global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
   const uint32_t ntask = min(ald(a.st + S_PULL2), a.ptask_cap);
   const uint32_t lane = lane_id();
   const uint32_t wv = threadIdx.x >> 6;
-  PushBuf pb{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u};
+  PushBuf pb{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u, false};
   for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;
   wave_sync();
   const uint32_t part = blockIdx.x % a.parts;
@@ -2003,8 +2035,19 @@ __global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
     const uint32_t pp = REPL ? push_part(a, gb, part) : part;
     pb_add(a, pb, imp, ((uint64_t)best << 32) | tk.x, light_queue<FLT>(a, pidl, best, pp, pc, QM_OWN), ws);
   }
-  pb_flush(flush_args(a), pb, ws);
+  pb_flush(flush_args(a, false), pb, ws);
   publish_stats(a.st, s_stat, ws, nullptr);
+}
+
+// The pull kernels reserve and write their queue entries without publishing them (one same-address
+// atomic per queue and flush less: on RMAT-24 the re-bucketing pull queues 7M vertices into a few
+// dozen queues, whose counters then serialised the kernel).  No claim runs between the persistent
+// launches, so afterwards every queue's unclaimed positions [head, tail) are written, and all of them are
+// published at once.
+__global__ void __launch_bounds__(256) k_pull_publish(KArgs a, uint32_t slot) {
+  if (!pull_requested(a, slot) || ald(a.st + PULL_OUTCOME) != 1u) return;
+  const uint32_t nq = nq_of(a);
+  for (uint32_t q = threadIdx.x; q < 2 * nq; q += blockDim.x) ast(qavail(a, q), ald(qtail(a, q)) - ald(qhead(a, q)));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2288,6 +2331,7 @@ static void launch_pull_t(const KArgs& a, uint32_t slot, uint32_t grid, hipStrea
   hipLaunchKernelGGL(k_pull_decide, dim3(1), dim3(256), 0, s, a, slot);
   hipLaunchKernelGGL((k_pull_a<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
   hipLaunchKernelGGL((k_pull_b<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
+  hipLaunchKernelGGL(k_pull_publish, dim3(1), dim3(256), 0, s, a, slot);
 }
 hipError_t launch_pull(const KArgs& a, bool is_float, bool replicated, uint32_t slot, uint32_t grid, hipStream_t s) {
   if (is_float) {
@@ -2303,15 +2347,26 @@ hipError_t launch_cleanup(const KArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_cleanup, dim3(1024), dim3(256), 0, s, a);
   return hipGetLastError();
 }
-// Per-vertex row record (row start, row end, light edges, 0) for the solver's item loads.
-__global__ void k_rinfo(const uint32_t* row_ptr, const uint32_t* lcnt, uint32_t n, uint4* rinfo) {
-  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
-    rinfo[v] = make_uint4(row_ptr[v], row_ptr[v + 1], lcnt[v], 0u);
+// Per-vertex row record (row start, row end, light edges, 0) for the solver's item loads, and the first
+// two heavy edges of each split row (read by the pull phases with the distances, so that most vertices
+// need no random row access).
+__global__ void k_rinfo(const uint32_t* row_ptr, const uint32_t* lcnt, uint32_t n, const uint2* cw2, uint4* rinfo,
+                        uint4* phead) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r0 = row_ptr[v], r1 = row_ptr[v + 1], lc = lcnt[v];
+    rinfo[v] = make_uint4(r0, r1, lc, 0u);
+    if (phead) {
+      const uint32_t h = r0 + lc;
+      const uint2 e1 = h < r1 ? cw2[h] : make_uint2(PHEAD_NONE, 0u);
+      const uint2 e2 = h + 1 < r1 ? cw2[h + 1] : make_uint2(PHEAD_NONE, 0u);
+      phead[v] = make_uint4(e1.x, e1.y, e2.x, e2.y);
+    }
+  }
 }
 
 hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t n, float delta,
                         float band_lo, float band_inv, bool is_float, uint2* cw2, uint32_t* lcnt, uint32_t* aux,
-                        uint4* rinfo, hipStream_t s) {
+                        uint4* rinfo, uint4* phead, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipError_t e = launch_fill_u32(aux, SPLIT_AUX_ROWS, 0, s);  // long-row count and the two work counters
   if (e != hipSuccess) return e;
@@ -2322,7 +2377,7 @@ hipError_t launch_split(const uint32_t* row_ptr, const uint32_t* col, const uint
                      is_float ? 1 : 0, cw2, lcnt, aux);
   if (rinfo)
     hipLaunchKernelGGL(k_rinfo, dim3(std::min<uint32_t>((n + 255) / 256, 8192)), dim3(256), 0, s, row_ptr, lcnt, n,
-                       rinfo);
+                       (const uint2*)cw2, rinfo, phead);
   return hipGetLastError();
 }
 
