@@ -679,8 +679,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": f"one solve: k_sssp<false, false, {head['params'].get('edge_windows') or 4}> "
-                               "(persistent launch), k_pull_count, k_pull_decide, k_pull_a, k_pull_b, k_sssp "
-                               "(continuation); HIP events around the sequence",
+                               "(persistent launch), k_pull_count, k_pull_decide, k_pull_a, k_pull_b, "
+                               "k_pull_publish, k_sssp (continuation); HIP events around the sequence",
                      "kernel_ms": head["kernel_ms_event"], "kernel_source_sha": sha,
                      "bytes_alg_per_launch": head["bytes_alg"],
                      "random_gather_ceiling": head["gather_ceiling"]},
@@ -689,7 +689,7 @@ def main():
                                                              "solve_ms_event", "gteps_kernel", "host_out_ms",
                                                              "host_out_gteps", "cold_first_solve_ms",
                                                              "cold_kernel_ms", "work", "work_efficiency", "cpu",
-                                                             "near_far", "cli")
+                                                             "near_far", "cli", "push_direction_only")
                                        if k in head}),
     }
     print(json.dumps(line))
