@@ -1673,16 +1673,20 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
       const adds_params p4 = p;
       const float lt = p4.light_delta > 0.f ? p4.light_delta : p4.delta;
       if (lt < 1e29f && !(p4.flags & ADDS_F_NO_PULL)) {
-        for (float div : {1.0f, 8.0f, 16.0f})
-          for (uint32_t od : {0u, 16u, 32u}) {
-            if (div == 1.0f && od) continue;
-            adds_params q = p4;
-            q.flags |= ADDS_F_PULL_SPLIT;
-            q.delta = lt / div;
-            q.light_delta = lt;
-            q.order_degree = od;
-            try_point(q);
-          }
+        // (the light threshold itself, and half of it: the pull then comes earlier, where the banded scan
+        // makes it cheap)
+        for (float lf : {1.0f, 0.5f})
+          for (float div : {1.0f, 8.0f, 12.0f, 16.0f})
+            for (uint32_t od : {0u, 12u, 16u, 32u}) {
+              if (div == 1.0f && od) continue;
+              adds_params q = p4;
+              q.flags |= ADDS_F_PULL_SPLIT;
+              q.light_delta = lt * lf;
+              q.delta = q.light_delta / div;
+              if (is_int && q.delta < 1.0f) continue;
+              q.order_degree = od;
+              try_point(q);
+            }
         if (p.flags & ADDS_F_PULL_SPLIT) {
           // more open buckets: the narrow buckets' hubs are expanded ahead of the head, and the head's
           // walk through the narrow buckets blocks less (4 parts leave probe lanes for 8 buckets)

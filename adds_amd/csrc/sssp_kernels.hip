@@ -1772,6 +1772,8 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
       ph[j] = a.phead[in ? x[j] - gb[j] : 0u];
     }
     {
+      // the second edge is gathered only if its band is not above the best value after the first: an
+      // extra round trip, but most second gathers (each a cache line) are saved
       bool g1[IPL], g2[IPL];
       uint32_t d1[IPL], d2[IPL];
 #pragma unroll
@@ -1780,15 +1782,20 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
         best[j] = dv[j];
         const uint32_t bd = heavy_band(wfloat<FLT>(dv[j]), a.band_lo, a.band_inv);
         g1[j] = uns[j] && ph[j].x != PHEAD_NONE && heavy_band(wfloat<FLT>(ph[j].y), a.band_lo, a.band_inv) <= bd;
-        g2[j] = uns[j] && ph[j].z != PHEAD_NONE && heavy_band(wfloat<FLT>(ph[j].w), a.band_lo, a.band_inv) <= bd;
         d1[j] = g1[j] ? a.dist[gb[j] + ph[j].x] : INF;
+      }
+#pragma unroll
+      for (int j = 0; j < IPL; j++) {
+        if (g1[j]) best[j] = min(best[j], pull_cand<FLT>(d1[j], ph[j].y));
+        g2[j] = uns[j] && ph[j].z != PHEAD_NONE &&
+                heavy_band(wfloat<FLT>(ph[j].w), a.band_lo, a.band_inv) <=
+                    heavy_band(wfloat<FLT>(best[j]), a.band_lo, a.band_inv);
         d2[j] = g2[j] ? a.dist[gb[j] + ph[j].z] : INF;
       }
       uint32_t nscan = 0;
       bool more[IPL];
 #pragma unroll
       for (int j = 0; j < IPL; j++) {
-        if (g1[j]) best[j] = min(best[j], pull_cand<FLT>(d1[j], ph[j].y));
         if (g2[j]) best[j] = min(best[j], pull_cand<FLT>(d2[j], ph[j].w));
         nscan += (uint32_t)g1[j] + (uint32_t)g2[j];
         // a third heavy edge (band at least the second's) can only matter if the second one's band is not
