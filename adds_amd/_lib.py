@@ -63,6 +63,7 @@ class Params(ctypes.Structure):
         ("pool_entries", ctypes.c_uint64),
         ("pull_delta", ctypes.c_float),
         ("order_degree", ctypes.c_uint32),
+        ("pull_early", ctypes.c_uint32),
     ]
 
     @classmethod

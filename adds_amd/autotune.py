@@ -27,7 +27,7 @@ DEFAULT_CACHE = os.path.join(_HERE, "tune_cache.json")
 PINNED = os.path.join(_HERE, "tuned_params.json")
 TUNED_FIELDS = ("delta", "wg_per_cu", "chunk", "nbuckets", "parts", "concurrency", "task_edges", "flags",
                 "light_delta", "edge_windows", "tail_items", "hub_degree", "pull_delta",
-                "order_degree")
+                "order_degree", "pull_early")
 
 
 def workload_key(host, k):

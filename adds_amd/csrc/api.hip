@@ -461,6 +461,7 @@ struct Resolved {
   uint64_t pull_min;               // heavy edges a bucket needs before a pull phase is considered
   uint32_t pull_at;                // ADDS_F_PULL_SPLIT: the bucket of the forced first pull (PULL_AT_OFF: off)
   uint32_t order_degree;           // ... and the light degree from which vertices wait for their bucket there
+  uint32_t pull_early;             // ... and the pending-item count at which the pull may come early
 };
 
 // Default bucket width.  The reference derives it from the mean weight and degree
@@ -554,6 +555,10 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
     if (key >= 1) r.pull_at = key - 1;
   }
   r.order_degree = r.pull_at != PULL_AT_OFF ? p->order_degree : 0u;
+  // an early pull leaves items queued below the split point; the launches after it must relax every edge
+  // of whatever they expand (all light, after a re-bucketing pull), and the queues' contents are carried
+  r.pull_early = (r.pull_at != PULL_AT_OFF && r.pull_delta > 0.f && (r.flags & ADDS_F_PULL_ALL_LIGHT))
+                     ? p->pull_early : 0u;
   r.ptask_cap = (uint32_t)std::min<uint64_t>(total_m / PULL_LONG + 4096, 1u << 24);
   return ADDS_OK;
 }
@@ -605,7 +610,7 @@ static adds_status ensure_workspace(Workspace& ws, const Resolved& r, uint32_t n
     if (ws.ptask) (void)hipFree(ws.ptask);
     ws.ptask = nullptr;
     ws.ptask_cap = 0;
-    HIPCHK(hipMalloc(&ws.ptask, (size_t)r.ptask_cap * 16));
+    HIPCHK(hipMalloc(&ws.ptask, ((size_t)r.ptask_cap + CARRY_UINT4) * 16));  // then the carry ranges
     ws.ptask_cap = r.ptask_cap;
   }
   if (ws.srcs_cap < nsrc) {
@@ -707,6 +712,8 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.band_inv = 1.0f / g->split_delta;
   a.ptask = ws.ptask;
   a.ptask_cap = ws.ptask_cap;
+  a.carry = ws.ptask ? (uint2*)(ws.ptask + ws.ptask_cap) : nullptr;
+  a.pull_early = r.pull_early;
   a.launch_idx = 0;
   return a;
 }
@@ -1704,6 +1711,15 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
             adds_params q = p;
             q.flags = (q.flags & ~ADDS_F_NO_LOCAL) | ADDS_F_LOCAL_SPARSE;
             try_point(q);
+          }
+          // an early pull (its queued items carried over), with every edge light after it
+          if (p.pull_delta > 0.f && (p.flags & ADDS_F_PULL_ALL_LIGHT)) {
+            const adds_params p6 = p;
+            for (uint32_t e : {65536u, 262144u}) {
+              adds_params q = p6;
+              q.pull_early = e;
+              try_point(q);
+            }
           }
         }
       }

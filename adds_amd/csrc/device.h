@@ -34,7 +34,8 @@ constexpr uint32_t S_TAIL = 1 * LINE;       // [0] 1 once the solve is in its ta
                                             // request, [4..5] u64 heavy edges queued in it, [6] pulls
                                             // performed, [7] the last request's outcome (1 pull, 2 push),
                                             // [8] the requester's 1/delta (float bits), [9] 1 once a pull
-                                            // has re-bucketed the solve (pull_delta), [10] its new head
+                                            // has re-bucketed the solve (pull_delta), [10] its new head,
+                                            // [11] 1 when the request left queued items to carry over
 constexpr uint32_t S_DONE = 2 * LINE;       // [0] done flag, [1] error code
 constexpr uint32_t S_POOL = 3 * LINE;       // next free pool chunk
 // (lines 4 and 5 are spare)
@@ -50,7 +51,7 @@ constexpr uint32_t S_PULL = (8 + VER_GROUPS) * LINE;
 constexpr uint32_t S_PULL2 = S_PULL + LINE;
 constexpr uint32_t PULL_GEN = S_TAIL + 2, PULL_BUCKET = S_TAIL + 3, PULL_H = S_TAIL + 4, PULL_DONE = S_TAIL + 6,
                    PULL_OUTCOME = S_TAIL + 7, PULL_IDL = S_TAIL + 8, PULL_REBUCKET = S_TAIL + 9,
-                   PULL_NEWHEAD = S_TAIL + 10;
+                   PULL_NEWHEAD = S_TAIL + 10, PULL_CARRY = S_TAIL + 11;
 // Statistics: STAT_COPIES copies of u64 stats[NSTATS] (two lines each).  Workgroup b adds its counters,
 // summed over its wavefronts in LDS, to copy b % STAT_COPIES, and the host adds the copies up: atomics on
 // one address serialise (about 12 ns each), and one per wavefront and counter on a single copy cost a
@@ -169,6 +170,12 @@ struct KArgs {
   // its first two heavy edges (col, w, col, w), PHEAD_NONE as the col of an absent one
   float band_lo, band_inv;
   const uint4* phead;
+  // early pull (split first bucket, every edge light after a re-bucketing pull): launch 0 asks for the
+  // pull once at most pull_early items are queued or in flight, after four times as many were seen; the
+  // items still queued then are carried over into the wide buckets.  carry: per queue (head, tail) at the
+  // request, 2 * nb * parts entries (after the pull tasks).
+  uint32_t pull_early;
+  uint2* carry;
   uint32_t ptask_cap;
   uint4* ptask;          // long-row tasks of a pull phase: (vertex-instance, first edge, end edge, 0)
 };
@@ -179,6 +186,7 @@ constexpr uint32_t PULL_LONG = 128;
 constexpr uint32_t PULL_AT_OFF = 0xFFFFFFFFu;
 constexpr uint32_t PHEAD_NONE = 0xFFFFFFFFu;  // no vertex id (vertex-instances stay below 2^32 - 1)
 constexpr uint32_t PULL_TASK = 1024;
+constexpr uint32_t CARRY_UINT4 = MAX_PROBE_QUEUES * 32;  // room for 2 * nb * parts (head, tail) pairs: nb <= 256, parts <= 8
 
 // Inclusive prefix sum over the 64 lanes of a wavefront (every lane active), with DPP: row shifts by
 // 1, 2, 4 and 8 scan each row of 16 lanes (lanes shifted in from outside the row add 0), then

@@ -1441,7 +1441,44 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         behind_any = beh && (busy_q[2] || busy_q[3]);
       }
       const bool hl = ballot(head_light) != 0, hh = ballot(head_heavy) != 0, ba = ballot(behind_any) != 0;
-      if (!hl && rdlane(lane == 0 ? ald(phase(a, hs)) : 1u, 0) == 0) {
+      // Early pull: once at most pull_early items are left at or below the split point, after at least
+      // four times as many were seen (the start of a solve has few too), the pull for the split point is
+      // asked for now; what is still queued is carried over by the pull kernels.
+      bool early = false;
+      if (defer && a.pull_early != 0 && c2 < a.pull_at) {
+        uint64_t pend = 0;  // (the buckets up to the split point; those beyond it hold unsettled vertices)
+        for (uint32_t sl = lane; sl < a.nb; sl += 64) {
+          const uint32_t b = c2 - 1 + ((sl - (c2 - 1)) & (a.nb - 1));  // the window's bucket in slot sl
+          if (b > a.pull_at) continue;
+          for (uint32_t pi = 0; pi < a.parts; pi++) {
+            const uint32_t ql = sl * a.parts + pi, qh = nq + ql;
+            const uint32_t hl0 = ald(qhead(a, ql)), tl0 = ald(qtail(a, ql));
+            const uint32_t hh0 = ald(qhead(a, qh)), th0 = ald(qtail(a, qh));
+            pend += (uint64_t)(tl0 - hl0) + (th0 - hh0) + ald(inflight(a, sl, pi)) + ald(inflight(a, sl, MAX_PARTS + pi));
+          }
+        }
+        pend = wave_sum64(pend);
+        uint32_t go = 0;
+        if (lane == 0) {
+          const uint32_t p32 = (uint32_t)min(pend, (uint64_t)0xFFFFFFFFu);
+          const uint32_t peak = max(ald(a.st + S_TAIL + 1), p32);
+          ast(a.st + S_TAIL + 1, peak);
+          if (pend <= a.pull_early && peak >= 4ull * a.pull_early) {
+            ast(a.st + PULL_BUCKET, a.pull_at);
+            ast(a.st + PULL_IDL, __float_as_uint(idl));
+            ast64((uint64_t*)(a.st + PULL_H), 0ull);
+            ast(a.st + PULL_CARRY, 1u);
+            mem_wait();
+            ast(a.st + PULL_GEN, pgen + 1);
+            trace_event(a, 5, a.pull_at, 0);
+            go = 1;
+          }
+        }
+        early = rdlane(go, 0) != 0;
+      }
+      if (early) {
+        // this launch ends: its wavefronts leave once they hold nothing
+      } else if (!hl && rdlane(lane == 0 ? ald(phase(a, hs)) : 1u, 0) == 0) {
         // The head bucket's light phase is done.  Its heavy edges are relaxed from the sending side
         // (open the heavy queues), or -- when the queues hold many heavy edges and a pull slot is left
         // -- from the receiving side by a pull phase: request it and let this launch end.
@@ -1473,7 +1510,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
           opened = true;
         }
       }
-      if (!hl && !hh && !ba && !(defer && c2 >= a.pull_at)) {
+      if (!early && !hl && !hh && !ba && !(defer && c2 >= a.pull_at)) {
         uint32_t firstj = 0xFFFFFFFFu;
         uint64_t pend = 0;  // items queued or in flight in the window (this lane's buckets)
         for (uint32_t j0 = 0; j0 < a.nb; j0 += 64) {
@@ -1690,11 +1727,16 @@ __global__ void __launch_bounds__(256) k_pull_decide(KArgs a, uint32_t slot) {
   const uint32_t nq = nq_of(a), cs = c & (a.nb - 1);
   if (rebucket) {
     // every queue emptied (positions stay reserved, so the chunk maps and the cleanup are unaffected),
-    // every slot's phase word and heavy-edge counters cleared; in-flight counts are zero (no launch runs)
+    // every slot's phase word and heavy-edge counters cleared; in-flight counts are zero (no launch runs).
+    // An early request leaves items queued: each queue's range is kept for k_pull_carry.
+    const bool carry = ald(a.st + PULL_CARRY) != 0 && a.carry;
     for (uint32_t q = threadIdx.x; q < 2 * nq; q += blockDim.x) {
-      ast(qhead(a, q), ald(qtail(a, q)));
+      const uint32_t t = ald(qtail(a, q));
+      if (carry) a.carry[q] = make_uint2(ald(qhead(a, q)), t);
+      ast(qhead(a, q), t);
       ast(qavail(a, q), 0u);
     }
+    if (threadIdx.x == 0) ast(a.st + S_TAIL + 1, 0u);  // the peak of the launches that follow starts afresh
     for (uint32_t i = threadIdx.x; i < a.nb * a.parts; i += blockDim.x) {
       const uint32_t sl = i / a.parts, p = i % a.parts;
       ast64(heavy_ctr(a, sl, p), 0ull);
@@ -2046,6 +2088,73 @@ __global__ void __launch_bounds__(256) k_pull_b(KArgs a, uint32_t slot) {
   publish_stats(a.st, s_stat, ws, nullptr);
 }
 
+// Early pull: the light-queue items still queued at the request whose vertex lies at or below the split
+// point, and still holds the item's distance, are queued again at the wide width (the pull saw their
+// distance, so their heavy edges are covered; their light edges are relaxed when they are expanded).
+// Items of vertices beyond the split point are not needed: k_pull_a queues every such vertex that has a
+// distance.  A reference item (one chunk of a long row) is carried as it is.
+template <bool FLT, bool REPL>
+__global__ void __launch_bounds__(256) k_pull_carry(KArgs a, uint32_t slot) {
+  __shared__ uint64_t s_pbe[WAVES_PER_WG][PB_CAP];
+  __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
+  __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
+  __shared__ uint64_t s_stat[WAVES_PER_WG][NSTATS];
+  if (!pull_requested(a, slot) || ald(a.st + PULL_OUTCOME) != 1u || ald(a.st + PULL_CARRY) == 0u ||
+      ald(a.st + PULL_REBUCKET) == 0u || !a.carry)
+    return;
+  const uint32_t c = ald(a.st + PULL_BUCKET);
+  const float idl = pull_req_idl(a);
+  const float pidl = a.inv_delta_resume;
+  const uint32_t pc = ald(a.st + PULL_NEWHEAD);
+  const uint32_t lane = lane_id();
+  const uint32_t wv = threadIdx.x >> 6;
+  PushBuf pb{s_pbe[wv], s_pbq[wv], s_tc[wv], 0u, false};
+  for (uint32_t i = lane; i < TC_SIZE; i += 64) s_tc[wv][i] = ~0ull;
+  wave_sync();
+  const uint32_t part = blockIdx.x % a.parts;
+  WaveStats ws;
+#pragma unroll
+  for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
+  const uint32_t nq = nq_of(a);
+  const uint32_t nwaves = gridDim.x * WAVES_PER_WG, wave_gid = blockIdx.x * WAVES_PER_WG + wv;
+  // 64-position windows of every light queue's range, dealt round-robin over the wavefronts
+  uint32_t widx = 0;
+  for (uint32_t q = 0; q < nq; q++) {
+    const uint2 ht = a.carry[q];
+    if (ht.y <= ht.x) continue;
+    const uint32_t nwin = (ht.y - ht.x + 63) / 64;
+    uint32_t first = (wave_gid + nwaves - widx % nwaves) % nwaves;  // this wavefront's first window here
+    for (uint32_t w = first; w < nwin; w += nwaves) {
+      const uint32_t pos = ht.x + w * 64 + lane;
+      const bool in = pos < ht.y;
+      uint64_t e = EMPTY64;
+      if (in) {
+        const uint32_t ph = a.tab[(size_t)q * a.maxv + (pos >> CH_SHIFT)];
+        if (ph < TAB_BAD) e = a.pool[((size_t)ph << CH_SHIFT) + (pos & (CH - 1))];
+      }
+      bool keep = false;
+      uint32_t qd = 0, gbv = 0;
+      if (e != EMPTY64) {
+        const uint32_t gid = (uint32_t)e, du = (uint32_t)(e >> 32);
+        uint32_t vtx = gid;
+        qd = du;
+        if (du == REF_TAG) {
+          const uint32_t* r = a.hrec + (size_t)(gid >> REF_CHUNK_BITS) * HREC_WORDS;
+          vtx = ald(r + 0);
+          qd = ald(r + 1);
+        }
+        keep = a.dist[vtx] >= qd && !unsettled<FLT>(qd, c, idl);
+        gbv = REPL ? vtx - vtx % a.n : 0u;
+      }
+      const uint32_t pp = REPL ? push_part(a, gbv, part) : part;
+      pb_add(a, pb, keep, e, light_queue<FLT>(a, pidl, qd, pp, pc, QM_OWN), ws);
+    }
+    widx += nwin;
+  }
+  pb_flush(flush_args(a, false), pb, ws);
+  publish_stats(a.st, s_stat, ws, nullptr);
+}
+
 // The pull kernels reserve and write their queue entries without publishing them (one same-address
 // atomic per queue and flush less: on RMAT-24 the re-bucketing pull queues 7M vertices into a few
 // dozen queues, whose counters then serialised the kernel).  No claim runs between the persistent
@@ -2336,6 +2445,7 @@ static void launch_pull_t(const KArgs& a, uint32_t slot, uint32_t grid, hipStrea
                                                                    PULL_COUNT_THREADS, 256));
   hipLaunchKernelGGL((k_pull_count<FLT, REPL>), dim3(cblocks), dim3(PULL_COUNT_THREADS), 0, s, a, slot);
   hipLaunchKernelGGL(k_pull_decide, dim3(1), dim3(256), 0, s, a, slot);
+  hipLaunchKernelGGL((k_pull_carry<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
   hipLaunchKernelGGL((k_pull_a<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
   hipLaunchKernelGGL((k_pull_b<FLT, REPL>), dim3(grid), dim3(256), 0, s, a, slot);
   hipLaunchKernelGGL(k_pull_publish, dim3(1), dim3(256), 0, s, a, slot);

@@ -99,7 +99,7 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
     if (colon != std::string::npos) *v = strtod(obj.c_str() + colon + 1, nullptr);
   };
   double delta = 0, light = 0, wg = 0, chunk = 0, nb = 0, parts = 0, conc = 0, te = 0, flags = 0, win = 0, tail = 0,
-         hub = 0, pdelta = 0, order = 0;
+         hub = 0, pdelta = 0, order = 0, early = 0;
   num("delta", &delta);
   num("light_delta", &light);
   num("wg_per_cu", &wg);
@@ -114,6 +114,7 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
   num("hub_degree", &hub);
   num("pull_delta", &pdelta);
   num("order_degree", &order);
+  num("pull_early", &early);
   p->delta = (float)delta;
   p->light_delta = (float)light;
   p->wg_per_cu = (uint32_t)wg;
@@ -129,6 +130,7 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
   p->hub_degree = (uint32_t)hub;
   p->pull_delta = (float)pdelta;
   p->order_degree = (uint32_t)order;
+  p->pull_early = (uint32_t)early;
   return true;
 }
 

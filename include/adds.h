@@ -84,6 +84,12 @@ typedef struct {
                             bucket order).  A vertex of many light edges expanded before its distance is
                             final rescans all of them; one of few costs little, and running those out of
                             order needs fewer, fuller buckets */
+  uint32_t pull_early;   /* with ADDS_F_PULL_SPLIT, ADDS_F_PULL_ALL_LIGHT and pull_delta: the split point's pull
+                            may come as soon as at most this many items are queued or in flight (after
+                            four times as many were seen) instead of after the last narrow bucket; the items
+                            still queued are carried over into the wide buckets, where every edge is light,
+                            so the last hops of the narrow buckets overlap the rest of the solve (0, the
+                            default: the pull waits for the split point) */
 } adds_params;
 
 /* flags */

@@ -286,6 +286,12 @@ SPLIT = L.F_PULL_SPLIT
     dict(flags=SPLIT | L.F_PULL_ALL_LIGHT | L.F_LOCAL_SPARSE, order_degree=8, pull_delta=16.0, chunk=64),
     dict(flags=SPLIT | L.F_PULL_ALL_LIGHT, order_degree=8, pull_delta=16.0, parts=4, concurrency=13),
     dict(flags=SPLIT, order_degree=8, parts=2, concurrency=8, nbuckets=8),  # concurrency capped at nb - 2
+    # early pull: queued items carried over into the wide buckets
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT, order_degree=8, pull_delta=16.0, pull_early=4096),
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT, pull_delta=16.0, pull_early=1 << 30),  # at the first chance
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT | L.F_LOCAL_SPARSE | L.F_NO_WAIT, order_degree=4, pull_delta=64.0,
+         pull_early=256, hub_degree=64, task_edges=32, parts=4, concurrency=8),
+    dict(flags=SPLIT, order_degree=8, pull_early=4096),  # no re-bucketing: pull_early is ignored
 ])
 def test_rmat_split_first_bucket(is_float, params):
     """ADDS_F_PULL_SPLIT: the buckets below the light threshold run their light phases only, then one
@@ -315,7 +321,8 @@ def test_split_first_bucket_runs_dry_before_the_pull():
     assert g.check_symmetric()
     dg = adds_amd.Graph(g)
     for params in (dict(), dict(pull_delta=4096.0), dict(pull_delta=4096.0, flags=L.F_PULL_ALL_LIGHT),
-                   dict(tail_items=4096), dict(order_degree=2)):
+                   dict(tail_items=4096), dict(order_degree=2),
+                   dict(pull_delta=4096.0, flags=L.F_PULL_ALL_LIGHT, pull_early=1 << 30)):
         params = dict(params)
         params["flags"] = params.get("flags", 0) | SPLIT
         for src in range(g.n):
@@ -327,14 +334,15 @@ def test_split_first_bucket_runs_dry_before_the_pull():
     assert list(dist.view(np.uint32)[:4]) == [0, 1, 2, 100002]
 
 
-@pytest.mark.parametrize("flags,order", [(SPLIT, 0), (SPLIT | L.F_REPLICA_PARTS | L.F_PULL_ALL_LIGHT, 0),
-                                         (SPLIT | L.F_REPLICA_PARTS | L.F_PULL_ALL_LIGHT, 3)])
-def test_multi_source_split(flags, order):
+@pytest.mark.parametrize("flags,order,early", [(SPLIT, 0, 0), (SPLIT | L.F_REPLICA_PARTS | L.F_PULL_ALL_LIGHT, 0, 0),
+                                               (SPLIT | L.F_REPLICA_PARTS | L.F_PULL_ALL_LIGHT, 3, 0),
+                                               (SPLIT | L.F_PULL_ALL_LIGHT, 3, 2048)])
+def test_multi_source_split(flags, order, early):
     g = adds_amd.HostGraph.rmat(13, 16, seed=9, wkind="f32", lo=1.0, hi=10.0)
     srcs = g.sample_component(g.max_degree_vertex(), 12, seed=4)
     dm, st = adds_amd.Graph(g).sssp_multi(srcs, timeout_ms=TIMEOUT, delta=0.5, light_delta=4.0, parts=4,
                                           flags=flags, pull_delta=16.0 if flags & L.F_PULL_ALL_LIGHT else 0.0,
-                                          order_degree=order)
+                                          order_degree=order, pull_early=early)
     assert st["pulls"] == 1, st
     for i, s in enumerate(srcs):
         _check(g, dm[i], s, True)
@@ -349,7 +357,7 @@ def test_captured_split_replays_and_weight_update():
     dist = torch.empty((1, g.n), dtype=torch.int32, device="cuda")
     sd = torch.from_numpy(src).cuda()
     sg = adds_amd.SolveGraph(dg, sd.data_ptr(), 1, dist.data_ptr(), delta=32.0, light_delta=256.0,
-                             flags=SPLIT | L.F_PULL_ALL_LIGHT, pull_delta=2048.0)
+                             flags=SPLIT | L.F_PULL_ALL_LIGHT, pull_delta=2048.0, order_degree=4, pull_early=2048)
     ref = None
     for i in range(3):
         dist.fill_(5)
