@@ -18,7 +18,8 @@ W_U32, W_F32 = 0, 1
 WK_U32_UNIFORM, WK_F32_UNIFORM, WK_F32_LOGNORMAL = 0, 1, 2
 # flags
 F_HOST_OUT, F_NO_SYNC, F_ATOMIC_PROBE, F_NO_LOCAL, F_NO_WAIT, F_REPLICA_PARTS = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20
-F_NO_PULL, F_PULL_EAGER, F_PULL_ALL_LIGHT, F_PULL_SPLIT, F_LOCAL_SPARSE = 0x40, 0x80, 0x100, 0x200, 0x400
+F_NO_PULL, F_PULL_EAGER, F_PULL_ALL_LIGHT, F_PULL_SPLIT, F_LOCAL_SPARSE, F_FULL_FIRST = (0x40, 0x80, 0x100, 0x200,
+                                                                                0x400, 0x800)
 
 INF_U32 = 0xFFFFFFFF
 INF_F32_BITS = 0x7F800000

@@ -695,6 +695,7 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.cached_probe = (r.flags & ADDS_F_ATOMIC_PROBE) ? 0u : 1u;
   a.sticky = 1;
   a.local_cont = (r.flags & ADDS_F_NO_LOCAL) ? 0u : (r.flags & ADDS_F_LOCAL_SPARSE) ? 2u : 1u;
+  a.full_first = (r.flags & ADDS_F_FULL_FIRST) ? 1u : 0u;
   a.nowait = (r.flags & ADDS_F_NO_WAIT) ? 1u : 0u;
   a.trace = ws.trace;
   a.trace_cap = TRACE_CAP;
@@ -1751,7 +1752,7 @@ static adds_status autotune_core(adds_graph* g, const uint32_t* srcs, uint32_t k
     }
     // the two flags again: the first pass can settle on a point from which the no-wait flush or local
     // continuation looks worse than it is
-    for (uint32_t f : {ADDS_F_NO_LOCAL, ADDS_F_NO_WAIT, ADDS_F_REPLICA_PARTS}) {
+    for (uint32_t f : {ADDS_F_NO_LOCAL, ADDS_F_NO_WAIT, ADDS_F_REPLICA_PARTS, ADDS_F_FULL_FIRST}) {
       if (f == ADDS_F_REPLICA_PARTS && !repl) continue;
       adds_params q = p;
       q.flags ^= f;

@@ -132,6 +132,7 @@ struct KArgs {
   uint64_t timeout_ticks;  // wall_clock64 ticks
   uint32_t cached_probe;   // probe queue counters with L2-cached loads (claims stay atomic)
   uint32_t sticky;         // after a full claim from the behind/head bucket, claim the same queue again
+  uint32_t full_first;     // claims try queues that hold a full chunk first (ADDS_F_FULL_FIRST)
   uint32_t local_cont;     // a light claim keeps up to 64 same-bucket improvements for itself (2: only a
                            // claim of at most chunk / 4 items)
   uint32_t nowait;         // flushes do not wait for their atomicMins (every improvement is queued)

@@ -921,7 +921,7 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
     const uint32_t c = got ? 0u : hld(a.st + S_CUR, cached);
     const bool head_heavy_open = !got && hld(phase(a, c & (a.nb - 1)), cached) != 0;
     uint32_t pq = 0, pbk = 0;
-    bool pv = false;
+    bool pv = false, pf = false;
     {
       const uint32_t grp = lane / a.parts;
       uint32_t j = 0xFFFFFFFFu;  // bucket c-1+j
@@ -938,10 +938,16 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       if (!got && j != 0xFFFFFFFFu && (c > 0 || j > 0)) {
         pbk = c - 1 + j;
         pq = (heavy ? nq : 0u) + (pbk & (a.nb - 1)) * a.parts + ((part + lane % a.parts) % a.parts);
-        pv = (int32_t)hld(qavail(a, pq), cached) > 0;
+        const int32_t av = (int32_t)hld(qavail(a, pq), cached);
+        pv = av > 0;
+        pf = av >= (int32_t)a.chunk;
       }
     }
     const uint64_t pm = got ? 0ull : ballot(pv);
+    // (full_first: queues that hold a full claim are tried before the priority order, so that when the
+    // head bucket is down to its last few items the wavefronts do not all contend for them while the
+    // open buckets ahead hold plenty)
+    const uint64_t pmf = (a.full_first && pm) ? ballot(pf) : 0ull;
     if (lane == 0 && !got) ws.v[ST_PROBE]++;
     if (pm) {
       if (!busy) {
@@ -952,9 +958,16 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
       // Take up to `chunk` from the published count (fetch_sub; a shortfall is given back), then
       // fetch_add the head by what was taken.  Published items are at positions the head has not
       // passed, so every claimed position has been reserved by a producer.
-      uint64_t qm = pm;
-      for (int tries = 0; tries < 3 && !got && qm; tries++) {
-        const uint32_t L = ffs64(qm);
+      uint64_t qf = pmf, qm = pm & ~pmf;
+      for (int tries = 0; tries < 3 && !got && (qf | qm); tries++) {
+        uint32_t L;
+        if (qf) {
+          L = ffs64(qf);
+          qf &= qf - 1;
+        } else {
+          L = ffs64(qm);
+          qm &= qm - 1;
+        }
         const uint32_t qq = rdlane(pq, L);
         const uint32_t bb = rdlane(pbk, L);
         if (try_claim(a, qq, bb, nq, h, cnt, ws)) {
@@ -968,7 +981,6 @@ __global__ void __launch_bounds__(WG_THREADS) k_sssp(KArgs a) {
         } else {
           missed = true;
         }
-        qm &= qm - 1;
       }
     }
     const uint64_t t1 = wall_clock64();

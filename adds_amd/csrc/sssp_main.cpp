@@ -124,7 +124,7 @@ static bool params_from_json(const std::string& text, const std::string& key, ad
   p->concurrency = (uint32_t)conc;
   p->task_edges = (uint32_t)te;
   p->flags = (uint32_t)flags & (ADDS_F_NO_LOCAL | ADDS_F_NO_WAIT | ADDS_F_NO_PULL | ADDS_F_PULL_ALL_LIGHT |
-                                ADDS_F_PULL_SPLIT | ADDS_F_LOCAL_SPARSE);  // tuning flags only
+                                ADDS_F_PULL_SPLIT | ADDS_F_LOCAL_SPARSE | ADDS_F_FULL_FIRST);  // tuning flags only
   p->edge_windows = (uint32_t)win;
   p->tail_items = (uint32_t)tail;
   p->hub_degree = (uint32_t)hub;

@@ -122,6 +122,9 @@ typedef struct {
                                  sparse stretches (chains of single hops at the end of a bucket) keep it,
                                  full claims hand their improvements to every wavefront (ignored with
                                  ADDS_F_NO_LOCAL) */
+#define ADDS_F_FULL_FIRST 0x800u /* a claim tries the open queues that hold a full chunk before the others
+                                 (head bucket first otherwise): when the head bucket is down to its last
+                                 items, wavefronts take the plentiful buckets ahead instead of contending */
 
 typedef struct {
   double ms;                  /* hipEvent time of the solve (init + persistent kernel + cleanup) */

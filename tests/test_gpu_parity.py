@@ -69,6 +69,8 @@ def test_grid2048_float_c2():
     dict(edge_windows=8, parts=4, concurrency=4),
     dict(tail_items=1 << 30, delta=64.0),  # one bucket once the pending work is a quarter of its peak
     dict(tail_items=4096, delta=16.0, parts=2),
+    dict(flags=0x800, parts=4, concurrency=8, delta=64.0),  # full queues first, 8 open buckets
+    dict(flags=0x800 | 0x400 | 0x10, parts=2, concurrency=13, chunk=64),  # sparse-only local continuation
 ])
 def test_param_sweep_identical(params):
     """Results never depend on the tuning point (delta, buckets, parts, chunk, hub split)."""

@@ -292,6 +292,8 @@ SPLIT = L.F_PULL_SPLIT
     dict(flags=SPLIT | L.F_PULL_ALL_LIGHT | L.F_LOCAL_SPARSE | L.F_NO_WAIT, order_degree=4, pull_delta=64.0,
          pull_early=256, hub_degree=64, task_edges=32, parts=4, concurrency=8),
     dict(flags=SPLIT, order_degree=8, pull_early=4096),  # no re-bucketing: pull_early is ignored
+    dict(flags=SPLIT | L.F_PULL_ALL_LIGHT | L.F_FULL_FIRST | L.F_LOCAL_SPARSE, order_degree=8, pull_delta=32.0,
+         pull_early=2048, parts=4, concurrency=8),
 ])
 def test_rmat_split_first_bucket(is_float, params):
     """ADDS_F_PULL_SPLIT: the buckets below the light threshold run their light phases only, then one
