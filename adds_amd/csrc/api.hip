@@ -58,6 +58,9 @@ struct Workspace {
   double timeout_ms = 0;
   uint4* ptask = nullptr;               // pull phases: long-row tasks
   uint32_t ptask_cap = 0;
+  uint64_t* pstage = nullptr;           // re-bucketing pulls: staged queue entries, one per vertex-instance
+  uint32_t* pcnt = nullptr;             // ... and their count per 256-instance chunk
+  uint64_t pstage_cap = 0;
 };
 
 // Every graph handle gets a number no other handle of this process has had, renewed whenever its
@@ -166,6 +169,8 @@ static adds_status free_workspace(Workspace& ws) {
   if (ws.srcs) (void)hipFree(ws.srcs);
   if (ws.trace) (void)hipFree(ws.trace);
   if (ws.ptask) (void)hipFree(ws.ptask);
+  if (ws.pstage) (void)hipFree(ws.pstage);
+  if (ws.pcnt) (void)hipFree(ws.pcnt);
   if (ws.host_state) (void)hipHostFree(ws.host_state);
   if (ws.host_abort) (void)hipHostFree(ws.host_abort);
   if (ws.ev0) (void)hipEventDestroy(ws.ev0);
@@ -462,6 +467,7 @@ struct Resolved {
   uint32_t pull_at;                // ADDS_F_PULL_SPLIT: the bucket of the forced first pull (PULL_AT_OFF: off)
   uint32_t order_degree;           // ... and the light degree from which vertices wait for their bucket there
   uint32_t pull_early;             // ... and the pending-item count at which the pull may come early
+  uint64_t total_n;                // vertex-instances of the solve
 };
 
 // Default bucket width.  The reference derives it from the mean weight and degree
@@ -537,6 +543,7 @@ static adds_status resolve(const adds_graph* g, const adds_params* p, uint64_t t
     return ADDS_EINVAL;
   }
   r.pull_min = std::max<uint64_t>(total_n / 2, 1u << 20);
+  r.total_n = total_n;
   // Split first bucket: the last bucket of width delta that no heavy edge can reach, i.e. one below the
   // bucket of the smallest heavy weight (the kernel's bucket key is monotone, and a heavy edge's target
   // distance is at least its weight).  Heavy: w > floor(light_delta) (int), w > light_delta (float).
@@ -612,6 +619,17 @@ static adds_status ensure_workspace(Workspace& ws, const Resolved& r, uint32_t n
     ws.ptask_cap = 0;
     HIPCHK(hipMalloc(&ws.ptask, ((size_t)r.ptask_cap + CARRY_UINT4) * 16));  // then the carry ranges
     ws.ptask_cap = r.ptask_cap;
+  }
+  if (r.pull_slots && r.pull_delta > 0.f && ws.pstage_cap < r.total_n) {
+    HIPCHK(hipDeviceSynchronize());
+    if (ws.pstage) (void)hipFree(ws.pstage);
+    if (ws.pcnt) (void)hipFree(ws.pcnt);
+    ws.pstage = nullptr;
+    ws.pcnt = nullptr;
+    ws.pstage_cap = 0;
+    HIPCHK(hipMalloc(&ws.pstage, (size_t)r.total_n * 8));
+    HIPCHK(hipMalloc(&ws.pcnt, ((size_t)r.total_n / 256 + 1) * 4));  // the pull's chunks of 256 vertex-instances
+    ws.pstage_cap = r.total_n;
   }
   if (ws.srcs_cap < nsrc) {
     if (ws.srcs) (void)hipFree(ws.srcs);
@@ -714,6 +732,8 @@ static KArgs make_args(const adds_graph* g, const Workspace& ws, const Resolved&
   a.ptask = ws.ptask;
   a.ptask_cap = ws.ptask_cap;
   a.carry = ws.ptask ? (uint2*)(ws.ptask + ws.ptask_cap) : nullptr;
+  a.pstage = (ws.pstage && ws.pstage_cap >= total_n) ? ws.pstage : nullptr;
+  a.pcnt = a.pstage ? ws.pcnt : nullptr;
   a.pull_early = r.pull_early;
   a.launch_idx = 0;
   return a;

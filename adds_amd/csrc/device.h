@@ -177,6 +177,11 @@ struct KArgs {
   // request, 2 * nb * parts entries (after the pull tasks).
   uint32_t pull_early;
   uint2* carry;
+  // re-bucketing pull: k_pull_a stages the vertices it queues at the new head per 256-vertex chunk
+  // (pstage[chunk * 256 ..], pcnt[chunk]) and each wavefront then reserves and writes all of its own at
+  // once (null: queued through the push buffer)
+  uint64_t* pstage;
+  uint32_t* pcnt;
   uint32_t ptask_cap;
   uint4* ptask;          // long-row tasks of a pull phase: (vertex-instance, first edge, end edge, 0)
 };

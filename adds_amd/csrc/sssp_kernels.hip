@@ -685,6 +685,7 @@ __device__ __forceinline__ void trace_hist(const KArgs& a, uint64_t* hl, uint64_
 }
 
 constexpr int IPL = 4;  // claimed items per lane: a claim takes at most 64 * IPL items
+static_assert(64 * IPL == 256, "the pull's staging counts are per 256 vertex-instances (api.hip)");
 
 // Pushes the reference items of a claim's split ranges into queue q in one go: one reservation for
 // all of them, the pool chunks that the reservation starts allocated in one step (consecutive physical
@@ -1789,11 +1790,12 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
   __shared__ uint32_t s_pbq[WAVES_PER_WG][PB_CAP];
   __shared__ uint64_t s_tc[WAVES_PER_WG][TC_SIZE];
   __shared__ uint64_t s_stat[WAVES_PER_WG][NSTATS];
+  __shared__ uint32_t s_chunk[WAVES_PER_WG][64];
   if (!pull_requested(a, slot) || ald(a.st + PULL_OUTCOME) != 1u) return;
   const uint32_t c = ald(a.st + PULL_BUCKET);
   const float idl = pull_req_idl(a);
-  // pushes: at the requester's width behind head c, or (re-bucketed) at the wide width behind its new head,
-  // and then every unsettled vertex with a distance is queued, improved or not
+  // pushes: at the requester's width behind head c, or (re-bucketing) at the wide width behind its new
+  // head, and then every unsettled vertex with a distance is queued, improved or not
   const bool rebucket = ald(a.st + PULL_REBUCKET) != 0;
   const float pidl = rebucket ? a.inv_delta_resume : idl;
   const uint32_t pc = rebucket ? ald(a.st + PULL_NEWHEAD) : c;
@@ -1810,7 +1812,17 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
   // chunks of 256 vertex-instances dealt round-robin over the wavefronts (no shared counter: one address
   // taking an atomic per chunk from every wavefront costs more than the scan itself)
   const uint32_t nwaves = gridDim.x * WAVES_PER_WG;
-  for (uint64_t ch = (uint64_t)blockIdx.x * WAVES_PER_WG + wv; ch * (64 * IPL) < a.total_n; ch += nwaves) {
+  const uint32_t wave_gid = blockIdx.x * WAVES_PER_WG + wv;
+  // Bulk queueing (re-bucketing pulls): the vertices queued at the new head in this wavefront's own part
+  // -- nearly all of them -- are staged per chunk and reserved and written at the end in one run, instead
+  // of a flush (a reservation per queue and a chunk-map round trip) per 256 entries.  A wavefront whose
+  // chunks could need more than 62 pool chunks queues through the push buffer.
+  const uint64_t nch = (a.total_n + 64 * IPL - 1) / (64 * IPL);
+  const uint64_t my_chunks = wave_gid < nch ? (nch - wave_gid + nwaves - 1) / nwaves : 0;
+  const bool bulk = rebucket && a.pstage && my_chunks * 64 * IPL <= 62ull * CH;
+  const uint32_t qbulk = (pc & (a.nb - 1)) * a.parts + part;
+  uint32_t stot = 0;  // entries staged by this wavefront
+  for (uint64_t ch = wave_gid; ch * (64 * IPL) < a.total_n; ch += nwaves) {
     const uint32_t base = (uint32_t)(ch * (64 * IPL));
     // items: vertex-instance x = base + j * 64 + lane (coalesced); range index k = lane * IPL + j
     uint32_t x[IPL], dv[IPL], rs[IPL], deg[IPL], gb[IPL], best[IPL], he[IPL];
@@ -2001,7 +2013,7 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
       }
     }
     wave_sync();
-    uint32_t nimp = 0;
+    uint32_t nimp = 0, sc = 0;
 #pragma unroll
     for (int j = 0; j < IPL; j++) {
       const uint32_t bj = s_best[wv][lane * IPL + j];
@@ -2010,16 +2022,56 @@ __global__ void __launch_bounds__(256) k_pull_a(KArgs a, uint32_t slot) {
       nimp += (uint32_t)__popcll(ballot(imp));
       const uint32_t pp = REPL ? push_part(a, gb[j], part) : part;
       const bool q = imp || (rebucket && uns[j] && bj < INF);
-      pb_add(a, pb, q, ((uint64_t)bj << 32) | x[j], light_queue<FLT>(a, pidl, bj, pp, pc, QM_OWN), ws);
+      const uint32_t lq = light_queue<FLT>(a, pidl, bj, pp, pc, QM_OWN);
+      const bool st = bulk && q && lq == qbulk;
+      const uint64_t sm = ballot(st);
+      if (st) a.pstage[(size_t)base + sc + mbcnt(sm)] = ((uint64_t)bj << 32) | x[j];
+      sc += (uint32_t)__popcll(sm);
+      pb_add(a, pb, q && !st, ((uint64_t)bj << 32) | x[j], lq, ws);
     }
     wave_sync();
     if (lane == 0) {
       ws.v[ST_EDGES] += total;
       ws.v[ST_PULL_EDGES] += total;
       ws.v[ST_PULL_IMPROVED] += nimp;
+      if (bulk) a.pcnt[ch] = sc;
     }
+    stot += sc;
   }
   pb_flush(flush_args(a, false), pb, ws);
+  if (bulk && stot) {
+    // reserve this wavefront's run, allocate the pool chunks whose first slot it holds (all of them
+    // before any lookup: see flush_buffer), look up the one it shares with the run before it, then copy
+    uint32_t rb = 0;
+    if (lane == 0) rb = afadd(qtail(a, qbulk), stot);
+    rb = rdlane(rb, 0);
+    const uint32_t j0 = rb >> CH_SHIFT, j1 = (rb + stot - 1) >> CH_SHIFT;  // j1 - j0 < 64 (62 chunks + 2)
+    const uint32_t jl = j0 + lane;
+    uint32_t ph = TAB_BAD;
+    if (jl <= j1) {
+      if (jl >= a.maxv) {
+        set_error(a, DERR_OVERFLOW);
+      } else if ((jl << CH_SHIFT) >= rb) {
+        ph = alloc_chunk(a, a.tab + (size_t)qbulk * a.maxv + jl);
+      }
+    }
+    if (jl <= j1 && jl < a.maxv && (jl << CH_SHIFT) < rb) ph = wait_tab(a, a.tab + (size_t)qbulk * a.maxv + jl);
+    s_chunk[wv][lane] = ph;
+    // the staged entries and counts were written by other lanes of this wavefront
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    wave_sync();
+    uint32_t off = 0;
+    for (uint64_t ch = wave_gid; ch * (64 * IPL) < a.total_n; ch += nwaves) {
+      const uint32_t cnt = a.pcnt[ch];
+      for (uint32_t i = lane; i < cnt; i += 64) {
+        const uint32_t pos = rb + off + i;
+        const uint32_t pch = s_chunk[wv][(pos >> CH_SHIFT) - j0];
+        if (pch < TAB_BAD) ast64(a.pool + ((size_t)pch << CH_SHIFT) + (pos & (CH - 1)), a.pstage[ch * (64 * IPL) + i]);
+      }
+      off += cnt;
+    }
+    if (lane == 0) ws.v[ST_PUSH] += stot;
+  }
   publish_stats(a.st, s_stat, ws, nullptr);
 }
 
