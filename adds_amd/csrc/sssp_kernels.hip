@@ -2181,39 +2181,44 @@ __global__ void __launch_bounds__(256) k_pull_carry(KArgs a, uint32_t slot) {
   for (int i = 0; i < NSTATS; i++) ws.v[i] = 0;
   const uint32_t nq = nq_of(a);
   const uint32_t nwaves = gridDim.x * WAVES_PER_WG, wave_gid = blockIdx.x * WAVES_PER_WG + wv;
-  // 64-position windows of every light queue's range, dealt round-robin over the wavefronts
+  // 64-position windows of every light queue's range, dealt round-robin over the wavefronts.  The ranges
+  // are read 64 queues per load and only the non-empty ones visited (a few dozen of the nb * parts).
   uint32_t widx = 0;
-  for (uint32_t q = 0; q < nq; q++) {
-    const uint2 ht = a.carry[q];
-    if (ht.y <= ht.x) continue;
-    const uint32_t nwin = (ht.y - ht.x + 63) / 64;
-    uint32_t first = (wave_gid + nwaves - widx % nwaves) % nwaves;  // this wavefront's first window here
-    for (uint32_t w = first; w < nwin; w += nwaves) {
-      const uint32_t pos = ht.x + w * 64 + lane;
-      const bool in = pos < ht.y;
-      uint64_t e = EMPTY64;
-      if (in) {
-        const uint32_t ph = a.tab[(size_t)q * a.maxv + (pos >> CH_SHIFT)];
-        if (ph < TAB_BAD) e = a.pool[((size_t)ph << CH_SHIFT) + (pos & (CH - 1))];
-      }
-      bool keep = false;
-      uint32_t qd = 0, gbv = 0;
-      if (e != EMPTY64) {
-        const uint32_t gid = (uint32_t)e, du = (uint32_t)(e >> 32);
-        uint32_t vtx = gid;
-        qd = du;
-        if (du == REF_TAG) {
-          const uint32_t* r = a.hrec + (size_t)(gid >> REF_CHUNK_BITS) * HREC_WORDS;
-          vtx = ald(r + 0);
-          qd = ald(r + 1);
+  for (uint32_t q0 = 0; q0 < nq; q0 += 64) {
+    const uint2 htl = q0 + lane < nq ? a.carry[q0 + lane] : make_uint2(0u, 0u);
+    uint64_t ne = ballot(htl.y > htl.x);
+    for (; ne; ne &= ne - 1) {
+      const uint32_t ql = (uint32_t)__builtin_ctzll(ne), q = q0 + ql;
+      const uint2 ht = make_uint2(rdlane(htl.x, ql), rdlane(htl.y, ql));
+      const uint32_t nwin = (ht.y - ht.x + 63) / 64;
+      uint32_t first = (wave_gid + nwaves - widx % nwaves) % nwaves;  // this wavefront's first window here
+      for (uint32_t w = first; w < nwin; w += nwaves) {
+        const uint32_t pos = ht.x + w * 64 + lane;
+        const bool in = pos < ht.y;
+        uint64_t e = EMPTY64;
+        if (in) {
+          const uint32_t ph = a.tab[(size_t)q * a.maxv + (pos >> CH_SHIFT)];
+          if (ph < TAB_BAD) e = a.pool[((size_t)ph << CH_SHIFT) + (pos & (CH - 1))];
         }
-        keep = a.dist[vtx] >= qd && !unsettled<FLT>(qd, c, idl);
-        gbv = REPL ? vtx - vtx % a.n : 0u;
+        bool keep = false;
+        uint32_t qd = 0, gbv = 0;
+        if (e != EMPTY64) {
+          const uint32_t gid = (uint32_t)e, du = (uint32_t)(e >> 32);
+          uint32_t vtx = gid;
+          qd = du;
+          if (du == REF_TAG) {
+            const uint32_t* r = a.hrec + (size_t)(gid >> REF_CHUNK_BITS) * HREC_WORDS;
+            vtx = ald(r + 0);
+            qd = ald(r + 1);
+          }
+          keep = a.dist[vtx] >= qd && !unsettled<FLT>(qd, c, idl);
+          gbv = REPL ? vtx - vtx % a.n : 0u;
+        }
+        const uint32_t pp = REPL ? push_part(a, gbv, part) : part;
+        pb_add(a, pb, keep, e, light_queue<FLT>(a, pidl, qd, pp, pc, QM_OWN), ws);
       }
-      const uint32_t pp = REPL ? push_part(a, gbv, part) : part;
-      pb_add(a, pb, keep, e, light_queue<FLT>(a, pidl, qd, pp, pc, QM_OWN), ws);
+      widx += nwin;
     }
-    widx += nwin;
   }
   pb_flush(flush_args(a, false), pb, ws);
   publish_stats(a.st, s_stat, ws, nullptr);
